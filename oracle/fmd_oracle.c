@@ -1,0 +1,818 @@
+/*
+ * fmd_oracle.c -- FP64 CPU restatement of the fuller_md force / integrator path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP kernels are
+ * compared against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it; the product library never links or calls it.
+ *
+ * Every function follows one reference function; the citation is given as
+ * alias:line with
+ *   C1 = src/fuller_LJ_npt_md_core_serial.cpp
+ *   LJ = src/fuller_LJ_npt_md_serial_omp_acc.cpp
+ *   AB = src/fuller_airebo_npt_md_serial_omp_acc.cpp
+ * The arithmetic order follows the reference so that the serial reference and
+ * this restatement agree to a few ulps on identical inputs.
+ *
+ * How each function is pinned (besides the golden vectors written by the
+ * reference build, tests/golden/):
+ *   orc_rigid_forces     golden vectors (C1 and LJ), Newton-3 on molecule sums,
+ *                        finite-difference gradient of the energy.
+ *   orc_rigid_nlist      golden lists, set equality with brute force.
+ *   orc_rigid_step_npt   golden multi-step trajectories (C1).
+ *   orc_compute_rebo     golden vectors (compat); physical mode by FD gradient
+ *                        and strain-derivative virial; dimer closed form.
+ *   orc_compute_lj       golden vectors; FD gradient.
+ *   orc_build_nlist_*    golden lists.
+ *   orc_airebo_step_npt  golden multi-step trajectories (AB).
+ */
+#include <math.h>
+#include <string.h>
+
+#include "fmd_oracle.h"
+
+#define H9(h, i, j) ((h)[3 * (i) + (j)])
+
+/* Units: A, amu, eV, fs (C1:84-87). */
+static const double CONV = 9.64853321e-3;
+static const double KB = 8.617333262e-5;
+static const double EV2GPA = 160.21766208;
+
+/* REBO-II C-C parameters (AB:110-116). */
+static const double Q_CC = 0.3134602960833, A_CC = 10953.544162170,
+                    ALPHA_CC = 4.7465390606595;
+static const double B1_CC = 12388.79197798, BETA1_CC = 4.7204523127;
+static const double B2_CC = 17.56740646509, BETA2_CC = 1.4332132499;
+static const double B3_CC = 30.71493208065, BETA3_CC = 1.3826912506;
+static const double DMIN_CC = 1.7, DMAX_CC = 2.0, BO_DELTA = 0.5;
+static const double G_A0 = 0.00020813, G_C0 = 330.0, G_D0 = 3.5;
+static const double PI_ = 3.14159265358979323846;
+
+static double clampd(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+/* mat_det9 C1:137-141 */
+double orc_mat_det9(const double *h) {
+  return H9(h, 0, 0) * (H9(h, 1, 1) * H9(h, 2, 2) - H9(h, 1, 2) * H9(h, 2, 1)) -
+         H9(h, 0, 1) * (H9(h, 1, 0) * H9(h, 2, 2) - H9(h, 1, 2) * H9(h, 2, 0)) +
+         H9(h, 0, 2) * (H9(h, 1, 0) * H9(h, 2, 1) - H9(h, 1, 1) * H9(h, 2, 0));
+}
+
+/* mat_inv9 C1:147-158 */
+void orc_mat_inv9(const double *h, double *hi) {
+  double d = orc_mat_det9(h), id = 1.0 / d;
+  hi[0] = id * (h[4] * h[8] - h[5] * h[7]);
+  hi[1] = id * (h[2] * h[7] - h[1] * h[8]);
+  hi[2] = id * (h[1] * h[5] - h[2] * h[4]);
+  hi[3] = id * (h[5] * h[6] - h[3] * h[8]);
+  hi[4] = id * (h[0] * h[8] - h[2] * h[6]);
+  hi[5] = id * (h[2] * h[3] - h[0] * h[5]);
+  hi[6] = id * (h[3] * h[7] - h[4] * h[6]);
+  hi[7] = id * (h[1] * h[6] - h[0] * h[7]);
+  hi[8] = id * (h[0] * h[4] - h[1] * h[3]);
+}
+
+/* mimg_flat C1:166-175: fractional round-to-nearest minimum image. */
+void orc_mimg(double *d, const double *hi, const double *h) {
+  double dx = d[0], dy = d[1], dz = d[2];
+  double s0 = hi[0] * dx + hi[1] * dy + hi[2] * dz;
+  double s1 = hi[3] * dx + hi[4] * dy + hi[5] * dz;
+  double s2 = hi[6] * dx + hi[7] * dy + hi[8] * dz;
+  s0 -= round(s0);
+  s1 -= round(s1);
+  s2 -= round(s2);
+  d[0] = h[0] * s0 + h[1] * s1 + h[2] * s2;
+  d[1] = h[3] * s0 + h[4] * s1 + h[5] * s2;
+  d[2] = h[6] * s0 + h[7] * s1 + h[8] * s2;
+}
+
+/* apply_pbc C1:309-320: wrap into [0,1) fractional. */
+void orc_apply_pbc(double *pos, const double *h, const double *hi, int n) {
+  for (int i = 0; i < n; i++) {
+    double px = pos[i * 3], py = pos[i * 3 + 1], pz = pos[i * 3 + 2];
+    double s0 = hi[0] * px + hi[1] * py + hi[2] * pz;
+    double s1 = hi[3] * px + hi[4] * py + hi[5] * pz;
+    double s2 = hi[6] * px + hi[7] * py + hi[8] * pz;
+    s0 -= floor(s0);
+    s1 -= floor(s1);
+    s2 -= floor(s2);
+    pos[i * 3] = h[0] * s0 + h[1] * s1 + h[2] * s2;
+    pos[i * 3 + 1] = h[3] * s0 + h[4] * s1 + h[5] * s2;
+    pos[i * 3 + 2] = h[6] * s0 + h[7] * s1 + h[8] * s2;
+  }
+}
+
+/* q2R_flat C1:186-191 (q = [w,x,y,z]). */
+void orc_q2R(const double *q, double *R) {
+  double w = q[0], x = q[1], y = q[2], z = q[3];
+  R[0] = 1 - 2 * (y * y + z * z);
+  R[1] = 2 * (x * y - w * z);
+  R[2] = 2 * (x * z + w * y);
+  R[3] = 2 * (x * y + w * z);
+  R[4] = 1 - 2 * (x * x + z * z);
+  R[5] = 2 * (y * z - w * x);
+  R[6] = 2 * (x * z - w * y);
+  R[7] = 2 * (y * z + w * x);
+  R[8] = 1 - 2 * (x * x + y * y);
+}
+
+/* qmul_flat C1:193-198 */
+static void qmul(const double *a, const double *b, double *o) {
+  o[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  o[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+  o[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+  o[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+}
+
+/* qnorm_flat C1:200-204 */
+static void qnorm(double *q) {
+  double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  double inv = 1.0 / n;
+  q[0] *= inv;
+  q[1] *= inv;
+  q[2] *= inv;
+  q[3] *= inv;
+}
+
+/* omega2dq_flat C1:206-216: exact rotation by |w| dt about w. */
+static void omega2dq(double wx, double wy, double wz, double dt, double *dq) {
+  double wm = sqrt(wx * wx + wy * wy + wz * wz);
+  double th = wm * dt * 0.5;
+  if (th < 1e-14) {
+    dq[0] = 1.0;
+    dq[1] = 0.5 * dt * wx;
+    dq[2] = 0.5 * dt * wy;
+    dq[3] = 0.5 * dt * wz;
+  } else {
+    double s = sin(th) / wm;
+    dq[0] = cos(th);
+    dq[1] = s * wx;
+    dq[2] = s * wy;
+    dq[3] = s * wz;
+  }
+}
+
+/* ---------------------------------------------------------------- LJ params */
+
+/* Rigid constants: sigma=3.431, eps=2.635e-3, rc=3 sigma, VSHFT with
+ * sigma/rc = 1/3 (C1:93-104, LJ:103-107). */
+void orc_lj_params_rigid(orc_lj_params *p) {
+  const double sigma = 3.431, eps = 2.635e-3;
+  const double rc = 3.0 * sigma;
+  const double sr = 1.0 / 3.0, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+  p->sig2 = sigma * sigma;
+  p->eps = eps;
+  p->rc2 = rc * rc;
+  p->vshift = 4.0 * eps * (sr6 * sr6 - sr6);
+}
+
+/* AIREBO intermolecular constants: sigma=3.40, eps=2.84e-3, rc=3 sigma,
+ * VSHFT with sigma/rc (AB:119-124). */
+void orc_lj_params_airebo(orc_lj_params *p) {
+  const double sigma = 3.40, eps = 2.84e-3;
+  const double rc = 3.0 * sigma;
+  const double sr = sigma / rc, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+  p->sig2 = sigma * sigma;
+  p->eps = eps;
+  p->rc2 = rc * rc;
+  p->vshift = 4 * eps * (sr6 * sr6 - sr6);
+}
+
+/* ------------------------------------------------------------- rigid body */
+
+/* nlist_build_sym C1:280-301: symmetric full list of molecule pairs with
+ * COM distance < rmcut+3 (minimum image), capped at max_neigh per molecule
+ * in discovery order.  Overflowing entries are dropped (the reference only
+ * warns). */
+void orc_rigid_nlist(const double *pos, const double *h, const double *hi,
+                     int n, double rmcut, int max_neigh, int *nl_count,
+                     int *nl_list) {
+  double rc2 = (rmcut + 3.0) * (rmcut + 3.0);
+  for (int i = 0; i < n; i++) nl_count[i] = 0;
+  for (int i = 0; i < n; i++) {
+    for (int j = i + 1; j < n; j++) {
+      double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                     pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(d, hi, h);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 < rc2) {
+        int ci = nl_count[i], cj = nl_count[j];
+        if (ci < max_neigh) {
+          nl_list[i * max_neigh + ci] = j;
+          nl_count[i]++;
+        }
+        if (cj < max_neigh) {
+          nl_list[j * max_neigh + cj] = i;
+          nl_count[j]++;
+        }
+      }
+    }
+  }
+}
+
+/* Lab-frame site offsets, C1:354-364. */
+void orc_rigid_lab(const double *qv, const double *body, int n, int natom,
+                   double *lab) {
+  for (int i = 0; i < n; i++) {
+    double R[9];
+    orc_q2R(&qv[i * 4], R);
+    for (int a = 0; a < natom; a++) {
+      double bx = body[a * 3], by = body[a * 3 + 1], bz = body[a * 3 + 2];
+      int idx = i * natom * 3 + a * 3;
+      lab[idx] = R[0] * bx + R[1] * by + R[2] * bz;
+      lab[idx + 1] = R[3] * bx + R[4] * by + R[5] * bz;
+      lab[idx + 2] = R[6] * bx + R[7] * by + R[8] * bz;
+    }
+  }
+}
+
+/* forces C1:346-444 (= LJ:271-400): site-site shifted LJ between rigid
+ * molecules over the symmetric molecule list.  Each molecule accumulates only
+ * its own force / torque; energy and virial carry a factor 1/2 for the double
+ * counting.  Returns the potential energy. */
+double orc_rigid_forces(double *Fv, double *Tv, double *Wm9, const double *pos,
+                        const double *qv, const double *body, const double *h,
+                        const double *hi, const int *nl_count,
+                        const int *nl_list, int max_neigh, int n, int natom,
+                        double rmcut2, const orc_lj_params *p, double *lab) {
+  orc_rigid_lab(qv, body, n, natom, lab);
+  for (int i = 0; i < n * 3; i++) {
+    Fv[i] = 0.0;
+    Tv[i] = 0.0;
+  }
+  for (int i = 0; i < 9; i++) Wm9[i] = 0.0;
+  const double sig2 = p->sig2, eps = p->eps, rc2 = p->rc2, vsh = p->vshift;
+  double Ep = 0.0;
+  for (int i = 0; i < n; i++) {
+    double fi0 = 0, fi1 = 0, fi2 = 0, ti0 = 0, ti1 = 0, ti2 = 0, my_Ep = 0;
+    double w00 = 0, w01 = 0, w02 = 0, w10 = 0, w11 = 0, w12 = 0, w20 = 0,
+           w21 = 0, w22 = 0;
+    int nni = nl_count[i];
+    for (int k = 0; k < nni; k++) {
+      int j = nl_list[i * max_neigh + k];
+      double dm[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                      pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(dm, hi, h);
+      double dmx = dm[0], dmy = dm[1], dmz = dm[2];
+      if (dmx * dmx + dmy * dmy + dmz * dmz > rmcut2) continue;
+      for (int ai = 0; ai < natom; ai++) {
+        int ia = i * natom * 3 + ai * 3;
+        double rax = lab[ia], ray = lab[ia + 1], raz = lab[ia + 2];
+        for (int bj = 0; bj < natom; bj++) {
+          int jb = j * natom * 3 + bj * 3;
+          double ddx = dmx + lab[jb] - rax;
+          double ddy = dmy + lab[jb + 1] - ray;
+          double ddz = dmz + lab[jb + 2] - raz;
+          double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (r2 < rc2) {
+            if (r2 < 0.25) r2 = 0.25;
+            double ri2 = 1.0 / r2;
+            double sr2 = sig2 * ri2;
+            double sr6 = sr2 * sr2 * sr2;
+            double sr12 = sr6 * sr6;
+            double fm = 24.0 * eps * (2.0 * sr12 - sr6) * ri2;
+            double fx = fm * ddx, fy = fm * ddy, fz = fm * ddz;
+            fi0 -= fx;
+            fi1 -= fy;
+            fi2 -= fz;
+            ti0 -= (ray * fz - raz * fy);
+            ti1 -= (raz * fx - rax * fz);
+            ti2 -= (rax * fy - ray * fx);
+            my_Ep += 0.5 * (4.0 * eps * (sr12 - sr6) - vsh);
+            w00 += 0.5 * ddx * fx;
+            w01 += 0.5 * ddx * fy;
+            w02 += 0.5 * ddx * fz;
+            w10 += 0.5 * ddy * fx;
+            w11 += 0.5 * ddy * fy;
+            w12 += 0.5 * ddy * fz;
+            w20 += 0.5 * ddz * fx;
+            w21 += 0.5 * ddz * fy;
+            w22 += 0.5 * ddz * fz;
+          }
+        }
+      }
+    }
+    Fv[i * 3] = fi0;
+    Fv[i * 3 + 1] = fi1;
+    Fv[i * 3 + 2] = fi2;
+    Tv[i * 3] = ti0;
+    Tv[i * 3 + 1] = ti1;
+    Tv[i * 3 + 2] = ti2;
+    Wm9[0] += w00;
+    Wm9[1] += w01;
+    Wm9[2] += w02;
+    Wm9[3] += w10;
+    Wm9[4] += w11;
+    Wm9[5] += w12;
+    Wm9[6] += w20;
+    Wm9[7] += w21;
+    Wm9[8] += w22;
+    Ep += my_Ep;
+  }
+  return Ep;
+}
+
+/* ke_trans / ke_rot C1:451-463 */
+double orc_ke_trans(const double *vel, int n, double mmol) {
+  double s = 0;
+  for (int i = 0; i < n; i++)
+    s += vel[i * 3] * vel[i * 3] + vel[i * 3 + 1] * vel[i * 3 + 1] +
+         vel[i * 3 + 2] * vel[i * 3 + 2];
+  return 0.5 * mmol * s / CONV;
+}
+
+double orc_ke_rot(const double *omg, int n, double i0) {
+  double s = 0;
+  for (int i = 0; i < n; i++)
+    s += omg[i * 3] * omg[i * 3] + omg[i * 3 + 1] * omg[i * 3 + 1] +
+         omg[i * 3 + 2] * omg[i * 3 + 2];
+  return 0.5 * i0 * s / CONV;
+}
+
+/* inst_T / inst_P C1:465-468 */
+double orc_inst_T(double ke, int nf) { return 2 * ke / (nf * KB); }
+double orc_inst_P(const double *W, double ket, double V) {
+  return (2 * ket + W[0] + W[4] + W[8]) / (3 * V) * EV2GPA;
+}
+
+/* make_npt C1:480-488 (rigid: Nf = 6N-3) and AB:512-515 (atomic: 3Na-3). */
+void orc_make_npt(orc_npt *s, double T, double Pe, int nf) {
+  s->xi = 0;
+  s->Q = fmax(nf * KB * T * 100.0 * 100.0, 1e-20);
+  for (int i = 0; i < 9; i++) s->Vg[i] = 0;
+  s->W = fmax((nf + 9) * KB * T * 1000.0 * 1000.0, 1e-20);
+  s->Pe = Pe;
+  s->Tt = T;
+  s->Nf = nf;
+}
+
+/* step_npt C1:504-601 (= LJ:434-538): one Nose-Hoover + Parrinello-Rahman
+ * velocity-Verlet step of rigid molecules.  Clamps: xi in +-0.1, diagonal
+ * Vg in +-0.01.  out[0] = Ep, out[1] = KE after the second half-kick. */
+void orc_rigid_step_npt(double *pos, double *vel, double *qv, double *omg,
+                        double *Fv, double *Tv, double *Wm9, double *h,
+                        double *hi, const double *body, double I0, double Mmol,
+                        int n, int natom, double rmcut2, double dt,
+                        orc_npt *npt, const int *nl_count, const int *nl_list,
+                        int max_neigh, const orc_lj_params *p, double *lab,
+                        double *out) {
+  double hdt = 0.5 * dt;
+  orc_mat_inv9(h, hi);
+  double V = fabs(orc_mat_det9(h));
+  double kt = orc_ke_trans(vel, n, Mmol), kr = orc_ke_rot(omg, n, I0),
+         KE = kt + kr;
+  npt->xi += hdt * (2 * KE - npt->Nf * KB * npt->Tt) / npt->Q;
+  npt->xi = clampd(npt->xi, -0.1, 0.1);
+  double dP = orc_inst_P(Wm9, kt, V) - npt->Pe;
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] += hdt * V * dP / (npt->W * EV2GPA);
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] = clampd(npt->Vg[a * 4], -0.01, 0.01);
+  double eps_tr = npt->Vg[0] * hi[0] + npt->Vg[4] * hi[4] + npt->Vg[8] * hi[8];
+  double sc_nh = exp(-hdt * npt->xi);
+  double sc_pr = exp(-hdt * eps_tr / 3.0);
+  double sc_v = sc_nh * sc_pr;
+  double cF = CONV / Mmol, cT = CONV / I0;
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++) {
+      vel[i * 3 + a] = vel[i * 3 + a] * sc_v + hdt * Fv[i * 3 + a] * cF;
+      omg[i * 3 + a] = omg[i * 3 + a] * sc_nh + hdt * Tv[i * 3 + a] * cT;
+    }
+  for (int i = 0; i < n; i++) {
+    double px = pos[i * 3], py = pos[i * 3 + 1], pz = pos[i * 3 + 2];
+    double vx = vel[i * 3], vy = vel[i * 3 + 1], vz = vel[i * 3 + 2];
+    double sx = hi[0] * px + hi[1] * py + hi[2] * pz;
+    double sy = hi[3] * px + hi[4] * py + hi[5] * pz;
+    double sz = hi[6] * px + hi[7] * py + hi[8] * pz;
+    double vsx = hi[0] * vx + hi[1] * vy + hi[2] * vz;
+    double vsy = hi[3] * vx + hi[4] * vy + hi[5] * vz;
+    double vsz = hi[6] * vx + hi[7] * vy + hi[8] * vz;
+    sx += dt * vsx;
+    sy += dt * vsy;
+    sz += dt * vsz;
+    sx -= floor(sx);
+    sy -= floor(sy);
+    sz -= floor(sz);
+    pos[i * 3] = sx;
+    pos[i * 3 + 1] = sy;
+    pos[i * 3 + 2] = sz;
+  }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) h[a * 3 + b] += dt * npt->Vg[a * 3 + b];
+  for (int i = 0; i < n; i++) {
+    double sx = pos[i * 3], sy = pos[i * 3 + 1], sz = pos[i * 3 + 2];
+    pos[i * 3] = h[0] * sx + h[1] * sy + h[2] * sz;
+    pos[i * 3 + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
+    pos[i * 3 + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
+  }
+  for (int i = 0; i < n; i++) {
+    double dq[4], tmp[4];
+    omega2dq(omg[i * 3], omg[i * 3 + 1], omg[i * 3 + 2], dt, dq);
+    qmul(&qv[i * 4], dq, tmp);
+    qv[i * 4] = tmp[0];
+    qv[i * 4 + 1] = tmp[1];
+    qv[i * 4 + 2] = tmp[2];
+    qv[i * 4 + 3] = tmp[3];
+    qnorm(&qv[i * 4]);
+  }
+  orc_mat_inv9(h, hi);
+  double Ep = orc_rigid_forces(Fv, Tv, Wm9, pos, qv, body, h, hi, nl_count,
+                               nl_list, max_neigh, n, natom, rmcut2, p, lab);
+  double eps_tr2 = npt->Vg[0] * hi[0] + npt->Vg[4] * hi[4] + npt->Vg[8] * hi[8];
+  double sc_v2 = sc_nh * exp(-hdt * eps_tr2 / 3.0);
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++) {
+      vel[i * 3 + a] = (vel[i * 3 + a] + hdt * Fv[i * 3 + a] * cF) * sc_v2;
+      omg[i * 3 + a] = (omg[i * 3 + a] + hdt * Tv[i * 3 + a] * cT) * sc_nh;
+    }
+  kt = orc_ke_trans(vel, n, Mmol);
+  kr = orc_ke_rot(omg, n, I0);
+  KE = kt + kr;
+  npt->xi += hdt * (2 * KE - npt->Nf * KB * npt->Tt) / npt->Q;
+  npt->xi = clampd(npt->xi, -0.1, 0.1);
+  double V2 = fabs(orc_mat_det9(h));
+  dP = orc_inst_P(Wm9, kt, V2) - npt->Pe;
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] += hdt * V2 * dP / (npt->W * EV2GPA);
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] = clampd(npt->Vg[a * 4], -0.01, 0.01);
+  out[0] = Ep;
+  out[1] = KE;
+}
+
+/* ------------------------------------------------------------------ AIREBO */
+
+/* fc_d / dfc_d AB:165-171: cosine switch between DMIN and DMAX. */
+double orc_rebo_fc(double r) {
+  if (r <= DMIN_CC) return 1.0;
+  if (r >= DMAX_CC) return 0.0;
+  return 0.5 * (1.0 + cos(PI_ * (r - DMIN_CC) / (DMAX_CC - DMIN_CC)));
+}
+double orc_rebo_dfc(double r) {
+  if (r <= DMIN_CC || r >= DMAX_CC) return 0.0;
+  return -0.5 * PI_ / (DMAX_CC - DMIN_CC) * sin(PI_ * (r - DMIN_CC) / (DMAX_CC - DMIN_CC));
+}
+/* VR_CC_d / dVR_CC_d AB:173-177 */
+double orc_rebo_vr(double r) { return (1.0 + Q_CC / r) * A_CC * exp(-ALPHA_CC * r); }
+double orc_rebo_dvr(double r) {
+  double ex = A_CC * exp(-ALPHA_CC * r);
+  return (-Q_CC / (r * r)) * ex + (1.0 + Q_CC / r) * (-ALPHA_CC) * ex;
+}
+/* VA_CC_d / dVA_CC_d AB:179-183 */
+double orc_rebo_va(double r) {
+  return B1_CC * exp(-BETA1_CC * r) + B2_CC * exp(-BETA2_CC * r) +
+         B3_CC * exp(-BETA3_CC * r);
+}
+double orc_rebo_dva(double r) {
+  return -BETA1_CC * B1_CC * exp(-BETA1_CC * r) - BETA2_CC * B2_CC * exp(-BETA2_CC * r) -
+         BETA3_CC * B3_CC * exp(-BETA3_CC * r);
+}
+/* G_C_d / dG_C_d AB:185-191: Brenner-1990 rational angular function. */
+double orc_rebo_G(double x) {
+  double c2 = G_C0 * G_C0, d2 = G_D0 * G_D0, hh = 1.0 + x;
+  return G_A0 * (1.0 + c2 / d2 - c2 / (d2 + hh * hh));
+}
+double orc_rebo_dG(double x) {
+  double c2 = G_C0 * G_C0, d2 = G_D0 * G_D0, hh = 1.0 + x, dn = d2 + hh * hh;
+  return G_A0 * 2.0 * c2 * hh / (dn * dn);
+}
+
+/* build_nlist_rebo AB:251-261: symmetric list of same-molecule atoms within
+ * rcut (reference 2.3 A), capped in discovery order. */
+void orc_build_nlist_rebo(const double *pos, const double *h, const double *hi,
+                          int na, const int *mol_id, double rcut, int max_neigh,
+                          int *nlc, int *nll) {
+  double rc2 = rcut * rcut;
+  for (int i = 0; i < na; i++) nlc[i] = 0;
+  for (int i = 0; i < na - 1; i++) {
+    int mi = mol_id[i];
+    for (int j = i + 1; j < na; j++) {
+      if (mol_id[j] != mi) continue;
+      double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                     pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(d, hi, h);
+      if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rc2) {
+        if (nlc[i] < max_neigh) {
+          nll[i * max_neigh + nlc[i]] = j;
+          nlc[i]++;
+        }
+        if (nlc[j] < max_neigh) {
+          nll[j * max_neigh + nlc[j]] = i;
+          nlc[j]++;
+        }
+      }
+    }
+  }
+}
+
+/* build_nlist_lj AB:263-272: half list (j > i) of other-molecule atoms
+ * within rlist (reference rc+2 = 12.2 A), keeping the first max_neigh
+ * (reference 400) in ascending j; the rest are silently dropped. */
+void orc_build_nlist_lj(const double *pos, const double *h, const double *hi,
+                        int na, const int *mol_id, double rlist, int max_neigh,
+                        int *nlc, int *nll) {
+  double rc2 = rlist * rlist;
+  for (int i = 0; i < na; i++) nlc[i] = 0;
+  for (int i = 0; i < na - 1; i++) {
+    int mi = mol_id[i];
+    for (int j = i + 1; j < na; j++) {
+      if (mol_id[j] == mi) continue;
+      double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                     pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(d, hi, h);
+      if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rc2) {
+        if (nlc[i] < max_neigh) {
+          nll[i * max_neigh + nlc[i]] = j;
+          nlc[i]++;
+        }
+      }
+    }
+  }
+}
+
+/* One angular term of the bond order around centre c for bond c->p with the
+ * third atom t (all displacement vectors taken from c).  Returns the two
+ * force vectors of the physical gradient: ft on t and fp on p (the centre
+ * receives -(ft+fp)).  coeff = 0.5 fc(r_cp) VA(r_cp) db/dGs. */
+static void rebo_angular(const double *rcp, double r_cp, const double *rct,
+                         double r_ct, double fc_ct, double dfc_ct, double coeff,
+                         double *ft, double *fp) {
+  double uct[3] = {rct[0] / r_ct, rct[1] / r_ct, rct[2] / r_ct};
+  double ucp[3] = {rcp[0] / r_cp, rcp[1] / r_cp, rcp[2] / r_cp};
+  double cs = (rcp[0] * rct[0] + rcp[1] * rct[1] + rcp[2] * rct[2]) / (r_cp * r_ct);
+  if (cs < -1.0) cs = -1.0;
+  if (cs > 1.0) cs = 1.0;
+  double g = orc_rebo_G(cs), dg = orc_rebo_dG(cs);
+  for (int a = 0; a < 3; a++) {
+    ft[a] = coeff * (dfc_ct * g * uct[a] + fc_ct * dg * (ucp[a] - cs * uct[a]) / r_ct);
+    fp[a] = coeff * fc_ct * dg * (uct[a] - cs * ucp[a]) / r_cp;
+  }
+}
+
+/* compute_rebo AB:275-421.
+ *   mode ORC_REBO_COMPAT: exactly the reference -- pair force and pair-only
+ *     virial per bond i<j; angular forces on k (and l) with the reference's
+ *     sign and without the d(cos)/d(r_j) term (AB:360-419).
+ *   mode ORC_REBO_PHYSICAL: the exact negative gradient of the same energy
+ *     (angular forces on k, j, i) and the full virial sum r_ab (x) F_b.
+ * Adds into F and vir9 (the caller zeroes them, as compute_forces does,
+ * AB:481-492).  Returns the REBO energy. */
+double orc_compute_rebo(double *F, double *vir9, const double *pos,
+                        const double *h, const double *hi, const int *nlc,
+                        const int *nll, int max_neigh, int na, int mode) {
+  double Ep = 0;
+  for (int i = 0; i < na; i++) {
+    int nni = nlc[i];
+    for (int jn = 0; jn < nni; jn++) {
+      int j = nll[i * max_neigh + jn];
+      if (j <= i) continue;
+      double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                     pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(d, hi, h);
+      double dx = d[0], dy = d[1], dz = d[2];
+      double rij = sqrt(dx * dx + dy * dy + dz * dz);
+      if (rij > DMAX_CC) continue;
+      double fcut = orc_rebo_fc(rij), dfcut = orc_rebo_dfc(rij);
+      if (fcut < 1e-15 && dfcut == 0) continue;
+      double vr = orc_rebo_vr(rij), dvr = orc_rebo_dvr(rij);
+      double va = orc_rebo_va(rij), dva = orc_rebo_dva(rij);
+      double rij_inv = 1.0 / rij;
+      double rhat0 = dx * rij_inv, rhat1 = dy * rij_inv, rhat2 = dz * rij_inv;
+
+      double Gs_ij = 0;
+      for (int kn = 0; kn < nni; kn++) {
+        int k = nll[i * max_neigh + kn];
+        if (k == j) continue;
+        double dk[3] = {pos[k * 3] - pos[i * 3], pos[k * 3 + 1] - pos[i * 3 + 1],
+                        pos[k * 3 + 2] - pos[i * 3 + 2]};
+        orc_mimg(dk, hi, h);
+        double rik = sqrt(dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2]);
+        if (rik > DMAX_CC) continue;
+        double fc_ik = orc_rebo_fc(rik);
+        if (fc_ik < 1e-15) continue;
+        double costh = (dx * dk[0] + dy * dk[1] + dz * dk[2]) / (rij * rik);
+        if (costh < -1.0) costh = -1.0;
+        if (costh > 1.0) costh = 1.0;
+        Gs_ij += fc_ik * orc_rebo_G(costh);
+      }
+      double bij = pow(1.0 + Gs_ij, -BO_DELTA);
+
+      double Gs_ji = 0;
+      int nnj = nlc[j];
+      for (int ln = 0; ln < nnj; ln++) {
+        int l = nll[j * max_neigh + ln];
+        if (l == i) continue;
+        double dl[3] = {pos[l * 3] - pos[j * 3], pos[l * 3 + 1] - pos[j * 3 + 1],
+                        pos[l * 3 + 2] - pos[j * 3 + 2]};
+        orc_mimg(dl, hi, h);
+        double rjl = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+        if (rjl > DMAX_CC) continue;
+        double fc_jl = orc_rebo_fc(rjl);
+        if (fc_jl < 1e-15) continue;
+        double costh = (-dx * dl[0] - dy * dl[1] - dz * dl[2]) / (rij * rjl);
+        if (costh < -1.0) costh = -1.0;
+        if (costh > 1.0) costh = 1.0;
+        Gs_ji += fc_jl * orc_rebo_G(costh);
+      }
+      double bji = pow(1.0 + Gs_ji, -BO_DELTA);
+      double bbar = 0.5 * (bij + bji);
+
+      Ep += fcut * (vr - bbar * va);
+      double fpair = (dfcut * (vr - bbar * va) + fcut * (dvr - bbar * dva)) * rij_inv;
+      F[i * 3] += fpair * dx;
+      F[i * 3 + 1] += fpair * dy;
+      F[i * 3 + 2] += fpair * dz;
+      F[j * 3] -= fpair * dx;
+      F[j * 3 + 1] -= fpair * dy;
+      F[j * 3 + 2] -= fpair * dz;
+      for (int a = 0; a < 3; a++) {
+        double da = d[a];
+        for (int b = 0; b < 3; b++) vir9[a * 3 + b] -= da * fpair * d[b];
+      }
+
+      /* angular terms: side 0 = centre i (partner j, thirds k),
+       *                side 1 = centre j (partner i, thirds l). */
+      for (int side = 0; side < 2; side++) {
+        int c = side == 0 ? i : j;
+        int pp = side == 0 ? j : i;
+        double Gs = side == 0 ? Gs_ij : Gs_ji;
+        if (!(fabs(Gs) > 1e-20 && va > 1e-20)) continue;
+        double dbp = -BO_DELTA * pow(1.0 + Gs, -BO_DELTA - 1.0);
+        double vh = 0.5 * fcut * va;
+        double rcp[3] = {side == 0 ? dx : -dx, side == 0 ? dy : -dy, side == 0 ? dz : -dz};
+        double rhc[3] = {side == 0 ? rhat0 : -rhat0, side == 0 ? rhat1 : -rhat1,
+                         side == 0 ? rhat2 : -rhat2};
+        int nnc = nlc[c];
+        for (int tn = 0; tn < nnc; tn++) {
+          int t = nll[c * max_neigh + tn];
+          if (t == pp) continue;
+          double dt_[3] = {pos[t * 3] - pos[c * 3], pos[t * 3 + 1] - pos[c * 3 + 1],
+                           pos[t * 3 + 2] - pos[c * 3 + 2]};
+          orc_mimg(dt_, hi, h);
+          double rct = sqrt(dt_[0] * dt_[0] + dt_[1] * dt_[1] + dt_[2] * dt_[2]);
+          if (rct > DMAX_CC) continue;
+          double fc_ct = orc_rebo_fc(rct);
+          if (fc_ct < 1e-15) continue;
+          double dfc_ct = orc_rebo_dfc(rct);
+          if (mode == ORC_REBO_COMPAT) {
+            /* AB:370-386 / 398-419, operation order kept. */
+            double rct_inv = 1.0 / rct;
+            double rk[3] = {dt_[0] * rct_inv, dt_[1] * rct_inv, dt_[2] * rct_inv};
+            double costh = (rcp[0] * dt_[0] + rcp[1] * dt_[1] + rcp[2] * dt_[2]) / (rij * rct);
+            if (costh < -1.0) costh = -1.0;
+            if (costh > 1.0) costh = 1.0;
+            double gv = orc_rebo_G(costh), dgv = orc_rebo_dG(costh), coeff = -vh * dbp;
+            for (int a = 0; a < 3; a++) {
+              double fk1 = coeff * dfc_ct * gv * rk[a];
+              double dc = (rhc[a] - costh * rk[a]) * rct_inv;
+              double fk = fk1 + coeff * fc_ct * dgv * dc;
+              F[t * 3 + a] += fk;
+              F[c * 3 + a] -= fk;
+            }
+          } else {
+            double ft[3], fp[3];
+            rebo_angular(rcp, rij, dt_, rct, fc_ct, dfc_ct, vh * dbp, ft, fp);
+            for (int a = 0; a < 3; a++) {
+              F[t * 3 + a] += ft[a];
+              F[pp * 3 + a] += fp[a];
+              F[c * 3 + a] -= ft[a] + fp[a];
+            }
+            for (int a = 0; a < 3; a++)
+              for (int b = 0; b < 3; b++)
+                vir9[a * 3 + b] += dt_[a] * ft[b] + rcp[a] * fp[b];
+          }
+        }
+      }
+    }
+  }
+  return Ep;
+}
+
+/* compute_lj AB:424-473: shifted 12-6 over the half list with r^2 <= rc^2
+ * (note <=), r^2 clamped at 0.25, Newton-3 pair forces and 9-component
+ * virial sum d_a fm d_b.  Adds into F / vir9.  Returns the energy. */
+double orc_compute_lj(double *F, double *vir9, const double *pos, const double *h,
+                      const double *hi, const int *nlc, const int *nll,
+                      int max_neigh, int na, const orc_lj_params *p) {
+  const double sig2 = p->sig2, eps = p->eps, rc2 = p->rc2, vsh = p->vshift;
+  double Ep = 0;
+  for (int i = 0; i < na; i++) {
+    int nni = nlc[i];
+    for (int jn = 0; jn < nni; jn++) {
+      int j = nll[i * max_neigh + jn];
+      double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                     pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(d, hi, h);
+      double dx = d[0], dy = d[1], dz = d[2];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > rc2) continue;
+      if (r2 < 0.25) r2 = 0.25;
+      double ri2 = 1.0 / r2, sr2 = sig2 * ri2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+      double fm = 24 * eps * (2 * sr12 - sr6) * ri2;
+      Ep += 4 * eps * (sr12 - sr6) - vsh;
+      F[i * 3] -= fm * dx;
+      F[i * 3 + 1] -= fm * dy;
+      F[i * 3 + 2] -= fm * dz;
+      F[j * 3] += fm * dx;
+      F[j * 3 + 1] += fm * dy;
+      F[j * 3 + 2] += fm * dz;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) vir9[a * 3 + b] += d[a] * fm * d[b];
+    }
+  }
+  return Ep;
+}
+
+/* compute_forces AB:477-495: zero, REBO, LJ.  out = {Ep, Ep_rebo, Ep_lj}. */
+void orc_airebo_forces(double *F, double *vir9, const double *pos, const double *h,
+                       const double *hi, const int *nlc_r, const int *nll_r,
+                       int max_r, const int *nlc_l, const int *nll_l, int max_l,
+                       int na, int rebo_mode, const orc_lj_params *p, double *out) {
+  for (int i = 0; i < na * 3; i++) F[i] = 0;
+  for (int i = 0; i < 9; i++) vir9[i] = 0;
+  double er = orc_compute_rebo(F, vir9, pos, h, hi, nlc_r, nll_r, max_r, na, rebo_mode);
+  double el = orc_compute_lj(F, vir9, pos, h, hi, nlc_l, nll_l, max_l, na, p);
+  out[0] = er + el;
+  out[1] = er;
+  out[2] = el;
+}
+
+/* ke_total AB:498-506 */
+double orc_ke_total(const double *vel, const double *mass, int na) {
+  double s = 0;
+  for (int i = 0; i < na; i++)
+    s += mass[i] * (vel[i * 3] * vel[i * 3] + vel[i * 3 + 1] * vel[i * 3 + 1] +
+                    vel[i * 3 + 2] * vel[i * 3 + 2]);
+  return 0.5 * s / CONV;
+}
+
+/* step_npt AB:518-590: atomic NH + PR velocity-Verlet step.  Clamps: xi in
+ * +-0.05, diagonal Vg in +-0.005.  Note h^-1 is refreshed after the cell
+ * update, before the fractional->Cartesian map (AB:555).
+ * out = {Ep, Ep_rebo, Ep_lj, KE}. */
+void orc_airebo_step_npt(double *pos, double *vel, double *F, double *vir9,
+                         double *h, double *hi, const double *mass, int na,
+                         double dt, orc_npt *npt, const int *nlc_r,
+                         const int *nll_r, int max_r, const int *nlc_l,
+                         const int *nll_l, int max_l, int rebo_mode,
+                         const orc_lj_params *p, double *out) {
+  double hdt = 0.5 * dt, V = fabs(orc_mat_det9(h));
+  double KE = orc_ke_total(vel, mass, na);
+  npt->xi += hdt * (2 * KE - npt->Nf * KB * npt->Tt) / npt->Q;
+  npt->xi = clampd(npt->xi, -0.05, 0.05);
+  double dP = orc_inst_P(vir9, KE, V) - npt->Pe;
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] += hdt * V * dP / (npt->W * EV2GPA);
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] = clampd(npt->Vg[a * 4], -0.005, 0.005);
+  double eps = npt->Vg[0] * hi[0] + npt->Vg[4] * hi[4] + npt->Vg[8] * hi[8];
+  double sc_nh = exp(-hdt * npt->xi), sc_pr = exp(-hdt * eps / 3.0), sc_v = sc_nh * sc_pr;
+  for (int i = 0; i < na; i++) {
+    double mi = CONV / mass[i];
+    vel[i * 3] = vel[i * 3] * sc_v + hdt * F[i * 3] * mi;
+    vel[i * 3 + 1] = vel[i * 3 + 1] * sc_v + hdt * F[i * 3 + 1] * mi;
+    vel[i * 3 + 2] = vel[i * 3 + 2] * sc_v + hdt * F[i * 3 + 2] * mi;
+  }
+  for (int i = 0; i < na; i++) {
+    double px = pos[i * 3], py = pos[i * 3 + 1], pz = pos[i * 3 + 2];
+    double vx = vel[i * 3], vy = vel[i * 3 + 1], vz = vel[i * 3 + 2];
+    double sx = hi[0] * px + hi[1] * py + hi[2] * pz, sy = hi[3] * px + hi[4] * py + hi[5] * pz,
+           sz = hi[6] * px + hi[7] * py + hi[8] * pz;
+    double vsx = hi[0] * vx + hi[1] * vy + hi[2] * vz, vsy = hi[3] * vx + hi[4] * vy + hi[5] * vz,
+           vsz = hi[6] * vx + hi[7] * vy + hi[8] * vz;
+    sx += dt * vsx;
+    sy += dt * vsy;
+    sz += dt * vsz;
+    sx -= floor(sx);
+    sy -= floor(sy);
+    sz -= floor(sz);
+    pos[i * 3] = sx;
+    pos[i * 3 + 1] = sy;
+    pos[i * 3 + 2] = sz;
+  }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) h[a * 3 + b] += dt * npt->Vg[a * 3 + b];
+  orc_mat_inv9(h, hi);
+  for (int i = 0; i < na; i++) {
+    double sx = pos[i * 3], sy = pos[i * 3 + 1], sz = pos[i * 3 + 2];
+    pos[i * 3] = h[0] * sx + h[1] * sy + h[2] * sz;
+    pos[i * 3 + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
+    pos[i * 3 + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
+  }
+  double fr[3];
+  orc_airebo_forces(F, vir9, pos, h, hi, nlc_r, nll_r, max_r, nlc_l, nll_l, max_l, na,
+                    rebo_mode, p, fr);
+  double eps2 = npt->Vg[0] * hi[0] + npt->Vg[4] * hi[4] + npt->Vg[8] * hi[8];
+  double sc_v2 = sc_nh * exp(-hdt * eps2 / 3.0);
+  for (int i = 0; i < na; i++) {
+    double mi = CONV / mass[i];
+    vel[i * 3] = (vel[i * 3] + hdt * F[i * 3] * mi) * sc_v2;
+    vel[i * 3 + 1] = (vel[i * 3 + 1] + hdt * F[i * 3 + 1] * mi) * sc_v2;
+    vel[i * 3 + 2] = (vel[i * 3 + 2] + hdt * F[i * 3 + 2] * mi) * sc_v2;
+  }
+  KE = orc_ke_total(vel, mass, na);
+  npt->xi += hdt * (2 * KE - npt->Nf * KB * npt->Tt) / npt->Q;
+  npt->xi = clampd(npt->xi, -0.05, 0.05);
+  double V2 = fabs(orc_mat_det9(h));
+  dP = orc_inst_P(vir9, KE, V2) - npt->Pe;
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] += hdt * V2 * dP / (npt->W * EV2GPA);
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] = clampd(npt->Vg[a * 4], -0.005, 0.005);
+  out[0] = fr[0];
+  out[1] = fr[1];
+  out[2] = fr[2];
+  out[3] = KE;
+}
