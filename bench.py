@@ -1,0 +1,141 @@
+"""Benchmark: rigid-body LJ C60 crystal NPT on one MI355X.
+
+Workload (the rigid-LJ configuration the headline metric is quoted on):
+C60 (.cc1 geometry) FCC 10x10x10 = 4000 molecules / 240,000 sites,
+a = 14.17 A, seed 42, Maxwell velocities and random orientations at 300 K,
+NPT at 0 GPa, dt = 1 fs; FP32 site-site forces with FP64 accumulation,
+FP64 integrator / virial / cell.  One "step" = one full NPT step (list rebuild
+every 25 steps included).  value = sites x steps / s over the timed steps.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line.  Extra objects: "roofline" for the force kernel (HIP
+events on the engine's stream around every force launch) and "cpu_baseline"
+(the reference OpenMP build in oracle/_ref, timed on this host on a bounded
+sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s (AIREBO & LJ) vs host OpenMP; surrogate speedup at force RMSE"
+NC = 10
+A_LATTICE = 14.17
+SEED = 42
+T_K = 300.0
+# Algorithmic work of the rigid force kernel: 69 FLOP per unique in-cutoff
+# site pair (11 displacement + r^2, 13 LJ scalar incl. energy, 3 force, 6 force
+# accumulation, 24 torque, 12 virial) x 163.2 unique pairs per site for the
+# C60 FCC crystal at a = 14.17 A.
+FLOP_PER_SITE_STEP = 69.0 * 163.2
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps of the CPU baseline sample (default: auto, ~20 s)")
+    return ap.parse_args()
+
+
+def build_system():
+    from fuller_md_en_amd import rigid, system
+    mol = system.molecule("C60")
+    a0 = system.default_a0(mol.dmol, "FCC", 1.0)
+    scale = A_LATTICE / a0
+    sysd = system.rigid_system(nc=NC, spec="C60", T=T_K, seed=SEED, init_scale=scale)
+    eng = rigid.RigidEngine(mol.coords, len(sysd["pos"]), mol.I0, mol.mass, rmol=mol.rmol,
+                            dt=1.0)
+    eng.set_state(pos=sysd["pos"], vel=sysd["vel"], qv=sysd["qv"], omg=sysd["omg"],
+                  h=sysd["h"], npt=rigid.make_npt(T_K, 0.0, len(sysd["pos"])))
+    return eng, mol, sysd, scale
+
+
+def cpu_baseline(scale: float, steps: int):
+    """Reference OpenMP build (oracle/_ref), same workload, bounded sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "fuller_LJ_npt_md_omp")
+    if not os.path.exists(exe):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    libdir = os.path.join(ROOT, "fuller_md_en_amd", "FullereneLib")
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    cmd = [exe, f"--cell={NC}", f"--temp={T_K:g}", "--pres=0", f"--step={steps}",
+           f"--seed={SEED}", f"--init_scale={scale:.9f}", f"--libdir={libdir}",
+           f"--mon={steps}"]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600,
+                         cwd="/tmp").stdout
+    m = re.search(r"Done \(([\d.]+) sec\)", out)
+    if not m:
+        return None
+    secs = float(m.group(1))
+    sites = 4 * NC ** 3 * 60
+    return {"value": sites * steps / secs, "unit": "atom-steps/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"fuller_LJ_npt_md_omp --cell={NC} --step={steps} (a=14.17 A, seed 42), "
+                      f"OMP_NUM_THREADS={cores}, its own 'Done' timer ({secs:.1f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "decomposition is not built.", file=sys.stderr)
+        sys.exit(2)
+    eng, mol, sysd, scale = build_system()
+    n_sites = len(sysd["pos"]) * mol.natom
+    eng.prepare()
+    step0 = 0
+    if args.warmup:
+        eng.run(step0, args.warmup, thermo=False)
+        step0 += args.warmup
+    eng.sync()
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    th = eng.run(step0, args.steps, thermo=True)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms, kn = eng.kernel_time()
+    value = n_sites * args.steps / dt
+    achieved = FLOP_PER_SITE_STEP * n_sites / (kms * 1e-3) / 1e12
+    last = th[args.steps - 1]
+    res = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "rigid-LJ C60 FCC 10x10x10 (4000 mol, 240000 sites), "
+                               "a=14.17 A, 300 K, 0 GPa, dt=1 fs, NPT",
+                   "sites": n_sites, "precision": "fp32 pair forces, fp64 integrator/virial"},
+        "roofline": {"bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_rigid_forces", "kernel_ms": kms, "launches": kn},
+        "final": {"T": last.T, "P": last.P, "Ep_per_mol": last.Ep / len(sysd["pos"])},
+    }
+    if not args.no_cpu:
+        steps = args.cpu_steps or 60
+        res["cpu_baseline"] = cpu_baseline(scale, steps)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

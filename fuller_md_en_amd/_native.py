@@ -1,0 +1,142 @@
+"""ctypes binding of libfmd.so (the C ABI declared in include/fmd.h).
+
+The library is built in-tree by ``fuller_md_en_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfmd.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "fmd.h")
+
+FMD_OK = 0
+
+
+class FmdError(RuntimeError):
+    pass
+
+
+class LJParams(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("eps", C.c_double), ("rcut", C.c_double),
+                ("vshift", C.c_double), ("rmin2", C.c_double)]
+
+
+class NPTState(C.Structure):
+    """NPTState of the reference (C1:478)."""
+    _fields_ = [("xi", C.c_double), ("Q", C.c_double), ("Vg", C.c_double * 9),
+                ("W", C.c_double), ("Pe", C.c_double), ("Tt", C.c_double), ("Nf", C.c_int)]
+
+    def to_array(self) -> np.ndarray:
+        return np.array([self.xi, self.Q, *self.Vg, self.W, self.Pe, self.Tt, float(self.Nf)])
+
+    @classmethod
+    def from_array(cls, a) -> "NPTState":
+        s = cls()
+        s.xi, s.Q = a[0], a[1]
+        for k in range(9):
+            s.Vg[k] = a[2 + k]
+        s.W, s.Pe, s.Tt, s.Nf = a[11], a[12], a[13], int(a[14])
+        return s
+
+
+class RigidConfig(C.Structure):
+    _fields_ = [("n_mol", C.c_int), ("natom", C.c_int), ("body", C.POINTER(C.c_double)),
+                ("I0", C.c_double), ("Mmol", C.c_double), ("lj", LJParams),
+                ("rmcut", C.c_double), ("skin", C.c_double), ("max_neigh", C.c_int),
+                ("nlup", C.c_int), ("dt", C.c_double), ("clamp_xi", C.c_double),
+                ("clamp_vg", C.c_double)]
+
+
+class StepCtrl(C.Structure):
+    _fields_ = [("Tt", C.c_double), ("rescale_T", C.c_double), ("flags", C.c_int),
+                ("pad", C.c_int)]
+
+
+class Thermo(C.Structure):
+    _fields_ = [("Ep", C.c_double), ("KE", C.c_double), ("KEt", C.c_double),
+                ("KEr", C.c_double), ("T", C.c_double), ("P", C.c_double), ("V", C.c_double),
+                ("xi", C.c_double), ("Ep_rebo", C.c_double), ("Ep_lj", C.c_double),
+                ("h", C.c_double * 9)]
+
+
+CTRL_ZERO_XI_VG = 1
+CTRL_ZERO_VG = 2
+CTRL_RESCALE = 4
+CTRL_RESCALE_ZERO_VG = 8
+CTRL_REBUILD = 16
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_d, _i, _vp = C.c_double, C.c_int, C.c_void_p
+
+
+def _optd(x):
+    """ndpointer that also accepts None."""
+    return x
+
+
+_SIGS = {
+    "fmd_last_error": (C.c_char_p, []),
+    "fmd_version": (_i, []),
+    "fmd_device_info": (_i, [C.c_char_p, _i, C.POINTER(_i), C.POINTER(_d)]),
+    "fmd_lj_params_rigid": (None, [C.POINTER(LJParams)]),
+    "fmd_lj_params_airebo": (None, [C.POINTER(LJParams)]),
+    "fmd_make_npt": (None, [C.POINTER(NPTState), _d, _d, _i]),
+    "fmd_rigid_nlist": (_i, [_dp, _dp, _dp, _i, _d, _i, _ip, _ip, C.POINTER(_i)]),
+    "fmd_rigid_forces": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i, _i,
+                              _d, C.POINTER(LJParams), _vp, C.POINTER(_d)]),
+    "fmd_rigid_step_npt": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _d, _d, _i,
+                                _i, _d, _d, C.POINTER(NPTState), _ip, _ip, _i,
+                                C.POINTER(LJParams), C.POINTER(_d), C.POINTER(_d)]),
+    "fmd_rigid_config_default": (None, [C.POINTER(RigidConfig), _d]),
+    "fmd_rigid_create": (_i, [C.POINTER(RigidConfig), C.POINTER(_vp)]),
+    "fmd_rigid_set_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_rigid_get_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_engine_prepare": (_i, [_vp]),
+    "fmd_engine_run": (_i, [_vp, _i, _i, _vp, _vp]),
+    "fmd_engine_forces": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "fmd_engine_counters": (_i, [_vp, C.POINTER(C.c_longlong)]),
+    "fmd_engine_set_timing": (_i, [_vp, _i]),
+    "fmd_engine_kernel_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
+    "fmd_engine_sync": (_i, [_vp]),
+    "fmd_engine_destroy": (None, [_vp]),
+}
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Load libfmd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FmdError(f"{LIB_PATH} is missing: run `python -m fuller_md_en_amd.build` "
+                           "(the HIP library is required; there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != FMD_OK:
+        raise FmdError(f"libfmd error {rc}: {lib().fmd_last_error().decode()}")
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared in include/fmd.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(fmd_[a-z0-9_]+)\s*\(", text)))
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

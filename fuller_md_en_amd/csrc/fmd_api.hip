@@ -1,0 +1,690 @@
+// fmd_api.hip -- host implementation of the C ABI in include/fmd.h:
+// the resident engine (HBM-resident state, stream-ordered steps) and the
+// drop-in entry points that mirror the reference's function signatures.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fmd.h"
+#include "fmd_engine.hpp"
+
+using namespace fmd;
+
+namespace fmd {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return set_error(FMD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+LJ32 make_lj32(const fmd_lj_params &lj) {
+  LJ32 p;
+  p.sig2 = (float)(lj.sigma * lj.sigma);
+  p.rc2 = (float)(lj.rcut * lj.rcut);
+  p.c48 = (float)(48.0 * lj.eps);
+  p.c24 = (float)(24.0 * lj.eps);
+  p.vs4 = (float)(lj.vshift / (4.0 * lj.eps));
+  p.rmin2 = (float)lj.rmin2;
+  return p;
+}
+
+}  // namespace fmd
+
+
+// Device address of RigidState::h (h then h^-1).
+static const double *state_hh(const RigidDev &d) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.state) +
+                                          offsetof(RigidState, h));
+}
+
+extern "C" {
+
+const char *fmd_last_error(void) { return g_err.c_str(); }
+int fmd_version(void) { return 100; }
+
+int fmd_device_info(char *name, int cap, int *cu_count, double *hbm_bytes) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  HIPCHK(hipGetDeviceProperties(&pr, dev));
+  if (name && cap > 0) snprintf(name, cap, "%s (%s)", pr.name, pr.gcnArchName);
+  if (cu_count) *cu_count = pr.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (double)pr.totalGlobalMem;
+  return FMD_OK;
+}
+
+void fmd_lj_params_rigid(fmd_lj_params *p) {
+  const double sigma = 3.431, eps = 2.635e-3;  // C1:93-96
+  const double sr = 1.0 / 3.0, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;  // C1:101-104
+  p->sigma = sigma;
+  p->eps = eps;
+  p->rcut = 3.0 * sigma;
+  p->vshift = 4.0 * eps * (sr6 * sr6 - sr6);
+  p->rmin2 = 0.25;
+}
+
+void fmd_lj_params_airebo(fmd_lj_params *p) {
+  const double sigma = 3.40, eps = 2.84e-3;  // AB:119-124
+  const double rc = 3.0 * sigma, sr = sigma / rc, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+  p->sigma = sigma;
+  p->eps = eps;
+  p->rcut = rc;
+  p->vshift = 4 * eps * (sr6 * sr6 - sr6);
+  p->rmin2 = 0.25;
+}
+
+void fmd_make_npt(fmd_npt_state *s, double T, double Pe, int Nf) {
+  s->xi = 0;
+  s->Q = std::max(Nf * kB * T * 100.0 * 100.0, 1e-20);
+  for (int i = 0; i < 9; i++) s->Vg[i] = 0;
+  s->W = std::max((Nf + 9) * kB * T * 1000.0 * 1000.0, 1e-20);
+  s->Pe = Pe;
+  s->Tt = T;
+  s->Nf = Nf;
+}
+
+void fmd_rigid_config_default(fmd_rigid_config *c, double rmol) {
+  memset(c, 0, sizeof *c);
+  fmd_lj_params_rigid(&c->lj);
+  c->rmcut = c->lj.rcut + 2 * rmol + 1.0;  // LJ:716
+  c->skin = 3.0;                           // C1:282
+  c->max_neigh = 80;                       // C1:121
+  c->nlup = 25;                            // C1:617
+  c->dt = 1.0;
+  c->clamp_xi = 0.1;
+  c->clamp_vg = 0.01;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ rigid engine
+namespace fmd {
+
+static void free_rigid(RigidDev &d) {
+  void *ptrs[] = {d.pos,      d.vel,      d.omg,   d.qv,    d.Fv,   d.Tv,     d.sites,
+                  d.body,     d.hh,       d.nl_count, d.nl_list, d.flags, d.molpart,
+                  d.blockpart, d.state,   d.mid,   d.ctrl,  d.thermo};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  memset(&d, 0, sizeof d);
+}
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+
+int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
+  memset(&d, 0, sizeof d);
+  const size_t N = p.N;
+  d.nblk = (p.N + 255) / 256;
+  hipError_t e = hipSuccess;
+#define AL(ptr, n) \
+  if (e == hipSuccess) e = dalloc(&(ptr), (n))
+  AL(d.pos, 3 * N);
+  AL(d.vel, 3 * N);
+  AL(d.omg, 3 * N);
+  AL(d.qv, 4 * N);
+  AL(d.Fv, 3 * N);
+  AL(d.Tv, 3 * N);
+  AL(d.sites, N * p.natom_pad);
+  AL(d.body, 4 * (size_t)p.natom_pad);
+  AL(d.hh, 18);
+  AL(d.nl_count, N);
+  AL(d.nl_list, N * p.max_neigh);
+  AL(d.flags, 4);
+  AL(d.molpart, N * kMolPart);
+  AL(d.blockpart, (size_t)d.nblk * kBlkPart);
+  AL(d.state, 1);
+  AL(d.mid, 1);
+  AL(d.ctrl, ring);
+  AL(d.thermo, ring);
+#undef AL
+  if (e != hipSuccess) {
+    free_rigid(d);
+    return hip_fail(e, "hipMalloc (rigid engine)");
+  }
+  (void)hipMemset(d.flags, 0, 4 * sizeof(int));
+  (void)hipMemset(d.nl_count, 0, N * sizeof(int));
+  (void)hipMemset(d.Fv, 0, 3 * N * sizeof(double));
+  (void)hipMemset(d.Tv, 0, 3 * N * sizeof(double));
+  (void)hipMemset(d.state, 0, sizeof(RigidState));
+  return FMD_OK;
+}
+
+int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0, double Mmol,
+                 const fmd_lj_params &lj, double rmcut2, double rlist, int max_neigh,
+                 double dt, double cxi, double cvg, std::vector<double> &body4) {
+  if (N < 1 || natom < 1 || natom > 128 || max_neigh < 1)
+    return set_error(FMD_ERR_ARG, "rigid: need N >= 1, 1 <= natom <= 128, max_neigh >= 1");
+  memset(&p, 0, sizeof p);
+  p.N = N;
+  p.natom = natom;
+  p.natom_pad = (natom + 3) & ~3;
+  p.max_neigh = max_neigh;
+  p.rmcut2 = rmcut2;
+  p.rlist2 = rlist * rlist;
+  p.dt = dt;
+  p.I0 = I0;
+  p.Mmol = Mmol;
+  p.clamp_xi = cxi;
+  p.clamp_vg = cvg;
+  p.lj = make_lj32(lj);
+  p.eps = lj.eps;
+  double rb = 0;
+  body4.assign(4 * (size_t)p.natom_pad, 0.0);
+  for (int a = 0; a < p.natom_pad; a++) {
+    if (a < natom) {
+      for (int k = 0; k < 3; k++) body4[4 * a + k] = body[3 * a + k];
+      rb = std::max(rb, std::sqrt(body[3 * a] * body[3 * a] + body[3 * a + 1] * body[3 * a + 1] +
+                                  body[3 * a + 2] * body[3 * a + 2]));
+    } else {
+      body4[4 * a + 3] = 1.0;
+    }
+  }
+  const double cr = lj.rcut + rb + 0.05;  // FP32 margin
+  p.cull_r2 = cr * cr;
+  return FMD_OK;
+}
+
+}  // namespace fmd
+
+
+
+int fmd::engine_init_common(fmd_engine *e, int ring) {
+  e->ring = ring;
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipHostMalloc((void **)&e->h_ctrl, ring * sizeof(StepCtrl)));
+  HIPCHK(hipHostMalloc((void **)&e->h_thermo, ring * sizeof(Thermo)));
+  return FMD_OK;
+}
+
+extern "C" {
+
+int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
+  if (!c || !out || !c->body) return set_error(FMD_ERR_ARG, "fmd_rigid_create: null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(FMD_ERR_NODEVICE, "fmd_rigid_create: no HIP device");
+  fmd_engine *e = new fmd_engine();
+  e->kind = FMD_KIND_RIGID;
+  std::vector<double> body4;
+  int rc = rigid_params(e->rp, c->n_mol, c->natom, c->body, c->I0, c->Mmol, c->lj,
+                        c->rmcut * c->rmcut, c->rmcut + c->skin, c->max_neigh, c->dt,
+                        c->clamp_xi, c->clamp_vg, body4);
+  if (rc == FMD_OK) rc = engine_init_common(e, 4096);
+  if (rc == FMD_OK) rc = rigid_alloc(e->rd, e->rp, e->ring);
+  if (rc != FMD_OK) {
+    fmd_engine_destroy(e);
+    return rc;
+  }
+  e->nlup = c->nlup > 0 ? c->nlup : 25;
+  hipError_t he = hipMemcpy(e->rd.body, body4.data(), body4.size() * sizeof(double),
+                            hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    fmd_engine_destroy(e);
+    return hip_fail(he, "upload body");
+  }
+  *out = e;
+  return FMD_OK;
+}
+
+int fmd_rigid_set_state(fmd_engine *e, const double *pos, const double *vel, const double *qv,
+                        const double *omg, const double *h, const fmd_npt_state *npt) {
+  if (!e || e->kind != FMD_KIND_RIGID) return set_error(FMD_ERR_ARG, "not a rigid engine");
+  RigidDev &d = e->rd;
+  const size_t N = e->rp.N;
+  hipStream_t s = e->stream;
+  if (pos) HIPCHK(hipMemcpyAsync(d.pos, pos, 3 * N * 8, hipMemcpyHostToDevice, s));
+  if (vel) HIPCHK(hipMemcpyAsync(d.vel, vel, 3 * N * 8, hipMemcpyHostToDevice, s));
+  if (qv) HIPCHK(hipMemcpyAsync(d.qv, qv, 4 * N * 8, hipMemcpyHostToDevice, s));
+  if (omg) HIPCHK(hipMemcpyAsync(d.omg, omg, 3 * N * 8, hipMemcpyHostToDevice, s));
+  RigidState S;
+  HIPCHK(hipMemcpyAsync(&S, d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h) {
+    for (int k = 0; k < 9; k++) S.h[k] = h[k];
+    inv9(S.h, S.hi);
+  }
+  if (npt) {
+    S.xi = npt->xi;
+    S.Q = npt->Q;
+    for (int k = 0; k < 9; k++) S.Vg[k] = npt->Vg[k];
+    S.W = npt->W;
+    S.Pe = npt->Pe;
+    S.Tt = npt->Tt;
+    S.Nf = npt->Nf;
+    e->tt = npt->Tt;
+  }
+  S.scale = 1.0;
+  HIPCHK(hipMemcpyAsync(d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  e->prepared = false;
+  return FMD_OK;
+}
+
+int fmd_rigid_get_state(fmd_engine *e, double *pos, double *vel, double *qv, double *omg,
+                        double *h, fmd_npt_state *npt) {
+  if (!e || e->kind != FMD_KIND_RIGID) return set_error(FMD_ERR_ARG, "not a rigid engine");
+  RigidDev &d = e->rd;
+  const size_t N = e->rp.N;
+  hipStream_t s = e->stream;
+  if (pos) HIPCHK(hipMemcpyAsync(pos, d.pos, 3 * N * 8, hipMemcpyDeviceToHost, s));
+  if (vel) HIPCHK(hipMemcpyAsync(vel, d.vel, 3 * N * 8, hipMemcpyDeviceToHost, s));
+  if (qv) HIPCHK(hipMemcpyAsync(qv, d.qv, 4 * N * 8, hipMemcpyDeviceToHost, s));
+  if (omg) HIPCHK(hipMemcpyAsync(omg, d.omg, 3 * N * 8, hipMemcpyDeviceToHost, s));
+  RigidState S;
+  HIPCHK(hipMemcpyAsync(&S, d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h)
+    for (int k = 0; k < 9; k++) h[k] = S.h[k];
+  if (npt) {
+    npt->xi = S.xi;
+    npt->Q = S.Q;
+    for (int k = 0; k < 9; k++) npt->Vg[k] = S.Vg[k];
+    npt->W = S.W;
+    npt->Pe = S.Pe;
+    npt->Tt = S.Tt;
+    npt->Nf = S.Nf;
+  }
+  return FMD_OK;
+}
+
+}  // extern "C"
+
+static int rigid_prepare(fmd_engine *e) {
+  RigidDev &d = e->rd;
+  const RigidParams &p = e->rp;
+  hipStream_t s = e->stream;
+  const double *hh = state_hh(d);
+  HIPCHK(rigid_launch_apply_pbc(d, p, hh, s));
+  HIPCHK(rigid_launch_nlist(d, p, hh, s));
+  HIPCHK(rigid_launch_sites(d, p, s));
+  HIPCHK(rigid_launch_forces(d, p, hh, s));
+  HIPCHK(rigid_launch_observe(d, p, kObsKE | kObsForce, s));
+  e->rebuilds++;
+  e->prepared = true;
+  return FMD_OK;
+}
+
+static int read_overflow(fmd_engine *e, int *flags_dev) {
+  int f = 0;
+  HIPCHK(hipMemcpyAsync(&f, flags_dev, sizeof f, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (f) {
+    e->overflow += f;
+    HIPCHK(hipMemsetAsync(flags_dev, 0, sizeof(int), e->stream));
+  }
+  return FMD_OK;
+}
+
+static int rigid_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+                     fmd_thermo *thermo) {
+  RigidDev &d = e->rd;
+  const RigidParams &p = e->rp;
+  hipStream_t s = e->stream;
+  if (!e->prepared) {
+    int rc = rigid_prepare(e);
+    if (rc) return rc;
+  }
+  for (int c0 = 0; c0 < nsteps; c0 += e->ring) {
+    const int n = std::min(e->ring, nsteps - c0);
+    for (int k = 0; k < n; k++) {
+      StepCtrl &c = e->h_ctrl[k];
+      if (ctrl) {
+        c.Tt = ctrl[c0 + k].Tt;
+        c.rescale_T = ctrl[c0 + k].rescale_T;
+        c.flags = ctrl[c0 + k].flags;
+      } else {
+        c.Tt = e->tt;
+        c.rescale_T = 0;
+        c.flags = 0;
+      }
+      c.pad = 0;
+    }
+    if (ctrl) e->tt = ctrl[c0 + n - 1].Tt;
+    HIPCHK(hipMemcpyAsync(d.ctrl, e->h_ctrl, n * sizeof(StepCtrl), hipMemcpyHostToDevice, s));
+    if (e->timing && (int)e->ev.size() < 2 * n) {
+      while ((int)e->ev.size() < 2 * n) {
+        hipEvent_t ev;
+        HIPCHK(hipEventCreate(&ev));
+        e->ev.push_back(ev);
+      }
+    }
+    for (int k = 0; k < n; k++) {
+      const long long g = (long long)first_step + c0 + k + 1;
+      const int fl = e->h_ctrl[k].flags;
+      const int rebuild = (g % e->nlup == 0) || (fl & FMD_CTRL_REBUILD);
+      const int rescale = (fl & FMD_CTRL_RESCALE) ? 1 : 0;
+      hipEvent_t e0 = e->timing ? e->ev[2 * k] : nullptr;
+      hipEvent_t e1 = e->timing ? e->ev[2 * k + 1] : nullptr;
+      HIPCHK(rigid_launch_step(d, p, k, k, rebuild, rescale, e0, e1, s));
+      e->rebuilds += rebuild;
+    }
+    if (thermo)
+      HIPCHK(hipMemcpyAsync(e->h_thermo, d.thermo, n * sizeof(Thermo), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (thermo) memcpy(thermo + c0, e->h_thermo, n * sizeof(Thermo));
+    if (e->timing) {
+      for (int k = 0; k < n; k++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]));
+        e->t_sum += ms;
+        e->t_n++;
+      }
+    }
+    e->steps += n;
+  }
+  return read_overflow(e, d.flags);
+}
+
+extern "C" {
+
+int fmd_engine_prepare(fmd_engine *e) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  if (e->kind == FMD_KIND_RIGID) {
+    int rc = rigid_prepare(e);
+    if (rc) return rc;
+    return read_overflow(e, e->rd.flags);
+  }
+  return engine_prepare_atomic(e);
+}
+
+int fmd_engine_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+                   fmd_thermo *thermo) {
+  if (!e || nsteps < 0) return set_error(FMD_ERR_ARG, "fmd_engine_run: bad argument");
+  static_assert(sizeof(fmd_thermo) == sizeof(Thermo), "thermo layout");
+  static_assert(sizeof(fmd_step_ctrl) == sizeof(StepCtrl), "ctrl layout");
+  if (nsteps == 0) return FMD_OK;
+  if (e->kind == FMD_KIND_RIGID) return rigid_run(e, first_step, nsteps, ctrl, thermo);
+  return engine_run_atomic(e, first_step, nsteps, ctrl, thermo);
+}
+
+int fmd_engine_forces(fmd_engine *e, double *F, double *T, double *W9, double *energies) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  if (e->kind != FMD_KIND_RIGID) return engine_forces_atomic(e, F, T, W9, energies);
+  if (!e->prepared) {
+    int rc = rigid_prepare(e);
+    if (rc) return rc;
+  }
+  RigidDev &d = e->rd;
+  const size_t N = e->rp.N;
+  hipStream_t s = e->stream;
+  if (F) HIPCHK(hipMemcpyAsync(F, d.Fv, 3 * N * 8, hipMemcpyDeviceToHost, s));
+  if (T) HIPCHK(hipMemcpyAsync(T, d.Tv, 3 * N * 8, hipMemcpyDeviceToHost, s));
+  RigidState S;
+  HIPCHK(hipMemcpyAsync(&S, d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (W9)
+    for (int k = 0; k < 9; k++) W9[k] = S.W9[k];
+  if (energies) {
+    energies[0] = S.Ep;
+    energies[1] = 0;
+    energies[2] = S.Ep;
+  }
+  return FMD_OK;
+}
+
+int fmd_engine_counters(fmd_engine *e, long long *out3) {
+  if (!e || !out3) return set_error(FMD_ERR_ARG, "null argument");
+  out3[0] = e->overflow;
+  out3[1] = e->rebuilds;
+  out3[2] = e->steps;
+  return FMD_OK;
+}
+
+int fmd_engine_set_timing(fmd_engine *e, int on) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  e->timing = on != 0;
+  e->t_sum = 0;
+  e->t_n = 0;
+  return FMD_OK;
+}
+
+int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  if (mean_ms) *mean_ms = e->t_n ? e->t_sum / e->t_n : 0.0;
+  if (n) *n = e->t_n;
+  e->t_sum = 0;
+  e->t_n = 0;
+  return FMD_OK;
+}
+
+int fmd_engine_sync(fmd_engine *e) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return FMD_OK;
+}
+
+void fmd_engine_destroy(fmd_engine *e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->kind == FMD_KIND_RIGID) free_rigid(e->rd);
+  engine_free_atomic(e);
+  for (hipEvent_t ev : e->ev) (void)hipEventDestroy(ev);
+  if (e->h_ctrl) (void)hipHostFree(e->h_ctrl);
+  if (e->h_thermo) (void)hipHostFree(e->h_thermo);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ rigid drop-ins
+// One per-process workspace engine, re-created when the shape changes.
+namespace {
+
+struct RigidWorkspace {
+  fmd_engine *e = nullptr;
+  int N = 0, natom = 0, max_neigh = 0;
+};
+RigidWorkspace g_rws;
+
+int rigid_workspace(int N, int natom, int max_neigh, const double *body, double I0, double Mmol,
+                    const fmd_lj_params *lj, double rmcut2, double dt, fmd_engine **out) {
+  fmd_lj_params ljd;
+  if (!lj) {
+    fmd_lj_params_rigid(&ljd);
+    lj = &ljd;
+  }
+  if (!g_rws.e || g_rws.N != N || g_rws.natom != natom || g_rws.max_neigh != max_neigh) {
+    if (g_rws.e) fmd_engine_destroy(g_rws.e);
+    g_rws = RigidWorkspace();
+    fmd_rigid_config c;
+    fmd_rigid_config_default(&c, 0.0);
+    c.n_mol = N;
+    c.natom = natom;
+    c.body = body;
+    c.I0 = I0;
+    c.Mmol = Mmol;
+    c.lj = *lj;
+    c.max_neigh = max_neigh;
+    int rc = fmd_rigid_create(&c, &g_rws.e);
+    if (rc) return rc;
+    g_rws.N = N;
+    g_rws.natom = natom;
+    g_rws.max_neigh = max_neigh;
+  }
+  fmd_engine *e = g_rws.e;
+  std::vector<double> body4;
+  int rc = rigid_params(e->rp, N, natom, body, I0, Mmol, *lj, rmcut2, 0.0, max_neigh, dt, 0.1,
+                        0.01, body4);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(e->rd.body, body4.data(), body4.size() * sizeof(double),
+                        hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  *out = e;
+  return FMD_OK;
+}
+
+int upload_lists(fmd_engine *e, const int *nl_count, const int *nl_list) {
+  const size_t N = e->rp.N;
+  HIPCHK(hipMemcpyAsync(e->rd.nl_count, nl_count, N * sizeof(int), hipMemcpyHostToDevice,
+                        e->stream));
+  HIPCHK(hipMemcpyAsync(e->rd.nl_list, nl_list, N * e->rp.max_neigh * sizeof(int),
+                        hipMemcpyHostToDevice, e->stream));
+  return FMD_OK;
+}
+
+// The reference's lab scratch (C1:354-364), filled on the host in FP64.
+void host_lab(const double *qv, const double *body, int N, int natom, double *lab) {
+  for (int i = 0; i < N; i++) {
+    double R[9];
+    q2R(&qv[i * 4], R);
+    for (int a = 0; a < natom; a++) {
+      const double *b = body + 3 * a;
+      double *o = lab + (size_t)i * natom * 3 + a * 3;
+      o[0] = R[0] * b[0] + R[1] * b[1] + R[2] * b[2];
+      o[1] = R[3] * b[0] + R[4] * b[1] + R[5] * b[2];
+      o[2] = R[6] * b[0] + R[7] * b[1] + R[8] * b[2];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmd_rigid_nlist(const double *pos, const double *h, const double *hi, int N, double rmcut,
+                    int max_neigh, int *nl_count, int *nl_list, int *overflow) {
+  if (!pos || !h || !hi || !nl_count || !nl_list || N < 1 || max_neigh < 1)
+    return set_error(FMD_ERR_ARG, "fmd_rigid_nlist: bad argument");
+  const double one[3] = {0, 0, 0};
+  fmd_engine *e;
+  int rc = rigid_workspace(N, 1, max_neigh, one, 1.0, 1.0, nullptr, 0.0, 1.0, &e);
+  if (rc) return rc;
+  e->rp.rlist2 = (rmcut + 3.0) * (rmcut + 3.0);  // C1:282
+  RigidDev &d = e->rd;
+  hipStream_t s = e->stream;
+  double hh[18];
+  for (int k = 0; k < 9; k++) {
+    hh[k] = h[k];
+    hh[9 + k] = hi[k];
+  }
+  HIPCHK(hipMemcpyAsync(d.pos, pos, 3 * (size_t)N * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(d.flags, 0, sizeof(int), s));
+  HIPCHK(rigid_launch_nlist(d, e->rp, d.hh, s));
+  int ovf = 0;
+  HIPCHK(hipMemcpyAsync(nl_count, d.nl_count, N * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(nl_list, d.nl_list, (size_t)N * max_neigh * sizeof(int),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&ovf, d.flags, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (overflow) *overflow = ovf;
+  return FMD_OK;
+}
+
+int fmd_rigid_forces(double *Fv, double *Tv, double *Wm9, const double *pos, const double *qv,
+                     const double *body, const double *h, const double *hi,
+                     const int *nl_count, const int *nl_list, int max_neigh, int N, int natom,
+                     double rmcut2, const fmd_lj_params *lj, double *lab, double *Ep) {
+  if (!Fv || !Tv || !Wm9 || !pos || !qv || !body || !h || !hi || !nl_count || !nl_list)
+    return set_error(FMD_ERR_ARG, "fmd_rigid_forces: null argument");
+  fmd_engine *e;
+  int rc = rigid_workspace(N, natom, max_neigh, body, 1.0, 1.0, lj, rmcut2, 1.0, &e);
+  if (rc) return rc;
+  RigidDev &d = e->rd;
+  hipStream_t s = e->stream;
+  double hh[18];
+  for (int k = 0; k < 9; k++) {
+    hh[k] = h[k];
+    hh[9 + k] = hi[k];
+  }
+  HIPCHK(hipMemcpyAsync(d.pos, pos, 3 * (size_t)N * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.qv, qv, 4 * (size_t)N * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
+  if ((rc = upload_lists(e, nl_count, nl_list))) return rc;
+  HIPCHK(rigid_launch_sites(d, e->rp, s));
+  HIPCHK(rigid_launch_forces(d, e->rp, d.hh, s));
+  HIPCHK(rigid_launch_reduce_molpart(d, e->rp, d.blockpart, s));
+  double out10[kMolPart];
+  HIPCHK(hipMemcpyAsync(Fv, d.Fv, 3 * (size_t)N * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(Tv, d.Tv, 3 * (size_t)N * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(out10, d.blockpart, sizeof out10, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < 9; k++) Wm9[k] = out10[k];
+  if (Ep) *Ep = out10[9];
+  if (lab) host_lab(qv, body, N, natom, lab);
+  return FMD_OK;
+}
+
+int fmd_rigid_step_npt(double *pos, double *vel, double *qv, double *omg, double *Fv,
+                       double *Tv, double *Wm9, double *h, double *hi, const double *body,
+                       double I0, double Mmol, int N, int natom, double rmcut2, double dt,
+                       fmd_npt_state *npt, const int *nl_count, const int *nl_list,
+                       int max_neigh, const fmd_lj_params *lj, double *Ep, double *KE) {
+  if (!pos || !vel || !qv || !omg || !Fv || !Tv || !Wm9 || !h || !hi || !body || !npt ||
+      !nl_count || !nl_list)
+    return set_error(FMD_ERR_ARG, "fmd_rigid_step_npt: null argument");
+  fmd_engine *e;
+  int rc = rigid_workspace(N, natom, max_neigh, body, I0, Mmol, lj, rmcut2, dt, &e);
+  if (rc) return rc;
+  RigidDev &d = e->rd;
+  hipStream_t s = e->stream;
+  const size_t n3 = 3 * (size_t)N * 8;
+  HIPCHK(hipMemcpyAsync(d.pos, pos, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.vel, vel, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.qv, qv, 4 * (size_t)N * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.omg, omg, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.Fv, Fv, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.Tv, Tv, n3, hipMemcpyHostToDevice, s));
+  if ((rc = upload_lists(e, nl_count, nl_list))) return rc;
+  RigidState S;
+  memset(&S, 0, sizeof S);
+  for (int k = 0; k < 9; k++) {
+    S.h[k] = h[k];
+    S.Vg[k] = npt->Vg[k];
+    S.W9[k] = Wm9[k];
+  }
+  inv9(S.h, S.hi);
+  S.xi = npt->xi;
+  S.Q = npt->Q;
+  S.W = npt->W;
+  S.Pe = npt->Pe;
+  S.Tt = npt->Tt;
+  S.Nf = npt->Nf;
+  S.scale = 1.0;
+  StepCtrl c = {npt->Tt, 0.0, 0, 0};
+  HIPCHK(hipMemcpyAsync(d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
+  HIPCHK(rigid_launch_observe(d, e->rp, kObsKE, s));
+  HIPCHK(rigid_launch_step(d, e->rp, 0, 0, 0, 0, nullptr, nullptr, s));
+  HIPCHK(hipMemcpyAsync(pos, d.pos, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(vel, d.vel, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(qv, d.qv, 4 * (size_t)N * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(omg, d.omg, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(Fv, d.Fv, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(Tv, d.Tv, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < 9; k++) {
+    h[k] = S.h[k];
+    hi[k] = S.hi[k];
+    Wm9[k] = S.W9[k];
+    npt->Vg[k] = S.Vg[k];
+  }
+  npt->xi = S.xi;
+  if (Ep) *Ep = S.Ep;
+  if (KE) *KE = S.KEt + S.KEr;
+  return FMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// fmd_common.hpp -- scalar math shared by the gfx950 kernels and the host
+// code: FP64 cell-matrix algebra, the fractional minimum image, quaternion
+// kinematics and the FP32 shifted-LJ pair term.
+//
+// Conventions follow the reference (src/fuller_LJ_npt_md_core_serial.cpp,
+// "C1"): h is row-major with the cell vectors as its columns (r = h s),
+// quaternions are [w,x,y,z], virial W_ab = sum d_a f_b with d the pair vector
+// from the first to the second atom and f the force on the second.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define FMD_HD __host__ __device__ __forceinline__
+#else
+#define FMD_HD inline
+#endif
+
+namespace fmd {
+
+// Units A, amu, eV, fs, K, GPa (C1:84-87).
+constexpr double kConv = 9.64853321e-3;  // eV fs^2 / (amu A^2)
+constexpr double kB = 8.617333262e-5;    // eV / K
+constexpr double kEv2GPa = 160.21766208;
+constexpr double kEv2KcalMol = 23.06054783;
+constexpr double kMassC = 12.011;
+
+// ------------------------------------------------------------ cell algebra
+FMD_HD double det9(const double *h) {  // C1:137-141
+  return h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+         h[2] * (h[3] * h[7] - h[4] * h[6]);
+}
+
+FMD_HD void inv9(const double *h, double *hi) {  // C1:147-158
+  double d = det9(h), id = 1.0 / d;
+  hi[0] = id * (h[4] * h[8] - h[5] * h[7]);
+  hi[1] = id * (h[2] * h[7] - h[1] * h[8]);
+  hi[2] = id * (h[1] * h[5] - h[2] * h[4]);
+  hi[3] = id * (h[5] * h[6] - h[3] * h[8]);
+  hi[4] = id * (h[0] * h[8] - h[2] * h[6]);
+  hi[5] = id * (h[2] * h[3] - h[0] * h[5]);
+  hi[6] = id * (h[3] * h[7] - h[4] * h[6]);
+  hi[7] = id * (h[1] * h[6] - h[0] * h[7]);
+  hi[8] = id * (h[0] * h[4] - h[1] * h[3]);
+}
+
+// Fractional round-half-away minimum image (C1:166-175).
+FMD_HD void mimg(double &dx, double &dy, double &dz, const double *hi, const double *h) {
+  double s0 = hi[0] * dx + hi[1] * dy + hi[2] * dz;
+  double s1 = hi[3] * dx + hi[4] * dy + hi[5] * dz;
+  double s2 = hi[6] * dx + hi[7] * dy + hi[8] * dz;
+  s0 -= round(s0);
+  s1 -= round(s1);
+  s2 -= round(s2);
+  dx = h[0] * s0 + h[1] * s1 + h[2] * s2;
+  dy = h[3] * s0 + h[4] * s1 + h[5] * s2;
+  dz = h[6] * s0 + h[7] * s1 + h[8] * s2;
+}
+
+// ------------------------------------------------------------ quaternions
+FMD_HD void q2R(const double *q, double *R) {  // C1:186-191
+  double w = q[0], x = q[1], y = q[2], z = q[3];
+  R[0] = 1 - 2 * (y * y + z * z);
+  R[1] = 2 * (x * y - w * z);
+  R[2] = 2 * (x * z + w * y);
+  R[3] = 2 * (x * y + w * z);
+  R[4] = 1 - 2 * (x * x + z * z);
+  R[5] = 2 * (y * z - w * x);
+  R[6] = 2 * (x * z - w * y);
+  R[7] = 2 * (y * z + w * x);
+  R[8] = 1 - 2 * (x * x + y * y);
+}
+
+// q <- normalise(q (x) dq(w dt)), dq the exact rotation by |w| dt about w
+// (C1:193-216, 567-574).
+FMD_HD void quat_advance(double *q, double wx, double wy, double wz, double dt) {
+  double dq[4];
+  double wm = sqrt(wx * wx + wy * wy + wz * wz);
+  double th = wm * dt * 0.5;
+  if (th < 1e-14) {
+    dq[0] = 1.0;
+    dq[1] = 0.5 * dt * wx;
+    dq[2] = 0.5 * dt * wy;
+    dq[3] = 0.5 * dt * wz;
+  } else {
+    double s = sin(th) / wm;
+    dq[0] = cos(th);
+    dq[1] = s * wx;
+    dq[2] = s * wy;
+    dq[3] = s * wz;
+  }
+  double o0 = q[0] * dq[0] - q[1] * dq[1] - q[2] * dq[2] - q[3] * dq[3];
+  double o1 = q[0] * dq[1] + q[1] * dq[0] + q[2] * dq[3] - q[3] * dq[2];
+  double o2 = q[0] * dq[2] - q[1] * dq[3] + q[2] * dq[0] + q[3] * dq[1];
+  double o3 = q[0] * dq[3] + q[1] * dq[2] - q[2] * dq[1] + q[3] * dq[0];
+  double inv = 1.0 / sqrt(o0 * o0 + o1 * o1 + o2 * o2 + o3 * o3);
+  q[0] = o0 * inv;
+  q[1] = o1 * inv;
+  q[2] = o2 * inv;
+  q[3] = o3 * inv;
+}
+
+// ------------------------------------------------------------ LJ pair term
+// Shifted 12-6 in FP32, with the energy kept in units of 4 eps and the shift
+// folded in (e = sr6 (sr6 - 1) - vshift / (4 eps)), so one pair costs one
+// reciprocal and 14 FMA-class operations.
+struct LJ32 {
+  float sig2;   // sigma^2
+  float rc2;    // cutoff^2 (pairs with r^2 < rc2, or <= for AIREBO)
+  float c48;    // 48 eps
+  float c24;    // 24 eps
+  float vs4;    // vshift / (4 eps)
+  float rmin2;  // r^2 clamp (0.25, C1:404)
+};
+
+// Reciprocal: the hardware v_rcp_f32 (1 ulp) on the device, IEEE on the host.
+FMD_HD float rcp32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// Returns fm / (48 eps): the force on the second atom is 48 eps fm' d (the
+// caller applies 48 eps once per partial sum); adds the reduced energy
+// sr6 (sr6 - 1) - vshift / (4 eps) into e.  One reciprocal, 10 VALU ops.
+FMD_HD float lj_fm(float r2, const LJ32 &p, float &e) {
+  float ri2 = rcp32(fmaxf(r2, p.rmin2));
+  float sr2 = p.sig2 * ri2;
+  float sr6 = sr2 * sr2 * sr2;
+  e += sr6 * (sr6 - 1.0f) - p.vs4;
+  return sr6 * (sr6 - 0.5f) * ri2;
+}
+
+}  // namespace fmd

@@ -1,0 +1,57 @@
+// fmd_engine.hpp -- the opaque fmd_engine of the C ABI and the helpers shared
+// by the rigid and atomic halves of the host implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/fmd.h"
+#include "fmd_atomic.hpp"
+#include "fmd_rigid.hpp"
+
+namespace fmd {
+
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+LJ32 make_lj32(const fmd_lj_params &lj);
+
+}  // namespace fmd
+
+#define HIPCHK(expr)                                        \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return fmd::hip_fail(_e, #expr);  \
+  } while (0)
+
+struct fmd_engine {
+  int kind = 0;
+  hipStream_t stream = nullptr;
+  int ring = 0;  // capacity of the per-step control / record rings
+  int nlup = 25;
+  double tt = 0;  // host mirror of the target temperature
+  bool prepared = false;
+  long long overflow = 0, rebuilds = 0, steps = 0;
+  fmd::StepCtrl *h_ctrl = nullptr;
+  fmd::Thermo *h_thermo = nullptr;
+  // rigid
+  fmd::RigidParams rp{};
+  fmd::RigidDev rd{};
+  // atomic
+  fmd::AtomicEngine *at = nullptr;
+  // timing of the force kernel (HIP events on `stream`)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double t_sum = 0;
+  int t_n = 0;
+};
+
+namespace fmd {
+int engine_init_common(fmd_engine *e, int ring);
+// Atomic (AIREBO) engine, fmd_atomic_api.hip.
+int engine_prepare_atomic(fmd_engine *e);
+int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+                      fmd_thermo *thermo);
+int engine_forces_atomic(fmd_engine *e, double *F, double *T, double *W9, double *energies);
+void engine_free_atomic(fmd_engine *e);
+}  // namespace fmd

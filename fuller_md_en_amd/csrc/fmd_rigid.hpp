@@ -1,0 +1,116 @@
+// fmd_rigid.hpp -- device data layout and launchers of the rigid-body engine
+// (site-site LJ between rigid molecules; reference
+// src/fuller_LJ_npt_md_core_serial.cpp "C1" and
+// src/fuller_LJ_npt_md_serial_omp_acc.cpp "LJ").
+//
+// HBM layout (all per-molecule arrays in the reference's flat AoS order):
+//   pos, vel, omg, Fv, Tv   double[N*3]
+//   qv                      double[N*4]           [w,x,y,z]
+//   sites                   float4[N*natom_pad]   lab-frame site offsets
+//                                                 R(q_i) body_a, FP32; padding
+//                                                 sites sit at +1e6 A
+//   nl_count / nl_list      int[N] / int[N*max_neigh]  symmetric full list
+//   molpart                 double[N*kMolPart]    per-molecule W9, E partials
+//   blockpart               double[nblk*kBlkPart] per-block KE / W / E sums
+//   RigidState              NPT + cell, FP64 (one per system)
+//   RigidMid                first-half results of the current step
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fmd_common.hpp"
+
+namespace fmd {
+
+constexpr int kMolPart = 10;  // W9 + E
+constexpr int kBlkPart = 12;  // sum v^2, sum w^2, W9, E
+
+// Per-step control word (host-decided phase logic of LJ:848-894).
+enum : int {
+  kCtrlZeroXiVg = 1,      // xi = 0, Vg = 0 before the step (first WARM step)
+  kCtrlZeroVg = 2,        // Vg = 0 before the step (COLD phase)
+  kCtrlRescale = 4,       // velocity rescale to rescale_T after the step
+  kCtrlRescaleZeroVg = 8, // ... and Vg = 0 (COLD)
+  kCtrlRebuild = 16,      // rebuild the molecule list before the step
+};
+
+struct StepCtrl {
+  double Tt;         // target temperature of this step
+  double rescale_T;  // target of the velocity rescale
+  int flags;
+  int pad;
+};
+
+struct RigidState {
+  double h[9], hi[9];  // kept adjacent: kernels take them as one 18-double block
+  double xi, Vg[9], Q, W, Pe, Tt;
+  double KEt, KEr, Ep, W9[9];  // observables of the last completed step
+  double scale;                // last velocity rescale factor (1 if none)
+  int Nf, step;
+};
+
+struct RigidMid {
+  double h[9], hi[9];
+  double xi, Vg[9];
+  double sc_nh;
+};
+
+// One monitor record per step, as LJ:874-876 / 896 compute them.
+struct Thermo {
+  double Ep, KE, KEt, KEr, T, P, V, xi, Ep_rebo, Ep_lj;
+  double h[9];
+};
+
+struct RigidParams {
+  int N, natom, natom_pad, max_neigh;
+  double rmcut2;      // molecule-pair cutoff^2 inside the force loop
+  double rlist2;      // list radius^2 (rmcut + skin)^2
+  double dt, I0, Mmol;
+  double clamp_xi, clamp_vg;
+  double cull_r2;     // (rc + max site radius)^2: per-lane neighbour cull
+  LJ32 lj;
+  double eps;         // LJ epsilon (energy scale of the reduced sums)
+};
+
+// Device buffers of one rigid system.
+struct RigidDev {
+  double *pos, *vel, *omg, *qv, *Fv, *Tv;
+  float4 *sites;
+  double *body;  // natom_pad x 4 doubles: x, y, z, padding flag (1 = padding)
+  double *hh;    // 18 doubles: h, h^-1 for the stateless entry points
+  int *nl_count, *nl_list, *flags;  // flags[0]: list overflow count
+  double *molpart, *blockpart;
+  RigidState *state;
+  RigidMid *mid;
+  StepCtrl *ctrl;     // ring of per-step controls
+  Thermo *thermo;     // ring of per-step records
+  int nblk;           // blocks of the per-molecule integrator kernels
+};
+
+// Launchers (rigid_kernels.hip).  All enqueue on `s` and return the HIP
+// status of the launch.
+// `hh` points at 18 device doubles (h then h^-1).
+hipError_t rigid_launch_sites(const RigidDev &d, const RigidParams &p, hipStream_t s);
+hipError_t rigid_launch_apply_pbc(const RigidDev &d, const RigidParams &p, const double *hh,
+                                  hipStream_t s);
+hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const double *hh,
+                              hipStream_t s);
+hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const double *hh,
+                               hipStream_t s);
+// Observables of the current state into RigidState: what = kObsKE | kObsForce.
+enum : int { kObsKE = 1, kObsForce = 2 };
+hipError_t rigid_launch_observe(const RigidDev &d, const RigidParams &p, int what,
+                                hipStream_t s);
+// One NPT step (LJ:847-894 loop body without I/O): optional list rebuild,
+// kick-drift + sites, forces, kick + reductions + NPT second half, optional
+// velocity rescale.  Controls come from ctrl[ctrl_slot]; the record goes to
+// thermo[thermo_slot].
+// ev0 / ev1 (may be null) are recorded around the force kernel.
+hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_slot,
+                             int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
+                             hipEvent_t ev1, hipStream_t s);
+// Sum of molpart (W9, E) into out10 on the device.
+hipError_t rigid_launch_reduce_molpart(const RigidDev &d, const RigidParams &p, double *out10,
+                                       hipStream_t s);
+
+}  // namespace fmd

@@ -1,0 +1,617 @@
+// rigid_kernels.hip -- gfx950 kernels of the rigid-body LJ engine.
+//
+//   k_rigid_nlist       symmetric molecule list, one wavefront per molecule,
+//                       ballot/popcount compaction keeps ascending order
+//                       (nlist_build_sym C1:280-301).
+//   k_rigid_sites       lab-frame FP32 site offsets R(q) body (C1:354-364).
+//   k_rigid_kick_drift  NPT first half + half-kick + fractional drift + cell
+//                       update + quaternion update + sites, wavefront per
+//                       molecule (step_npt (A)-(G), C1:512-574).
+//   k_rigid_forces      site-site LJ force / torque / virial (forces
+//                       C1:346-444): one 256-thread workgroup per molecule,
+//                       the 4 wavefronts split its neighbour list, lane = site
+//                       of molecule i, the sites of neighbour j are
+//                       wavefront-uniform and arrive through scalar loads.
+//   k_rigid_kick2       second half-kick + per-block KE / virial partials
+//                       (step_npt (I), C1:581-589).
+//   k_rigid_finish      fixed-order reduction + NPT second half (J)(K)
+//                       (C1:591-598), thermo record, optional rescale factor
+//                       (LJ:879-894).
+//
+// No floating-point atomics anywhere: every sum has a fixed order, so runs are
+// bitwise repeatable.
+#include <cstddef>
+
+#include "fmd_rigid.hpp"
+
+namespace fmd {
+
+namespace {
+
+constexpr int kXcd = 8;  // MI355X: workgroups are dealt round-robin to 8 XCDs
+
+__device__ __forceinline__ double clampd(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Sum of a double over the 64 lanes of a wavefront (butterfly, fixed order).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Lab-frame offsets of the sites of molecule i (lanes a and a+64 ...).
+template <int NS>
+__device__ __forceinline__ void write_sites(int i, const double *q, const double *body,
+                                            float4 *sites, int natom_pad, int lane) {
+  double R[9];
+  q2R(q, R);
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int a = lane + 64 * s;
+    if (a < natom_pad) {
+      const double *b = body + 4 * a;
+      float4 o;
+      if (b[3] != 0.0) {
+        o = make_float4(1e6f, 1e6f, 1e6f, 0.f);
+      } else {
+        o.x = (float)(R[0] * b[0] + R[1] * b[1] + R[2] * b[2]);
+        o.y = (float)(R[3] * b[0] + R[4] * b[1] + R[5] * b[2]);
+        o.z = (float)(R[6] * b[0] + R[7] * b[1] + R[8] * b[2]);
+        o.w = 0.f;
+      }
+      sites[(size_t)i * natom_pad + a] = o;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ list
+__global__ __launch_bounds__(64) void k_rigid_nlist(const double *__restrict__ pos,
+                                                    const double *__restrict__ hh,
+                                                    int *__restrict__ nl_count,
+                                                    int *__restrict__ nl_list,
+                                                    int *__restrict__ flags, RigidParams p) {
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double *h = hh, *hi = hh + 9;
+  const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+  int count = 0;
+  for (int j0 = 0; j0 < p.N; j0 += 64) {
+    int j = j0 + lane;
+    bool in = false;
+    if (j < p.N && j != i) {
+      double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
+      mimg(dx, dy, dz, hi, h);
+      in = dx * dx + dy * dy + dz * dz < p.rlist2;
+    }
+    unsigned long long m = __ballot(in);
+    int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (in && count + rank < p.max_neigh) nl_list[(size_t)i * p.max_neigh + count + rank] = j;
+    count += __popcll(m);
+  }
+  if (lane == 0) {
+    nl_count[i] = count < p.max_neigh ? count : p.max_neigh;
+    if (count > p.max_neigh) atomicAdd(&flags[0], 1);
+  }
+}
+
+// ------------------------------------------------------------------ sites
+template <int NS>
+__global__ __launch_bounds__(256) void k_rigid_sites(const double *__restrict__ qv,
+                                                     const double *__restrict__ body,
+                                                     float4 *__restrict__ sites, RigidParams p) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= p.N) return;
+  double q[4] = {qv[4 * i], qv[4 * i + 1], qv[4 * i + 2], qv[4 * i + 3]};
+  write_sites<NS>(i, q, body, sites, p.natom_pad, lane);
+}
+
+__global__ __launch_bounds__(256) void k_rigid_apply_pbc(double *__restrict__ pos,
+                                                         const double *__restrict__ hh, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double *h = hh, *hi = hh + 9;
+  double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  double s0 = hi[0] * px + hi[1] * py + hi[2] * pz;
+  double s1 = hi[3] * px + hi[4] * py + hi[5] * pz;
+  double s2 = hi[6] * px + hi[7] * py + hi[8] * pz;
+  s0 -= floor(s0);
+  s1 -= floor(s1);
+  s2 -= floor(s2);
+  pos[3 * i] = h[0] * s0 + h[1] * s1 + h[2] * s2;
+  pos[3 * i + 1] = h[3] * s0 + h[4] * s1 + h[5] * s2;
+  pos[3 * i + 2] = h[6] * s0 + h[7] * s1 + h[8] * s2;
+}
+
+// ------------------------------------------------------- NPT first half
+struct Half1 {
+  double hi[9], h1[9], hi1[9], Vg1[9];
+  double xi1, sc_nh, sc_v;
+};
+
+// step_npt (A)(B) and the scaling factors, C1:512-531, with the per-step
+// phase controls of LJ:851-857.
+__device__ __forceinline__ void npt_half1(const RigidState &S, const StepCtrl &c,
+                                          const RigidParams &p, Half1 &o) {
+  const double hdt = 0.5 * p.dt;
+  double xi = S.xi, Vg[9];
+  for (int k = 0; k < 9; k++) Vg[k] = S.Vg[k];
+  if (c.flags & kCtrlZeroXiVg) {
+    xi = 0;
+    for (int k = 0; k < 9; k++) Vg[k] = 0;
+  }
+  if (c.flags & kCtrlZeroVg)
+    for (int k = 0; k < 9; k++) Vg[k] = 0;
+  inv9(S.h, o.hi);
+  double V = fabs(det9(S.h));
+  double KE = S.KEt + S.KEr;
+  xi += hdt * (2 * KE - S.Nf * kB * c.Tt) / S.Q;
+  xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
+  double dP = (2 * S.KEt + S.W9[0] + S.W9[4] + S.W9[8]) / (3 * V) * kEv2GPa - S.Pe;
+  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V * dP / (S.W * kEv2GPa);
+  for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
+  double eps_tr = Vg[0] * o.hi[0] + Vg[4] * o.hi[4] + Vg[8] * o.hi[8];
+  o.sc_nh = exp(-hdt * xi);
+  o.sc_v = o.sc_nh * exp(-hdt * eps_tr / 3.0);
+  for (int k = 0; k < 9; k++) {
+    o.h1[k] = S.h[k] + p.dt * Vg[k];
+    o.Vg1[k] = Vg[k];
+  }
+  inv9(o.h1, o.hi1);
+  o.xi1 = xi;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_rigid_kick_drift(RigidDev d, RigidParams p,
+                                                          int ctrl_slot) {
+  const RigidState &S = *d.state;
+  const StepCtrl c = d.ctrl[ctrl_slot];
+  Half1 H;
+  npt_half1(S, c, p, H);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    RigidMid &M = *d.mid;
+    for (int k = 0; k < 9; k++) {
+      M.h[k] = H.h1[k];
+      M.hi[k] = H.hi1[k];
+      M.Vg[k] = H.Vg1[k];
+    }
+    M.xi = H.xi1;
+    M.sc_nh = H.sc_nh;
+  }
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= p.N) return;
+  double q[4] = {0, 0, 0, 0};
+  if (lane == 0) {
+    const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
+    double v[3], w[3], r[3];
+    for (int a = 0; a < 3; a++) {
+      v[a] = d.vel[3 * i + a] * H.sc_v + hdt * d.Fv[3 * i + a] * cF;
+      w[a] = d.omg[3 * i + a] * H.sc_nh + hdt * d.Tv[3 * i + a] * cT;
+      r[a] = d.pos[3 * i + a];
+    }
+    const double *hi = H.hi;
+    double sx = hi[0] * r[0] + hi[1] * r[1] + hi[2] * r[2];
+    double sy = hi[3] * r[0] + hi[4] * r[1] + hi[5] * r[2];
+    double sz = hi[6] * r[0] + hi[7] * r[1] + hi[8] * r[2];
+    double vsx = hi[0] * v[0] + hi[1] * v[1] + hi[2] * v[2];
+    double vsy = hi[3] * v[0] + hi[4] * v[1] + hi[5] * v[2];
+    double vsz = hi[6] * v[0] + hi[7] * v[1] + hi[8] * v[2];
+    sx += p.dt * vsx;
+    sy += p.dt * vsy;
+    sz += p.dt * vsz;
+    sx -= floor(sx);
+    sy -= floor(sy);
+    sz -= floor(sz);
+    const double *h = H.h1;
+    d.pos[3 * i] = h[0] * sx + h[1] * sy + h[2] * sz;
+    d.pos[3 * i + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
+    d.pos[3 * i + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
+    for (int a = 0; a < 3; a++) {
+      d.vel[3 * i + a] = v[a];
+      d.omg[3 * i + a] = w[a];
+    }
+    for (int a = 0; a < 4; a++) q[a] = d.qv[4 * i + a];
+    quat_advance(q, w[0], w[1], w[2], p.dt);
+    for (int a = 0; a < 4; a++) d.qv[4 * i + a] = q[a];
+  }
+  for (int a = 0; a < 4; a++) q[a] = __shfl(q[a], 0, 64);
+  write_sites<NS>(i, q, d.body, d.sites, p.natom_pad, lane);
+}
+
+// ------------------------------------------------------------------ forces
+// Per lane (site a of molecule i, lab offset r_a) and neighbour j (minimum
+// image COM vector dm): G_aj = sum_b f_ab, f_ab = fm(d) d, d = dm + r_b - r_a,
+// summed in FP32 over the <= natom sites b, then promoted to FP64.
+//   force on i      F_i = -sum_a sum_j G_aj
+//   torque          T_i = sum_a r_a x (-sum_j G_aj)               (C1:415-417)
+//   virial          W  += 1/2 sum_j dm (x) G_aj - r_a (x) sum_j G_aj
+// The virial line is the reference's sum_pairs 1/2 d (x) f (C1:421-423)
+// rewritten with the symmetric list: the r_b part of pair (i,a,j,b) equals
+// minus the r_a part of its mirror (j,b,i,a), so the per-pair tensor never
+// has to be formed.
+template <int NS>
+__global__ __launch_bounds__(256) void k_rigid_forces(
+    const double *__restrict__ pos, const float4 *__restrict__ sites,
+    const int *__restrict__ nl_count, const int *__restrict__ nl_list,
+    const double *__restrict__ hh, double *__restrict__ Fv, double *__restrict__ Tv,
+    double *__restrict__ molpart, RigidParams p) {
+  // XCD-aware mapping: the workgroups one XCD receives (bid = x mod 8) take a
+  // contiguous range of molecules, so neighbours share that XCD's L2.
+  const int per = (p.N + kXcd - 1) / kXcd;
+  const int bid = blockIdx.x;
+  const int i = (bid % kXcd) * per + bid / kXcd;
+  if (i >= p.N) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  const double *h = hh, *hi = hh + 9;
+  const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+  const int npad = p.natom_pad;
+
+  float rax[NS], ray[NS], raz[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int a = lane + 64 * s;
+    if (a < p.natom) {
+      float4 r = sites[(size_t)i * npad + a];
+      rax[s] = r.x;
+      ray[s] = r.y;
+      raz[s] = r.z;
+    } else {  // idle lane: far away from everything
+      rax[s] = ray[s] = raz[s] = -3.0e5f;
+    }
+  }
+  double GA[NS][3], WA[NS][9];
+  float EA[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    for (int k = 0; k < 3; k++) GA[s][k] = 0;
+    for (int k = 0; k < 9; k++) WA[s][k] = 0;
+    EA[s] = 0;
+  }
+  const LJ32 lj = p.lj;
+  const int nn = nl_count[i];
+  const int *li = nl_list + (size_t)i * p.max_neigh;
+  for (int k = wv; k < nn; k += 4) {
+    const int j = uniform(li[k]);
+    double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
+    mimg(dx, dy, dz, hi, h);
+    if (dx * dx + dy * dy + dz * dz > p.rmcut2) continue;
+    float cx[NS], cy[NS], cz[NS];
+    bool act = false;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      cx[s] = (float)dx - rax[s];
+      cy[s] = (float)dy - ray[s];
+      cz[s] = (float)dz - raz[s];
+      act |= cx[s] * cx[s] + cy[s] * cy[s] + cz[s] * cz[s] < (float)p.cull_r2;
+    }
+    if (!__any(act)) continue;
+    float gx[NS], gy[NS], gz[NS], e[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) gx[s] = gy[s] = gz[s] = e[s] = 0.f;
+    const float4 *sj = sites + (size_t)j * npad;
+    for (int b = 0; b < npad; b += 4) {
+      // the four sites of this group are loaded together (one 64-byte scalar
+      // load) before any of the divergent cutoff branches
+      float4 rb[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) rb[u] = sj[b + u];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          float ex = rb[u].x + cx[s], ey = rb[u].y + cy[s], ez = rb[u].z + cz[s];
+          float r2 = ex * ex + ey * ey + ez * ez;
+          if (r2 < lj.rc2) {
+            float fm = lj_fm(r2, lj, e[s]);
+            gx[s] += fm * ex;
+            gy[s] += fm * ey;
+            gz[s] += fm * ez;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const double c48 = 48.0 * p.eps;
+      double g0 = c48 * gx[s], g1 = c48 * gy[s], g2 = c48 * gz[s];
+      GA[s][0] += g0;
+      GA[s][1] += g1;
+      GA[s][2] += g2;
+      WA[s][0] += dx * g0;
+      WA[s][1] += dx * g1;
+      WA[s][2] += dx * g2;
+      WA[s][3] += dy * g0;
+      WA[s][4] += dy * g1;
+      WA[s][5] += dy * g2;
+      WA[s][6] += dz * g0;
+      WA[s][7] += dz * g1;
+      WA[s][8] += dz * g2;
+      EA[s] += e[s];
+    }
+  }
+  // Per-lane molecule contributions: F(3) T(3) W(9) E(1).
+  double acc[16];
+  for (int k = 0; k < 16; k++) acc[k] = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    double r0 = rax[s], r1 = ray[s], r2 = raz[s];
+    double f0 = -GA[s][0], f1 = -GA[s][1], f2 = -GA[s][2];
+    acc[0] += f0;
+    acc[1] += f1;
+    acc[2] += f2;
+    acc[3] += r1 * f2 - r2 * f1;
+    acc[4] += r2 * f0 - r0 * f2;
+    acc[5] += r0 * f1 - r1 * f0;
+    const double ra[3] = {r0, r1, r2};
+    for (int x = 0; x < 3; x++)
+      for (int y = 0; y < 3; y++) acc[6 + 3 * x + y] += 0.5 * WA[s][3 * x + y] - ra[x] * GA[s][y];
+    acc[15] += (double)EA[s];
+  }
+  __shared__ double red[4][16];
+  for (int k = 0; k < 16; k++) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int k = threadIdx.x;
+    double v = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    if (k < 3)
+      Fv[3 * i + k] = v;
+    else if (k < 6)
+      Tv[3 * i + k - 3] = v;
+    else if (k < 15)
+      molpart[(size_t)i * kMolPart + (k - 6)] = v;
+    else
+      molpart[(size_t)i * kMolPart + 9] = 2.0 * p.eps * v;  // 1/2 * 4 eps * sum e
+  }
+}
+
+// ------------------------------------------------------ kick 2 + partials
+// Block sums of 12 doubles in a fixed order.
+__device__ __forceinline__ void block_sum12(double *v, double *out) {
+  __shared__ double red[4][kBlkPart];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < kBlkPart; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < kBlkPart) {
+    const int k = threadIdx.x;
+    out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, int kick) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v[kBlkPart];
+  for (int k = 0; k < kBlkPart; k++) v[k] = 0;
+  if (i < p.N) {
+    double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
+    double wx = d.omg[3 * i], wy = d.omg[3 * i + 1], wz = d.omg[3 * i + 2];
+    if (kick) {
+      const RigidMid &M = *d.mid;
+      const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
+      double eps_tr2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
+      double sc_v2 = M.sc_nh * exp(-hdt * eps_tr2 / 3.0);
+      vx = (vx + hdt * d.Fv[3 * i] * cF) * sc_v2;
+      vy = (vy + hdt * d.Fv[3 * i + 1] * cF) * sc_v2;
+      vz = (vz + hdt * d.Fv[3 * i + 2] * cF) * sc_v2;
+      wx = (wx + hdt * d.Tv[3 * i] * cT) * M.sc_nh;
+      wy = (wy + hdt * d.Tv[3 * i + 1] * cT) * M.sc_nh;
+      wz = (wz + hdt * d.Tv[3 * i + 2] * cT) * M.sc_nh;
+      d.vel[3 * i] = vx;
+      d.vel[3 * i + 1] = vy;
+      d.vel[3 * i + 2] = vz;
+      d.omg[3 * i] = wx;
+      d.omg[3 * i + 1] = wy;
+      d.omg[3 * i + 2] = wz;
+    }
+    v[0] = vx * vx + vy * vy + vz * vz;
+    v[1] = wx * wx + wy * wy + wz * wz;
+    for (int k = 0; k < kMolPart; k++) v[2 + k] = d.molpart[(size_t)i * kMolPart + k];
+  }
+  block_sum12(v, d.blockpart + (size_t)blockIdx.x * kBlkPart);
+}
+
+// Fixed-order reduction of the block partials (one workgroup).
+__device__ __forceinline__ void reduce_blocks(const double *bp, int nblk, double *sums) {
+  double v[kBlkPart];
+  for (int k = 0; k < kBlkPart; k++) v[k] = 0;
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int k = 0; k < kBlkPart; k++) v[k] += bp[(size_t)b * kBlkPart + k];
+  __shared__ double tot[kBlkPart];
+  block_sum12(v, tot);
+  __syncthreads();
+  for (int k = 0; k < kBlkPart; k++) sums[k] = tot[k];
+}
+
+__global__ __launch_bounds__(256) void k_rigid_observe(RigidDev d, RigidParams p, int what) {
+  double sums[kBlkPart];
+  reduce_blocks(d.blockpart, d.nblk, sums);
+  if (threadIdx.x != 0) return;
+  RigidState &S = *d.state;
+  if (what & kObsKE) {
+    S.KEt = 0.5 * p.Mmol * sums[0] / kConv;
+    S.KEr = 0.5 * p.I0 * sums[1] / kConv;
+  }
+  if (what & kObsForce) {
+    for (int k = 0; k < 9; k++) S.W9[k] = sums[2 + k];
+    S.Ep = sums[11];
+  }
+}
+
+// step_npt (J)(K) C1:591-598 plus the monitor quantities of LJ:874-876 and
+// the COLD/WARM rescale of LJ:879-894 (the velocities are scaled by
+// k_rigid_rescale; KE is rescaled by s^2 here).
+__global__ __launch_bounds__(256) void k_rigid_finish(RigidDev d, RigidParams p, int ctrl_slot,
+                                                      int thermo_slot) {
+  double sums[kBlkPart];
+  reduce_blocks(d.blockpart, d.nblk, sums);
+  if (threadIdx.x != 0) return;
+  RigidState &S = *d.state;
+  const RigidMid &M = *d.mid;
+  const StepCtrl c = d.ctrl[ctrl_slot];
+  const double hdt = 0.5 * p.dt;
+  double KEt = 0.5 * p.Mmol * sums[0] / kConv;
+  double KEr = 0.5 * p.I0 * sums[1] / kConv;
+  double KE = KEt + KEr;
+  double W9[9];
+  for (int k = 0; k < 9; k++) W9[k] = sums[2 + k];
+  double Ep = sums[11];
+  double xi = M.xi, Vg[9];
+  for (int k = 0; k < 9; k++) Vg[k] = M.Vg[k];
+  xi += hdt * (2 * KE - S.Nf * kB * c.Tt) / S.Q;
+  xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
+  double V2 = fabs(det9(M.h));
+  double P = (2 * KEt + W9[0] + W9[4] + W9[8]) / (3 * V2) * kEv2GPa;
+  double dP = P - S.Pe;
+  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * dP / (S.W * kEv2GPa);
+  for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
+  double T = 2 * KE / (S.Nf * kB);
+  double scale = 1.0;
+  if ((c.flags & kCtrlRescale) && T > 0.1) {
+    double tgt = c.rescale_T > 0.1 ? c.rescale_T : 0.1;
+    scale = sqrt(tgt / T);
+    KEt *= scale * scale;
+    KEr *= scale * scale;
+    KE = KEt + KEr;
+    T = 2 * KE / (S.Nf * kB);
+    xi = 0;
+    if (c.flags & kCtrlRescaleZeroVg)
+      for (int k = 0; k < 9; k++) Vg[k] = 0;
+  }
+  for (int k = 0; k < 9; k++) {
+    S.h[k] = M.h[k];
+    S.hi[k] = M.hi[k];
+    S.Vg[k] = Vg[k];
+    S.W9[k] = W9[k];
+  }
+  S.xi = xi;
+  S.Tt = c.Tt;
+  S.KEt = KEt;
+  S.KEr = KEr;
+  S.Ep = Ep;
+  S.scale = scale;
+  S.step += 1;
+  Thermo &t = d.thermo[thermo_slot];
+  t.Ep = Ep;
+  t.KE = KE;
+  t.KEt = KEt;
+  t.KEr = KEr;
+  t.T = T;
+  t.P = P;
+  t.V = V2;
+  t.xi = xi;
+  t.Ep_rebo = 0;
+  t.Ep_lj = Ep;
+  for (int k = 0; k < 9; k++) t.h[k] = M.h[k];
+}
+
+__global__ __launch_bounds__(256) void k_rigid_rescale(RigidDev d, RigidParams p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.N) return;
+  const double s = d.state->scale;
+  for (int a = 0; a < 3; a++) {
+    d.vel[3 * i + a] *= s;
+    d.omg[3 * i + a] *= s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rigid_reduce_molpart(const double *__restrict__ molpart,
+                                                              int n, double *__restrict__ out) {
+  double v[kBlkPart];
+  for (int k = 0; k < kBlkPart; k++) v[k] = 0;
+  for (int i = threadIdx.x; i < n; i += 256)
+    for (int k = 0; k < kMolPart; k++) v[k] += molpart[(size_t)i * kMolPart + k];
+  __shared__ double tot[kBlkPart];
+  block_sum12(v, tot);
+  __syncthreads();
+  if (threadIdx.x < kMolPart) out[threadIdx.x] = tot[threadIdx.x];
+}
+
+inline int ns_of(const RigidParams &p) { return p.natom <= 64 ? 1 : 2; }
+
+}  // namespace
+
+hipError_t rigid_launch_sites(const RigidDev &d, const RigidParams &p, hipStream_t s) {
+  dim3 g((p.N + 3) / 4);
+  if (ns_of(p) == 1)
+    k_rigid_sites<1><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, p);
+  else
+    k_rigid_sites<2><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, p);
+  return hipGetLastError();
+}
+
+hipError_t rigid_launch_apply_pbc(const RigidDev &d, const RigidParams &p, const double *hh,
+                                  hipStream_t s) {
+  k_rigid_apply_pbc<<<(p.N + 255) / 256, 256, 0, s>>>(d.pos, hh, p.N);
+  return hipGetLastError();
+}
+
+hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const double *hh,
+                              hipStream_t s) {
+  k_rigid_nlist<<<p.N, 64, 0, s>>>(d.pos, hh, d.nl_count, d.nl_list, d.flags, p);
+  return hipGetLastError();
+}
+
+hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const double *hh,
+                               hipStream_t s) {
+  const int per = (p.N + kXcd - 1) / kXcd;
+  dim3 g(per * kXcd);
+  if (ns_of(p) == 1)
+    k_rigid_forces<1><<<g, 256, 0, s>>>(d.pos, d.sites, d.nl_count, d.nl_list, hh, d.Fv, d.Tv,
+                                        d.molpart, p);
+  else
+    k_rigid_forces<2><<<g, 256, 0, s>>>(d.pos, d.sites, d.nl_count, d.nl_list, hh, d.Fv, d.Tv,
+                                        d.molpart, p);
+  return hipGetLastError();
+}
+
+hipError_t rigid_launch_observe(const RigidDev &d, const RigidParams &p, int what,
+                                hipStream_t s) {
+  k_rigid_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0);
+  k_rigid_observe<<<1, 256, 0, s>>>(d, p, what);
+  return hipGetLastError();
+}
+
+hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_slot,
+                             int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
+                             hipEvent_t ev1, hipStream_t s) {
+  hipError_t e;
+  const double *state_hh = reinterpret_cast<const double *>(
+      reinterpret_cast<const char *>(d.state) + offsetof(RigidState, h));
+  const double *mid_hh = reinterpret_cast<const double *>(
+      reinterpret_cast<const char *>(d.mid) + offsetof(RigidMid, h));
+  if (rebuild) {
+    if ((e = rigid_launch_nlist(d, p, state_hh, s)) != hipSuccess) return e;
+  }
+  dim3 g((p.N + 3) / 4);
+  if (ns_of(p) == 1)
+    k_rigid_kick_drift<1><<<g, 256, 0, s>>>(d, p, ctrl_slot);
+  else
+    k_rigid_kick_drift<2><<<g, 256, 0, s>>>(d, p, ctrl_slot);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
+  if ((e = rigid_launch_forces(d, p, mid_hh, s)) != hipSuccess) return e;
+  if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
+  k_rigid_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
+  k_rigid_finish<<<1, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+  if (rescale) k_rigid_rescale<<<(p.N + 255) / 256, 256, 0, s>>>(d, p);
+  return hipGetLastError();
+}
+
+hipError_t rigid_launch_reduce_molpart(const RigidDev &d, const RigidParams &p, double *out10,
+                                       hipStream_t s) {
+  k_rigid_reduce_molpart<<<1, 256, 0, s>>>(d.molpart, p.N, out10);
+  return hipGetLastError();
+}
+
+}  // namespace fmd

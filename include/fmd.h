@@ -1,0 +1,211 @@
+/*
+ * fmd.h -- C ABI of the MI355X fuller_md force engine (libfmd.so).
+ *
+ * Two layers:
+ *
+ *  1. Drop-in entry points with the arguments of the reference's force,
+ *     list and integrator functions (host pointers, reference layouts).  Each
+ *     call copies its inputs to the GPU, runs the gfx950 kernels and copies
+ *     the outputs back, so a reference driver can swap one call at a time.
+ *     Reference files are cited as
+ *       C1 = src/fuller_LJ_npt_md_core_serial.cpp
+ *       LJ = src/fuller_LJ_npt_md_serial_omp_acc.cpp
+ *       AB = src/fuller_airebo_npt_md_serial_omp_acc.cpp
+ *
+ *  2. A resident engine (fmd_engine_*) that keeps the whole NPT state in HBM
+ *     and runs many steps per call: the fast path used by the C++ drivers and
+ *     the benchmark.
+ *
+ * Layout conventions (all as in the reference): flat AoS x,y,z per particle;
+ * quaternions [w,x,y,z]; the cell matrix h is row-major with the cell vectors
+ * as its columns (r = h s); virial W[3a+b] = sum d_a f_b; units A, amu, eV,
+ * fs, K, GPa.
+ *
+ * Errors: every function returns FMD_OK (0) or a negative status;
+ * fmd_last_error() gives a message (thread-local).  Kernels never fail
+ * silently: list overflow, which the reference ignores, is reported through
+ * fmd_engine_counters().
+ *
+ * Threading: one engine per host thread; the drop-in entry points share one
+ * per-process device workspace and must be called from one thread at a time.
+ */
+#ifndef FMD_H
+#define FMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMD_OK 0
+#define FMD_ERR_ARG -1
+#define FMD_ERR_HIP -2
+#define FMD_ERR_CAPACITY -3
+#define FMD_ERR_STATE -4
+#define FMD_ERR_NODEVICE -5
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fmd_last_error(void);
+/* ABI version (major * 100 + minor). */
+int fmd_version(void);
+/* Name and compute-unit count of the current HIP device. */
+int fmd_device_info(char *name, int cap, int *cu_count, double *hbm_bytes);
+
+/* Shifted 12-6 Lennard-Jones: V = 4 eps ((s/r)^12 - (s/r)^6) - vshift for
+ * r < rcut (rigid, C1:403) or r <= rcut (AIREBO, AB:441); r^2 is clamped at
+ * rmin2 (0.25 in the reference, C1:404). */
+typedef struct {
+  double sigma, eps, rcut, vshift, rmin2;
+} fmd_lj_params;
+
+/* Reference constants: rigid sigma=3.431 eps=2.635e-3 rc=3 sigma (C1:93-104);
+ * AIREBO sigma=3.40 eps=2.84e-3 rc=3 sigma (AB:119-124). */
+void fmd_lj_params_rigid(fmd_lj_params *p);
+void fmd_lj_params_airebo(fmd_lj_params *p);
+
+/* NPTState of the reference (C1:478 / AB:511). */
+typedef struct {
+  double xi, Q, Vg[9], W, Pe, Tt;
+  int Nf;
+} fmd_npt_state;
+
+/* make_npt (C1:480-488; AB:512-515): Q = Nf kB T 1e4, W = (Nf+9) kB T 1e6. */
+void fmd_make_npt(fmd_npt_state *s, double T, double Pe, int Nf);
+
+/* ------------------------------------------------- system construction */
+
+/* load_cc1 (LJ:178-207, AB:195-210): coords (>= 84 x 3) centred on the
+ * geometric centre; props = {Rmol, Dmol, I0, Mmol}. */
+int fmd_load_cc1(const char *path, double *coords, int *natom, double *props);
+/* resolve_fullerene (LJ:210-226): species spec -> .cc1 path and label. */
+int fmd_resolve_fullerene(const char *spec, const char *libdir, char *path, int pcap,
+                          char *label, int lcap);
+/* default_a0 (LJ:251-254). */
+double fmd_default_a0(double dmax, const char *crystal, double scale);
+/* make_fcc / make_hcp / make_bcc (LJ:229-250); returns the molecule count
+ * (or a negative status).  pos holds 4 nc^3 (FCC) or 2 nc^3 entries. */
+int fmd_make_crystal(const char *crystal, double a, int nc, double *pos, double *h);
+/* Seeded initial state of LJ:790-801 (std::mt19937 + normal_distribution). */
+int fmd_rigid_init_state(int n, unsigned seed, double T, double Mmol, double I0, double *vel,
+                         double *omg, double *qv);
+/* Seeded atomic velocities of AB:808-815. */
+int fmd_atomic_init_velocities(int na, unsigned seed, double T, const double *mass,
+                               double *vel);
+/* First n draws of normal_distribution<double>(0,1) over mt19937(seed). */
+int fmd_normal_stream(unsigned seed, int n, double *out);
+
+/* ----------------------------------------------------- rigid drop-ins */
+
+/* Replaces nlist_build_sym (C1:280-301, LJ:257-268): symmetric list of
+ * molecule pairs with minimum-image COM distance < rmcut + 3, ascending,
+ * capped at max_neigh (reference MAX_NEIGH = 80, C1:121).  *overflow (may be
+ * NULL) receives the number of molecules whose list was truncated. */
+int fmd_rigid_nlist(const double *pos, const double *h, const double *hi, int N,
+                    double rmcut, int max_neigh, int *nl_count, int *nl_list,
+                    int *overflow);
+
+/* Replaces forces (C1:346-351, LJ:271-274): site-site LJ between rigid
+ * molecules.  Writes Fv, Tv (N x 3), Wm9 (9) and *Ep.  lab (N*natom*3, may be
+ * NULL) receives the lab-frame site offsets as in the reference.  lj may be
+ * NULL for the rigid reference constants.  natom <= 128. */
+int fmd_rigid_forces(double *Fv, double *Tv, double *Wm9, const double *pos,
+                     const double *qv, const double *body, const double *h,
+                     const double *hi, const int *nl_count, const int *nl_list,
+                     int max_neigh, int N, int natom, double rmcut2,
+                     const fmd_lj_params *lj, double *lab, double *Ep);
+
+/* Replaces step_npt (C1:504-510, LJ:434-439): one NH + PR velocity-Verlet
+ * step of rigid molecules (clamps xi +-0.1, Vg +-0.01).  Updates pos, vel,
+ * qv, omg, Fv, Tv, Wm9, h, hi and *npt in place; *Ep, *KE as the reference
+ * returns them. */
+int fmd_rigid_step_npt(double *pos, double *vel, double *qv, double *omg, double *Fv,
+                       double *Tv, double *Wm9, double *h, double *hi,
+                       const double *body, double I0, double Mmol, int N, int natom,
+                       double rmcut2, double dt, fmd_npt_state *npt,
+                       const int *nl_count, const int *nl_list, int max_neigh,
+                       const fmd_lj_params *lj, double *Ep, double *KE);
+
+/* ------------------------------------------------------- resident engine */
+
+typedef struct fmd_engine fmd_engine;
+
+#define FMD_KIND_RIGID 1
+#define FMD_KIND_ATOMIC 2
+
+typedef struct {
+  int n_mol;            /* molecules */
+  int natom;            /* sites per molecule (<= 128) */
+  const double *body;   /* natom x 3 body-frame offsets from the COM (A) */
+  double I0, Mmol;      /* isotropic inertia (amu A^2), molecular mass (amu) */
+  fmd_lj_params lj;
+  double rmcut;         /* molecule-pair cutoff (reference rc + 2 Rmol + 1) */
+  double skin;          /* list radius = rmcut + skin (reference 3.0) */
+  int max_neigh;        /* reference 80 */
+  int nlup;             /* list rebuild interval in global steps (25) */
+  double dt;            /* fs */
+  double clamp_xi, clamp_vg; /* 0.1, 0.01 (C1:520, 525) */
+} fmd_rigid_config;
+
+/* Defaults of the reference rigid programs for the given molecule radius. */
+void fmd_rigid_config_default(fmd_rigid_config *c, double rmol);
+
+int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out);
+
+/* State in reference layouts.  Any pointer may be NULL (left unchanged). */
+int fmd_rigid_set_state(fmd_engine *e, const double *pos, const double *vel,
+                        const double *qv, const double *omg, const double *h,
+                        const fmd_npt_state *npt);
+int fmd_rigid_get_state(fmd_engine *e, double *pos, double *vel, double *qv, double *omg,
+                        double *h, fmd_npt_state *npt);
+
+/* Per-step phase control (the COLD / WARM / PROD logic of LJ:848-894). */
+#define FMD_CTRL_ZERO_XI_VG 1  /* xi = 0, Vg = 0 before the step */
+#define FMD_CTRL_ZERO_VG 2     /* Vg = 0 before the step */
+#define FMD_CTRL_RESCALE 4     /* rescale velocities to rescale_T after it */
+#define FMD_CTRL_RESCALE_ZERO_VG 8
+#define FMD_CTRL_REBUILD 16    /* force a list rebuild before the step */
+typedef struct {
+  double Tt;         /* target temperature of the step */
+  double rescale_T;
+  int flags;
+  int pad;
+} fmd_step_ctrl;
+
+/* Monitor record of one step (the quantities LJ:874-876 / AB:896 print). */
+typedef struct {
+  double Ep, KE, KEt, KEr, T, P, V, xi, Ep_rebo, Ep_lj;
+  double h[9];
+} fmd_thermo;
+
+/* Prepare the current state as the reference does before its loop
+ * (LJ:813-828): wrap positions into the cell, build the list, compute forces
+ * and the observables the first step needs. */
+int fmd_engine_prepare(fmd_engine *e);
+
+/* Run nsteps steps whose global indices are first_step+1 .. first_step+nsteps.
+ * ctrl: nsteps entries, or NULL (production: Tt = the state's Tt).  The list
+ * is rebuilt before every global step divisible by nlup (LJ:859) or when
+ * FMD_CTRL_REBUILD is set.  thermo: nsteps records, or NULL. */
+int fmd_engine_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+                   fmd_thermo *thermo);
+
+/* Forces of the current state: rigid -> F = Fv (N x 3), T = Tv; atomic ->
+ * F (Na x 3), T unused.  W9, energies {Ep, Ep_rebo, Ep_lj}.  Any may be NULL. */
+int fmd_engine_forces(fmd_engine *e, double *F, double *T, double *W9, double *energies);
+
+/* Counters: [0] list overflows since creation, [1] list rebuilds, [2] steps. */
+int fmd_engine_counters(fmd_engine *e, long long *out3);
+
+/* Mean duration (ms) of the force kernel over the launches timed since the
+ * last call (HIP events on the engine's stream; enabled by
+ * fmd_engine_set_timing(e, 1)).  *n receives the number of launches. */
+int fmd_engine_set_timing(fmd_engine *e, int on);
+int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n);
+
+/* Wait for all work of the engine. */
+int fmd_engine_sync(fmd_engine *e);
+void fmd_engine_destroy(fmd_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMD_H */

@@ -50,6 +50,7 @@ def core():
         L.refc_init.restype = _i
         L.refc_normal_stream.argtypes = [C.c_uint, _i, _dp]
         L.refc_apply_pbc.argtypes = [_dp, _dp, _dp, _i]
+        L.refc_mat_inv9.argtypes = [_dp, _dp]
         _libs["core"] = L
     return _libs["core"]
 
@@ -89,6 +90,7 @@ def airebo():
         L.refab_step_npt.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _d, _dp, _ip,
                                      _ip, _ip, _ip, _dp]
         L.refab_apply_pbc.argtypes = [_dp, _dp, _dp, _i]
+        L.refab_mat_inv9.argtypes = [_dp, _dp]
         _libs["airebo"] = L
     return _libs["airebo"]
 
