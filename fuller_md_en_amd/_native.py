@@ -52,6 +52,14 @@ class RigidConfig(C.Structure):
                 ("clamp_vg", C.c_double)]
 
 
+class AtomicConfig(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("mol_id", C.POINTER(C.c_int)),
+                ("mass", C.POINTER(C.c_double)), ("lj", LJParams), ("lj_skin", C.c_double),
+                ("lj_max_neigh", C.c_int), ("rebo_rlist", C.c_double),
+                ("rebo_max_neigh", C.c_int), ("rebo_mode", C.c_int), ("nlup", C.c_int),
+                ("dt", C.c_double), ("clamp_xi", C.c_double), ("clamp_vg", C.c_double)]
+
+
 class StepCtrl(C.Structure):
     _fields_ = [("Tt", C.c_double), ("rescale_T", C.c_double), ("flags", C.c_int),
                 ("pad", C.c_int)]
@@ -75,11 +83,6 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _d, _i, _vp = C.c_double, C.c_int, C.c_void_p
 
 
-def _optd(x):
-    """ndpointer that also accepts None."""
-    return x
-
-
 _SIGS = {
     "fmd_last_error": (C.c_char_p, []),
     "fmd_version": (_i, []),
@@ -97,6 +100,16 @@ _SIGS = {
     "fmd_rigid_create": (_i, [C.POINTER(RigidConfig), C.POINTER(_vp)]),
     "fmd_rigid_set_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
     "fmd_rigid_get_state": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_airebo_nlist_rebo": (_i, [_dp, _dp, _dp, _i, _ip, _ip, _ip]),
+    "fmd_airebo_nlist_lj": (_i, [_dp, _dp, _dp, _i, _ip, _ip, _ip]),
+    "fmd_airebo_compute_forces": (_i, [_dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _i, _ip,
+                                       _i, _dp]),
+    "fmd_airebo_step_npt": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _d,
+                                 C.POINTER(NPTState), _ip, _ip, _ip, _ip, _ip, _i, _dp]),
+    "fmd_atomic_config_default": (None, [C.POINTER(AtomicConfig)]),
+    "fmd_atomic_create": (_i, [C.POINTER(AtomicConfig), C.POINTER(_vp)]),
+    "fmd_atomic_set_state": (_i, [_vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_atomic_get_state": (_i, [_vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
     "fmd_engine_prepare": (_i, [_vp]),
     "fmd_engine_run": (_i, [_vp, _i, _i, _vp, _vp]),
     "fmd_engine_forces": (_i, [_vp, _vp, _vp, _vp, _vp]),

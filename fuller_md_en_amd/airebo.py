@@ -1,0 +1,156 @@
+"""AIREBO path (REBO bond order + intermolecular LJ), computed on the GPU.
+
+The functions keep the names and argument meaning of the reference's
+(src/fuller_airebo_npt_md_serial_omp_acc.cpp "AB"): build_nlist_rebo,
+build_nlist_lj, compute_forces, step_npt.  ``AtomicEngine`` is the
+HBM-resident fast path.  ``mode``: REBO_COMPAT (the reference's forces and
+pair-only virial) or REBO_PHYSICAL (exact gradient, full virial).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import NPTState  # noqa: F401
+
+REBO_COMPAT = 0
+REBO_PHYSICAL = 1
+MAX_REBO_NEIGH = 12  # AB:126
+MAX_LJ_NEIGH = 400  # AB:127
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def make_npt(T: float, Pe: float, Na: int) -> N.NPTState:
+    """make_npt (AB:512-515): Nf = 3 Na - 3."""
+    s = N.NPTState()
+    N.lib().fmd_make_npt(C.byref(s), T, Pe, 3 * Na - 3)
+    return s
+
+
+def build_nlist_rebo(pos, h, hi, mol_id):
+    """build_nlist_rebo (AB:251-261) -> (nlc[Na], nll[Na, 12])."""
+    pos = _c(pos).reshape(-1)
+    na = pos.size // 3
+    nlc = np.zeros(na, np.int32)
+    nll = np.zeros(na * MAX_REBO_NEIGH, np.int32)
+    N.check(N.lib().fmd_airebo_nlist_rebo(pos, _c(h), _c(hi), na, _c(mol_id, np.int32), nlc,
+                                          nll))
+    return nlc, nll.reshape(na, MAX_REBO_NEIGH)
+
+
+def build_nlist_lj(pos, h, hi, mol_id):
+    """build_nlist_lj (AB:263-272) -> (nlc[Na], nll[Na, 400])."""
+    pos = _c(pos).reshape(-1)
+    na = pos.size // 3
+    nlc = np.zeros(na, np.int32)
+    nll = np.zeros(na * MAX_LJ_NEIGH, np.int32)
+    N.check(N.lib().fmd_airebo_nlist_lj(pos, _c(h), _c(hi), na, _c(mol_id, np.int32), nlc, nll))
+    return nlc, nll.reshape(na, MAX_LJ_NEIGH)
+
+
+def compute_forces(pos, h, hi, lists, mol_id, mode=REBO_COMPAT):
+    """compute_forces (AB:477-495) -> ((Ep, Ep_rebo, Ep_lj), F[Na,3], vir9[9])."""
+    pos = _c(pos).reshape(-1)
+    na = pos.size // 3
+    nlc_r, nll_r, nlc_l, nll_l = (_c(x, np.int32).reshape(-1) for x in lists)
+    F, W, out = np.zeros(na * 3), np.zeros(9), np.zeros(3)
+    N.check(N.lib().fmd_airebo_compute_forces(F, W, pos, _c(h), _c(hi), nlc_r, nll_r, nlc_l,
+                                              nll_l, na, _c(mol_id, np.int32), mode, out))
+    return tuple(out), F.reshape(na, 3), W
+
+
+def step_npt(pos, vel, F, vir9, h, hi, mass, dt, npt, lists, mol_id, mode=REBO_COMPAT):
+    """step_npt (AB:518-590): in place like the reference; returns
+    (Ep, Ep_rebo, Ep_lj, KE)."""
+    for a in (pos, vel, F, vir9, h, hi):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+            raise TypeError("step_npt updates its arrays in place: pass C-contiguous float64")
+    na = pos.size // 3
+    nlc_r, nll_r, nlc_l, nll_l = (_c(x, np.int32).reshape(-1) for x in lists)
+    out = np.zeros(4)
+    N.check(N.lib().fmd_airebo_step_npt(pos.reshape(-1), vel.reshape(-1), F.reshape(-1), vir9, h,
+                                        hi, _c(mass), na, dt, C.byref(npt), nlc_r, nll_r, nlc_l,
+                                        nll_l, _c(mol_id, np.int32), mode, out))
+    return tuple(out)
+
+
+class AtomicEngine:
+    """HBM-resident AIREBO NPT engine (fmd_atomic_create / fmd_engine_*)."""
+
+    def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
+                 nlup=20, lj=None):
+        self._mol = _c(mol_id, np.int32)
+        self._mass = _c(mass)
+        cfg = N.AtomicConfig()
+        N.lib().fmd_atomic_config_default(C.byref(cfg))
+        cfg.n_atoms = len(self._mol)
+        cfg.mol_id = self._mol.ctypes.data_as(C.POINTER(C.c_int))
+        cfg.mass = self._mass.ctypes.data_as(C.POINTER(C.c_double))
+        cfg.dt, cfg.rebo_mode, cfg.lj_max_neigh, cfg.nlup = dt, mode, lj_max_neigh, nlup
+        if lj is not None:
+            cfg.lj = lj
+        self.cfg = cfg
+        self.n = len(self._mol)
+        h = C.c_void_p()
+        N.check(N.lib().fmd_atomic_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            N.lib().fmd_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, pos=None, vel=None, h=None, npt=None):
+        arrs = [None if a is None else _c(a).reshape(-1) for a in (pos, vel, h)]
+        N.check(N.lib().fmd_atomic_set_state(self._h, *[N.ptr(a) for a in arrs],
+                                             C.byref(npt) if npt is not None else None))
+
+    def get_state(self):
+        pos, vel, h = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros(9)
+        npt = N.NPTState()
+        N.check(N.lib().fmd_atomic_get_state(self._h, N.ptr(pos), N.ptr(vel), N.ptr(h),
+                                             C.byref(npt)))
+        return dict(pos=pos, vel=vel, h=h, npt=npt)
+
+    def prepare(self):
+        N.check(N.lib().fmd_engine_prepare(self._h))
+
+    def forces(self):
+        F, W, E = np.zeros((self.n, 3)), np.zeros(9), np.zeros(3)
+        N.check(N.lib().fmd_engine_forces(self._h, N.ptr(F), None, N.ptr(W), N.ptr(E)))
+        return tuple(E), F, W
+
+    def run(self, first_step, nsteps, ctrl=None, thermo=True):
+        ctrl_arr = (N.StepCtrl * nsteps)(*ctrl) if ctrl is not None else None
+        th = (N.Thermo * nsteps)() if thermo else None
+        N.check(N.lib().fmd_engine_run(self._h, first_step, nsteps,
+                                       C.cast(ctrl_arr, C.c_void_p) if ctrl_arr else None,
+                                       C.cast(th, C.c_void_p) if th else None))
+        return th
+
+    def counters(self):
+        out = (C.c_longlong * 3)()
+        N.check(N.lib().fmd_engine_counters(self._h, out))
+        return list(out)
+
+    def set_timing(self, on=True):
+        N.check(N.lib().fmd_engine_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self):
+        ms, n = C.c_double(0), C.c_int(0)
+        N.check(N.lib().fmd_engine_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sync(self):
+        N.check(N.lib().fmd_engine_sync(self._h))

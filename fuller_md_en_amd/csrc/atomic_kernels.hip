@@ -1,0 +1,895 @@
+// atomic_kernels.hip -- gfx950 kernels of the atomic (AIREBO) engine.
+//
+//   k_at_centers      molecule centres / radii / FP32 offset tiles, one
+//                     wavefront per molecule (per step)
+//   k_at_mol_nlist    molecule neighbour list, ballot compaction (rebuild)
+//   k_at_rebo_nlist   same-molecule REBO list, reference order (build_nlist_rebo
+//                     AB:251-261; rebuild)
+//   k_at_scan_half    the reference's LJ half list walk (build_nlist_lj
+//                     AB:263-272): per atom, ascending j > i of other
+//                     molecules within rc + 2, first 400 kept.  Writes the
+//                     truncation point jmax (engine) or the list itself
+//                     (drop-in).  FP64 with the reference's operation order.
+//   k_at_kick_drift   NPT first half + half-kick + drift + cell update
+//                     (step_npt AB:522-568)
+//   k_rebo_pass1/2    REBO bond order (compute_rebo AB:275-421), see
+//                     fmd_rebo_passes.hpp
+//   k_at_lj           intermolecular LJ (compute_lj AB:424-473): one
+//                     256-thread workgroup per molecule, its 4 wavefronts
+//                     split the molecule list, lane = atom, neighbour atoms
+//                     wavefront-uniform through scalar loads; the half-list
+//                     truncation becomes a per-pair predicate on jmax
+//   k_at_kick2        second half-kick + KE partials (AB:570-581)
+//   k_at_finish       reductions + NPT second half + thermo (AB:581-589, 896)
+// No floating-point atomics: all sums have a fixed order.
+#include <climits>
+#include <cstddef>
+
+#include "fmd_atomic.hpp"
+
+namespace fmd {
+
+namespace {
+
+constexpr int kXcd = 8;
+
+__device__ __forceinline__ double clampd(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// |mimg(pos_j - pos_i)|^2 in FP64 with the reference's operation order and
+// no contraction (AB:144-147, 257-259), so list decisions at the cutoff agree
+// bit for bit with the reference builder.
+__device__ __noinline__ double r2_ref(const double *__restrict__ pos, int i, int j,
+                                      const double *__restrict__ h,
+                                      const double *__restrict__ hi) {
+#pragma clang fp contract(off)
+  double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+         dz = pos[3 * j + 2] - pos[3 * i + 2];
+  double s0 = hi[0] * dx + hi[1] * dy + hi[2] * dz, s1 = hi[3] * dx + hi[4] * dy + hi[5] * dz,
+         s2 = hi[6] * dx + hi[7] * dy + hi[8] * dz;
+  s0 -= round(s0);
+  s1 -= round(s1);
+  s2 -= round(s2);
+  dx = h[0] * s0 + h[1] * s1 + h[2] * s2;
+  dy = h[3] * s0 + h[4] * s1 + h[5] * s2;
+  dz = h[6] * s0 + h[7] * s1 + h[8] * s2;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Block sum of K doubles in a fixed order (256 threads).
+template <int K>
+__device__ __forceinline__ void block_sum(const double *v, double *out) {
+  __shared__ double red[4][K];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < K; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
+  __syncthreads();
+}
+
+// ----------------------------------------------------------------- pbc
+__global__ __launch_bounds__(256) void k_at_apply_pbc(double *__restrict__ pos,
+                                                      const double *__restrict__ hh, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double *h = hh, *hi = hh + 9;
+  double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  double s0 = hi[0] * px + hi[1] * py + hi[2] * pz;
+  double s1 = hi[3] * px + hi[4] * py + hi[5] * pz;
+  double s2 = hi[6] * px + hi[7] * py + hi[8] * pz;
+  s0 -= floor(s0);
+  s1 -= floor(s1);
+  s2 -= floor(s2);
+  pos[3 * i] = h[0] * s0 + h[1] * s1 + h[2] * s2;
+  pos[3 * i + 1] = h[3] * s0 + h[4] * s1 + h[5] * s2;
+  pos[3 * i + 2] = h[6] * s0 + h[7] * s1 + h[8] * s2;
+}
+
+// ------------------------------------------------------------- centres
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
+                                                    const double *__restrict__ hh) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= p.Nm) return;
+  const double *h = hh, *hi = hh + 9;
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
+  const double rx = d.pos[3 * s0], ry = d.pos[3 * s0 + 1], rz = d.pos[3 * s0 + 2];
+  double da[NS][3];
+  double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    da[s][0] = da[s][1] = da[s][2] = 0;
+    if (a < n) {
+      double dx = d.pos[3 * (s0 + a)] - rx, dy = d.pos[3 * (s0 + a) + 1] - ry,
+             dz = d.pos[3 * (s0 + a) + 2] - rz;
+      mimg(dx, dy, dz, hi, h);
+      da[s][0] = dx;
+      da[s][1] = dy;
+      da[s][2] = dz;
+      sx += dx;
+      sy += dy;
+      sz += dz;
+    }
+  }
+  const double mx = wave_sum(sx) / n, my = wave_sum(sy) / n, mz = wave_sum(sz) / n;
+  float rmax = 0.f;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < p.apad) {
+      float4 o;
+      if (a < n) {
+        o = make_float4((float)(da[s][0] - mx), (float)(da[s][1] - my), (float)(da[s][2] - mz),
+                        0.f);
+        rmax = fmaxf(rmax, sqrtf(o.x * o.x + o.y * o.y + o.z * o.z));
+      } else {
+        o = make_float4(1e6f, 1e6f, 1e6f, 0.f);
+      }
+      d.off[(size_t)m * p.apad + a] = o;
+    }
+  }
+  rmax = wave_max(rmax);
+  if (lane == 0) {
+    d.center[3 * m] = rx + mx;
+    d.center[3 * m + 1] = ry + my;
+    d.center[3 * m + 2] = rz + mz;
+    d.radius[m] = rmax;
+  }
+}
+
+// --------------------------------------------------------- molecule list
+__global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p,
+                                                     const double *__restrict__ hh) {
+  const int m = blockIdx.x, lane = threadIdx.x;
+  const double *h = hh, *hi = hh + 9;
+  const double cx = d.center[3 * m], cy = d.center[3 * m + 1], cz = d.center[3 * m + 2];
+  const double rm = d.radius[m];
+  int count = 0;
+  for (int n0 = 0; n0 < p.Nm; n0 += 64) {
+    const int q = n0 + lane;
+    bool in = false;
+    if (q < p.Nm && q != m) {
+      if (p.small_box) {
+        in = true;
+      } else {
+        double dx = d.center[3 * q] - cx, dy = d.center[3 * q + 1] - cy,
+               dz = d.center[3 * q + 2] - cz;
+        mimg(dx, dy, dz, hi, h);
+        const double rl = p.mol_rlist + rm + d.radius[q];
+        in = dx * dx + dy * dy + dz * dz < rl * rl;
+      }
+    }
+    unsigned long long bits = __ballot(in);
+    int rank = __popcll(bits & ((1ull << lane) - 1ull));
+    if (in && count + rank < p.mol_cap) d.mol_nl[(size_t)m * p.mol_cap + count + rank] = q;
+    count += __popcll(bits);
+  }
+  if (lane == 0) {
+    d.mol_nl_count[m] = count < p.mol_cap ? count : p.mol_cap;
+    if (count > p.mol_cap) atomicAdd(&d.flags[1], 1);
+  }
+}
+
+// -------------------------------------------------------------- REBO list
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_rebo_nlist(AtomicDev d, AtomicParams p,
+                                                       const double *__restrict__ hh) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= p.Nm) return;
+  const double *h = hh, *hi = hh + 9;
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
+  int cnt[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) cnt[s] = 0;
+  for (int b = 0; b < n; b++) {
+    const int j = s0 + b;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int a = lane + 64 * s;
+      const int i = s0 + a;
+      if (a < n && j != i && r2_ref(d.pos, i, j, h, hi) < p.rebo_rlist2) {
+        if (cnt[s] < p.maxr) d.nll_r[(size_t)i * p.maxr + cnt[s]] = j;
+        cnt[s]++;
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < n) {
+      d.nlc_r[s0 + a] = cnt[s] < p.maxr ? cnt[s] : p.maxr;
+      if (cnt[s] > p.maxr) atomicAdd(&d.flags[0], 1);
+    }
+  }
+}
+
+// ------------------------------------------------- LJ half-list walk
+// WRITE = 0: truncation point jmax per atom (tile layout) and jmax_min;
+// WRITE = 1: the reference-layout half list itself (nlc_l / nll_l).
+template <int NS, int WRITE>
+__global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams p,
+                                                      const double *__restrict__ hh,
+                                                      int *__restrict__ nlc_l,
+                                                      int *__restrict__ nll_l, int cap) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= p.Nm) return;
+  const double *h = hh, *hi = hh + 9;
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
+  int cnt[NS], jm[NS];
+  bool done[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    cnt[s] = 0;
+    jm[s] = INT_MAX;
+    done[s] = lane + 64 * s >= n;
+  }
+  const int nq = d.mol_nl_count[m];
+  for (int k = 0; k < nq; k++) {
+    const int q = uniform(d.mol_nl[(size_t)m * p.mol_cap + k]);
+    if (q < m) continue;  // molecule ranges ascend: its atoms are all < i
+    const int t0 = d.mol_start[q], tn = d.mol_start[q + 1] - t0;
+    for (int b = 0; b < tn; b++) {
+      const int j = t0 + b;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        if (done[s]) continue;
+        const int i = s0 + lane + 64 * s;
+        if (r2_ref(d.pos, i, j, h, hi) < p.lj_rlist2) {
+          if (WRITE) nll_l[(size_t)i * cap + cnt[s]] = j;
+          cnt[s]++;
+          if (cnt[s] == cap) {
+            jm[s] = j;
+            done[s] = true;
+          }
+        }
+      }
+    }
+    bool all = true;
+#pragma unroll
+    for (int s = 0; s < NS; s++) all = all && done[s];
+    if (__all(all)) break;
+  }
+  int mn = INT_MAX;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < n) {
+      if (WRITE)
+        nlc_l[s0 + a] = cnt[s];
+      else
+        d.jmax[(size_t)m * p.apad + a] = jm[s];
+      mn = min(mn, jm[s]);
+    } else if (!WRITE && a < p.apad) {
+      d.jmax[(size_t)m * p.apad + a] = INT_MAX;
+    }
+  }
+  mn = wave_min(mn);
+  if (!WRITE && lane == 0) d.jmax_min[m] = mn;
+}
+
+// jmax from a reference-layout half list (the drop-in path).
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_jmax_from_list(AtomicDev d, AtomicParams p,
+                                                           const int *__restrict__ nlc_l,
+                                                           const int *__restrict__ nll_l,
+                                                           int cap) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= p.Nm) return;
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
+  int mn = INT_MAX;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < p.apad) {
+      int jm = INT_MAX;
+      if (a < n) {
+        const int i = s0 + a;
+        if (nlc_l[i] >= cap) jm = nll_l[(size_t)i * cap + cap - 1];
+      }
+      d.jmax[(size_t)m * p.apad + a] = jm;
+      mn = min(mn, jm);
+    }
+  }
+  mn = wave_min(mn);
+  if (lane == 0) d.jmax_min[m] = mn;
+}
+
+__global__ __launch_bounds__(256) void k_at_fill_int(int *__restrict__ a, int v, size_t n) {
+  size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ------------------------------------------------------- NPT first half
+struct AtHalf1 {
+  double hi[9], h1[9], hi1[9], Vg1[9];
+  double xi1, sc_nh, sc_v;
+};
+
+// step_npt AB:522-555 with the phase controls of AB:875-879.
+__device__ __forceinline__ void at_half1(const AtomicState &S, const StepCtrl &c,
+                                         const AtomicParams &p, AtHalf1 &o) {
+  const double hdt = 0.5 * p.dt;
+  double xi = S.xi, Vg[9];
+  for (int k = 0; k < 9; k++) Vg[k] = S.Vg[k];
+  if (c.flags & kCtrlZeroXiVg) {
+    xi = 0;
+    for (int k = 0; k < 9; k++) Vg[k] = 0;
+  }
+  if (c.flags & kCtrlZeroVg)
+    for (int k = 0; k < 9; k++) Vg[k] = 0;
+  inv9(S.h, o.hi);
+  const double V = fabs(det9(S.h));
+  xi += hdt * (2 * S.KE - S.Nf * kB * c.Tt) / S.Q;
+  xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
+  const double dP = (2 * S.KE + S.W9[0] + S.W9[4] + S.W9[8]) / (3 * V) * kEv2GPa - S.Pe;
+  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V * dP / (S.W * kEv2GPa);
+  for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
+  const double eps = Vg[0] * o.hi[0] + Vg[4] * o.hi[4] + Vg[8] * o.hi[8];
+  o.sc_nh = exp(-hdt * xi);
+  o.sc_v = o.sc_nh * exp(-hdt * eps / 3.0);
+  for (int k = 0; k < 9; k++) {
+    o.h1[k] = S.h[k] + p.dt * Vg[k];
+    o.Vg1[k] = Vg[k];
+  }
+  inv9(o.h1, o.hi1);
+  o.xi1 = xi;
+}
+
+__global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams p,
+                                                       int ctrl_slot) {
+  const AtomicState &S = *d.state;
+  const StepCtrl c = d.ctrl[ctrl_slot];
+  AtHalf1 H;
+  at_half1(S, c, p, H);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    AtomicMid &M = *d.mid;
+    for (int k = 0; k < 9; k++) {
+      M.h[k] = H.h1[k];
+      M.hi[k] = H.hi1[k];
+      M.Vg[k] = H.Vg1[k];
+    }
+    M.xi = H.xi1;
+    M.sc_nh = H.sc_nh;
+  }
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.Na) return;
+  const double hdt = 0.5 * p.dt, mi = kConv / d.mass[i];
+  double v[3], r[3];
+  for (int a = 0; a < 3; a++) {
+    v[a] = d.vel[3 * i + a] * H.sc_v + hdt * d.F[3 * i + a] * mi;
+    r[a] = d.pos[3 * i + a];
+  }
+  const double *hi = H.hi;
+  double sx = hi[0] * r[0] + hi[1] * r[1] + hi[2] * r[2];
+  double sy = hi[3] * r[0] + hi[4] * r[1] + hi[5] * r[2];
+  double sz = hi[6] * r[0] + hi[7] * r[1] + hi[8] * r[2];
+  sx += p.dt * (hi[0] * v[0] + hi[1] * v[1] + hi[2] * v[2]);
+  sy += p.dt * (hi[3] * v[0] + hi[4] * v[1] + hi[5] * v[2]);
+  sz += p.dt * (hi[6] * v[0] + hi[7] * v[1] + hi[8] * v[2]);
+  sx -= floor(sx);
+  sy -= floor(sy);
+  sz -= floor(sz);
+  const double *h = H.h1;
+  d.pos[3 * i] = h[0] * sx + h[1] * sy + h[2] * sz;
+  d.pos[3 * i + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
+  d.pos[3 * i + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
+  for (int a = 0; a < 3; a++) d.vel[3 * i + a] = v[a];
+}
+
+// ------------------------------------------------------------------ REBO
+__global__ __launch_bounds__(256) void k_rebo_pass1(AtomicDev d, AtomicParams p,
+                                                    const double *__restrict__ hh) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= p.Na * p.maxr) return;
+  const int i = t / p.maxr, jn = t - i * p.maxr;
+  if (jn >= d.nlc_r[i]) return;
+  rebo_pass1<float>(i, jn, d.pos, hh, hh + 9, d.nlc_r, d.nll_r, p.maxr, d.bonds);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rebo_pass2(AtomicDev d, AtomicParams p) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  double part[kAtPart];
+  for (int k = 0; k < kAtPart; k++) part[k] = 0;
+  if (x < p.Na) {
+    double o[13];
+    rebo_pass2<float, MODE>(x, d.nlc_r, d.nll_r, p.maxr, d.bonds, o);
+    for (int k = 0; k < 3; k++) d.Frebo[3 * x + k] = o[k];
+    for (int k = 0; k < 9; k++) part[k] = o[4 + k];
+    part[9] = o[3];
+  }
+  block_sum<kAtPart>(part, d.rebo_part + (size_t)blockIdx.x * kAtPart);
+}
+
+// -------------------------------------------------------------------- LJ
+// One group of four neighbour atoms against the lane's atom(s).
+// PRED: the reference half-list truncation (pair kept iff j <= jmax(i) for
+// the lower index i).  PERPAIR: per-pair minimum image (cells narrower than
+// twice the list reach).
+template <int NS, bool PRED, bool PERPAIR>
+__device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, int jb0,
+                                         const float *cx, const float *cy, const float *cz,
+                                         const int *ia, const int *ia_jmax, bool j_above,
+                                         const LJ32 &lj, const float *hf, const float *hif,
+                                         const double *pos, const double *h, const double *hi,
+                                         float *gx, float *gy, float *gz, float *e,
+                                         float *wd) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      float ex = rb[u].x + cx[s], ey = rb[u].y + cy[s], ez = rb[u].z + cz[s];
+      if (PERPAIR) {
+        float s0 = hif[0] * ex + hif[1] * ey + hif[2] * ez;
+        float s1 = hif[3] * ex + hif[4] * ey + hif[5] * ez;
+        float s2 = hif[6] * ex + hif[7] * ey + hif[8] * ez;
+        s0 -= rintf(s0);
+        s1 -= rintf(s1);
+        s2 -= rintf(s2);
+        ex = hf[0] * s0 + hf[1] * s1 + hf[2] * s2;
+        ey = hf[3] * s0 + hf[4] * s1 + hf[5] * s2;
+        ez = hf[6] * s0 + hf[7] * s1 + hf[8] * s2;
+      }
+      const float r2 = ex * ex + ey * ey + ez * ez;
+      bool in = r2 <= lj.rc2_hi;
+      // within FP32 rounding of the cutoff: the reference's own FP64 test
+      if (in && r2 > lj.rc2_lo) in = r2_ref(pos, ia[s], jb0 + u, h, hi) <= lj.rc2d;
+      if (PRED) in = in && (j_above ? (jb0 + u <= ia_jmax[s]) : (ia[s] <= jb_jmax[u]));
+      if (in) {
+        const float fm = lj_fm(r2, lj, e[s]);
+        gx[s] += fm * ex;
+        gy[s] += fm * ey;
+        gz[s] += fm * ez;
+        if (PERPAIR) {
+          wd[6 * s + 0] += fm * ex * ex;
+          wd[6 * s + 1] += fm * ex * ey;
+          wd[6 * s + 2] += fm * ex * ez;
+          wd[6 * s + 3] += fm * ey * ey;
+          wd[6 * s + 4] += fm * ey * ez;
+          wd[6 * s + 5] += fm * ez * ez;
+        }
+      }
+    }
+  }
+}
+
+// Per lane (atom a of molecule i, offset r_a) and neighbour molecule j
+// (minimum-image centre vector dm): G_aj = sum_b f_ab, FP32 over the <= apad
+// atoms b, promoted to FP64.  Force on a: -48 eps sum_j G_aj (plus its REBO
+// force); virial 1/2 sum_j dm (x) G_aj - r_a (x) sum_j G_aj (the
+// reference's sum over pairs of d (x) f, AB:465-472, regrouped through the
+// symmetric pair set); energy 1/2 per ordered pair.
+template <int NS, bool PERPAIR>
+__global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p,
+                                               const double *__restrict__ hh) {
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int bid = blockIdx.x;
+  const int mi = (bid % kXcd) * per + bid / kXcd;
+  if (mi >= p.Nm) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  const double *h = hh, *hi = hh + 9;
+  float hf[9], hif[9];
+  for (int k = 0; k < 9; k++) {
+    hf[k] = (float)h[k];
+    hif[k] = (float)hi[k];
+  }
+  const int apad = p.apad;
+  const int s0 = d.mol_start[mi], n = d.mol_start[mi + 1] - s0;
+  const double cix = d.center[3 * mi], ciy = d.center[3 * mi + 1], ciz = d.center[3 * mi + 2];
+  float rax[NS], ray[NS], raz[NS];
+  int ia[NS], iam[NS];  // atom index, its truncation point
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < n) {
+      const float4 r = d.off[(size_t)mi * apad + a];
+      rax[s] = r.x;
+      ray[s] = r.y;
+      raz[s] = r.z;
+      iam[s] = d.jmax[(size_t)mi * apad + a];
+    } else {
+      rax[s] = ray[s] = raz[s] = -3.0e5f;
+      iam[s] = INT_MAX;
+    }
+    ia[s] = a < n ? s0 + a : s0;  // idle lanes never pass the cutoff test
+  }
+  double GA[NS][3], WA[NS][9];
+  float EA[NS], WD[NS][6];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    for (int k = 0; k < 3; k++) GA[s][k] = 0;
+    for (int k = 0; k < 9; k++) WA[s][k] = 0;
+    for (int k = 0; k < 6; k++) WD[s][k] = 0;
+    EA[s] = 0;
+  }
+  const LJ32 lj = p.lj;
+  const int i_last = s0 + n - 1;
+  const int jmin_i = d.jmax_min[mi];
+  const int nq = d.mol_nl_count[mi];
+  const int *lq = d.mol_nl + (size_t)mi * p.mol_cap;
+  for (int k = wv; k < nq; k += 4) {
+    const int mj = uniform(lq[k]);
+    double dx = d.center[3 * mj] - cix, dy = d.center[3 * mj + 1] - ciy,
+           dz = d.center[3 * mj + 2] - ciz;
+    mimg(dx, dy, dz, hi, h);
+    const float rcut = sqrtf(lj.rc2) + d.radius[mj] + 0.05f;
+    float cx[NS], cy[NS], cz[NS];
+    bool act = PERPAIR;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      // dm - r_a in FP64, rounded once per lane (no shared rounding of dm)
+      cx[s] = (float)(dx - (double)rax[s]);
+      cy[s] = (float)(dy - (double)ray[s]);
+      cz[s] = (float)(dz - (double)raz[s]);
+      act |= cx[s] * cx[s] + cy[s] * cy[s] + cz[s] * cz[s] < rcut * rcut;
+    }
+    if (!__any(act)) continue;
+    const int t0 = d.mol_start[mj];
+    const int j_last = d.mol_start[mj + 1] - 1;
+    const bool j_above = mj > mi;
+    // truncation can only bite where the lower-index side has a finite jmax
+    const bool pred = p.lj_compat && (j_above ? (j_last > jmin_i) : (i_last > d.jmax_min[mj]));
+    float gx[NS], gy[NS], gz[NS], e[NS], wd[NS * 6];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      gx[s] = gy[s] = gz[s] = e[s] = 0.f;
+      for (int q = 0; q < 6; q++) wd[6 * s + q] = 0.f;
+    }
+    const float4 *sj = d.off + (size_t)mj * apad;
+    const int *jj = d.jmax + (size_t)mj * apad;
+    for (int b = 0; b < apad; b += 4) {
+      float4 rb[4];
+      int jm[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) rb[u] = sj[b + u];
+      if (pred) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) jm[u] = jj[b + u];
+        lj_group<NS, true, PERPAIR>(rb, jm, t0 + b, cx, cy, cz, ia, iam, j_above, lj, hf, hif,
+                                    d.pos, h, hi, gx, gy, gz, e, wd);
+      } else {
+        lj_group<NS, false, PERPAIR>(rb, jm, t0 + b, cx, cy, cz, ia, iam, j_above, lj, hf, hif,
+                                     d.pos, h, hi, gx, gy, gz, e, wd);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const double g0 = gx[s], g1 = gy[s], g2 = gz[s];
+      GA[s][0] += g0;
+      GA[s][1] += g1;
+      GA[s][2] += g2;
+      if (!PERPAIR) {
+        WA[s][0] += dx * g0;
+        WA[s][1] += dx * g1;
+        WA[s][2] += dx * g2;
+        WA[s][3] += dy * g0;
+        WA[s][4] += dy * g1;
+        WA[s][5] += dy * g2;
+        WA[s][6] += dz * g0;
+        WA[s][7] += dz * g1;
+        WA[s][8] += dz * g2;
+      }
+      for (int q = 0; q < 6; q++) WD[s][q] += wd[6 * s + q];
+      EA[s] += e[s];
+    }
+  }
+  __shared__ double fsum[4][2 * 64][3];
+  double acc[10];
+  for (int k = 0; k < 10; k++) acc[k] = 0;
+  const double c48 = 48.0 * p.eps;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    for (int k = 0; k < 3; k++) fsum[wv][lane + 64 * s][k] = -c48 * GA[s][k];
+    if (PERPAIR) {
+      // sum over ordered pairs of 1/2 fm d d^T (symmetric)
+      const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+      for (int q = 0; q < 9; q++) acc[q] += 0.5 * c48 * (double)WD[s][map[q]];
+    } else {
+      const double ra[3] = {rax[s], ray[s], raz[s]};
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++)
+          acc[3 * x + y] += c48 * (0.5 * WA[s][3 * x + y] - ra[x] * GA[s][y]);
+    }
+    acc[9] += (double)EA[s];
+  }
+  __shared__ double red[4][10];
+  for (int k = 0; k < 10; k++) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = v;
+  }
+  __syncthreads();
+  // per-atom forces: the 4 wavefronts' partial sums in a fixed order
+  for (int a = threadIdx.x; a < n; a += 256) {
+    const int x = s0 + a;
+    for (int k = 0; k < 3; k++) {
+      const double f = ((fsum[0][a][k] + fsum[1][a][k]) + fsum[2][a][k]) + fsum[3][a][k];
+      d.F[3 * x + k] = d.Frebo[3 * x + k] + f;
+    }
+  }
+  if (threadIdx.x < 10) {
+    const int k = threadIdx.x;
+    double v = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    if (k == 9) v *= 2.0 * p.eps;  // 1/2 * 4 eps * sum of reduced energies
+    d.lj_part[(size_t)mi * kAtPart + k] = v;
+  }
+}
+
+// ------------------------------------------------------- kick 2 + partials
+__global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, int kick) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v2[1] = {0};
+  if (i < p.Na) {
+    double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
+    if (kick) {
+      const AtomicMid &M = *d.mid;
+      const double hdt = 0.5 * p.dt, mi = kConv / d.mass[i];
+      const double eps2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
+      const double sc_v2 = M.sc_nh * exp(-hdt * eps2 / 3.0);
+      vx = (vx + hdt * d.F[3 * i] * mi) * sc_v2;
+      vy = (vy + hdt * d.F[3 * i + 1] * mi) * sc_v2;
+      vz = (vz + hdt * d.F[3 * i + 2] * mi) * sc_v2;
+      d.vel[3 * i] = vx;
+      d.vel[3 * i + 1] = vy;
+      d.vel[3 * i + 2] = vz;
+    }
+    v2[0] = d.mass[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  block_sum<1>(v2, d.ke_part + blockIdx.x);
+}
+
+// Fixed-order sums of the three partial arrays: s[0] = sum m v^2,
+// s[1..9] = W9, s[10] = E_rebo, s[11] = E_lj.
+__device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams &p,
+                                          double *sums) {
+  double v[12];
+  for (int k = 0; k < 12; k++) v[k] = 0;
+  for (int b = threadIdx.x; b < d.nblk; b += 256) {
+    v[0] += d.ke_part[b];
+    for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[(size_t)b * kAtPart + k];
+    v[10] += d.rebo_part[(size_t)b * kAtPart + 9];
+  }
+  for (int m = threadIdx.x; m < p.Nm; m += 256) {
+    for (int k = 0; k < 9; k++) v[1 + k] += d.lj_part[(size_t)m * kAtPart + k];
+    v[11] += d.lj_part[(size_t)m * kAtPart + 9];
+  }
+  __shared__ double tot[12];
+  block_sum<12>(v, tot);
+  for (int k = 0; k < 12; k++) sums[k] = tot[k];
+}
+
+__global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p, int what) {
+  double s[12];
+  at_reduce(d, p, s);
+  if (threadIdx.x != 0) return;
+  AtomicState &S = *d.state;
+  if (what & kAtObsKE) S.KE = 0.5 * s[0] / kConv;
+  if (what & kAtObsForce) {
+    for (int k = 0; k < 9; k++) S.W9[k] = s[1 + k];
+    S.Ep_rebo = s[10];
+    S.Ep_lj = s[11];
+    S.Ep = s[10] + s[11];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
+                                                   int thermo_slot) {
+  double s[12];
+  at_reduce(d, p, s);
+  if (threadIdx.x != 0) return;
+  AtomicState &S = *d.state;
+  const AtomicMid &M = *d.mid;
+  const StepCtrl c = d.ctrl[ctrl_slot];
+  const double hdt = 0.5 * p.dt;
+  double KE = 0.5 * s[0] / kConv;
+  double W9[9];
+  for (int k = 0; k < 9; k++) W9[k] = s[1 + k];
+  const double Er = s[10], El = s[11];
+  double xi = M.xi, Vg[9];
+  for (int k = 0; k < 9; k++) Vg[k] = M.Vg[k];
+  xi += hdt * (2 * KE - S.Nf * kB * c.Tt) / S.Q;
+  xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
+  const double V2 = fabs(det9(M.h));
+  const double P = (2 * KE + W9[0] + W9[4] + W9[8]) / (3 * V2) * kEv2GPa;
+  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * (P - S.Pe) / (S.W * kEv2GPa);
+  for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
+  double T = 2 * KE / (S.Nf * kB);
+  double scale = 1.0;
+  if ((c.flags & kCtrlRescale) && T > 0.1) {  // AB:898-908
+    const double tgt = c.rescale_T > 0.1 ? c.rescale_T : 0.1;
+    scale = sqrt(tgt / T);
+    KE *= scale * scale;
+    T = 2 * KE / (S.Nf * kB);
+    xi = 0;
+    if (c.flags & kCtrlRescaleZeroVg)
+      for (int k = 0; k < 9; k++) Vg[k] = 0;
+  }
+  for (int k = 0; k < 9; k++) {
+    S.h[k] = M.h[k];
+    S.hi[k] = M.hi[k];
+    S.Vg[k] = Vg[k];
+    S.W9[k] = W9[k];
+  }
+  S.xi = xi;
+  S.Tt = c.Tt;
+  S.KE = KE;
+  S.Ep = Er + El;
+  S.Ep_rebo = Er;
+  S.Ep_lj = El;
+  S.scale = scale;
+  S.step += 1;
+  Thermo &t = d.thermo[thermo_slot];
+  t.Ep = Er + El;
+  t.KE = KE;
+  t.KEt = KE;
+  t.KEr = 0;
+  t.T = T;
+  t.P = P;
+  t.V = V2;
+  t.xi = xi;
+  t.Ep_rebo = Er;
+  t.Ep_lj = El;
+  for (int k = 0; k < 9; k++) t.h[k] = M.h[k];
+}
+
+__global__ __launch_bounds__(256) void k_at_rescale(AtomicDev d, AtomicParams p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.Na) return;
+  const double s = d.state->scale;
+  for (int a = 0; a < 3; a++) d.vel[3 * i + a] *= s;
+}
+
+inline int ns_of(const AtomicParams &p) { return p.apad <= 64 ? 1 : 2; }
+inline const double *mid_hh(const AtomicDev &d) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.mid) +
+                                          offsetof(AtomicMid, h));
+}
+inline const double *state_hh(const AtomicDev &d) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.state) +
+                                          offsetof(AtomicState, h));
+}
+
+}  // namespace
+
+hipError_t at_launch_apply_pbc(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                               hipStream_t s) {
+  k_at_apply_pbc<<<(p.Na + 255) / 256, 256, 0, s>>>(d.pos, hh, p.Na);
+  return hipGetLastError();
+}
+
+hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                             hipStream_t s) {
+  dim3 g((p.Nm + 3) / 4);
+  if (ns_of(p) == 1)
+    k_at_centers<1><<<g, 256, 0, s>>>(d, p, hh);
+  else
+    k_at_centers<2><<<g, 256, 0, s>>>(d, p, hh);
+  return hipGetLastError();
+}
+
+hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                           int rebo_list, hipStream_t s) {
+  k_at_mol_nlist<<<p.Nm, 64, 0, s>>>(d, p, hh);
+  dim3 g((p.Nm + 3) / 4);
+  const bool one = ns_of(p) == 1;
+  if (rebo_list) {
+    if (one)
+      k_at_rebo_nlist<1><<<g, 256, 0, s>>>(d, p, hh);
+    else
+      k_at_rebo_nlist<2><<<g, 256, 0, s>>>(d, p, hh);
+  }
+  if (p.lj_compat) {
+    if (one)
+      k_at_scan_half<1, 0><<<g, 256, 0, s>>>(d, p, hh, nullptr, nullptr, p.lj_cap);
+    else
+      k_at_scan_half<2, 0><<<g, 256, 0, s>>>(d, p, hh, nullptr, nullptr, p.lj_cap);
+  } else {
+    const size_t n = (size_t)p.Nm * p.apad;
+    k_at_fill_int<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d.jmax, INT_MAX, n);
+    k_at_fill_int<<<(p.Nm + 255) / 256, 256, 0, s>>>(d.jmax_min, INT_MAX, p.Nm);
+  }
+  return hipGetLastError();
+}
+
+hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
+                                    const int *nlc_l, const int *nll_l, int cap,
+                                    hipStream_t s) {
+  dim3 g((p.Nm + 3) / 4);
+  if (ns_of(p) == 1)
+    k_at_jmax_from_list<1><<<g, 256, 0, s>>>(d, p, nlc_l, nll_l, cap);
+  else
+    k_at_jmax_from_list<2><<<g, 256, 0, s>>>(d, p, nlc_l, nll_l, cap);
+  return hipGetLastError();
+}
+
+hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                              int *nlc_l, int *nll_l, int cap, hipStream_t s) {
+  dim3 g((p.Nm + 3) / 4);
+  if (ns_of(p) == 1)
+    k_at_scan_half<1, 1><<<g, 256, 0, s>>>(d, p, hh, nlc_l, nll_l, cap);
+  else
+    k_at_scan_half<2, 1><<<g, 256, 0, s>>>(d, p, hh, nlc_l, nll_l, cap);
+  return hipGetLastError();
+}
+
+hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                            hipStream_t s) {
+  k_rebo_pass1<<<(p.Na * p.maxr + 255) / 256, 256, 0, s>>>(d, p, hh);
+  if (p.rebo_mode == kReboPhysical)
+    k_rebo_pass2<kReboPhysical><<<d.nblk, 256, 0, s>>>(d, p);
+  else
+    k_rebo_pass2<kReboCompat><<<d.nblk, 256, 0, s>>>(d, p);
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  dim3 g(per * kXcd);
+  const bool one = ns_of(p) == 1;
+  if (p.small_box) {
+    if (one)
+      k_at_lj<1, true><<<g, 256, 0, s>>>(d, p, hh);
+    else
+      k_at_lj<2, true><<<g, 256, 0, s>>>(d, p, hh);
+  } else {
+    if (one)
+      k_at_lj<1, false><<<g, 256, 0, s>>>(d, p, hh);
+    else
+      k_at_lj<2, false><<<g, 256, 0, s>>>(d, p, hh);
+  }
+  return hipGetLastError();
+}
+
+hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
+                             hipStream_t s) {
+  k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0);
+  k_at_observe<<<1, 256, 0, s>>>(d, p, what);
+  return hipGetLastError();
+}
+
+hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
+                          int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
+                          hipEvent_t ev1, hipStream_t s) {
+  hipError_t e;
+  if (rebuild) {
+    // AB:881-887: both lists from the current positions and cell
+    if ((e = at_launch_centers(d, p, state_hh(d), s)) != hipSuccess) return e;
+    if ((e = at_launch_lists(d, p, state_hh(d), 1, s)) != hipSuccess) return e;
+  }
+  k_at_kick_drift<<<d.nblk, 256, 0, s>>>(d, p, ctrl_slot);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = at_launch_centers(d, p, mid_hh(d), s)) != hipSuccess) return e;
+  if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
+  if ((e = at_launch_forces(d, p, mid_hh(d), s)) != hipSuccess) return e;
+  if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
+  k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
+  k_at_finish<<<1, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+  if (rescale) k_at_rescale<<<d.nblk, 256, 0, s>>>(d, p);
+  return hipGetLastError();
+}
+
+}  // namespace fmd

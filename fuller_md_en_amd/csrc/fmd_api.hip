@@ -38,6 +38,9 @@ LJ32 make_lj32(const fmd_lj_params &lj) {
   LJ32 p;
   p.sig2 = (float)(lj.sigma * lj.sigma);
   p.rc2 = (float)(lj.rcut * lj.rcut);
+  p.rc2d = lj.rcut * lj.rcut;
+  p.rc2_lo = (float)(p.rc2d * (1.0 - 1e-5));
+  p.rc2_hi = (float)(p.rc2d * (1.0 + 1e-5));
   p.c48 = (float)(48.0 * lj.eps);
   p.c24 = (float)(24.0 * lj.eps);
   p.vs4 = (float)(lj.vshift / (4.0 * lj.eps));
@@ -118,7 +121,7 @@ void fmd_rigid_config_default(fmd_rigid_config *c, double rmol) {
 namespace fmd {
 
 static void free_rigid(RigidDev &d) {
-  void *ptrs[] = {d.pos,      d.vel,      d.omg,   d.qv,    d.Fv,   d.Tv,     d.sites,
+  void *ptrs[] = {d.pos,      d.vel,      d.omg,   d.qv,    d.Fv,   d.Tv,     d.sites, d.sites_d,
                   d.body,     d.hh,       d.nl_count, d.nl_list, d.flags, d.molpart,
                   d.blockpart, d.state,   d.mid,   d.ctrl,  d.thermo};
   for (void *p : ptrs)
@@ -145,6 +148,7 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   AL(d.Fv, 3 * N);
   AL(d.Tv, 3 * N);
   AL(d.sites, N * p.natom_pad);
+  AL(d.sites_d, 3 * N * p.natom_pad);
   AL(d.body, 4 * (size_t)p.natom_pad);
   AL(d.hh, 18);
   AL(d.nl_count, N);

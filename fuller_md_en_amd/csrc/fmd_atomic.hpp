@@ -1,10 +1,118 @@
-// fmd_atomic.hpp -- device data layout of the atomic (AIREBO) engine.
+// fmd_atomic.hpp -- device data layout and launchers of the atomic (AIREBO)
+// engine: REBO bond-order forces within molecules plus shifted LJ between
+// molecules (reference src/fuller_airebo_npt_md_serial_omp_acc.cpp, "AB").
+//
+// HBM layout:
+//   pos, vel, F, Frebo     double[Na*3]   reference AoS order
+//   mass                   double[Na]
+//   mol_start              int[Nm+1]      atoms of molecule m are the contiguous
+//                                         range [mol_start[m], mol_start[m+1])
+//   center                 double[Nm*3]   molecule centre (mean of the
+//                                         unwrapped atoms), FP64
+//   radius                 float[Nm]      max |atom - centre|
+//   off                    float4[Nm*apad] atom offsets from the centre, one
+//                                         padded tile per molecule (padding
+//                                         sites at +1e6 A)
+//   jmax                   int[Nm*apad]   per atom: largest j kept in its
+//                                         400-entry half list (INT_MAX if
+//                                         the list is not truncated)
+//   jmax_min               int[Nm]
+//   mol_nl_count / mol_nl  molecule neighbour list (ascending)
+//   nlc_r / nll_r          REBO list, reference layout (stride maxr = 12)
+//   bonds                  per directed REBO entry (fmd_rebo_passes.hpp)
+//   rebo_part, lj_part, ke_part  fixed-order partial sums
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "fmd_common.hpp"
+#include "fmd_rebo_passes.hpp"
+#include "fmd_rigid.hpp"  // StepCtrl, Thermo
 
 namespace fmd {
-struct AtomicEngine;
+
+constexpr int kAtPart = 10;  // W9 + E
+
+struct AtomicState {
+  double h[9], hi[9];  // adjacent: kernels take them as one 18-double block
+  double xi, Vg[9], Q, W, Pe, Tt;
+  double KE, Ep, Ep_rebo, Ep_lj, W9[9];
+  double scale;
+  int Nf, step;
+};
+
+struct AtomicMid {
+  double h[9], hi[9];
+  double xi, Vg[9];
+  double sc_nh;
+};
+
+struct AtomicParams {
+  int Na, Nm;
+  int apad;        // atoms per molecule tile (max molecule size, rounded up to 4)
+  int maxr;        // REBO list stride (12, AB:126)
+  int mol_cap;     // molecule list stride
+  int lj_cap;      // half-list cap of the reference (400, AB:127)
+  int lj_compat;   // 1: pairs limited by the reference's truncated half list
+  int rebo_mode;   // kReboCompat / kReboPhysical
+  int small_box;   // 1: per-pair minimum image (cell narrower than the list)
+  double rebo_rlist2;  // REBO list radius^2 (2.3^2, AB:253)
+  double lj_rlist2;    // LJ half-list radius^2 ((rc + 2)^2, AB:265)
+  double mol_rlist;    // molecule list radius beyond the two molecule radii
+  double dt, clamp_xi, clamp_vg;
+  LJ32 lj;
+  double eps;
+};
+
+struct AtomicDev {
+  double *pos, *vel, *F, *Frebo, *mass;
+  int *mol_start;
+  double *center;
+  float *radius;
+  float4 *off;
+  int *jmax, *jmax_min;
+  int *mol_nl_count, *mol_nl;
+  int *nlc_r, *nll_r;
+  ReboBonds bonds;
+  double *rebo_part, *lj_part, *ke_part;
+  double *hh;  // 18 doubles for the stateless entry points
+  AtomicState *state;
+  AtomicMid *mid;
+  StepCtrl *ctrl;
+  Thermo *thermo;
+  int *flags;  // [0] REBO list overflow, [1] molecule list overflow
+  int nblk;    // ceil(Na / 256)
+};
+
+// Launchers (atomic_kernels.hip).  hh: 18 device doubles (h then h^-1).
+hipError_t at_launch_apply_pbc(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                               hipStream_t s);
+// Molecule centres, radii and offset tiles from pos.
+hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                             hipStream_t s);
+// Molecule list, REBO list and (compat) truncation points; needs centres.
+hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                           int rebo_list, hipStream_t s);
+// REBO + LJ forces into F (and the partial sums); needs centres and lists.
+hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                            hipStream_t s);
+enum : int { kAtObsKE = 1, kAtObsForce = 2 };
+hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
+                             hipStream_t s);
+hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
+                          int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
+                          hipEvent_t ev1, hipStream_t s);
+// jmax from a reference-layout half list (stride cap) for the drop-in path.
+hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
+                                    const int *nlc_l, const int *nll_l, int cap,
+                                    hipStream_t s);
+// Reference-layout half list (AB:263-272) built on the GPU.
+hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, const double *hh,
+                              int *nlc_l, int *nll_l, int cap, hipStream_t s);
+
+struct AtomicEngine {
+  AtomicParams p{};
+  AtomicDev d{};
+};
+
 }  // namespace fmd

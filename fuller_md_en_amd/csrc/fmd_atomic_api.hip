@@ -1,17 +1,634 @@
-// fmd_atomic_api.hip -- host side of the atomic (AIREBO) engine.
+// fmd_atomic_api.hip -- host side of the atomic (AIREBO) engine: creation,
+// state transfer, the stream-ordered step loop and the drop-in entry points
+// with the signatures of the reference's AIREBO functions.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
 #include "fmd_engine.hpp"
+
+using namespace fmd;
+
+namespace {
+
+const double *at_state_hh(const AtomicDev &d) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.state) +
+                                          offsetof(AtomicState, h));
+}
+
+// Molecule ranges from mol_id: every molecule's atoms must be one contiguous
+// run (the reference builds atoms molecule by molecule, AB:781-786).
+int mol_ranges(const int *mol_id, int na, std::vector<int> &start, int &apad) {
+  start.clear();
+  start.push_back(0);
+  std::vector<int> seen;
+  int maxn = 0;
+  for (int i = 1; i <= na; i++) {
+    if (i == na || mol_id[i] != mol_id[i - 1]) {
+      start.push_back(i);
+      maxn = std::max(maxn, start.back() - start[start.size() - 2]);
+    }
+  }
+  // an id reappearing after its run ended means a non-contiguous molecule
+  std::vector<int> ids;
+  for (size_t m = 0; m + 1 < start.size(); m++) ids.push_back(mol_id[start[m]]);
+  std::vector<int> sorted = ids;
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    return set_error(FMD_ERR_ARG, "atomic: the atoms of each molecule must be contiguous");
+  if (maxn > 128) return set_error(FMD_ERR_ARG, "atomic: molecules of more than 128 atoms");
+  apad = (maxn + 3) & ~3;
+  return FMD_OK;
+}
+
+// Smallest perpendicular width of the cell h (columns are the cell vectors).
+double min_width(const double *h) {
+  const double a[3] = {h[0], h[3], h[6]}, b[3] = {h[1], h[4], h[7]}, c[3] = {h[2], h[5], h[8]};
+  auto cross_norm = [](const double *u, const double *v) {
+    const double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2],
+                 z = u[0] * v[1] - u[1] * v[0];
+    return std::sqrt(x * x + y * y + z * z);
+  };
+  const double V = std::fabs(det9(h));
+  return std::min(V / cross_norm(b, c), std::min(V / cross_norm(c, a), V / cross_norm(a, b)));
+}
+
+// Largest atom distance from its molecule's centre (host, FP64).
+double max_radius(const double *pos, const std::vector<int> &start, const double *h) {
+  double hi[9];
+  inv9(h, hi);
+  double rmax = 0;
+  for (size_t m = 0; m + 1 < start.size(); m++) {
+    const int s0 = start[m], n = start[m + 1] - s0;
+    std::vector<double> d(3 * n);
+    double c[3] = {0, 0, 0};
+    for (int a = 0; a < n; a++) {
+      double dx = pos[3 * (s0 + a)] - pos[3 * s0], dy = pos[3 * (s0 + a) + 1] - pos[3 * s0 + 1],
+             dz = pos[3 * (s0 + a) + 2] - pos[3 * s0 + 2];
+      mimg(dx, dy, dz, hi, h);
+      d[3 * a] = dx;
+      d[3 * a + 1] = dy;
+      d[3 * a + 2] = dz;
+      c[0] += dx;
+      c[1] += dy;
+      c[2] += dz;
+    }
+    for (int k = 0; k < 3; k++) c[k] /= n;
+    for (int a = 0; a < n; a++) {
+      const double x = d[3 * a] - c[0], y = d[3 * a + 1] - c[1], z = d[3 * a + 2] - c[2];
+      rmax = std::max(rmax, std::sqrt(x * x + y * y + z * z));
+    }
+  }
+  return rmax;
+}
+
+void free_atomic_dev(AtomicDev &d) {
+  void *ptrs[] = {d.pos,       d.vel,       d.F,        d.Frebo,     d.mass,       d.mol_start,
+                  d.center,    d.radius,    d.off,      d.jmax,      d.jmax_min,   d.mol_nl_count,
+                  d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
+                  d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.lj_part, d.ke_part,
+                  d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  memset(&d, 0, sizeof d);
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+
+int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring) {
+  memset(&d, 0, sizeof d);
+  const size_t Na = p.Na, Nm = p.Nm, ns = Na * p.maxr, nt = Nm * p.apad;
+  d.nblk = (p.Na + 255) / 256;
+  hipError_t e = hipSuccess;
+#define AL(ptr, n) \
+  if (e == hipSuccess) e = dalloc(&(ptr), (n))
+  AL(d.pos, 3 * Na);
+  AL(d.vel, 3 * Na);
+  AL(d.F, 3 * Na);
+  AL(d.Frebo, 3 * Na);
+  AL(d.mass, Na);
+  AL(d.mol_start, Nm + 1);
+  AL(d.center, 3 * Nm);
+  AL(d.radius, Nm);
+  AL(d.off, nt);
+  AL(d.jmax, nt);
+  AL(d.jmax_min, Nm);
+  AL(d.mol_nl_count, Nm);
+  AL(d.mol_nl, Nm * p.mol_cap);
+  AL(d.nlc_r, Na);
+  AL(d.nll_r, ns);
+  AL(d.bonds.bd, 4 * ns);
+  AL(d.bonds.ba, 4 * ns);
+  AL(d.bonds.bp, 4 * ns);
+  AL(d.bonds.bq, 4 * ns);
+  AL(d.bonds.bflag, ns);
+  AL(d.bonds.brev, ns);
+  AL(d.rebo_part, (size_t)d.nblk * kAtPart);
+  AL(d.lj_part, Nm * kAtPart);
+  AL(d.ke_part, d.nblk);
+  AL(d.hh, 18);
+  AL(d.state, 1);
+  AL(d.mid, 1);
+  AL(d.ctrl, ring);
+  AL(d.thermo, ring);
+  AL(d.flags, 4);
+#undef AL
+  if (e != hipSuccess) {
+    free_atomic_dev(d);
+    return hip_fail(e, "hipMalloc (atomic engine)");
+  }
+  (void)hipMemset(d.flags, 0, 4 * sizeof(int));
+  (void)hipMemset(d.F, 0, 3 * Na * sizeof(double));
+  (void)hipMemset(d.state, 0, sizeof(AtomicState));
+  (void)hipMemset(d.nlc_r, 0, Na * sizeof(int));
+  (void)hipMemset(d.mol_nl_count, 0, Nm * sizeof(int));
+  return FMD_OK;
+}
+
+void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Nm, int apad) {
+  memset(&p, 0, sizeof p);
+  p.Na = c.n_atoms;
+  p.Nm = Nm;
+  p.apad = apad;
+  p.maxr = c.rebo_max_neigh > 0 ? c.rebo_max_neigh : 12;
+  p.mol_cap = 160;
+  p.lj_cap = c.lj_max_neigh;
+  p.lj_compat = c.lj_max_neigh > 0;
+  p.rebo_mode = c.rebo_mode;
+  p.rebo_rlist2 = c.rebo_rlist * c.rebo_rlist;
+  const double rl = c.lj.rcut + c.lj_skin;
+  p.lj_rlist2 = rl * rl;  // (LJ_RCUT + 2.0)^2, AB:265
+  p.mol_rlist = rl + 1.0;  // + 1 A for centre drift between rebuilds
+  p.dt = c.dt;
+  p.clamp_xi = c.clamp_xi;
+  p.clamp_vg = c.clamp_vg;
+  p.lj = make_lj32(c.lj);
+  p.eps = c.lj.eps;
+}
+
+// The molecule-tile LJ formulation (one centre image per molecule pair)
+// equals the per-atom minimum image whenever rc + 2 R_max fits in half the
+// narrowest cell width; otherwise every pair takes its own image.
+int decide_small_box(const AtomicParams &p, const double *h, double rmax) {
+  const double rc = std::sqrt((double)p.lj.rc2d);
+  return (rc + 2.0 * rmax + 1.0) >= 0.5 * min_width(h) ? 1 : 0;
+}
+
+}  // namespace
 
 namespace fmd {
 
-int engine_prepare_atomic(fmd_engine *) {
-  return set_error(FMD_ERR_STATE, "atomic engine: not available");
+int engine_prepare_atomic(fmd_engine *e) {
+  AtomicEngine &A = *e->at;
+  const AtomicDev &d = A.d;
+  const AtomicParams &p = A.p;
+  hipStream_t s = e->stream;
+  const double *hh = at_state_hh(d);
+  HIPCHK(at_launch_apply_pbc(d, p, hh, s));  // AB:852
+  HIPCHK(at_launch_centers(d, p, hh, s));
+  HIPCHK(at_launch_lists(d, p, hh, 1, s));
+  HIPCHK(at_launch_forces(d, p, hh, s));
+  HIPCHK(at_launch_observe(d, p, kAtObsKE | kAtObsForce, s));
+  e->rebuilds++;
+  e->prepared = true;
+  int f[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  e->overflow += f[0] + f[1];
+  if (f[0] || f[1]) HIPCHK(hipMemset(d.flags, 0, 2 * sizeof(int)));
+  return FMD_OK;
 }
-int engine_run_atomic(fmd_engine *, int, int, const fmd_step_ctrl *, fmd_thermo *) {
-  return set_error(FMD_ERR_STATE, "atomic engine: not available");
+
+int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+                      fmd_thermo *thermo) {
+  AtomicEngine &A = *e->at;
+  const AtomicDev &d = A.d;
+  const AtomicParams &p = A.p;
+  hipStream_t s = e->stream;
+  if (!e->prepared) {
+    int rc = engine_prepare_atomic(e);
+    if (rc) return rc;
+  }
+  for (int c0 = 0; c0 < nsteps; c0 += e->ring) {
+    const int n = std::min(e->ring, nsteps - c0);
+    for (int k = 0; k < n; k++) {
+      StepCtrl &c = e->h_ctrl[k];
+      if (ctrl) {
+        c.Tt = ctrl[c0 + k].Tt;
+        c.rescale_T = ctrl[c0 + k].rescale_T;
+        c.flags = ctrl[c0 + k].flags;
+      } else {
+        c.Tt = e->tt;
+        c.rescale_T = 0;
+        c.flags = 0;
+      }
+      c.pad = 0;
+    }
+    if (ctrl) e->tt = ctrl[c0 + n - 1].Tt;
+    HIPCHK(hipMemcpyAsync(d.ctrl, e->h_ctrl, n * sizeof(StepCtrl), hipMemcpyHostToDevice, s));
+    while (e->timing && (int)e->ev.size() < 2 * n) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      e->ev.push_back(ev);
+    }
+    for (int k = 0; k < n; k++) {
+      const long long g = (long long)first_step + c0 + k + 1;
+      const int fl = e->h_ctrl[k].flags;
+      const int rebuild = (g % e->nlup == 0) || (fl & FMD_CTRL_REBUILD);  // AB:881
+      const int rescale = (fl & FMD_CTRL_RESCALE) ? 1 : 0;
+      HIPCHK(at_launch_step(d, p, k, k, rebuild, rescale, e->timing ? e->ev[2 * k] : nullptr,
+                            e->timing ? e->ev[2 * k + 1] : nullptr, s));
+      e->rebuilds += rebuild;
+    }
+    if (thermo)
+      HIPCHK(hipMemcpyAsync(e->h_thermo, d.thermo, n * sizeof(Thermo), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (thermo) memcpy(thermo + c0, e->h_thermo, n * sizeof(Thermo));
+    if (e->timing) {
+      for (int k = 0; k < n; k++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]));
+        e->t_sum += ms;
+        e->t_n++;
+      }
+    }
+    e->steps += n;
+  }
+  int f[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (f[0] || f[1]) {
+    e->overflow += f[0] + f[1];
+    HIPCHK(hipMemset(d.flags, 0, 2 * sizeof(int)));
+  }
+  return FMD_OK;
 }
-int engine_forces_atomic(fmd_engine *, double *, double *, double *, double *) {
-  return set_error(FMD_ERR_STATE, "atomic engine: not available");
+
+int engine_forces_atomic(fmd_engine *e, double *F, double *T, double *W9, double *energies) {
+  (void)T;
+  if (!e->at) return set_error(FMD_ERR_STATE, "engine has no atomic system");
+  if (!e->prepared) {
+    int rc = engine_prepare_atomic(e);
+    if (rc) return rc;
+  }
+  const AtomicDev &d = e->at->d;
+  hipStream_t s = e->stream;
+  if (F) HIPCHK(hipMemcpyAsync(F, d.F, 3 * (size_t)e->at->p.Na * 8, hipMemcpyDeviceToHost, s));
+  AtomicState S;
+  HIPCHK(hipMemcpyAsync(&S, d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (W9)
+    for (int k = 0; k < 9; k++) W9[k] = S.W9[k];
+  if (energies) {
+    energies[0] = S.Ep;
+    energies[1] = S.Ep_rebo;
+    energies[2] = S.Ep_lj;
+  }
+  return FMD_OK;
 }
-void engine_free_atomic(fmd_engine *) {}
+
+void engine_free_atomic(fmd_engine *e) {
+  if (!e->at) return;
+  free_atomic_dev(e->at->d);
+  delete e->at;
+  e->at = nullptr;
+}
 
 }  // namespace fmd
+
+extern "C" {
+
+void fmd_atomic_config_default(fmd_atomic_config *c) {
+  memset(c, 0, sizeof *c);
+  fmd_lj_params_airebo(&c->lj);
+  c->lj_skin = 2.0;
+  c->lj_max_neigh = 400;
+  c->rebo_rlist = 2.3;
+  c->rebo_max_neigh = 12;
+  c->rebo_mode = FMD_REBO_COMPAT;
+  c->nlup = 20;
+  c->dt = 0.5;
+  c->clamp_xi = 0.05;
+  c->clamp_vg = 0.005;
+}
+
+int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
+  if (!c || !out || !c->mol_id || !c->mass || c->n_atoms < 1)
+    return set_error(FMD_ERR_ARG, "fmd_atomic_create: bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(FMD_ERR_NODEVICE, "fmd_atomic_create: no HIP device");
+  std::vector<int> start;
+  int apad = 0;
+  int rc = mol_ranges(c->mol_id, c->n_atoms, start, apad);
+  if (rc) return rc;
+  fmd_engine *e = new fmd_engine();
+  e->kind = FMD_KIND_ATOMIC;
+  e->at = new AtomicEngine();
+  e->nlup = c->nlup > 0 ? c->nlup : 20;
+  fill_params(e->at->p, *c, (int)start.size() - 1, apad);
+  rc = engine_init_common(e, 2048);
+  if (rc == FMD_OK) rc = atomic_alloc(e->at->d, e->at->p, e->ring);
+  if (rc == FMD_OK) {
+    hipError_t he = hipMemcpy(e->at->d.mol_start, start.data(), start.size() * sizeof(int),
+                              hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+      he = hipMemcpy(e->at->d.mass, c->mass, c->n_atoms * sizeof(double), hipMemcpyHostToDevice);
+    if (he != hipSuccess) rc = hip_fail(he, "upload molecule layout");
+  }
+  if (rc) {
+    fmd_engine_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return FMD_OK;
+}
+
+int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel, const double *h,
+                         const fmd_npt_state *npt) {
+  if (!e || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const size_t n3 = 3 * (size_t)A.p.Na * 8;
+  if (pos) HIPCHK(hipMemcpyAsync(A.d.pos, pos, n3, hipMemcpyHostToDevice, s));
+  if (vel) HIPCHK(hipMemcpyAsync(A.d.vel, vel, n3, hipMemcpyHostToDevice, s));
+  AtomicState S;
+  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h) {
+    for (int k = 0; k < 9; k++) S.h[k] = h[k];
+    inv9(S.h, S.hi);
+  }
+  if (npt) {
+    S.xi = npt->xi;
+    S.Q = npt->Q;
+    for (int k = 0; k < 9; k++) S.Vg[k] = npt->Vg[k];
+    S.W = npt->W;
+    S.Pe = npt->Pe;
+    S.Tt = npt->Tt;
+    S.Nf = npt->Nf;
+    e->tt = npt->Tt;
+  }
+  S.scale = 1.0;
+  if (pos) {
+    std::vector<int> start(A.p.Nm + 1);
+    HIPCHK(hipMemcpy(start.data(), A.d.mol_start, start.size() * sizeof(int),
+                     hipMemcpyDeviceToHost));
+    A.p.small_box = decide_small_box(A.p, S.h, max_radius(pos, start, S.h));
+  }
+  HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  e->prepared = false;
+  return FMD_OK;
+}
+
+int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
+                         fmd_npt_state *npt) {
+  if (!e || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const size_t n3 = 3 * (size_t)A.p.Na * 8;
+  if (pos) HIPCHK(hipMemcpyAsync(pos, A.d.pos, n3, hipMemcpyDeviceToHost, s));
+  if (vel) HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
+  AtomicState S;
+  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h)
+    for (int k = 0; k < 9; k++) h[k] = S.h[k];
+  if (npt) {
+    npt->xi = S.xi;
+    npt->Q = S.Q;
+    for (int k = 0; k < 9; k++) npt->Vg[k] = S.Vg[k];
+    npt->W = S.W;
+    npt->Pe = S.Pe;
+    npt->Tt = S.Tt;
+    npt->Nf = S.Nf;
+  }
+  return FMD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- AIREBO drop-ins
+namespace {
+
+constexpr int kRefMaxRebo = 12, kRefMaxLj = 400;  // AB:126-127
+
+struct AtomicWorkspace {
+  fmd_engine *e = nullptr;
+  std::vector<int> mol_id;
+};
+AtomicWorkspace g_aws;
+
+// Workspace engine for this molecule layout (re-created when it changes).
+int atomic_workspace(int na, const int *mol_id, int rebo_mode, fmd_engine **out) {
+  if (!mol_id || na < 1) return set_error(FMD_ERR_ARG, "AIREBO drop-in: need mol_id and Na >= 1");
+  bool same = g_aws.e && (int)g_aws.mol_id.size() == na &&
+              std::equal(g_aws.mol_id.begin(), g_aws.mol_id.end(), mol_id);
+  if (!same) {
+    if (g_aws.e) fmd_engine_destroy(g_aws.e);
+    g_aws = AtomicWorkspace();
+    std::vector<double> mass(na, kMassC);
+    fmd_atomic_config c;
+    fmd_atomic_config_default(&c);
+    c.n_atoms = na;
+    c.mol_id = mol_id;
+    c.mass = mass.data();
+    int rc = fmd_atomic_create(&c, &g_aws.e);
+    if (rc) return rc;
+    g_aws.mol_id.assign(mol_id, mol_id + na);
+  }
+  g_aws.e->at->p.rebo_mode = rebo_mode;
+  *out = g_aws.e;
+  return FMD_OK;
+}
+
+// Upload pos and the (h, h^-1) pair, decide the image mode.
+int stage_positions(fmd_engine *e, const double *pos, const double *h, const double *hi) {
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  double hh[18];
+  for (int k = 0; k < 9; k++) {
+    hh[k] = h[k];
+    hh[9 + k] = hi[k];
+  }
+  HIPCHK(hipMemcpyAsync(A.d.pos, pos, 3 * (size_t)A.p.Na * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
+  std::vector<int> start(A.p.Nm + 1);
+  HIPCHK(hipMemcpy(start.data(), A.d.mol_start, start.size() * sizeof(int),
+                   hipMemcpyDeviceToHost));
+  A.p.small_box = decide_small_box(A.p, h, max_radius(pos, start, h));
+  return FMD_OK;
+}
+
+// Caller's REBO list (stride 12) and the truncation points of its LJ half
+// list (stride 400) onto the device.
+int stage_lists(fmd_engine *e, const int *nlc_r, const int *nll_r, const int *nlc_l,
+                const int *nll_l) {
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const size_t na = A.p.Na;
+  HIPCHK(hipMemcpyAsync(A.d.nlc_r, nlc_r, na * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.nll_r, nll_r, na * kRefMaxRebo * sizeof(int), hipMemcpyHostToDevice,
+                        s));
+  int *dl = nullptr;
+  int *dc = nullptr;
+  HIPCHK(hipMallocAsync((void **)&dl, na * kRefMaxLj * sizeof(int), s));
+  HIPCHK(hipMallocAsync((void **)&dc, na * sizeof(int), s));
+  HIPCHK(hipMemcpyAsync(dc, nlc_l, na * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dl, nll_l, na * kRefMaxLj * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(at_launch_jmax_from_list(A.d, A.p, dc, dl, kRefMaxLj, s));
+  HIPCHK(hipFreeAsync(dl, s));
+  HIPCHK(hipFreeAsync(dc, s));
+  return FMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmd_airebo_nlist_rebo(const double *pos, const double *h, const double *hi, int Na,
+                          const int *mol_id, int *nlc, int *nll) {
+  if (!pos || !h || !hi || !nlc || !nll) return set_error(FMD_ERR_ARG, "nlist_rebo: null argument");
+  fmd_engine *e;
+  int rc = atomic_workspace(Na, mol_id, FMD_REBO_COMPAT, &e);
+  if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
+  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 1, s));
+  HIPCHK(hipMemcpyAsync(nlc, A.d.nlc_r, (size_t)Na * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(nll, A.d.nll_r, (size_t)Na * kRefMaxRebo * sizeof(int),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FMD_OK;
+}
+
+int fmd_airebo_nlist_lj(const double *pos, const double *h, const double *hi, int Na,
+                        const int *mol_id, int *nlc, int *nll) {
+  if (!pos || !h || !hi || !nlc || !nll) return set_error(FMD_ERR_ARG, "nlist_lj: null argument");
+  fmd_engine *e;
+  int rc = atomic_workspace(Na, mol_id, FMD_REBO_COMPAT, &e);
+  if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  int *dl = nullptr, *dc = nullptr;
+  HIPCHK(hipMallocAsync((void **)&dl, (size_t)Na * kRefMaxLj * sizeof(int), s));
+  HIPCHK(hipMallocAsync((void **)&dc, (size_t)Na * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(dl, 0, (size_t)Na * kRefMaxLj * sizeof(int), s));
+  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
+  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 0, s));
+  HIPCHK(at_launch_halflist(A.d, A.p, A.d.hh, dc, dl, kRefMaxLj, s));
+  HIPCHK(hipMemcpyAsync(nlc, dc, (size_t)Na * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(nll, dl, (size_t)Na * kRefMaxLj * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipFreeAsync(dl, s));
+  HIPCHK(hipFreeAsync(dc, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FMD_OK;
+}
+
+int fmd_airebo_compute_forces(double *F, double *vir9, const double *pos, const double *h,
+                              const double *hi, const int *nlc_r, const int *nll_r,
+                              const int *nlc_l, const int *nll_l, int Na, const int *mol_id,
+                              int rebo_mode, double *out3) {
+  if (!F || !vir9 || !pos || !h || !hi || !nlc_r || !nll_r || !nlc_l || !nll_l)
+    return set_error(FMD_ERR_ARG, "compute_forces: null argument");
+  fmd_engine *e;
+  int rc = atomic_workspace(Na, mol_id, rebo_mode, &e);
+  if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const int lj_compat = A.p.lj_compat;
+  A.p.lj_compat = 0;  // the molecule list only; jmax comes from the caller's list
+  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
+  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 0, s));
+  A.p.lj_compat = lj_compat;
+  if ((rc = stage_lists(e, nlc_r, nll_r, nlc_l, nll_l))) return rc;
+  HIPCHK(at_launch_forces(A.d, A.p, A.d.hh, s));
+  HIPCHK(at_launch_observe(A.d, A.p, kAtObsForce, s));
+  AtomicState S;
+  HIPCHK(hipMemcpyAsync(F, A.d.F, 3 * (size_t)Na * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < 9; k++) vir9[k] = S.W9[k];
+  if (out3) {
+    out3[0] = S.Ep;
+    out3[1] = S.Ep_rebo;
+    out3[2] = S.Ep_lj;
+  }
+  return FMD_OK;
+}
+
+int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
+                        double *hi, const double *mass, int Na, double dt,
+                        fmd_npt_state *npt, const int *nlc_r, const int *nll_r,
+                        const int *nlc_l, const int *nll_l, const int *mol_id,
+                        int rebo_mode, double *out4) {
+  if (!pos || !vel || !F || !vir9 || !h || !hi || !mass || !npt || !nlc_r || !nll_r ||
+      !nlc_l || !nll_l)
+    return set_error(FMD_ERR_ARG, "step_npt: null argument");
+  fmd_engine *e;
+  int rc = atomic_workspace(Na, mol_id, rebo_mode, &e);
+  if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const size_t n3 = 3 * (size_t)Na * 8;
+  A.p.dt = dt;
+  HIPCHK(hipMemcpyAsync(A.d.vel, vel, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.F, F, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.mass, mass, (size_t)Na * 8, hipMemcpyHostToDevice, s));
+  // the molecule list from the current positions, then the caller's lists
+  const int lj_compat = A.p.lj_compat;
+  A.p.lj_compat = 0;
+  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
+  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 0, s));
+  A.p.lj_compat = lj_compat;
+  if ((rc = stage_lists(e, nlc_r, nll_r, nlc_l, nll_l))) return rc;
+  AtomicState S;
+  memset(&S, 0, sizeof S);
+  for (int k = 0; k < 9; k++) {
+    S.h[k] = h[k];
+    S.Vg[k] = npt->Vg[k];
+    S.W9[k] = vir9[k];
+  }
+  inv9(S.h, S.hi);
+  S.xi = npt->xi;
+  S.Q = npt->Q;
+  S.W = npt->W;
+  S.Pe = npt->Pe;
+  S.Tt = npt->Tt;
+  S.Nf = npt->Nf;
+  S.scale = 1.0;
+  StepCtrl c = {npt->Tt, 0.0, 0, 0};
+  HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
+  HIPCHK(at_launch_observe(A.d, A.p, kAtObsKE, s));
+  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, nullptr, s));
+  HIPCHK(hipMemcpyAsync(pos, A.d.pos, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(F, A.d.F, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < 9; k++) {
+    h[k] = S.h[k];
+    hi[k] = S.hi[k];
+    vir9[k] = S.W9[k];
+    npt->Vg[k] = S.Vg[k];
+  }
+  npt->xi = S.xi;
+  if (out4) {
+    out4[0] = S.Ep;
+    out4[1] = S.Ep_rebo;
+    out4[2] = S.Ep_lj;
+    out4[3] = S.KE;
+  }
+  return FMD_OK;
+}
+
+}  // extern "C"

@@ -58,6 +58,32 @@ FMD_HD void mimg(double &dx, double &dy, double &dz, const double *hi, const dou
   dz = h[6] * s0 + h[7] * s1 + h[8] * s2;
 }
 
+// The same minimum image with the reference's operation order and no FMA
+// contraction: bitwise the reference's FP64 result, for decisions (cutoff,
+// list membership) that must agree with it exactly.
+FMD_HD void mimg_exact(double &dx, double &dy, double &dz, const double *hi, const double *h) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  double s0 = hi[0] * dx + hi[1] * dy + hi[2] * dz;
+  double s1 = hi[3] * dx + hi[4] * dy + hi[5] * dz;
+  double s2 = hi[6] * dx + hi[7] * dy + hi[8] * dz;
+  s0 -= round(s0);
+  s1 -= round(s1);
+  s2 -= round(s2);
+  dx = h[0] * s0 + h[1] * s1 + h[2] * s2;
+  dy = h[3] * s0 + h[4] * s1 + h[5] * s2;
+  dz = h[6] * s0 + h[7] * s1 + h[8] * s2;
+}
+
+// dx^2 + dy^2 + dz^2 without contraction (the reference's r^2).
+FMD_HD double norm2_exact(double dx, double dy, double dz) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  return dx * dx + dy * dy + dz * dz;
+}
+
 // ------------------------------------------------------------ quaternions
 FMD_HD void q2R(const double *q, double *R) {  // C1:186-191
   double w = q[0], x = q[1], y = q[2], z = q[3];
@@ -70,6 +96,30 @@ FMD_HD void q2R(const double *q, double *R) {  // C1:186-191
   R[6] = 2 * (x * z - w * y);
   R[7] = 2 * (y * z + w * x);
   R[8] = 1 - 2 * (x * x + y * y);
+}
+
+// Lab-frame site offset R(q) b with the reference's operation order and no
+// contraction (C1:186-191, 357-362): bitwise the reference's lab array.
+FMD_HD void lab_exact(const double *q, const double *b, double *o) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  double w = q[0], x = q[1], y = q[2], z = q[3];
+  double R0 = 1 - 2 * (y * y + z * z), R1 = 2 * (x * y - w * z), R2 = 2 * (x * z + w * y);
+  double R3 = 2 * (x * y + w * z), R4 = 1 - 2 * (x * x + z * z), R5 = 2 * (y * z - w * x);
+  double R6 = 2 * (x * z - w * y), R7 = 2 * (y * z + w * x), R8 = 1 - 2 * (x * x + y * y);
+  o[0] = R0 * b[0] + R1 * b[1] + R2 * b[2];
+  o[1] = R3 * b[0] + R4 * b[1] + R5 * b[2];
+  o[2] = R6 * b[0] + R7 * b[1] + R8 * b[2];
+}
+
+// Squared length of dm + rb - ra with the reference's order (C1:398-401).
+FMD_HD double pair_r2_exact(const double *dm, const double *rb, const double *ra) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  double x = dm[0] + rb[0] - ra[0], y = dm[1] + rb[1] - ra[1], z = dm[2] + rb[2] - ra[2];
+  return x * x + y * y + z * z;
 }
 
 // q <- normalise(q (x) dq(w dt)), dq the exact rotation by |w| dt about w
@@ -108,6 +158,9 @@ FMD_HD void quat_advance(double *q, double wx, double wy, double wz, double dt) 
 struct LJ32 {
   float sig2;   // sigma^2
   float rc2;    // cutoff^2 (pairs with r^2 < rc2, or <= for AIREBO)
+  float rc2_lo, rc2_hi;  // rc2 (1 -+ 1e-5): outside this band the FP32 r^2
+                         // decides; inside it the pair is re-tested in FP64
+  double rc2d;           // the FP64 cutoff^2 of that re-test
   float c48;    // 48 eps
   float c24;    // 24 eps
   float vs4;    // vshift / (4 eps)

@@ -76,6 +76,8 @@ struct RigidParams {
 struct RigidDev {
   double *pos, *vel, *omg, *qv, *Fv, *Tv;
   float4 *sites;
+  double *sites_d;  // N*natom_pad*3: the same offsets in FP64, bitwise the
+                    // reference's lab array, for cutoff re-tests
   double *body;  // natom_pad x 4 doubles: x, y, z, padding flag (1 = padding)
   double *hh;    // 18 doubles: h, h^-1 for the stateless entry points
   int *nl_count, *nl_list, *flags;  // flags[0]: list overflow count

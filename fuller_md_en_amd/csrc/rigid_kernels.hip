@@ -43,27 +43,28 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Lab-frame offsets of the sites of molecule i (lanes a and a+64 ...).
+// Lab-frame offsets of the sites of molecule i (lanes a and a+64): FP32 for
+// the pair loop and FP64 (bitwise the reference's lab array) for re-tests.
 template <int NS>
 __device__ __forceinline__ void write_sites(int i, const double *q, const double *body,
-                                            float4 *sites, int natom_pad, int lane) {
-  double R[9];
-  q2R(q, R);
+                                            float4 *sites, double *sites_d, int natom_pad,
+                                            int lane) {
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     int a = lane + 64 * s;
     if (a < natom_pad) {
       const double *b = body + 4 * a;
-      float4 o;
+      double o[3];
       if (b[3] != 0.0) {
-        o = make_float4(1e6f, 1e6f, 1e6f, 0.f);
+        o[0] = o[1] = o[2] = 1e6;
       } else {
-        o.x = (float)(R[0] * b[0] + R[1] * b[1] + R[2] * b[2]);
-        o.y = (float)(R[3] * b[0] + R[4] * b[1] + R[5] * b[2]);
-        o.z = (float)(R[6] * b[0] + R[7] * b[1] + R[8] * b[2]);
-        o.w = 0.f;
+        lab_exact(q, b, o);
       }
-      sites[(size_t)i * natom_pad + a] = o;
+      const size_t t = (size_t)i * natom_pad + a;
+      sites[t] = make_float4((float)o[0], (float)o[1], (float)o[2], 0.f);
+      sites_d[3 * t] = o[0];
+      sites_d[3 * t + 1] = o[1];
+      sites_d[3 * t + 2] = o[2];
     }
   }
 }
@@ -84,8 +85,8 @@ __global__ __launch_bounds__(64) void k_rigid_nlist(const double *__restrict__ p
     bool in = false;
     if (j < p.N && j != i) {
       double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
-      mimg(dx, dy, dz, hi, h);
-      in = dx * dx + dy * dy + dz * dz < p.rlist2;
+      mimg_exact(dx, dy, dz, hi, h);
+      in = norm2_exact(dx, dy, dz) < p.rlist2;
     }
     unsigned long long m = __ballot(in);
     int rank = __popcll(m & ((1ull << lane) - 1ull));
@@ -102,12 +103,14 @@ __global__ __launch_bounds__(64) void k_rigid_nlist(const double *__restrict__ p
 template <int NS>
 __global__ __launch_bounds__(256) void k_rigid_sites(const double *__restrict__ qv,
                                                      const double *__restrict__ body,
-                                                     float4 *__restrict__ sites, RigidParams p) {
+                                                     float4 *__restrict__ sites,
+                                                     double *__restrict__ sites_d,
+                                                     RigidParams p) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= p.N) return;
   double q[4] = {qv[4 * i], qv[4 * i + 1], qv[4 * i + 2], qv[4 * i + 3]};
-  write_sites<NS>(i, q, body, sites, p.natom_pad, lane);
+  write_sites<NS>(i, q, body, sites, sites_d, p.natom_pad, lane);
 }
 
 __global__ __launch_bounds__(256) void k_rigid_apply_pbc(double *__restrict__ pos,
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(256) void k_rigid_kick_drift(RigidDev d, RigidParam
     for (int a = 0; a < 4; a++) d.qv[4 * i + a] = q[a];
   }
   for (int a = 0; a < 4; a++) q[a] = __shfl(q[a], 0, 64);
-  write_sites<NS>(i, q, d.body, d.sites, p.natom_pad, lane);
+  write_sites<NS>(i, q, d.body, d.sites, d.sites_d, p.natom_pad, lane);
 }
 
 // ------------------------------------------------------------------ forces
@@ -237,9 +240,9 @@ __global__ __launch_bounds__(256) void k_rigid_kick_drift(RigidDev d, RigidParam
 template <int NS>
 __global__ __launch_bounds__(256) void k_rigid_forces(
     const double *__restrict__ pos, const float4 *__restrict__ sites,
-    const int *__restrict__ nl_count, const int *__restrict__ nl_list,
-    const double *__restrict__ hh, double *__restrict__ Fv, double *__restrict__ Tv,
-    double *__restrict__ molpart, RigidParams p) {
+    const double *__restrict__ sites_d, const int *__restrict__ nl_count,
+    const int *__restrict__ nl_list, const double *__restrict__ hh, double *__restrict__ Fv,
+    double *__restrict__ Tv, double *__restrict__ molpart, RigidParams p) {
   // XCD-aware mapping: the workgroups one XCD receives (bid = x mod 8) take a
   // contiguous range of molecules, so neighbours share that XCD's L2.
   const int per = (p.N + kXcd - 1) / kXcd;
@@ -253,17 +256,21 @@ __global__ __launch_bounds__(256) void k_rigid_forces(
   const int npad = p.natom_pad;
 
   float rax[NS], ray[NS], raz[NS];
+  double rad[NS][3];  // FP64 offsets (reference lab values)
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     int a = lane + 64 * s;
     if (a < p.natom) {
-      float4 r = sites[(size_t)i * npad + a];
-      rax[s] = r.x;
-      ray[s] = r.y;
-      raz[s] = r.z;
+      const size_t t = (size_t)i * npad + a;
+      rad[s][0] = sites_d[3 * t];
+      rad[s][1] = sites_d[3 * t + 1];
+      rad[s][2] = sites_d[3 * t + 2];
     } else {  // idle lane: far away from everything
-      rax[s] = ray[s] = raz[s] = -3.0e5f;
+      rad[s][0] = rad[s][1] = rad[s][2] = -3.0e5;
     }
+    rax[s] = (float)rad[s][0];
+    ray[s] = (float)rad[s][1];
+    raz[s] = (float)rad[s][2];
   }
   double GA[NS][3], WA[NS][9];
   float EA[NS];
@@ -279,15 +286,18 @@ __global__ __launch_bounds__(256) void k_rigid_forces(
   for (int k = wv; k < nn; k += 4) {
     const int j = uniform(li[k]);
     double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
-    mimg(dx, dy, dz, hi, h);
+    mimg_exact(dx, dy, dz, hi, h);
     if (dx * dx + dy * dy + dz * dz > p.rmcut2) continue;
+    const double dm[3] = {dx, dy, dz};
     float cx[NS], cy[NS], cz[NS];
     bool act = false;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      cx[s] = (float)dx - rax[s];
-      cy[s] = (float)dy - ray[s];
-      cz[s] = (float)dz - raz[s];
+      // dm - r_a formed in FP64 and rounded once: no FP32 rounding of the
+      // (up to 18 A) centre vector that all 3600 site pairs would share
+      cx[s] = (float)(dx - rad[s][0]);
+      cy[s] = (float)(dy - rad[s][1]);
+      cz[s] = (float)(dz - rad[s][2]);
       act |= cx[s] * cx[s] + cy[s] * cy[s] + cz[s] * cz[s] < (float)p.cull_r2;
     }
     if (!__any(act)) continue;
@@ -307,11 +317,17 @@ __global__ __launch_bounds__(256) void k_rigid_forces(
         for (int s = 0; s < NS; s++) {
           float ex = rb[u].x + cx[s], ey = rb[u].y + cy[s], ez = rb[u].z + cz[s];
           float r2 = ex * ex + ey * ey + ez * ez;
-          if (r2 < lj.rc2) {
-            float fm = lj_fm(r2, lj, e[s]);
-            gx[s] += fm * ex;
-            gy[s] += fm * ey;
-            gz[s] += fm * ez;
+          if (r2 < lj.rc2_hi) {
+            bool in = r2 < lj.rc2_lo;
+            if (!in)  // within FP32 rounding of the cutoff: the reference's test
+              in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + b + u), rad[s]) <
+                   lj.rc2d;
+            if (in) {
+              float fm = lj_fm(r2, lj, e[s]);
+              gx[s] += fm * ex;
+              gy[s] += fm * ey;
+              gz[s] += fm * ez;
+            }
           }
         }
       }
@@ -544,9 +560,9 @@ inline int ns_of(const RigidParams &p) { return p.natom <= 64 ? 1 : 2; }
 hipError_t rigid_launch_sites(const RigidDev &d, const RigidParams &p, hipStream_t s) {
   dim3 g((p.N + 3) / 4);
   if (ns_of(p) == 1)
-    k_rigid_sites<1><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, p);
+    k_rigid_sites<1><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, d.sites_d, p);
   else
-    k_rigid_sites<2><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, p);
+    k_rigid_sites<2><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, d.sites_d, p);
   return hipGetLastError();
 }
 
@@ -567,11 +583,11 @@ hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const do
   const int per = (p.N + kXcd - 1) / kXcd;
   dim3 g(per * kXcd);
   if (ns_of(p) == 1)
-    k_rigid_forces<1><<<g, 256, 0, s>>>(d.pos, d.sites, d.nl_count, d.nl_list, hh, d.Fv, d.Tv,
-                                        d.molpart, p);
+    k_rigid_forces<1><<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.nl_count, d.nl_list, hh,
+                                        d.Fv, d.Tv, d.molpart, p);
   else
-    k_rigid_forces<2><<<g, 256, 0, s>>>(d.pos, d.sites, d.nl_count, d.nl_list, hh, d.Fv, d.Tv,
-                                        d.molpart, p);
+    k_rigid_forces<2><<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.nl_count, d.nl_list, hh,
+                                        d.Fv, d.Tv, d.molpart, p);
   return hipGetLastError();
 }
 

@@ -124,6 +124,44 @@ int fmd_rigid_step_npt(double *pos, double *vel, double *qv, double *omg, double
                        const int *nl_count, const int *nl_list, int max_neigh,
                        const fmd_lj_params *lj, double *Ep, double *KE);
 
+/* ---------------------------------------------------- AIREBO drop-ins */
+
+#define FMD_REBO_COMPAT 0   /* the reference's REBO forces and pair-only virial */
+#define FMD_REBO_PHYSICAL 1 /* exact gradient of the same energy, full virial */
+
+/* Replaces build_nlist_rebo (AB:251-261): symmetric same-molecule list within
+ * 2.3 A, ascending, stride 12 (MAX_REBO_NEIGH, AB:126). */
+int fmd_airebo_nlist_rebo(const double *pos, const double *h, const double *hi, int Na,
+                          const int *mol_id, int *nlc, int *nll);
+
+/* Replaces build_nlist_lj (AB:263-272): half list (j > i) of other-molecule
+ * atoms within rc + 2 A, ascending, the first 400 kept (MAX_LJ_NEIGH, AB:127),
+ * stride 400.  Atoms of one molecule must be contiguous (as the reference
+ * builds them, AB:781-786). */
+int fmd_airebo_nlist_lj(const double *pos, const double *h, const double *hi, int Na,
+                        const int *mol_id, int *nlc, int *nll);
+
+/* Replaces compute_forces (AB:477-495): REBO + intermolecular LJ forces F
+ * (Na x 3), virial vir9 and out3 = {Ep, Ep_rebo, Ep_lj}.  The REBO list is
+ * used as given; the LJ half list is used through its truncation points (the
+ * last kept j of every full 400-entry list), its pairs being regenerated on
+ * the GPU, which is the same pair set when the list was built from these
+ * positions (or within its 2 A skin).  mol_id (not an argument of the
+ * reference function) gives the molecule of each atom. */
+int fmd_airebo_compute_forces(double *F, double *vir9, const double *pos, const double *h,
+                              const double *hi, const int *nlc_r, const int *nll_r,
+                              const int *nlc_l, const int *nll_l, int Na, const int *mol_id,
+                              int rebo_mode, double *out3);
+
+/* Replaces step_npt (AB:518-590): one atomic NH + PR step (clamps xi +-0.05,
+ * Vg +-0.005); updates pos, vel, F, vir9, h, hi, *npt in place;
+ * out4 = {Ep, Ep_rebo, Ep_lj, KE}. */
+int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
+                        double *hi, const double *mass, int Na, double dt,
+                        fmd_npt_state *npt, const int *nlc_r, const int *nll_r,
+                        const int *nlc_l, const int *nll_l, const int *mol_id,
+                        int rebo_mode, double *out4);
+
 /* ------------------------------------------------------- resident engine */
 
 typedef struct fmd_engine fmd_engine;
@@ -156,6 +194,29 @@ int fmd_rigid_set_state(fmd_engine *e, const double *pos, const double *vel,
                         const fmd_npt_state *npt);
 int fmd_rigid_get_state(fmd_engine *e, double *pos, double *vel, double *qv, double *omg,
                         double *h, fmd_npt_state *npt);
+
+typedef struct {
+  int n_atoms;
+  const int *mol_id;     /* molecule of each atom; each molecule's atoms contiguous */
+  const double *mass;    /* amu */
+  fmd_lj_params lj;      /* intermolecular LJ */
+  double lj_skin;        /* half-list radius = rcut + lj_skin (2.0, AB:265) */
+  int lj_max_neigh;      /* 400 (AB:127): the reference's truncation; 0 = none */
+  double rebo_rlist;     /* 2.3 (AB:116) */
+  int rebo_max_neigh;    /* 12 (AB:126) */
+  int rebo_mode;         /* FMD_REBO_COMPAT / FMD_REBO_PHYSICAL */
+  int nlup;              /* list rebuild interval (20, AB:860) */
+  double dt;             /* fs (0.5) */
+  double clamp_xi, clamp_vg; /* 0.05, 0.005 (AB:524, 530) */
+} fmd_atomic_config;
+
+/* Defaults of the reference AIREBO program. */
+void fmd_atomic_config_default(fmd_atomic_config *c);
+int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out);
+int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel,
+                         const double *h, const fmd_npt_state *npt);
+int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
+                         fmd_npt_state *npt);
 
 /* Per-step phase control (the COLD / WARM / PROD logic of LJ:848-894). */
 #define FMD_CTRL_ZERO_XI_VG 1  /* xi = 0, Vg = 0 before the step */

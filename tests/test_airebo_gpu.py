@@ -1,0 +1,154 @@
+"""AIREBO (REBO bond order + intermolecular LJ) on the GPU against the
+reference build's golden vectors and the FP64 oracle.
+
+Tolerances: forces / virial max|a-b| / max|b| <= 1e-5, energies relative
+<= 1e-5, lists exact.  Cases: 1x1x1 and 2x2x2 C60 crystals jittered by
++-0.01 A (per-pair images: the cells are narrower than the LJ reach), and
+3x3x3 (molecule-image path) built on the fly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden, rel_max, unpack_list
+from fuller_md_en_amd import airebo
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def crystal(nc, seed, jitter):
+    """Jittered C60 FCC crystal the way the AIREBO program builds it."""
+    from fuller_md_en_amd import system
+    mol = system.molecule("C60")
+    a0 = system.default_a0(mol.dmol, "FCC", 1.0)
+    cen, h = system.make_crystal("FCC", a0, nc)
+    pos = (cen[:, None, :] + mol.coords[None]).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    pos = pos + rng.uniform(-jitter, jitter, pos.shape)
+    mol_id = np.repeat(np.arange(1, len(cen) + 1), mol.natom).astype(np.int32)
+    p = pos.reshape(-1).copy()
+    oracle.lib().orc_apply_pbc(p, h, oracle.inv9(h), len(pos))
+    return p.reshape(-1, 3), h, mol_id
+
+
+def test_rebo_and_lj_lists_equal_reference():
+    g = golden("airebo_nc1.npz")
+    hi = oracle.inv9(g["h"])
+    cr, lr = airebo.build_nlist_rebo(g["pos"], g["h"], hi, g["mol_id"])
+    cl, ll = airebo.build_nlist_lj(g["pos"], g["h"], hi, g["mol_id"])
+    assert (cr == g["nlc_r"]).all() and (cl == g["nlc_l"]).all()
+    assert (lr == unpack_list(g["nlc_r"], g["nll_r"], 12)).all()
+    assert (ll == unpack_list(g["nlc_l"], g["nll_l"], 400)).all()
+
+
+def test_lj_truncated_list_equals_reference():
+    g = golden("airebo_nc2.npz")
+    cl, ll = airebo.build_nlist_lj(g["pos"], g["h"], oracle.inv9(g["h"]), g["mol_id"])
+    assert (cl == g["nlc_l"]).all()
+    last = np.array([ll[i, cl[i] - 1] if cl[i] else -1 for i in range(len(cl))])
+    assert (last == g["lj_last"]).all()
+    ocl, oll = oracle.nlist_lj(g["pos"], g["h"], g["mol_id"])
+    assert (ll == oll).all()
+
+
+@pytest.mark.parametrize("name", ["airebo_nc1.npz", "airebo_nc2.npz"])
+def test_compute_forces_compat_matches_reference(name):
+    g = golden(name)
+    print(f"case {name}", flush=True)
+    cr, lr = oracle.nlist_rebo(g["pos"], g["h"], g["mol_id"])
+    cl, ll = oracle.nlist_lj(g["pos"], g["h"], g["mol_id"])
+    E, F, W = airebo.compute_forces(g["pos"], g["h"], oracle.inv9(g["h"]), (cr, lr, cl, ll),
+                                    g["mol_id"], airebo.REBO_COMPAT)
+    assert rel_max(F, g["F"]) < TOL
+    assert rel_max(W, g["W"]) < TOL
+    for a, b in zip(E, g["E"]):
+        assert rel(a, b) < TOL
+
+
+@pytest.mark.parametrize("nc,jit", [(2, 0.01), (2, 0.1)])
+def test_compute_forces_physical_matches_oracle(nc, jit):
+    pos, h, mol = crystal(nc, 31 + nc, jit)
+    lists = (*oracle.nlist_rebo(pos, h, mol), *oracle.nlist_lj(pos, h, mol))
+    E0, F0, W0 = oracle.airebo_forces(pos, h, lists, oracle.REBO_PHYSICAL)
+    E, F, W = airebo.compute_forces(pos, h, oracle.inv9(h), lists, mol, airebo.REBO_PHYSICAL)
+    assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
+    for a, b in zip(E, E0):
+        assert rel(a, b) < TOL
+
+
+def test_step_npt_dropin_matches_reference():
+    g = golden("airebo_nc1.npz")
+    cr, lr = oracle.nlist_rebo(g["pos"], g["h"], g["mol_id"])
+    cl, ll = oracle.nlist_lj(g["pos"], g["h"], g["mol_id"])
+    P, V, F, W = g["pos"].copy(), g["vel"].copy(), g["F"].copy(), g["W"].copy()
+    h, hi = g["h"].copy(), oracle.inv9(g["h"])
+    npt = airebo.NPTState.from_array(g["npt0"])
+    traj = []
+    for _ in range(5):
+        traj.append(airebo.step_npt(P, V, F, W, h, hi, g["mass"], float(g["dt"]), npt,
+                                    (cr, lr, cl, ll), g["mol_id"]))
+    traj = np.array(traj)
+    for k in range(4):
+        assert rel_max(traj[:, k], g["traj"][:, k]) < TOL
+    assert np.abs(P - g["pos_end"]).max() < 1e-8
+    assert rel_max(V, g["vel_end"]) < 1e-5
+    assert rel_max(F, g["F_end"]) < TOL
+    assert np.abs(h - g["h_end"]).max() < 1e-10
+
+
+def _engine(pos, h, mol, vel=None, mode=airebo.REBO_COMPAT, T=300.0):
+    mass = np.full(len(pos), 12.011)
+    eng = airebo.AtomicEngine(mol, mass, mode=mode)
+    eng.set_state(pos=pos, vel=np.zeros_like(pos) if vel is None else vel, h=h,
+                  npt=airebo.make_npt(T, 0.0, len(pos)))
+    return eng
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_engine_forces_nc3_against_oracle(mode):
+    """3x3x3 (6480 atoms): molecule-image LJ path, truncation active."""
+    pos, h, mol = crystal(3, 7, 0.01)
+    lists = (*oracle.nlist_rebo(pos, h, mol), *oracle.nlist_lj(pos, h, mol))
+    E0, F0, W0 = oracle.airebo_forces(pos, h, lists, mode)
+    eng = _engine(pos, h, mol, mode=mode)
+    eng.prepare()
+    E, F, W = eng.forces()
+    assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
+    for a, b in zip(E, E0):
+        assert rel(a, b) < TOL
+    assert eng.counters()[0] == 0
+    eng.close()
+
+
+def test_engine_trajectory_matches_reference():
+    g = golden("airebo_nc1.npz")
+    eng = _engine(g["pos"], g["h"], g["mol_id"], vel=g["vel"])
+    eng.set_state(npt=airebo.NPTState.from_array(g["npt0"]))
+    th = eng.run(0, 5)
+    ep = np.array([[t.Ep, t.Ep_rebo, t.Ep_lj, t.KE] for t in th])
+    for k in range(4):
+        assert rel_max(ep[:, k], g["traj"][:, k]) < TOL
+    st = eng.get_state()
+    assert np.abs(st["pos"] - g["pos_end"]).max() < 1e-8
+    assert rel_max(st["vel"], g["vel_end"]) < 1e-5
+    eng.close()
+
+
+def test_engine_deterministic_and_list_rebuild():
+    pos, h, mol = crystal(3, 11, 0.01)
+    from fuller_md_en_amd import system
+    vel = system.atomic_init_velocities(np.full(len(pos), 12.011), 42, 300.0)
+    out = []
+    for _ in range(2):
+        eng = _engine(pos, h, mol, vel=vel)
+        th = eng.run(0, 25)  # crosses the rebuild at step 20
+        out.append((eng.get_state()["pos"].copy(), th[24].Ep))
+        assert eng.counters()[1] >= 2
+        eng.close()
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
