@@ -434,8 +434,12 @@ __global__ __launch_bounds__(256) void k_rebo_pass2(AtomicDev d, AtomicParams p)
 // PRED: the reference half-list truncation (pair kept iff j <= jmax(i) for
 // the lower index i).  PERPAIR: per-pair minimum image (cells narrower than
 // twice the list reach).
+// PERPAIR re-images every pair vector, which would fold the far-away
+// sentinels of idle lanes and padding sites back into the cell, so that path
+// masks them explicitly (lane_ok, jb <= j_last).
 template <int NS, bool PRED, bool PERPAIR>
 __device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, int jb0,
+                                         int j_last, const bool *lane_ok,
                                          const float *cx, const float *cy, const float *cz,
                                          const int *ia, const int *ia_jmax, bool j_above,
                                          const LJ32 &lj, const float *hf, const float *hif,
@@ -460,6 +464,7 @@ __device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, i
       }
       const float r2 = ex * ex + ey * ey + ez * ez;
       bool in = r2 <= lj.rc2_hi;
+      if (PERPAIR) in = in && lane_ok[s] && jb0 + u <= j_last;
       // within FP32 rounding of the cutoff: the reference's own FP64 test
       if (in && r2 > lj.rc2_lo) in = r2_ref(pos, ia[s], jb0 + u, h, hi) <= lj.rc2d;
       if (PRED) in = in && (j_above ? (jb0 + u <= ia_jmax[s]) : (ia[s] <= jb_jmax[u]));
@@ -507,6 +512,7 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p,
   const double cix = d.center[3 * mi], ciy = d.center[3 * mi + 1], ciz = d.center[3 * mi + 2];
   float rax[NS], ray[NS], raz[NS];
   int ia[NS], iam[NS];  // atom index, its truncation point
+  bool lane_ok[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int a = lane + 64 * s;
@@ -521,6 +527,7 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p,
       iam[s] = INT_MAX;
     }
     ia[s] = a < n ? s0 + a : s0;  // idle lanes never pass the cutoff test
+    lane_ok[s] = a < n;
   }
   double GA[NS][3], WA[NS][9];
   float EA[NS], WD[NS][6];
@@ -574,10 +581,12 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p,
       if (pred) {
 #pragma unroll
         for (int u = 0; u < 4; u++) jm[u] = jj[b + u];
-        lj_group<NS, true, PERPAIR>(rb, jm, t0 + b, cx, cy, cz, ia, iam, j_above, lj, hf, hif,
+        lj_group<NS, true, PERPAIR>(rb, jm, t0 + b, j_last, lane_ok, cx, cy, cz, ia, iam,
+                                    j_above, lj, hf, hif,
                                     d.pos, h, hi, gx, gy, gz, e, wd);
       } else {
-        lj_group<NS, false, PERPAIR>(rb, jm, t0 + b, cx, cy, cz, ia, iam, j_above, lj, hf, hif,
+        lj_group<NS, false, PERPAIR>(rb, jm, t0 + b, j_last, lane_ok, cx, cy, cz, ia, iam,
+                                     j_above, lj, hf, hif,
                                      d.pos, h, hi, gx, gy, gz, e, wd);
       }
     }

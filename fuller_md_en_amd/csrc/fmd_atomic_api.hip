@@ -501,6 +501,7 @@ int fmd_airebo_nlist_rebo(const double *pos, const double *h, const double *hi, 
   if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
   AtomicEngine &A = *e->at;
   hipStream_t s = e->stream;
+  HIPCHK(hipMemsetAsync(A.d.nll_r, 0, (size_t)Na * kRefMaxRebo * sizeof(int), s));
   HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
   HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 1, s));
   HIPCHK(hipMemcpyAsync(nlc, A.d.nlc_r, (size_t)Na * sizeof(int), hipMemcpyDeviceToHost, s));
