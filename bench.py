@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the CPU baseline sample (default: auto, ~20 s)")
+    ap.add_argument("--workload", choices=["rigid", "airebo"], default="rigid",
+                    help="rigid: C60 FCC 10^3 rigid LJ (default); airebo: C60 FCC 8^3 "
+                         "flexible AIREBO (122,880 atoms)")
+    ap.add_argument("--nc", type=int, default=0, help="crystal repeats (default per workload)")
+    ap.add_argument("--rebo", choices=["compat", "physical"], default="compat")
     return ap.parse_args()
 
 
@@ -94,12 +99,89 @@ def cpu_baseline(scale: float, steps: int):
                       f"OMP_NUM_THREADS={cores}, its own 'Done' timer ({secs:.1f} s)"}
 
 
+# AIREBO: 6.94 kFLOP/atom-step for the intermolecular LJ (42 FLOP x 165.3
+# unique in-cutoff pairs per atom, exact list) -- the dominant kernel.
+FLOP_PER_ATOM_STEP_AIREBO_LJ = 42.0 * 165.3
+
+
+def airebo_cpu_baseline(steps: int):
+    """Reference serial AIREBO build (faster than its OpenMP build, which
+    anti-scales), bounded sample at 5x5x5 (30,000 atoms)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "fuller_airebo_npt_md_serial")
+    if not os.path.exists(exe):
+        return None
+    libdir = os.path.join(ROOT, "fuller_md_en_amd", "FullereneLib")
+    cmd = [exe, "--cell=5", "--temp=300", "--pres=0", f"--step={steps}", "--seed=42",
+           f"--libdir={libdir}", f"--mon={steps}"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, cwd="/tmp").stdout
+    m = re.search(r"Done \(([\d.]+) sec\)", out)
+    if not m:
+        return None
+    secs = float(m.group(1))
+    return {"value": 30000 * steps / secs, "unit": "atom-steps/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"fuller_airebo_npt_md_serial --cell=5 --step={steps} (30,000 atoms, "
+                      f"1 core; its O(N^2) list makes larger cells slower per atom), "
+                      f"'Done' timer {secs:.1f} s"}
+
+
+def main_airebo(args):
+    import numpy as np
+    from fuller_md_en_amd import airebo, system
+    nc = args.nc or 8
+    mol = system.molecule("C60")
+    a0 = system.default_a0(mol.dmol, "FCC", 1.0)
+    cen, h = system.make_crystal("FCC", a0, nc)
+    pos = (cen[:, None, :] + mol.coords[None]).reshape(-1, 3)
+    na = len(pos)
+    mol_id = np.repeat(np.arange(1, len(cen) + 1), mol.natom).astype(np.int32)
+    mass = np.full(na, system.MASS_C)
+    vel = system.atomic_init_velocities(mass, SEED, T_K)
+    mode = airebo.REBO_PHYSICAL if args.rebo == "physical" else airebo.REBO_COMPAT
+    eng = airebo.AtomicEngine(mol_id, mass, dt=0.5, mode=mode)
+    eng.set_state(pos=pos, vel=vel, h=h, npt=airebo.make_npt(T_K, 0.0, na))
+    eng.prepare()
+    step0 = 0
+    if args.warmup:
+        eng.run(step0, args.warmup, thermo=False)
+        step0 += args.warmup
+    eng.sync()
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    th = eng.run(step0, args.steps, thermo=True)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms, kn = eng.kernel_time()
+    value = na * args.steps / dt
+    achieved = FLOP_PER_ATOM_STEP_AIREBO_LJ * na / (kms * 1e-3) / 1e12
+    last = th[args.steps - 1]
+    res = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"AIREBO C60 FCC {nc}x{nc}x{nc} ({len(cen)} mol, {na} atoms), "
+                               f"REBO {args.rebo} + LJ (400-entry half-list semantics), "
+                               "300 K, 0 GPa, dt=0.5 fs, NPT", "atoms": na},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "REBO + k_at_lj (force phase)", "kernel_ms": kms, "launches": kn},
+        "final": {"T": last.T, "P": last.P, "E_rebo_per_mol": last.Ep_rebo / len(cen),
+                  "E_lj_per_mol": last.Ep_lj / len(cen)},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = airebo_cpu_baseline(args.cpu_steps or 20)
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "decomposition is not built.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "airebo":
+        return main_airebo(args)
     eng, mol, sysd, scale = build_system()
     n_sites = len(sysd["pos"]) * mol.natom
     eng.prepare()

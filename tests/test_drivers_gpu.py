@@ -1,0 +1,164 @@
+"""The C++ drivers (fuller_LJ_npt_md, fuller_airebo_npt_md) end to end.
+
+* The reference's smoke script (src/Test_fuller.sh:197-279) runs each program
+  with --step=200 --mon=100, plus the OVITO and restart variants, and passes
+  on exit 0 with "Done" in the output; the same invocations here.
+* Output-level parity: the reference programs (built from their sources in
+  oracle/_ref, where available) and these drivers started from identical
+  states -- the same seeded initial state, or the same restart file in either
+  direction -- print the same monitor rows to their printed precision.
+* Restart round trip, stop control (abort.md / stop.md), COLD/WARM phases.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "fuller_md_en_amd", "bin")
+LIB = os.path.join(ROOT, "fuller_md_en_amd", "FullereneLib")
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+pytestmark = pytest.mark.gpu
+
+
+def run(exe, args, cwd, timeout=300):
+    print(f"run {os.path.basename(exe)} {' '.join(args)}", flush=True)
+    r = subprocess.run([exe, *args, f"--libdir={LIB}"], cwd=cwd, capture_output=True, text=True,
+                       timeout=timeout)
+    return r.returncode, r.stdout + r.stderr
+
+
+def rows(out):
+    """Monitor rows: (step, phase, numbers...)."""
+    res = []
+    for line in out.splitlines():
+        m = re.match(r"\s+(\d+)\s+(COLD|WARM|PROD)\s+(.*)$", line)
+        if m:
+            res.append((int(m.group(1)), m.group(2), [float(x) for x in m.group(3).split()]))
+    return res
+
+
+LJ = os.path.join(BIN, "fuller_LJ_npt_md")
+AB = os.path.join(BIN, "fuller_airebo_npt_md")
+
+
+@pytest.mark.parametrize("exe,extra", [
+    (LJ, []), (LJ, ["--ovito=100"]), (LJ, ["--restart=100"]),
+    (AB, []), (AB, ["--ovito=100"]), (AB, ["--restart=100"])])
+def test_smoke_like_reference_script(tmp_path, exe, extra):
+    rc, out = run(exe, ["--step=200", "--mon=100", *extra], tmp_path)
+    assert rc == 0 and "Done" in out, out[-2000:]
+    names = os.listdir(tmp_path)
+    if "--ovito=100" in extra:
+        xyz = [n for n in names if n.startswith("ovito_traj")]
+        assert xyz
+        lines = open(tmp_path / xyz[0]).read().splitlines()
+        nat = int(lines[0])
+        assert lines[1].startswith("Lattice=") and len(lines) == 2 * (nat + 2)
+    if "--restart=100" in extra:
+        rst = sorted(n for n in names if n.endswith(".rst"))
+        assert len(rst) == 2, rst  # step 100 and the final step
+
+
+def _compare(ours, theirs, cols_tol):
+    assert len(ours) == len(theirs) and len(ours) > 0
+    for (s1, p1, a), (s2, p2, b) in zip(ours, theirs):
+        assert s1 == s2 and p1 == p2
+        for k, tol in cols_tol.items():
+            assert abs(a[k] - b[k]) <= tol, (s1, k, a[k], b[k])
+
+
+needs_ref = pytest.mark.skipif(not os.path.exists(os.path.join(REF, "fuller_LJ_npt_md_serial")),
+                               reason="reference build (oracle/_ref) not present")
+
+# printed columns: T %7.1f, P %9.3f, a %8.3f, Ecoh %10.5f  (one unit in the last digit)
+LJ_TOL = {0: 0.11, 1: 0.0011, 2: 0.0011, 3: 1.1e-5}
+
+
+@needs_ref
+def test_lj_driver_rows_equal_reference(tmp_path):
+    args = ["--cell=2", "--temp=300", "--step=60", "--mon=10", "--seed=7"]
+    rc1, ours = run(LJ, args, tmp_path)
+    rc2, theirs = run(os.path.join(REF, "fuller_LJ_npt_md_serial"), args, tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    _compare(rows(ours), rows(theirs), LJ_TOL)
+
+
+@needs_ref
+def test_lj_cold_warm_phases_equal_reference(tmp_path):
+    args = ["--cell=2", "--temp=300", "--coldstart=20", "--warmup=20", "--step=40",
+            "--mon=10", "--warmup_mon=freq"]
+    rc1, ours = run(LJ, args, tmp_path)
+    rc2, theirs = run(os.path.join(REF, "fuller_LJ_npt_md_serial"), args, tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    r1, r2 = rows(ours), rows(theirs)
+    assert {p for _, p, _ in r1} == {"COLD", "WARM", "PROD"}
+    _compare(r1, r2, LJ_TOL)
+
+
+@needs_ref
+def test_restart_files_cross_load(tmp_path):
+    """Our restart resumed by the reference and theirs by us: same next rows."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    base = ["--cell=2", "--temp=300", "--step=40", "--mon=10", "--restart=20", "--seed=11"]
+    ref_exe = os.path.join(REF, "fuller_LJ_npt_md_serial")
+    assert run(LJ, base, a)[0] == 0
+    assert run(ref_exe, base, b)[0] == 0
+    ours = sorted(p for p in os.listdir(a) if p.endswith("_20.rst"))[0]
+    theirs = sorted(p for p in os.listdir(b) if p.endswith("_20.rst"))[0]
+    # the two step-20 states are the same run to within FP32 force rounding
+    r_ours = [l for l in open(a / ours) if l.startswith("MOL")]
+    r_theirs = [l for l in open(b / theirs) if l.startswith("MOL")]
+    va = np.array([[float(x) for x in l.split()[2:]] for l in r_ours])
+    vb = np.array([[float(x) for x in l.split()[2:]] for l in r_theirs])
+    assert np.abs(va[:, :3] - vb[:, :3]).max() < 1e-6  # positions, A
+    cont = ["--cell=2", "--temp=300", "--step=40", "--mon=10", "--seed=11"]
+    _, o1 = run(ref_exe, [*cont, f"--resfile={a / ours}"], tmp_path)
+    _, o2 = run(LJ, [*cont, f"--resfile={b / theirs}"], tmp_path)
+    _, o3 = run(LJ, [*cont, f"--resfile={a / ours}"], tmp_path)
+    assert "Restarting from global step 20" in o1 and "Restarting from global step 20" in o2
+    _compare(rows(o3), rows(o1), LJ_TOL)  # same file, two programs
+    _compare(rows(o2), rows(o1), LJ_TOL)
+
+
+def test_restart_round_trip_continues_the_run(tmp_path):
+    base = ["--cell=2", "--temp=300", "--step=40", "--mon=10", "--restart=20", "--seed=5"]
+    rc, full = run(LJ, base, tmp_path)
+    assert rc == 0
+    rst = sorted(p for p in os.listdir(tmp_path) if p.endswith("_20.rst"))[0]
+    rc, resumed = run(LJ, ["--cell=2", "--temp=300", "--step=40", "--mon=10", "--seed=5",
+                           f"--resfile={tmp_path / rst}"], tmp_path)
+    assert rc == 0
+    r_full = [r for r in rows(full) if r[0] > 20]
+    _compare(rows(resumed), r_full, LJ_TOL)
+
+
+def test_stop_control(tmp_path):
+    os.mkdir(tmp_path / "abort.md")
+    rc, out = run(LJ, ["--cell=2", "--step=100", "--mon=10", "--restart=50"], tmp_path)
+    assert rc == 0 and "abort.md detected at step 10" in out and "Done" in out
+    assert any(p.endswith("_010.rst") for p in os.listdir(tmp_path))  # 3 digits of 100
+    shutil.rmtree(tmp_path / "abort.md")
+    os.mkdir(tmp_path / "stop.md")
+    rc, out = run(LJ, ["--cell=2", "--step=100", "--mon=10", "--restart=50"], tmp_path)
+    assert "stop.md detected at step 10" in out and "Stopped at restart checkpoint (step 50)" in out
+    rc, out = run(AB, ["--cell=1", "--step=100", "--mon=10"], tmp_path)
+    assert "stop.md detected at step 10" in out and "Done" in out
+
+
+AB_TOL = {0: 0.11, 1: 0.0011, 2: 0.0011, 3: 1.1e-4, 4: 1.1e-4, 5: 1.1e-4}
+
+
+@needs_ref
+def test_airebo_driver_rows_equal_reference(tmp_path):
+    args = ["--cell=2", "--temp=300", "--step=20", "--mon=5", "--seed=3"]
+    rc1, ours = run(AB, args, tmp_path)
+    rc2, theirs = run(os.path.join(REF, "fuller_airebo_npt_md_serial"), args, tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    _compare(rows(ours), rows(theirs), AB_TOL)
