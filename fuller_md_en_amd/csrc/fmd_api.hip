@@ -113,6 +113,7 @@ void fmd_rigid_config_default(fmd_rigid_config *c, double rmol) {
   c->dt = 1.0;
   c->clamp_xi = 0.1;
   c->clamp_vg = 0.01;
+  c->half_pairs = 1;
 }
 
 }  // extern "C"
@@ -121,9 +122,11 @@ void fmd_rigid_config_default(fmd_rigid_config *c, double rmol) {
 namespace fmd {
 
 static void free_rigid(RigidDev &d) {
-  void *ptrs[] = {d.pos,      d.vel,      d.omg,   d.qv,    d.Fv,   d.Tv,     d.sites, d.sites_d,
-                  d.body,     d.hh,       d.nl_count, d.nl_list, d.flags, d.molpart,
-                  d.blockpart, d.state,   d.mid,   d.ctrl,  d.thermo};
+  void *ptrs[] = {d.pos,      d.vel,      d.omg,      d.qv,       d.Fv,       d.Tv,
+                  d.sites,    d.sites_d,  d.body,     d.qbody,    d.qcen,     d.hh,
+                  d.nl_count, d.nl_list,  d.hl_count, d.hl_list,  d.in_count, d.in_list,
+                  d.pairout,  d.flags,    d.molpart,  d.blockpart, d.state,   d.mid,
+                  d.ctrl,     d.thermo};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   memset(&d, 0, sizeof d);
@@ -137,7 +140,7 @@ static hipError_t dalloc(T **p, size_t n) {
 int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   memset(&d, 0, sizeof d);
   const size_t N = p.N;
-  d.nblk = (p.N + 255) / 256;
+  d.nblk = (p.N + 3) / 4;  // k_rigid_kick2: one wavefront per molecule
   hipError_t e = hipSuccess;
 #define AL(ptr, n) \
   if (e == hipSuccess) e = dalloc(&(ptr), (n))
@@ -153,6 +156,15 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   AL(d.hh, 18);
   AL(d.nl_count, N);
   AL(d.nl_list, N * p.max_neigh);
+  if (p.use_half) {
+    AL(d.qbody, 4 * (size_t)p.nq);
+    AL(d.qcen, N * p.nq);
+    AL(d.hl_count, N);
+    AL(d.hl_list, N * p.max_neigh);
+    AL(d.in_count, N);
+    AL(d.in_list, N * p.max_neigh);
+    AL(d.pairout, 6 * N * p.max_neigh);
+  }
   AL(d.flags, 4);
   AL(d.molpart, N * kMolPart);
   AL(d.blockpart, (size_t)d.nblk * kBlkPart);
@@ -173,9 +185,39 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   return FMD_OK;
 }
 
+// Body sites regrouped into spatially compact quads (groups of 4 consecutive
+// device sites) for the half-pair kernel's per-quad cull.  Greedy: the site
+// farthest from the centroid of those left, then its 3 nearest left.  The
+// order is internal to the device: sums over sites do not depend on it.
+static void quad_order(const double *body, int natom, std::vector<int> &perm) {
+  std::vector<int> left(natom);
+  for (int a = 0; a < natom; a++) left[a] = a;
+  perm.clear();
+  auto d2 = [&](int a, const double *c) {
+    double x = body[3 * a] - c[0], y = body[3 * a + 1] - c[1], z = body[3 * a + 2] - c[2];
+    return x * x + y * y + z * z;
+  };
+  while (!left.empty()) {
+    double c[3] = {0, 0, 0};
+    for (int a : left)
+      for (int k = 0; k < 3; k++) c[k] += body[3 * a + k] / left.size();
+    size_t far = 0;
+    for (size_t t = 1; t < left.size(); t++)
+      if (d2(left[t], c) > d2(left[far], c)) far = t;
+    const int s0 = left[far];
+    std::stable_sort(left.begin(), left.end(), [&](int x, int y) {
+      return d2(x, body + 3 * s0) < d2(y, body + 3 * s0);
+    });
+    const size_t take = std::min<size_t>(4, left.size());
+    for (size_t t = 0; t < take; t++) perm.push_back(left[t]);
+    left.erase(left.begin(), left.begin() + take);
+  }
+}
+
 int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0, double Mmol,
                  const fmd_lj_params &lj, double rmcut2, double rlist, int max_neigh,
-                 double dt, double cxi, double cvg, std::vector<double> &body4) {
+                 double dt, double cxi, double cvg, std::vector<double> &body4,
+                 std::vector<double> *qbody) {
   if (N < 1 || natom < 1 || natom > 128 || max_neigh < 1)
     return set_error(FMD_ERR_ARG, "rigid: need N >= 1, 1 <= natom <= 128, max_neigh >= 1");
   memset(&p, 0, sizeof p);
@@ -192,19 +234,41 @@ int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0
   p.clamp_vg = cvg;
   p.lj = make_lj32(lj);
   p.eps = lj.eps;
+  std::vector<int> perm(natom);
+  for (int a = 0; a < natom; a++) perm[a] = a;
+  if (qbody) quad_order(body, natom, perm);
   double rb = 0;
   body4.assign(4 * (size_t)p.natom_pad, 0.0);
   for (int a = 0; a < p.natom_pad; a++) {
     if (a < natom) {
-      for (int k = 0; k < 3; k++) body4[4 * a + k] = body[3 * a + k];
-      rb = std::max(rb, std::sqrt(body[3 * a] * body[3 * a] + body[3 * a + 1] * body[3 * a + 1] +
-                                  body[3 * a + 2] * body[3 * a + 2]));
+      const double *b = body + 3 * perm[a];
+      for (int k = 0; k < 3; k++) body4[4 * a + k] = b[k];
+      rb = std::max(rb, std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]));
     } else {
       body4[4 * a + 3] = 1.0;
     }
   }
   const double cr = lj.rcut + rb + 0.05;  // FP32 margin
   p.cull_r2 = cr * cr;
+  if (qbody) {
+    p.use_half = 1;
+    p.nq = p.natom_pad / 4;
+    p.reach = (float)cr;
+    qbody->assign(4 * (size_t)p.nq, 0.0);
+    for (int q = 0; q < p.nq; q++) {
+      double c[3] = {0, 0, 0}, r = 0;
+      int n = 0;
+      for (int a = 4 * q; a < 4 * q + 4 && a < natom; a++, n++)
+        for (int k = 0; k < 3; k++) c[k] += body4[4 * a + k];
+      for (int k = 0; k < 3; k++) c[k] /= n;
+      for (int a = 4 * q; a < 4 * q + 4 && a < natom; a++) {
+        double x = body4[4 * a] - c[0], y = body4[4 * a + 1] - c[1], z = body4[4 * a + 2] - c[2];
+        r = std::max(r, std::sqrt(x * x + y * y + z * z));
+      }
+      for (int k = 0; k < 3; k++) (*qbody)[4 * q + k] = c[k];
+      (*qbody)[4 * q + 3] = r;
+    }
+  }
   return FMD_OK;
 }
 
@@ -229,10 +293,10 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
     return set_error(FMD_ERR_NODEVICE, "fmd_rigid_create: no HIP device");
   fmd_engine *e = new fmd_engine();
   e->kind = FMD_KIND_RIGID;
-  std::vector<double> body4;
+  std::vector<double> body4, qbody;
   int rc = rigid_params(e->rp, c->n_mol, c->natom, c->body, c->I0, c->Mmol, c->lj,
                         c->rmcut * c->rmcut, c->rmcut + c->skin, c->max_neigh, c->dt,
-                        c->clamp_xi, c->clamp_vg, body4);
+                        c->clamp_xi, c->clamp_vg, body4, c->half_pairs ? &qbody : nullptr);
   if (rc == FMD_OK) rc = engine_init_common(e, 4096);
   if (rc == FMD_OK) rc = rigid_alloc(e->rd, e->rp, e->ring);
   if (rc != FMD_OK) {
@@ -242,6 +306,9 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
   e->nlup = c->nlup > 0 ? c->nlup : 25;
   hipError_t he = hipMemcpy(e->rd.body, body4.data(), body4.size() * sizeof(double),
                             hipMemcpyHostToDevice);
+  if (he == hipSuccess && e->rp.use_half)
+    he = hipMemcpy(e->rd.qbody, qbody.data(), qbody.size() * sizeof(double),
+                   hipMemcpyHostToDevice);
   if (he != hipSuccess) {
     fmd_engine_destroy(e);
     return hip_fail(he, "upload body");
@@ -320,6 +387,7 @@ static int rigid_prepare(fmd_engine *e) {
   const double *hh = state_hh(d);
   HIPCHK(rigid_launch_apply_pbc(d, p, hh, s));
   HIPCHK(rigid_launch_nlist(d, p, hh, s));
+  if (p.use_half) HIPCHK(rigid_launch_halflist(d, p, s));
   HIPCHK(rigid_launch_sites(d, p, s));
   HIPCHK(rigid_launch_forces(d, p, hh, s));
   HIPCHK(rigid_launch_observe(d, p, kObsKE | kObsForce, s));
@@ -431,6 +499,25 @@ int fmd_engine_forces(fmd_engine *e, double *F, double *T, double *W9, double *e
   RigidDev &d = e->rd;
   const size_t N = e->rp.N;
   hipStream_t s = e->stream;
+  if (e->rp.use_half && e->steps > 0) {
+    // steps keep only the virial trace: the full tensor from the both-sides
+    // kernel at the current state
+    double out10[kMolPart];
+    HIPCHK(rigid_launch_forces(d, e->rp, state_hh(d), s));
+    HIPCHK(rigid_launch_reduce_molpart(d, e->rp, d.blockpart, s));
+    if (F) HIPCHK(hipMemcpyAsync(F, d.Fv, 3 * N * 8, hipMemcpyDeviceToHost, s));
+    if (T) HIPCHK(hipMemcpyAsync(T, d.Tv, 3 * N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(out10, d.blockpart, sizeof out10, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (W9)
+      for (int k = 0; k < 9; k++) W9[k] = out10[k];
+    if (energies) {
+      energies[0] = out10[9];
+      energies[1] = 0;
+      energies[2] = out10[9];
+    }
+    return FMD_OK;
+  }
   if (F) HIPCHK(hipMemcpyAsync(F, d.Fv, 3 * N * 8, hipMemcpyDeviceToHost, s));
   if (T) HIPCHK(hipMemcpyAsync(T, d.Tv, 3 * N * 8, hipMemcpyDeviceToHost, s));
   RigidState S;
@@ -513,6 +600,7 @@ int rigid_workspace(int N, int natom, int max_neigh, const double *body, double 
     g_rws = RigidWorkspace();
     fmd_rigid_config c;
     fmd_rigid_config_default(&c, 0.0);
+    c.half_pairs = 0;
     c.n_mol = N;
     c.natom = natom;
     c.body = body;
@@ -529,7 +617,7 @@ int rigid_workspace(int N, int natom, int max_neigh, const double *body, double 
   fmd_engine *e = g_rws.e;
   std::vector<double> body4;
   int rc = rigid_params(e->rp, N, natom, body, I0, Mmol, *lj, rmcut2, 0.0, max_neigh, dt, 0.1,
-                        0.01, body4);
+                        0.01, body4, nullptr);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(e->rd.body, body4.data(), body4.size() * sizeof(double),
                         hipMemcpyHostToDevice, e->stream));

@@ -63,11 +63,14 @@ struct Thermo {
 
 struct RigidParams {
   int N, natom, natom_pad, max_neigh;
+  int nq;             // site quads per molecule (natom_pad / 4)
+  int use_half;       // time stepping with the half-pair kernel (trace virial)
   double rmcut2;      // molecule-pair cutoff^2 inside the force loop
   double rlist2;      // list radius^2 (rmcut + skin)^2
   double dt, I0, Mmol;
   double clamp_xi, clamp_vg;
   double cull_r2;     // (rc + max site radius)^2: per-lane neighbour cull
+  float reach;        // rc + max site radius + FP32 margin: quad cull radius
   LJ32 lj;
   double eps;         // LJ epsilon (energy scale of the reduced sums)
 };
@@ -79,14 +82,20 @@ struct RigidDev {
   double *sites_d;  // N*natom_pad*3: the same offsets in FP64, bitwise the
                     // reference's lab array, for cutoff re-tests
   double *body;  // natom_pad x 4 doubles: x, y, z, padding flag (1 = padding)
+  double *qbody; // nq x 4 doubles: body-frame quad centre and radius
+  float4 *qcen;  // N*nq: lab-frame quad centre offsets (w = quad radius)
   double *hh;    // 18 doubles: h, h^-1 for the stateless entry points
   int *nl_count, *nl_list, *flags;  // flags[0]: list overflow count
+  // pairs owned by each molecule (half list) and, per molecule, the slots of
+  // the pairs owned by its neighbours that carry its force / torque
+  int *hl_count, *hl_list, *in_count, *in_list;
+  double *pairout;  // N*max_neigh*6: force and torque on the non-owner
   double *molpart, *blockpart;
   RigidState *state;
   RigidMid *mid;
   StepCtrl *ctrl;     // ring of per-step controls
   Thermo *thermo;     // ring of per-step records
-  int nblk;           // blocks of the per-molecule integrator kernels
+  int nblk;           // blocks of k_rigid_kick2 (4 molecules each): partials in blockpart
 };
 
 // Launchers (rigid_kernels.hip).  All enqueue on `s` and return the HIP
@@ -97,6 +106,8 @@ hipError_t rigid_launch_apply_pbc(const RigidDev &d, const RigidParams &p, const
                                   hipStream_t s);
 hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const double *hh,
                               hipStream_t s);
+// Half list and incoming slots from the symmetric list.
+hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, hipStream_t s);
 hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const double *hh,
                                hipStream_t s);
 // Observables of the current state into RigidState: what = kObsKE | kObsForce.

@@ -36,11 +36,33 @@ __device__ __forceinline__ double clampd(double x, double lo, double hi) {
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Sum of a double over the 64 lanes of a wavefront (butterfly, fixed order).
+// One DPP lane permutation of a double (both halves), all rows and banks.
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double readlane64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum of a double over the 64 lanes of a wavefront, the same value in every
+// lane and a fixed order: DPP pairings inside each row of 16 (lane ^ 1,
+// lane ^ 2, 8-lane and 16-lane mirrors; each pairing adds two equal-size
+// partial sums, so both partners get bitwise the same value), then the four
+// row sums.  Must be called with the whole wavefront active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
+  v += dpp64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp64<0x141>(v);  // row_half_mirror
+  v += dpp64<0x140>(v);  // row_mirror
+  return (readlane64(v, 0) + readlane64(v, 16)) + (readlane64(v, 32) + readlane64(v, 48));
 }
 
 // Lab-frame offsets of the sites of molecule i (lanes a and a+64): FP32 for
@@ -66,6 +88,17 @@ __device__ __forceinline__ void write_sites(int i, const double *q, const double
       sites_d[3 * t + 1] = o[1];
       sites_d[3 * t + 2] = o[2];
     }
+  }
+}
+
+// Lab-frame centres of the site quads (lanes q < nq), w = quad radius.
+__device__ __forceinline__ void write_quads(int i, const double *q, const double *qbody,
+                                            float4 *qcen, int nq, int lane) {
+  if (lane < nq) {
+    double o[3];
+    lab_exact(q, qbody + 4 * lane, o);
+    qcen[(size_t)i * nq + lane] =
+        make_float4((float)o[0], (float)o[1], (float)o[2], (float)qbody[4 * lane + 3]);
   }
 }
 
@@ -99,18 +132,63 @@ __global__ __launch_bounds__(64) void k_rigid_nlist(const double *__restrict__ p
   }
 }
 
+// Ownership of a molecule pair: i owns (i, j) iff (j - i) mod N < N / 2 (ties
+// to the lower index), so every molecule owns about half of its neighbours
+// whatever its index.
+__device__ __forceinline__ bool owns(int i, int j, int N) {
+  const int dlt = (j - i + N) % N;
+  return 2 * dlt < N || (2 * dlt == N && i < j);
+}
+
+// From the symmetric list: the owned neighbours of i (hl, in list order) and
+// the pairout slots (owner * max_neigh + position in its half list) of the
+// pairs owned by i's neighbours, where i's force and torque arrive.
+__global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int cnt = d.nl_count[i], cap = p.max_neigh;
+  int nh = 0, nin = 0;
+  for (int k0 = 0; k0 < cnt; k0 += 64) {
+    const int k = k0 + lane;
+    const int j = k < cnt ? d.nl_list[(size_t)i * cap + k] : -1;
+    const bool mine = j >= 0 && owns(i, j, p.N);
+    bool in = false;
+    int slot = 0;
+    if (j >= 0 && !mine) {  // position of i among j's owned neighbours
+      const int cj = d.nl_count[j];
+      int r = 0;
+      for (int t = 0; t < cj; t++) {
+        const int jj = d.nl_list[(size_t)j * cap + t];
+        if (jj == i) {
+          in = true;
+          break;
+        }
+        if (owns(j, jj, p.N)) r++;
+      }
+      slot = j * cap + r;
+    }
+    unsigned long long bm = __ballot(mine), bi = __ballot(in);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (mine) d.hl_list[(size_t)i * cap + nh + __popcll(bm & below)] = j;
+    if (in) d.in_list[(size_t)i * cap + nin + __popcll(bi & below)] = slot;
+    nh += __popcll(bm);
+    nin += __popcll(bi);
+    if (j >= 0 && !mine && !in) atomicAdd(&d.flags[0], 1);  // asymmetric (overflowed) list
+  }
+  if (lane == 0) {
+    d.hl_count[i] = nh;
+    d.in_count[i] = nin;
+  }
+}
+
 // ------------------------------------------------------------------ sites
 template <int NS>
-__global__ __launch_bounds__(256) void k_rigid_sites(const double *__restrict__ qv,
-                                                     const double *__restrict__ body,
-                                                     float4 *__restrict__ sites,
-                                                     double *__restrict__ sites_d,
-                                                     RigidParams p) {
+__global__ __launch_bounds__(256) void k_rigid_sites(RigidDev d, RigidParams p) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= p.N) return;
-  double q[4] = {qv[4 * i], qv[4 * i + 1], qv[4 * i + 2], qv[4 * i + 3]};
-  write_sites<NS>(i, q, body, sites, sites_d, p.natom_pad, lane);
+  double q[4] = {d.qv[4 * i], d.qv[4 * i + 1], d.qv[4 * i + 2], d.qv[4 * i + 3]};
+  write_sites<NS>(i, q, d.body, d.sites, d.sites_d, p.natom_pad, lane);
+  write_quads(i, q, d.qbody, d.qcen, p.nq, lane);
 }
 
 __global__ __launch_bounds__(256) void k_rigid_apply_pbc(double *__restrict__ pos,
@@ -224,6 +302,7 @@ __global__ __launch_bounds__(256) void k_rigid_kick_drift(RigidDev d, RigidParam
   }
   for (int a = 0; a < 4; a++) q[a] = __shfl(q[a], 0, 64);
   write_sites<NS>(i, q, d.body, d.sites, d.sites_d, p.natom_pad, lane);
+  write_quads(i, q, d.qbody, d.qcen, p.nq, lane);
 }
 
 // ------------------------------------------------------------------ forces
@@ -389,6 +468,184 @@ __global__ __launch_bounds__(256) void k_rigid_forces(
   }
 }
 
+// Half-pair variant for time stepping: every molecule pair once, by its
+// owner.  Per (lane a, owned neighbour j) the FP32 sum G_aj over j's sites as
+// in k_rigid_forces gives i's force and torque; the same G_aj summed over the
+// wavefront gives j's: F_j = sum_a G_aj and, since every pair force is along
+// its pair vector, tau_j = sum_a r_a x G_aj - dm x F_j (total torque of the
+// pair interaction about any point vanishes).  Only neighbour sites in quads
+// within reach of molecule i are visited.  The virial is the trace
+// sum_pairs d.f (what the pressure needs); energy is per pair, no 1/2.
+template <int NS>
+__global__ __launch_bounds__(256) void k_rigid_forces_half(
+    const double *__restrict__ pos, const float4 *__restrict__ sites,
+    const double *__restrict__ sites_d, const float4 *__restrict__ qcen,
+    const int *__restrict__ hl_count, const int *__restrict__ hl_list,
+    const double *__restrict__ hh, double *__restrict__ Fv, double *__restrict__ Tv,
+    double *__restrict__ molpart, double *__restrict__ pairout, RigidParams p) {
+  const int per = (p.N + kXcd - 1) / kXcd;
+  const int bid = blockIdx.x;
+  const int i = (bid % kXcd) * per + bid / kXcd;
+  if (i >= p.N) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  const double *h = hh, *hi = hh + 9;
+  const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+  const int npad = p.natom_pad, nq = p.nq;
+  float rax[NS], ray[NS], raz[NS];
+  double rad[NS][3];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    int a = lane + 64 * s;
+    if (a < p.natom) {
+      const size_t t = (size_t)i * npad + a;
+      rad[s][0] = sites_d[3 * t];
+      rad[s][1] = sites_d[3 * t + 1];
+      rad[s][2] = sites_d[3 * t + 2];
+    } else {
+      rad[s][0] = rad[s][1] = rad[s][2] = -3.0e5;
+    }
+    rax[s] = (float)rad[s][0];
+    ray[s] = (float)rad[s][1];
+    raz[s] = (float)rad[s][2];
+  }
+  double GA[NS][3];
+  double EA = 0, TA = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) GA[s][0] = GA[s][1] = GA[s][2] = 0;
+  const LJ32 lj = p.lj;
+  const double c48 = 48.0 * p.eps;
+  const int nn = hl_count[i];
+  const int *li = hl_list + (size_t)i * p.max_neigh;
+  for (int k = wv; k < nn; k += 4) {
+    const int j = uniform(li[k]);
+    double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
+    mimg_exact(dx, dy, dz, hi, h);
+    unsigned long long qmask = 0;
+    if (dx * dx + dy * dy + dz * dz <= p.rmcut2) {
+      bool near = false;
+      if (lane < nq) {
+        const float4 c = qcen[(size_t)j * nq + lane];
+        const float qx = (float)dx + c.x, qy = (float)dy + c.y, qz = (float)dz + c.z;
+        const float rr = p.reach + c.w;
+        near = qx * qx + qy * qy + qz * qz < rr * rr;
+      }
+      qmask = __ballot(near);
+    }
+    double* out = pairout + ((size_t)i * p.max_neigh + k) * 6;
+    if (qmask == 0) {
+      if (lane < 6) out[lane] = 0.0;
+      continue;
+    }
+    const double dm[3] = {dx, dy, dz};
+    float cx[NS], cy[NS], cz[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      cx[s] = (float)(dx - rad[s][0]);
+      cy[s] = (float)(dy - rad[s][1]);
+      cz[s] = (float)(dz - rad[s][2]);
+    }
+    float gx[NS], gy[NS], gz[NS], e = 0.f, tr = 0.f;
+#pragma unroll
+    for (int s = 0; s < NS; s++) gx[s] = gy[s] = gz[s] = 0.f;
+    const float4 *sj = sites + (size_t)j * npad;
+    while (qmask) {
+      const int q = __builtin_ctzll(qmask);
+      qmask &= qmask - 1ull;
+      float4 rb[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) rb[u] = sj[4 * q + u];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          const float ex = rb[u].x + cx[s], ey = rb[u].y + cy[s], ez = rb[u].z + cz[s];
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          if (r2 < lj.rc2_hi) {
+            bool in = r2 < lj.rc2_lo;
+            if (!in)
+              in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + 4 * q + u), rad[s]) <
+                   lj.rc2d;
+            if (in) {
+              const float fm = lj_fm(r2, lj, e);
+              tr += fm * r2;
+              gx[s] += fm * ex;
+              gy[s] += fm * ey;
+              gz[s] += fm * ez;
+            }
+          }
+        }
+      }
+    }
+    double fj[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const double g0 = c48 * gx[s], g1 = c48 * gy[s], g2 = c48 * gz[s];
+      GA[s][0] += g0;
+      GA[s][1] += g1;
+      GA[s][2] += g2;
+      fj[0] += g0;
+      fj[1] += g1;
+      fj[2] += g2;
+      tq[0] += rad[s][1] * g2 - rad[s][2] * g1;
+      tq[1] += rad[s][2] * g0 - rad[s][0] * g2;
+      tq[2] += rad[s][0] * g1 - rad[s][1] * g0;
+    }
+    EA += (double)e;
+    TA += (double)tr;
+    for (int c = 0; c < 3; c++) {
+      fj[c] = wave_sum(fj[c]);
+      tq[c] = wave_sum(tq[c]);
+    }
+    if (lane == 0) {
+      out[0] = fj[0];
+      out[1] = fj[1];
+      out[2] = fj[2];
+      out[3] = tq[0] - (dy * fj[2] - dz * fj[1]);
+      out[4] = tq[1] - (dz * fj[0] - dx * fj[2]);
+      out[5] = tq[2] - (dx * fj[1] - dy * fj[0]);
+    }
+  }
+  // i's own share: F(3) T(3) E tr
+  double acc[8];
+  for (int k = 0; k < 8; k++) acc[k] = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const double f0 = -GA[s][0], f1 = -GA[s][1], f2 = -GA[s][2];
+    acc[0] += f0;
+    acc[1] += f1;
+    acc[2] += f2;
+    acc[3] += rad[s][1] * f2 - rad[s][2] * f1;
+    acc[4] += rad[s][2] * f0 - rad[s][0] * f2;
+    acc[5] += rad[s][0] * f1 - rad[s][1] * f0;
+  }
+  acc[6] = EA;
+  acc[7] = TA;
+  __shared__ double red[4][8];
+  for (int k = 0; k < 8; k++) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int k = threadIdx.x;
+    const double v = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    if (k < 3)
+      Fv[3 * i + k] = v;
+    else if (k < 6)
+      Tv[3 * i + k - 3] = v;
+    else if (k == 6)
+      molpart[(size_t)i * kMolPart + 9] = 4.0 * p.eps * v;
+    else {
+      // trace only: W = tr/3 on the diagonal (the pressure is all the step uses)
+      const double t3 = c48 * v / 3.0;
+      double *m = molpart + (size_t)i * kMolPart;
+      m[0] = m[4] = m[8] = t3;
+      m[1] = m[2] = m[3] = m[5] = m[6] = m[7] = 0.0;
+    }
+  }
+}
+
 // ------------------------------------------------------ kick 2 + partials
 // Block sums of 12 doubles in a fixed order.
 __device__ __forceinline__ void block_sum12(double *v, double *out) {
@@ -405,34 +662,60 @@ __device__ __forceinline__ void block_sum12(double *v, double *out) {
   }
 }
 
+// Second half-kick, one wavefront per molecule.  In pair-once mode the
+// molecule's force and torque are its own share plus the shares its
+// neighbours computed for it (pairout slots listed in in_list), gathered by
+// the lanes and summed in a fixed order.  Block partials of KE, W, E follow.
 __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, int kick) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   double v[kBlkPart];
   for (int k = 0; k < kBlkPart; k++) v[k] = 0;
   if (i < p.N) {
-    double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
-    double wx = d.omg[3 * i], wy = d.omg[3 * i + 1], wz = d.omg[3 * i + 2];
-    if (kick) {
-      const RigidMid &M = *d.mid;
-      const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
-      double eps_tr2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
-      double sc_v2 = M.sc_nh * exp(-hdt * eps_tr2 / 3.0);
-      vx = (vx + hdt * d.Fv[3 * i] * cF) * sc_v2;
-      vy = (vy + hdt * d.Fv[3 * i + 1] * cF) * sc_v2;
-      vz = (vz + hdt * d.Fv[3 * i + 2] * cF) * sc_v2;
-      wx = (wx + hdt * d.Tv[3 * i] * cT) * M.sc_nh;
-      wy = (wy + hdt * d.Tv[3 * i + 1] * cT) * M.sc_nh;
-      wz = (wz + hdt * d.Tv[3 * i + 2] * cT) * M.sc_nh;
-      d.vel[3 * i] = vx;
-      d.vel[3 * i + 1] = vy;
-      d.vel[3 * i + 2] = vz;
-      d.omg[3 * i] = wx;
-      d.omg[3 * i + 1] = wy;
-      d.omg[3 * i + 2] = wz;
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    if (kick && p.use_half) {
+      const int nin = d.in_count[i];
+      for (int m = lane; m < nin; m += 64) {
+        const double *o = d.pairout + (size_t)d.in_list[(size_t)i * p.max_neigh + m] * 6;
+        for (int c = 0; c < 6; c++) f[c] += o[c];
+      }
+      for (int c = 0; c < 6; c++) f[c] = wave_sum(f[c]);
     }
-    v[0] = vx * vx + vy * vy + vz * vz;
-    v[1] = wx * wx + wy * wy + wz * wz;
-    for (int k = 0; k < kMolPart; k++) v[2 + k] = d.molpart[(size_t)i * kMolPart + k];
+    if (lane == 0) {
+      double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
+      double wx = d.omg[3 * i], wy = d.omg[3 * i + 1], wz = d.omg[3 * i + 2];
+      if (kick) {
+        double F[3], T[3];
+        for (int c = 0; c < 3; c++) {
+          F[c] = d.Fv[3 * i + c] + f[c];
+          T[c] = d.Tv[3 * i + c] + f[3 + c];
+        }
+        if (p.use_half)
+          for (int c = 0; c < 3; c++) {
+            d.Fv[3 * i + c] = F[c];
+            d.Tv[3 * i + c] = T[c];
+          }
+        const RigidMid &M = *d.mid;
+        const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
+        double eps_tr2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
+        double sc_v2 = M.sc_nh * exp(-hdt * eps_tr2 / 3.0);
+        vx = (vx + hdt * F[0] * cF) * sc_v2;
+        vy = (vy + hdt * F[1] * cF) * sc_v2;
+        vz = (vz + hdt * F[2] * cF) * sc_v2;
+        wx = (wx + hdt * T[0] * cT) * M.sc_nh;
+        wy = (wy + hdt * T[1] * cT) * M.sc_nh;
+        wz = (wz + hdt * T[2] * cT) * M.sc_nh;
+        d.vel[3 * i] = vx;
+        d.vel[3 * i + 1] = vy;
+        d.vel[3 * i + 2] = vz;
+        d.omg[3 * i] = wx;
+        d.omg[3 * i + 1] = wy;
+        d.omg[3 * i + 2] = wz;
+      }
+      v[0] = vx * vx + vy * vy + vz * vz;
+      v[1] = wx * wx + wy * wy + wz * wz;
+      for (int k = 0; k < kMolPart; k++) v[2 + k] = d.molpart[(size_t)i * kMolPart + k];
+    }
   }
   block_sum12(v, d.blockpart + (size_t)blockIdx.x * kBlkPart);
 }
@@ -560,9 +843,9 @@ inline int ns_of(const RigidParams &p) { return p.natom <= 64 ? 1 : 2; }
 hipError_t rigid_launch_sites(const RigidDev &d, const RigidParams &p, hipStream_t s) {
   dim3 g((p.N + 3) / 4);
   if (ns_of(p) == 1)
-    k_rigid_sites<1><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, d.sites_d, p);
+    k_rigid_sites<1><<<g, 256, 0, s>>>(d, p);
   else
-    k_rigid_sites<2><<<g, 256, 0, s>>>(d.qv, d.body, d.sites, d.sites_d, p);
+    k_rigid_sites<2><<<g, 256, 0, s>>>(d, p);
   return hipGetLastError();
 }
 
@@ -591,6 +874,24 @@ hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const do
   return hipGetLastError();
 }
 
+hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, hipStream_t s) {
+  k_rigid_halflist<<<p.N, 64, 0, s>>>(d, p);
+  return hipGetLastError();
+}
+
+static hipError_t rigid_launch_forces_half(const RigidDev &d, const RigidParams &p,
+                                           const double *hh, hipStream_t s) {
+  const int per = (p.N + kXcd - 1) / kXcd;
+  dim3 g(per * kXcd);
+  if (ns_of(p) == 1)
+    k_rigid_forces_half<1><<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.qcen, d.hl_count,
+                                             d.hl_list, hh, d.Fv, d.Tv, d.molpart, d.pairout, p);
+  else
+    k_rigid_forces_half<2><<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.qcen, d.hl_count,
+                                             d.hl_list, hh, d.Fv, d.Tv, d.molpart, d.pairout, p);
+  return hipGetLastError();
+}
+
 hipError_t rigid_launch_observe(const RigidDev &d, const RigidParams &p, int what,
                                 hipStream_t s) {
   k_rigid_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0);
@@ -608,6 +909,7 @@ hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_s
       reinterpret_cast<const char *>(d.mid) + offsetof(RigidMid, h));
   if (rebuild) {
     if ((e = rigid_launch_nlist(d, p, state_hh, s)) != hipSuccess) return e;
+    if (p.use_half && (e = rigid_launch_halflist(d, p, s)) != hipSuccess) return e;
   }
   dim3 g((p.N + 3) / 4);
   if (ns_of(p) == 1)
@@ -616,7 +918,11 @@ hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_s
     k_rigid_kick_drift<2><<<g, 256, 0, s>>>(d, p, ctrl_slot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-  if ((e = rigid_launch_forces(d, p, mid_hh, s)) != hipSuccess) return e;
+  if (p.use_half)
+    e = rigid_launch_forces_half(d, p, mid_hh, s);
+  else
+    e = rigid_launch_forces(d, p, mid_hh, s);
+  if (e != hipSuccess) return e;
   if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
   k_rigid_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
   k_rigid_finish<<<1, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);

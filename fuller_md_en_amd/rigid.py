@@ -93,7 +93,7 @@ class RigidEngine:
     """HBM-resident rigid-body NPT engine (fmd_rigid_create / fmd_engine_*)."""
 
     def __init__(self, body, n_mol, I0, Mmol, rmol, dt=1.0, lj=None, nlup=25,
-                 max_neigh=MAX_NEIGH, rmcut=None, skin=3.0):
+                 max_neigh=MAX_NEIGH, rmcut=None, skin=3.0, half_pairs=True):
         self.body = _c(body).reshape(-1, 3)
         cfg = N.RigidConfig()
         N.lib().fmd_rigid_config_default(C.byref(cfg), rmol)
@@ -103,6 +103,7 @@ class RigidEngine:
         cfg.body = self._body_flat.ctypes.data_as(C.POINTER(C.c_double))
         cfg.I0, cfg.Mmol, cfg.dt, cfg.nlup, cfg.max_neigh = I0, Mmol, dt, nlup, max_neigh
         cfg.skin = skin
+        cfg.half_pairs = int(half_pairs)
         if lj is not None:
             cfg.lj = lj
             cfg.rmcut = lj.rcut + 2 * rmol + 1.0

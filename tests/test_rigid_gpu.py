@@ -126,6 +126,56 @@ def test_engine_trajectory_matches_reference():
     eng.close()
 
 
+@pytest.mark.parametrize("half", [True, False])
+@pytest.mark.parametrize("tag", ["C60", "C70", "C84"])
+def test_engine_steps_match_reference(tag, half):
+    """Resident engine, both pair modes, against the reference's 3 NPT steps.
+
+    C70 leaves a last site quad with two real sites; C84 (84 sites) runs the
+    two-sites-per-lane kernels."""
+    g = golden(f"rigid_cc1_{tag}.npz")
+    print(f"case {tag} half={half}", flush=True)
+    from fuller_md_en_amd._native import NPTState
+    eng = rigid.RigidEngine(g["body"], len(g["pos"]), float(g["I0"]), float(g["Mmol"]),
+                            rmol=float(g["Rmol"]), rmcut=float(g["rmcut"]), half_pairs=half)
+    eng.set_state(pos=g["pos"], vel=g["vel"], qv=g["qv"], omg=g["omg"], h=g["h"],
+                  npt=NPTState.from_array(g["npt0"]))
+    th = eng.run(0, 3)
+    assert max(rel(t.Ep, b) for t, b in zip(th, g["traj_Ep"])) < TOL
+    assert max(rel(t.KE, b) for t, b in zip(th, g["traj_KE"])) < TOL
+    st = eng.get_state()
+    assert np.abs(st["pos"] - g["pos_end"]).max() < 1e-8
+    assert rel_max(st["vel"], g["vel_end"]) < 1e-6
+    assert rel_max(st["omg"], g["omg_end"]) < 1e-6
+    assert np.abs(st["qv"] - g["qv_end"]).max() < 1e-9
+    assert np.abs(st["h"] - g["h_end"]).max() < 1e-10
+    ep, F, T, W = eng.forces()
+    assert rel_max(F, g["Fv_end"]) < TOL and rel_max(T, g["Tv_end"]) < TOL
+    eng.close()
+
+
+def test_engine_half_pairs_equal_full_pairs():
+    """108 molecules, 20 steps with a list rebuild: the pair-once kernel (own
+    share + gathered neighbour shares, trace virial) against the both-sides
+    kernel."""
+    g = golden("rigid_core_nc3.npz")
+    out = []
+    for half in (True, False):
+        eng = rigid.RigidEngine(g["body"], len(g["pos"]), float(g["I0"]), float(g["Mmol"]),
+                                rmol=3.55, rmcut=float(g["rmcut"]), nlup=10, half_pairs=half)
+        eng.set_state(pos=g["pos"], vel=g["vel"], qv=g["qv"], omg=g["omg"], h=g["h"],
+                      npt=rigid.make_npt(300.0, 0.0, len(g["pos"])))
+        th = eng.run(0, 20)
+        out.append((eng.get_state(), np.array([[t.Ep, t.KE, t.P] for t in th])))
+        assert eng.counters()[0] == 0
+        eng.close()
+    (s1, t1), (s2, t2) = out
+    for k in range(3):
+        assert rel_max(t1[:, k], t2[:, k]) < 1e-6
+    assert np.abs(s1["pos"] - s2["pos"]).max() < 1e-8
+    assert rel_max(s1["vel"], s2["vel"]) < 1e-6 and rel_max(s1["omg"], s2["omg"]) < 1e-6
+
+
 def test_engine_deterministic():
     g = golden("rigid_core_nc2.npz")
     runs = []
