@@ -44,14 +44,16 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (default: 10000 rigid, 2000 airebo, 400 batch)")
+    ap.add_argument("--warmup", type=int, default=-1,
+                    help="untimed steps first (default: 100 rigid / airebo, 20 batch)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the CPU baseline sample (default: auto, ~20 s)")
-    ap.add_argument("--workload", choices=["rigid", "airebo"], default="rigid",
+    ap.add_argument("--workload", choices=["rigid", "airebo", "batch"], default="rigid",
                     help="rigid: C60 FCC 10^3 rigid LJ (default); airebo: C60 FCC 8^3 "
-                         "flexible AIREBO (122,880 atoms)")
+                         "flexible AIREBO (122,880 atoms); batch: 64 AIREBO NPT replicas")
     ap.add_argument("--nc", type=int, default=0, help="crystal repeats (default per workload)")
     ap.add_argument("--rebo", choices=["compat", "physical"], default="compat")
     return ap.parse_args()
@@ -174,14 +176,112 @@ def main_airebo(args):
     print(json.dumps(res))
 
 
+def batch_replicas(seed=2024, n=64):
+    """Seeded replica table: nc in {3,4,5,6}, C60-Ih or C70-D5h, T ~ U(100, 1500) K,
+    P ~ U(0, 5) GPa."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [dict(nc=int(rng.integers(3, 7)), spec=("C60", "C70")[int(rng.integers(0, 2))],
+                 T=float(rng.uniform(100.0, 1500.0)), P=float(rng.uniform(0.0, 5.0)))
+            for _ in range(n)]
+
+
+def batch_cpu_baseline(table, budget_s=20.0):
+    """Reference serial AIREBO, one process per replica run one after another
+    (the reference has no batching): replicas of the table in order, 10 steps
+    each, until about budget_s of CPU time is spent."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "fuller_airebo_npt_md_serial")
+    if not os.path.exists(exe):
+        return None
+    libdir = os.path.join(ROOT, "fuller_md_en_amd", "FullereneLib")
+    work, secs, used = 0.0, 0.0, []
+    for r, rep in enumerate(table):
+        steps = 10
+        cmd = [exe, f"--cell={rep['nc']}", f"--temp={rep['T']:.3f}", f"--pres={rep['P']:.3f}",
+               f"--fullerene={rep['spec']}", f"--step={steps}", "--seed=42",
+               f"--libdir={libdir}", f"--mon={steps}"]
+        t0 = time.perf_counter()
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600, cwd="/tmp").stdout
+        wall = time.perf_counter() - t0
+        m = re.search(r"Done \(([\d.]+) sec\)", out)
+        na = re.search(r"Natom=(\d+)", out)
+        if not m or not na:
+            return None
+        work += int(na.group(1)) * steps
+        secs += wall
+        used.append(r)
+        if secs > budget_s:
+            break
+    return {"value": work / secs, "unit": "atom-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"fuller_airebo_npt_md_serial, replicas {used[0]}..{used[-1]} of the same "
+                      f"table run one after another, 10 steps each, process wall time "
+                      f"{secs:.1f} s"}
+
+
+def main_batch(args):
+    import numpy as np
+    from fuller_md_en_amd import airebo, system
+    table = batch_replicas()
+    systems = []
+    for r, rep in enumerate(table):
+        s = system.airebo_system(rep["nc"], rep["spec"], T=rep["T"], seed=1000 + r)
+        systems.append(s)
+    na = sum(len(s["pos"]) for s in systems)
+    mode = airebo.REBO_PHYSICAL if args.rebo == "physical" else airebo.REBO_COMPAT
+    b = airebo.ReplicaBatch([(s["mol_id"], s["mass"]) for s in systems], mode=mode)
+    for r, (s, rep) in enumerate(zip(systems, table)):
+        b.set_state(r, pos=s["pos"], vel=s["vel"], h=s["h"],
+                    npt=airebo.make_npt(rep["T"], rep["P"], len(s["pos"])))
+    b.prepare()
+    step0 = 0
+    if args.warmup:
+        b.run(step0, args.warmup, thermo=False)
+        step0 += args.warmup
+    b.sync()
+    b.set_timing(True)
+    t0 = time.perf_counter()
+    th = b.run(step0, args.steps, thermo=True)
+    b.sync()
+    dt = time.perf_counter() - t0
+    kms, kn = b.kernel_time()
+    value = na * args.steps / dt
+    achieved = FLOP_PER_ATOM_STEP_AIREBO_LJ * na / (kms * 1e-3) / 1e12
+    Ts = np.array([t.T for t in th[-1]])
+    res = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(table)} AIREBO NPT replicas (C60-Ih / C70-D5h FCC, "
+                               f"nc 3..6, T 100-1500 K, P 0-5 GPa, seeded table), "
+                               f"REBO {args.rebo} + LJ, dt=0.5 fs, graph-replayed steps",
+                   "atoms": na, "replicas": len(table),
+                   "sizes": sorted({len(s["pos"]) for s in systems})},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "REBO + k_at_lj (force phase, all replicas)", "kernel_ms": kms,
+                     "launches": kn},
+        "final": {"T_min": float(Ts.min()), "T_max": float(Ts.max())},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = batch_cpu_baseline(table)
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "decomposition is not built.", file=sys.stderr)
         sys.exit(2)
+    if not args.steps:
+        args.steps = {"rigid": 10000, "airebo": 2000, "batch": 400}[args.workload]
+    if args.warmup < 0:
+        args.warmup = 20 if args.workload == "batch" else 100
     if args.workload == "airebo":
         return main_airebo(args)
+    if args.workload == "batch":
+        return main_batch(args)
     eng, mol, sysd, scale = build_system()
     n_sites = len(sysd["pos"]) * mol.natom
     eng.prepare()

@@ -110,6 +110,12 @@ _SIGS = {
     "fmd_atomic_create": (_i, [C.POINTER(AtomicConfig), C.POINTER(_vp)]),
     "fmd_atomic_set_state": (_i, [_vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
     "fmd_atomic_get_state": (_i, [_vp, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "fmd_batch_size": (_i, [_vp, C.POINTER(_i)]),
+    "fmd_batch_set_state": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_batch_get_state": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(NPTState)]),
+    "fmd_batch_forces": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "fmd_engine_set_graph": (_i, [_vp, _i]),
     "fmd_engine_prepare": (_i, [_vp]),
     "fmd_engine_run": (_i, [_vp, _i, _i, _vp, _vp]),
     "fmd_engine_forces": (_i, [_vp, _vp, _vp, _vp, _vp]),

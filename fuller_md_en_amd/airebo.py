@@ -154,3 +154,97 @@ class AtomicEngine:
 
     def sync(self):
         N.check(N.lib().fmd_engine_sync(self._h))
+
+    def set_graph(self, on=True):
+        N.check(N.lib().fmd_engine_set_graph(self._h, 1 if on else 0))
+
+
+class ReplicaBatch:
+    """R independent AIREBO NPT replicas in one engine (fmd_batch_*): every
+    kernel launch of a step covers all replicas; steps replay from captured
+    HIP graphs.  replicas: list of (mol_id, mass) per replica."""
+
+    def __init__(self, replicas, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH, nlup=20,
+                 lj=None, graph=True):
+        self._keep = []
+        cfgs = (N.AtomicConfig * len(replicas))()
+        for r, (mol_id, mass) in enumerate(replicas):
+            mol, m = _c(mol_id, np.int32), _c(mass)
+            self._keep += [mol, m]
+            N.lib().fmd_atomic_config_default(C.byref(cfgs[r]))
+            cfgs[r].n_atoms = len(mol)
+            cfgs[r].mol_id = mol.ctypes.data_as(C.POINTER(C.c_int))
+            cfgs[r].mass = m.ctypes.data_as(C.POINTER(C.c_double))
+            cfgs[r].dt, cfgs[r].rebo_mode = dt, mode
+            cfgs[r].lj_max_neigh, cfgs[r].nlup = lj_max_neigh, nlup
+            if lj is not None:
+                cfgs[r].lj = lj
+        self.sizes = [len(mol_id) for mol_id, _ in replicas]
+        self.R = len(replicas)
+        h = C.c_void_p()
+        N.check(N.lib().fmd_batch_create(C.cast(cfgs, C.c_void_p), self.R, C.byref(h)))
+        self._h = h
+        N.check(N.lib().fmd_engine_set_graph(self._h, 1 if graph else 0))
+
+    def close(self):
+        if self._h:
+            N.lib().fmd_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, r, pos=None, vel=None, h=None, npt=None):
+        arrs = [None if a is None else _c(a).reshape(-1) for a in (pos, vel, h)]
+        N.check(N.lib().fmd_batch_set_state(self._h, r, *[N.ptr(a) for a in arrs],
+                                            C.byref(npt) if npt is not None else None))
+
+    def get_state(self, r):
+        n = self.sizes[r]
+        pos, vel, h = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(9)
+        npt = N.NPTState()
+        N.check(N.lib().fmd_batch_get_state(self._h, r, N.ptr(pos), N.ptr(vel), N.ptr(h),
+                                            C.byref(npt)))
+        return dict(pos=pos, vel=vel, h=h, npt=npt)
+
+    def prepare(self):
+        N.check(N.lib().fmd_engine_prepare(self._h))
+
+    def forces(self, r):
+        F, W, E = np.zeros((self.sizes[r], 3)), np.zeros(9), np.zeros(3)
+        N.check(N.lib().fmd_batch_forces(self._h, r, N.ptr(F), N.ptr(W), N.ptr(E)))
+        return tuple(E), F, W
+
+    def run(self, first_step, nsteps, ctrl=None, thermo=True):
+        """ctrl: nsteps x R StepCtrl (row-major) or None.  Returns the records
+        as an (nsteps, R) object array of Thermo, or None."""
+        ctrl_arr = None
+        if ctrl is not None:
+            flat = [c for row in ctrl for c in row]
+            ctrl_arr = (N.StepCtrl * len(flat))(*flat)
+        th = (N.Thermo * (nsteps * self.R))() if thermo else None
+        N.check(N.lib().fmd_engine_run(self._h, first_step, nsteps,
+                                       C.cast(ctrl_arr, C.c_void_p) if ctrl_arr else None,
+                                       C.cast(th, C.c_void_p) if th else None))
+        if th is None:
+            return None
+        return [[th[k * self.R + r] for r in range(self.R)] for k in range(nsteps)]
+
+    def counters(self):
+        out = (C.c_longlong * 3)()
+        N.check(N.lib().fmd_engine_counters(self._h, out))
+        return list(out)
+
+    def set_timing(self, on=True):
+        N.check(N.lib().fmd_engine_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self):
+        ms, n = C.c_double(0), C.c_int(0)
+        N.check(N.lib().fmd_engine_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sync(self):
+        N.check(N.lib().fmd_engine_sync(self._h))

@@ -73,6 +73,8 @@ __device__ __noinline__ double r2_ref(const double *__restrict__ pos, int i, int
   return dx * dx + dy * dy + dz * dz;
 }
 
+__device__ __forceinline__ const double *hh_of(HHRef hh, int r) { return hh.p + r * hh.stride; }
+
 // Block sum of K doubles in a fixed order (256 threads).
 template <int K>
 __device__ __forceinline__ void block_sum(const double *v, double *out) {
@@ -92,10 +94,11 @@ __device__ __forceinline__ void block_sum(const double *v, double *out) {
 
 // ----------------------------------------------------------------- pbc
 __global__ __launch_bounds__(256) void k_at_apply_pbc(double *__restrict__ pos,
-                                                      const double *__restrict__ hh, int n) {
+                                                      const int *__restrict__ atom_rep,
+                                                      HHRef hh, int n) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const double *h = hh, *hi = hh + 9;
+  const double *h = hh_of(hh, atom_rep[i]), *hi = h + 9;
   double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
   double s0 = hi[0] * px + hi[1] * py + hi[2] * pz;
   double s1 = hi[3] * px + hi[4] * py + hi[5] * pz;
@@ -110,12 +113,11 @@ __global__ __launch_bounds__(256) void k_at_apply_pbc(double *__restrict__ pos,
 
 // ------------------------------------------------------------- centres
 template <int NS>
-__global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
-                                                    const double *__restrict__ hh) {
+__global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p, HHRef hh) {
   const int lane = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (m >= p.Nm) return;
-  const double *h = hh, *hi = hh + 9;
+  const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
   const double rx = d.pos[3 * s0], ry = d.pos[3 * s0 + 1], rz = d.pos[3 * s0 + 2];
   double da[NS][3];
@@ -163,17 +165,18 @@ __global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
 }
 
 // --------------------------------------------------------- molecule list
-__global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p,
-                                                     const double *__restrict__ hh) {
+__global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p, HHRef hh) {
   const int m = blockIdx.x, lane = threadIdx.x;
-  const double *h = hh, *hi = hh + 9;
+  const int r = d.mol_rep[m];
+  const double *h = hh_of(hh, r), *hi = h + 9;
   const double cx = d.center[3 * m], cy = d.center[3 * m + 1], cz = d.center[3 * m + 2];
   const double rm = d.radius[m];
+  const int q0 = d.rep_mol[r], q1 = d.rep_mol[r + 1];  // molecules of this replica
   int count = 0;
-  for (int n0 = 0; n0 < p.Nm; n0 += 64) {
+  for (int n0 = q0; n0 < q1; n0 += 64) {
     const int q = n0 + lane;
     bool in = false;
-    if (q < p.Nm && q != m) {
+    if (q < q1 && q != m) {
       if (p.small_box) {
         in = true;
       } else {
@@ -197,12 +200,11 @@ __global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p
 
 // -------------------------------------------------------------- REBO list
 template <int NS>
-__global__ __launch_bounds__(256) void k_at_rebo_nlist(AtomicDev d, AtomicParams p,
-                                                       const double *__restrict__ hh) {
+__global__ __launch_bounds__(256) void k_at_rebo_nlist(AtomicDev d, AtomicParams p, HHRef hh) {
   const int lane = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (m >= p.Nm) return;
-  const double *h = hh, *hi = hh + 9;
+  const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
   int cnt[NS];
 #pragma unroll
@@ -233,14 +235,13 @@ __global__ __launch_bounds__(256) void k_at_rebo_nlist(AtomicDev d, AtomicParams
 // WRITE = 0: truncation point jmax per atom (tile layout) and jmax_min;
 // WRITE = 1: the reference-layout half list itself (nlc_l / nll_l).
 template <int NS, int WRITE>
-__global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams p,
-                                                      const double *__restrict__ hh,
+__global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams p, HHRef hh,
                                                       int *__restrict__ nlc_l,
                                                       int *__restrict__ nll_l, int cap) {
   const int lane = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (m >= p.Nm) return;
-  const double *h = hh, *hi = hh + 9;
+  const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
   int cnt[NS], jm[NS];
   bool done[NS];
@@ -363,14 +364,18 @@ __device__ __forceinline__ void at_half1(const AtomicState &S, const StepCtrl &c
   o.xi1 = xi;
 }
 
+// One block per integrator block of a single replica; every thread derives
+// its replica's first-half NPT scalars (redundantly, from the same inputs).
 __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams p,
                                                        int ctrl_slot) {
-  const AtomicState &S = *d.state;
-  const StepCtrl c = d.ctrl[ctrl_slot];
+  const int b = blockIdx.x;
+  const int rep = d.blk_rep[b];
+  const AtomicState &S = d.state[rep];
+  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + rep];
   AtHalf1 H;
   at_half1(S, c, p, H);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    AtomicMid &M = *d.mid;
+  if (b == d.rep_blk[rep] && threadIdx.x == 0) {
+    AtomicMid &M = d.mid[rep];
     for (int k = 0; k < 9; k++) {
       M.h[k] = H.h1[k];
       M.hi[k] = H.hi1[k];
@@ -379,8 +384,8 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
     M.xi = H.xi1;
     M.sc_nh = H.sc_nh;
   }
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= p.Na) return;
+  const int i = d.blk_a0[b] + threadIdx.x;
+  if (i >= d.blk_a1[b]) return;
   const double hdt = 0.5 * p.dt, mi = kConv / d.mass[i];
   double v[3], r[3];
   for (int a = 0; a < 3; a++) {
@@ -405,21 +410,21 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 }
 
 // ------------------------------------------------------------------ REBO
-__global__ __launch_bounds__(256) void k_rebo_pass1(AtomicDev d, AtomicParams p,
-                                                    const double *__restrict__ hh) {
+__global__ __launch_bounds__(256) void k_rebo_pass1(AtomicDev d, AtomicParams p, HHRef hh) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= p.Na * p.maxr) return;
   const int i = t / p.maxr, jn = t - i * p.maxr;
   if (jn >= d.nlc_r[i]) return;
-  rebo_pass1<float>(i, jn, d.pos, hh, hh + 9, d.nlc_r, d.nll_r, p.maxr, d.bonds);
+  const double *h = hh_of(hh, d.atom_rep[i]);
+  rebo_pass1<float>(i, jn, d.pos, h, h + 9, d.nlc_r, d.nll_r, p.maxr, d.bonds);
 }
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rebo_pass2(AtomicDev d, AtomicParams p) {
-  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int x = d.blk_a0[blockIdx.x] + threadIdx.x;
   double part[kAtPart];
   for (int k = 0; k < kAtPart; k++) part[k] = 0;
-  if (x < p.Na) {
+  if (x < d.blk_a1[blockIdx.x]) {
     double o[13];
     rebo_pass2<float, MODE>(x, d.nlc_r, d.nll_r, p.maxr, d.bonds, o);
     for (int k = 0; k < 3; k++) d.Frebo[3 * x + k] = o[k];
@@ -493,15 +498,14 @@ __device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, i
 // reference's sum over pairs of d (x) f, AB:465-472, regrouped through the
 // symmetric pair set); energy 1/2 per ordered pair.
 template <int NS, bool PERPAIR>
-__global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p,
-                                               const double *__restrict__ hh) {
+__global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRef hh) {
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int bid = blockIdx.x;
   const int mi = (bid % kXcd) * per + bid / kXcd;
   if (mi >= p.Nm) return;
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
-  const double *h = hh, *hi = hh + 9;
+  const double *h = hh_of(hh, d.mol_rep[mi]), *hi = h + 9;
   float hf[9], hif[9];
   for (int k = 0; k < 9; k++) {
     hf[k] = (float)h[k];
@@ -654,12 +658,12 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p,
 
 // ------------------------------------------------------- kick 2 + partials
 __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, int kick) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = d.blk_a0[blockIdx.x] + threadIdx.x;
   double v2[1] = {0};
-  if (i < p.Na) {
+  if (i < d.blk_a1[blockIdx.x]) {
     double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
     if (kick) {
-      const AtomicMid &M = *d.mid;
+      const AtomicMid &M = d.mid[d.blk_rep[blockIdx.x]];
       const double hdt = 0.5 * p.dt, mi = kConv / d.mass[i];
       const double eps2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
       const double sc_v2 = M.sc_nh * exp(-hdt * eps2 / 3.0);
@@ -677,16 +681,19 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
 
 // Fixed-order sums of the three partial arrays: s[0] = sum m v^2,
 // s[1..9] = W9, s[10] = E_rebo, s[11] = E_lj.
-__device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams &p,
-                                          double *sums) {
+// Replica r's sums (one workgroup; the strided order is the same whatever
+// the replica's offset, so a replica sums bitwise as it would alone).
+__device__ __forceinline__ void at_reduce(const AtomicDev &d, int r, double *sums) {
   double v[12];
   for (int k = 0; k < 12; k++) v[k] = 0;
-  for (int b = threadIdx.x; b < d.nblk; b += 256) {
+  const int b0 = d.rep_blk[r], b1 = d.rep_blk[r + 1];
+  for (int b = b0 + threadIdx.x; b < b1; b += 256) {
     v[0] += d.ke_part[b];
     for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[(size_t)b * kAtPart + k];
     v[10] += d.rebo_part[(size_t)b * kAtPart + 9];
   }
-  for (int m = threadIdx.x; m < p.Nm; m += 256) {
+  const int m0 = d.rep_mol[r], m1 = d.rep_mol[r + 1];
+  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
     for (int k = 0; k < 9; k++) v[1 + k] += d.lj_part[(size_t)m * kAtPart + k];
     v[11] += d.lj_part[(size_t)m * kAtPart + 9];
   }
@@ -695,11 +702,13 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
   for (int k = 0; k < 12; k++) sums[k] = tot[k];
 }
 
+// One workgroup per replica.
 __global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p, int what) {
+  const int r = blockIdx.x;
   double s[12];
-  at_reduce(d, p, s);
+  at_reduce(d, r, s);
   if (threadIdx.x != 0) return;
-  AtomicState &S = *d.state;
+  AtomicState &S = d.state[r];
   if (what & kAtObsKE) S.KE = 0.5 * s[0] / kConv;
   if (what & kAtObsForce) {
     for (int k = 0; k < 9; k++) S.W9[k] = s[1 + k];
@@ -709,14 +718,16 @@ __global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p,
   }
 }
 
+// One workgroup per replica.
 __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
                                                    int thermo_slot) {
+  const int r = blockIdx.x;
   double s[12];
-  at_reduce(d, p, s);
+  at_reduce(d, r, s);
   if (threadIdx.x != 0) return;
-  AtomicState &S = *d.state;
-  const AtomicMid &M = *d.mid;
-  const StepCtrl c = d.ctrl[ctrl_slot];
+  AtomicState &S = d.state[r];
+  const AtomicMid &M = d.mid[r];
+  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + r];
   const double hdt = 0.5 * p.dt;
   double KE = 0.5 * s[0] / kConv;
   double W9[9];
@@ -755,7 +766,7 @@ __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, 
   S.Ep_lj = El;
   S.scale = scale;
   S.step += 1;
-  Thermo &t = d.thermo[thermo_slot];
+  Thermo &t = d.thermo[(size_t)thermo_slot * p.R + r];
   t.Ep = Er + El;
   t.KE = KE;
   t.KEt = KE;
@@ -769,32 +780,25 @@ __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, 
   for (int k = 0; k < 9; k++) t.h[k] = M.h[k];
 }
 
+// Velocity rescale by each replica's factor (1 where its control did not ask).
 __global__ __launch_bounds__(256) void k_at_rescale(AtomicDev d, AtomicParams p) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= p.Na) return;
-  const double s = d.state->scale;
+  const int i = d.blk_a0[blockIdx.x] + threadIdx.x;
+  if (i >= d.blk_a1[blockIdx.x]) return;
+  const double s = d.state[d.blk_rep[blockIdx.x]].scale;
   for (int a = 0; a < 3; a++) d.vel[3 * i + a] *= s;
 }
 
 inline int ns_of(const AtomicParams &p) { return p.apad <= 64 ? 1 : 2; }
-inline const double *mid_hh(const AtomicDev &d) {
-  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.mid) +
-                                          offsetof(AtomicMid, h));
-}
-inline const double *state_hh(const AtomicDev &d) {
-  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.state) +
-                                          offsetof(AtomicState, h));
-}
 
 }  // namespace
 
-hipError_t at_launch_apply_pbc(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_apply_pbc(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                                hipStream_t s) {
-  k_at_apply_pbc<<<(p.Na + 255) / 256, 256, 0, s>>>(d.pos, hh, p.Na);
+  k_at_apply_pbc<<<(p.Na + 255) / 256, 256, 0, s>>>(d.pos, d.atom_rep, hh, p.Na);
   return hipGetLastError();
 }
 
-hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                              hipStream_t s) {
   dim3 g((p.Nm + 3) / 4);
   if (ns_of(p) == 1)
@@ -804,7 +808,7 @@ hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, const do
   return hipGetLastError();
 }
 
-hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                            int rebo_list, hipStream_t s) {
   k_at_mol_nlist<<<p.Nm, 64, 0, s>>>(d, p, hh);
   dim3 g((p.Nm + 3) / 4);
@@ -839,7 +843,7 @@ hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
   return hipGetLastError();
 }
 
-hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                               int *nlc_l, int *nll_l, int cap, hipStream_t s) {
   dim3 g((p.Nm + 3) / 4);
   if (ns_of(p) == 1)
@@ -849,7 +853,7 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, const d
   return hipGetLastError();
 }
 
-hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                             hipStream_t s) {
   k_rebo_pass1<<<(p.Na * p.maxr + 255) / 256, 256, 0, s>>>(d, p, hh);
   if (p.rebo_mode == kReboPhysical)
@@ -876,27 +880,27 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, const dou
 hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
                              hipStream_t s) {
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0);
-  k_at_observe<<<1, 256, 0, s>>>(d, p, what);
+  k_at_observe<<<p.R, 256, 0, s>>>(d, p, what);
   return hipGetLastError();
 }
 
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
                           int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
-                          hipEvent_t ev1, hipStream_t s) {
+                          hipEvent_t ev1, unsigned ev_flags, hipStream_t s) {
   hipError_t e;
   if (rebuild) {
     // AB:881-887: both lists from the current positions and cell
-    if ((e = at_launch_centers(d, p, state_hh(d), s)) != hipSuccess) return e;
-    if ((e = at_launch_lists(d, p, state_hh(d), 1, s)) != hipSuccess) return e;
+    if ((e = at_launch_centers(d, p, hh_state(d), s)) != hipSuccess) return e;
+    if ((e = at_launch_lists(d, p, hh_state(d), 1, s)) != hipSuccess) return e;
   }
   k_at_kick_drift<<<d.nblk, 256, 0, s>>>(d, p, ctrl_slot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = at_launch_centers(d, p, mid_hh(d), s)) != hipSuccess) return e;
-  if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-  if ((e = at_launch_forces(d, p, mid_hh(d), s)) != hipSuccess) return e;
-  if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
+  if ((e = at_launch_centers(d, p, hh_mid(d), s)) != hipSuccess) return e;
+  if (ev0 && (e = hipEventRecordWithFlags(ev0, s, ev_flags)) != hipSuccess) return e;
+  if ((e = at_launch_forces(d, p, hh_mid(d), s)) != hipSuccess) return e;
+  if (ev1 && (e = hipEventRecordWithFlags(ev1, s, ev_flags)) != hipSuccess) return e;
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
-  k_at_finish<<<1, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+  k_at_finish<<<p.R, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);
   if (rescale) k_at_rescale<<<d.nblk, 256, 0, s>>>(d, p);
   return hipGetLastError();
 }

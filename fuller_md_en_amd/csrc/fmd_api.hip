@@ -276,11 +276,11 @@ int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0
 
 
 
-int fmd::engine_init_common(fmd_engine *e, int ring) {
+int fmd::engine_init_common(fmd_engine *e, int ring, int R) {
   e->ring = ring;
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-  HIPCHK(hipHostMalloc((void **)&e->h_ctrl, ring * sizeof(StepCtrl)));
-  HIPCHK(hipHostMalloc((void **)&e->h_thermo, ring * sizeof(Thermo)));
+  HIPCHK(hipHostMalloc((void **)&e->h_ctrl, (size_t)ring * R * sizeof(StepCtrl)));
+  HIPCHK(hipHostMalloc((void **)&e->h_thermo, (size_t)ring * R * sizeof(Thermo)));
   return FMD_OK;
 }
 
@@ -297,7 +297,7 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
   int rc = rigid_params(e->rp, c->n_mol, c->natom, c->body, c->I0, c->Mmol, c->lj,
                         c->rmcut * c->rmcut, c->rmcut + c->skin, c->max_neigh, c->dt,
                         c->clamp_xi, c->clamp_vg, body4, c->half_pairs ? &qbody : nullptr);
-  if (rc == FMD_OK) rc = engine_init_common(e, 4096);
+  if (rc == FMD_OK) rc = engine_init_common(e, 4096, 1);
   if (rc == FMD_OK) rc = rigid_alloc(e->rd, e->rp, e->ring);
   if (rc != FMD_OK) {
     fmd_engine_destroy(e);
@@ -380,6 +380,142 @@ int fmd_rigid_get_state(fmd_engine *e, double *pos, double *vel, double *qv, dou
 
 }  // extern "C"
 
+// ------------------------------------------------------ step sequencing
+namespace fmd {
+
+int engine_graph(fmd_engine *e, const std::string &key, const std::function<int()> &launch,
+                 hipGraphExec_t *out) {
+  for (auto &g : e->graphs)
+    if (g.first == key) {
+      *out = g.second;
+      return FMD_OK;
+    }
+  if (e->graphs.size() >= 32) engine_drop_graphs(e);
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch();
+  hipGraph_t graph = nullptr;
+  hipError_t he = hipStreamEndCapture(e->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (he != hipSuccess) return hip_fail(he, "hipGraphInstantiate");
+  e->graphs.emplace_back(key, exec);
+  *out = exec;
+  return FMD_OK;
+}
+
+void engine_drop_graphs(fmd_engine *e) {
+  for (auto &g : e->graphs) (void)hipGraphExecDestroy(g.second);
+  e->graphs.clear();
+}
+
+int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+              fmd_thermo *thermo, double *tt, StepCtrl *d_ctrl, Thermo *d_thermo,
+              const StepLaunch &launch) {
+  hipStream_t s = e->stream;
+  std::vector<char> rebuild(e->ring), rescale(e->ring);
+  for (int c0 = 0; c0 < nsteps; c0 += e->ring) {
+    const int n = std::min(e->ring, nsteps - c0);
+    for (int k = 0; k < n; k++) {
+      const long long g = (long long)first_step + c0 + k + 1;
+      int any_rebuild = 0, any_rescale = 0;
+      for (int r = 0; r < R; r++) {
+        StepCtrl &c = e->h_ctrl[(size_t)k * R + r];
+        if (ctrl) {
+          const fmd_step_ctrl &u = ctrl[(size_t)(c0 + k) * R + r];
+          c.Tt = u.Tt;
+          c.rescale_T = u.rescale_T;
+          c.flags = u.flags;
+        } else {
+          c.Tt = tt[r];
+          c.rescale_T = 0;
+          c.flags = 0;
+        }
+        c.pad = 0;
+        any_rebuild |= c.flags & FMD_CTRL_REBUILD;
+        any_rescale |= c.flags & FMD_CTRL_RESCALE;
+      }
+      rebuild[k] = (g % e->nlup == 0) || any_rebuild;
+      rescale[k] = any_rescale != 0;
+    }
+    if (ctrl)
+      for (int r = 0; r < R; r++) tt[r] = ctrl[(size_t)(c0 + n - 1) * R + r].Tt;
+    const size_t cb = (size_t)R * sizeof(StepCtrl), tb = (size_t)R * sizeof(Thermo);
+    while (e->timing && (int)e->ev.size() < 2 * n) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      e->ev.push_back(ev);
+    }
+    auto take_times = [&](int k0, int m) -> int {
+      for (int k = 0; k < m; k++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[2 * (k0 + k)], e->ev[2 * (k0 + k) + 1]));
+        e->t_sum += ms;
+        e->t_n++;
+      }
+      return FMD_OK;
+    };
+    if (!e->use_graph) {
+      HIPCHK(hipMemcpyAsync(d_ctrl, e->h_ctrl, n * cb, hipMemcpyHostToDevice, s));
+      for (int k = 0; k < n; k++) {
+        hipEvent_t e0 = e->timing ? e->ev[2 * k] : nullptr;
+        hipEvent_t e1 = e->timing ? e->ev[2 * k + 1] : nullptr;
+        HIPCHK(launch(k, rebuild[k], rescale[k], e0, e1, hipEventRecordDefault));
+      }
+      if (thermo)
+        HIPCHK(hipMemcpyAsync(e->h_thermo, d_thermo, n * tb, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (e->timing && take_times(0, n)) return FMD_ERR_HIP;
+    } else {
+      // runs of steps with a list rebuild only at their first step, each
+      // replayed from a graph captured once per (length, rebuild, rescale)
+      // pattern; controls go to slots [0, K) before and records come back
+      // from them after every replay (stream order keeps them apart)
+      constexpr int kMaxGraphSteps = 64;
+      for (int k = 0; k < n;) {
+        int K = 1;
+        while (k + K < n && !rebuild[k + K] && K < kMaxGraphSteps) K++;
+        std::string key = std::to_string(K) + (rebuild[k] ? "R" : "-") + (e->timing ? "T" : "-");
+        for (int j = 0; j < K; j++) key += rescale[k + j] ? '1' : '0';
+        hipGraphExec_t exec;
+        int rc = engine_graph(e, key, [&]() -> int {
+          for (int j = 0; j < K; j++) {
+            hipEvent_t e0 = e->timing ? e->ev[2 * j] : nullptr;
+            hipEvent_t e1 = e->timing ? e->ev[2 * j + 1] : nullptr;
+            HIPCHK(launch(j, j == 0 ? rebuild[k] : 0, rescale[k + j], e0, e1,
+                          hipEventRecordExternal));
+          }
+          return FMD_OK;
+        }, &exec);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(d_ctrl, e->h_ctrl + (size_t)k * R, K * cb, hipMemcpyHostToDevice,
+                              s));
+        HIPCHK(hipGraphLaunch(exec, s));
+        if (thermo)
+          HIPCHK(hipMemcpyAsync(e->h_thermo + (size_t)k * R, d_thermo, K * tb,
+                                hipMemcpyDeviceToHost, s));
+        if (e->timing) {
+          HIPCHK(hipStreamSynchronize(s));
+          if (take_times(0, K)) return FMD_ERR_HIP;
+        }
+        k += K;
+      }
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    for (int k = 0; k < n; k++) e->rebuilds += rebuild[k];
+    if (thermo) memcpy(thermo + (size_t)c0 * R, e->h_thermo, n * tb);
+    e->steps += n;
+  }
+  return FMD_OK;
+}
+
+}  // namespace fmd
+
 static int rigid_prepare(fmd_engine *e) {
   RigidDev &d = e->rd;
   const RigidParams &p = e->rp;
@@ -411,59 +547,16 @@ static int rigid_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_c
                      fmd_thermo *thermo) {
   RigidDev &d = e->rd;
   const RigidParams &p = e->rp;
-  hipStream_t s = e->stream;
   if (!e->prepared) {
     int rc = rigid_prepare(e);
     if (rc) return rc;
   }
-  for (int c0 = 0; c0 < nsteps; c0 += e->ring) {
-    const int n = std::min(e->ring, nsteps - c0);
-    for (int k = 0; k < n; k++) {
-      StepCtrl &c = e->h_ctrl[k];
-      if (ctrl) {
-        c.Tt = ctrl[c0 + k].Tt;
-        c.rescale_T = ctrl[c0 + k].rescale_T;
-        c.flags = ctrl[c0 + k].flags;
-      } else {
-        c.Tt = e->tt;
-        c.rescale_T = 0;
-        c.flags = 0;
-      }
-      c.pad = 0;
-    }
-    if (ctrl) e->tt = ctrl[c0 + n - 1].Tt;
-    HIPCHK(hipMemcpyAsync(d.ctrl, e->h_ctrl, n * sizeof(StepCtrl), hipMemcpyHostToDevice, s));
-    if (e->timing && (int)e->ev.size() < 2 * n) {
-      while ((int)e->ev.size() < 2 * n) {
-        hipEvent_t ev;
-        HIPCHK(hipEventCreate(&ev));
-        e->ev.push_back(ev);
-      }
-    }
-    for (int k = 0; k < n; k++) {
-      const long long g = (long long)first_step + c0 + k + 1;
-      const int fl = e->h_ctrl[k].flags;
-      const int rebuild = (g % e->nlup == 0) || (fl & FMD_CTRL_REBUILD);
-      const int rescale = (fl & FMD_CTRL_RESCALE) ? 1 : 0;
-      hipEvent_t e0 = e->timing ? e->ev[2 * k] : nullptr;
-      hipEvent_t e1 = e->timing ? e->ev[2 * k + 1] : nullptr;
-      HIPCHK(rigid_launch_step(d, p, k, k, rebuild, rescale, e0, e1, s));
-      e->rebuilds += rebuild;
-    }
-    if (thermo)
-      HIPCHK(hipMemcpyAsync(e->h_thermo, d.thermo, n * sizeof(Thermo), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (thermo) memcpy(thermo + c0, e->h_thermo, n * sizeof(Thermo));
-    if (e->timing) {
-      for (int k = 0; k < n; k++) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]));
-        e->t_sum += ms;
-        e->t_n++;
-      }
-    }
-    e->steps += n;
-  }
+  auto launch = [&](int slot, int rebuild, int rescale, hipEvent_t e0, hipEvent_t e1,
+                    unsigned evf) -> hipError_t {
+    return rigid_launch_step(d, p, slot, slot, rebuild, rescale, e0, e1, evf, e->stream);
+  };
+  int rc = run_steps(e, 1, first_step, nsteps, ctrl, thermo, &e->tt, d.ctrl, d.thermo, launch);
+  if (rc) return rc;
   return read_overflow(e, d.flags);
 }
 
@@ -549,6 +642,12 @@ int fmd_engine_set_timing(fmd_engine *e, int on) {
   return FMD_OK;
 }
 
+int fmd_engine_set_graph(fmd_engine *e, int on) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  e->use_graph = on != 0;
+  return FMD_OK;
+}
+
 int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n) {
   if (!e) return set_error(FMD_ERR_ARG, "null engine");
   if (mean_ms) *mean_ms = e->t_n ? e->t_sum / e->t_n : 0.0;
@@ -567,6 +666,7 @@ int fmd_engine_sync(fmd_engine *e) {
 void fmd_engine_destroy(fmd_engine *e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  engine_drop_graphs(e);
   if (e->kind == FMD_KIND_RIGID) free_rigid(e->rd);
   engine_free_atomic(e);
   for (hipEvent_t ev : e->ev) (void)hipEventDestroy(ev);
@@ -758,7 +858,7 @@ int fmd_rigid_step_npt(double *pos, double *vel, double *qv, double *omg, double
   HIPCHK(hipMemcpyAsync(d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
   HIPCHK(rigid_launch_observe(d, e->rp, kObsKE, s));
-  HIPCHK(rigid_launch_step(d, e->rp, 0, 0, 0, 0, nullptr, nullptr, s));
+  HIPCHK(rigid_launch_step(d, e->rp, 0, 0, 0, 0, nullptr, nullptr, 0, s));
   HIPCHK(hipMemcpyAsync(pos, d.pos, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(vel, d.vel, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(qv, d.qv, 4 * (size_t)N * 8, hipMemcpyDeviceToHost, s));

@@ -21,9 +21,20 @@
 //   nlc_r / nll_r          REBO list, reference layout (stride maxr = 12)
 //   bonds                  per directed REBO entry (fmd_rebo_passes.hpp)
 //   rebo_part, lj_part, ke_part  fixed-order partial sums
+//
+// Replicas: R independent systems share every array, concatenated replica
+// after replica (atoms, molecules).  Molecule and atom ranges of replica r
+// are [rep_mol[r], rep_mol[r+1]) and [rep_atom[r], rep_atom[r+1]); mol_rep /
+// atom_rep map back.  Per-atom integrator blocks never straddle replicas:
+// block b covers atoms [blk_a0[b], blk_a1[b]) of replica blk_rep[b], and
+// replica r owns blocks [rep_blk[r], rep_blk[r+1]).  state, mid: one per
+// replica; ctrl / thermo rings: slot-major, R entries per step slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
 
 #include "fmd_common.hpp"
 #include "fmd_rebo_passes.hpp"
@@ -48,14 +59,15 @@ struct AtomicMid {
 };
 
 struct AtomicParams {
-  int Na, Nm;
+  int Na, Nm;      // totals over all replicas
+  int R;           // replicas
   int apad;        // atoms per molecule tile (max molecule size, rounded up to 4)
   int maxr;        // REBO list stride (12, AB:126)
   int mol_cap;     // molecule list stride
   int lj_cap;      // half-list cap of the reference (400, AB:127)
   int lj_compat;   // 1: pairs limited by the reference's truncated half list
   int rebo_mode;   // kReboCompat / kReboPhysical
-  int small_box;   // 1: per-pair minimum image (cell narrower than the list)
+  int small_box;   // 1: per-pair minimum image (a cell narrower than the list)
   double rebo_rlist2;  // REBO list radius^2 (2.3^2, AB:253)
   double lj_rlist2;    // LJ half-list radius^2 ((rc + 2)^2, AB:265)
   double mol_rlist;    // molecule list radius beyond the two molecule radii
@@ -81,38 +93,69 @@ struct AtomicDev {
   StepCtrl *ctrl;
   Thermo *thermo;
   int *flags;  // [0] REBO list overflow, [1] molecule list overflow
-  int nblk;    // ceil(Na / 256)
+  int *mol_rep, *atom_rep, *rep_mol, *rep_atom;
+  int *blk_a0, *blk_a1, *blk_rep, *rep_blk;
+  int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)
 };
 
-// Launchers (atomic_kernels.hip).  hh: 18 device doubles (h then h^-1).
-hipError_t at_launch_apply_pbc(const AtomicDev &d, const AtomicParams &p, const double *hh,
+// h and h^-1 of each replica: 18 doubles at p + r * stride (stride 0: one
+// cell for all, the stateless entry points).
+struct HHRef {
+  const double *p;
+  int stride;
+};
+inline HHRef hh_state(const AtomicDev &d) {
+  static_assert(sizeof(AtomicState) % 8 == 0, "AtomicState stride");
+  return {reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.state) +
+                                           offsetof(AtomicState, h)),
+          (int)(sizeof(AtomicState) / 8)};
+}
+inline HHRef hh_mid(const AtomicDev &d) {
+  static_assert(sizeof(AtomicMid) % 8 == 0, "AtomicMid stride");
+  return {reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.mid) +
+                                           offsetof(AtomicMid, h)),
+          (int)(sizeof(AtomicMid) / 8)};
+}
+inline HHRef hh_fixed(const double *hh) { return {hh, 0}; }
+
+// Launchers (atomic_kernels.hip).  hh: the cells (HHRef).
+hipError_t at_launch_apply_pbc(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                                hipStream_t s);
 // Molecule centres, radii and offset tiles from pos.
-hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                              hipStream_t s);
 // Molecule list, REBO list and (compat) truncation points; needs centres.
-hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                            int rebo_list, hipStream_t s);
 // REBO + LJ forces into F (and the partial sums); needs centres and lists.
-hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                             hipStream_t s);
 enum : int { kAtObsKE = 1, kAtObsForce = 2 };
 hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
                              hipStream_t s);
+// One NPT step of every replica; controls from ctrl[ctrl_slot * R + r],
+// records to thermo[thermo_slot * R + r].  rescale: launch the velocity
+// rescale (it acts on the replicas whose control asks for it).
+// ev0 / ev1 (may be null) bracket the force phase; ev_flags is passed to
+// hipEventRecordWithFlags (hipEventRecordExternal inside a graph capture).
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
                           int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
-                          hipEvent_t ev1, hipStream_t s);
+                          hipEvent_t ev1, unsigned ev_flags, hipStream_t s);
 // jmax from a reference-layout half list (stride cap) for the drop-in path.
 hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
                                     const int *nlc_l, const int *nll_l, int cap,
                                     hipStream_t s);
 // Reference-layout half list (AB:263-272) built on the GPU.
-hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, const double *hh,
+hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                               int *nlc_l, int *nll_l, int cap, hipStream_t s);
 
 struct AtomicEngine {
   AtomicParams p{};
   AtomicDev d{};
+  // host copies of the replica tables
+  std::vector<int> rep_atom, rep_mol, mol_start;
+  std::vector<double> tt;  // per replica: target temperature (host mirror)
+  std::vector<int> small;  // per replica: cell narrower than the LJ reach
 };
 
 }  // namespace fmd

@@ -14,11 +14,6 @@ using namespace fmd;
 
 namespace {
 
-const double *at_state_hh(const AtomicDev &d) {
-  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.state) +
-                                          offsetof(AtomicState, h));
-}
-
 // Molecule ranges from mol_id: every molecule's atoms must be one contiguous
 // run (the reference builds atoms molecule by molecule, AB:781-786).
 int mol_ranges(const int *mol_id, int na, std::vector<int> &start, int &apad) {
@@ -90,7 +85,9 @@ void free_atomic_dev(AtomicDev &d) {
                   d.center,    d.radius,    d.off,      d.jmax,      d.jmax_min,   d.mol_nl_count,
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.lj_part, d.ke_part,
-                  d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags};
+                  d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
+                  d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
+                  d.blk_rep,   d.rep_blk};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   memset(&d, 0, sizeof d);
@@ -101,10 +98,66 @@ hipError_t dalloc(T **p, size_t n) {
   return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
 }
 
-int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring) {
+// Host image of the replica tables (fmd_atomic.hpp).
+struct Layout {
+  std::vector<int> mol_start, mol_rep, atom_rep, rep_mol, rep_atom;
+  std::vector<int> blk_a0, blk_a1, blk_rep, rep_blk;
+};
+
+// Concatenate the replicas' molecule layouts.
+int build_layout(const fmd_atomic_config *c, int R, Layout &L, int &apad) {
+  L = Layout();
+  L.mol_start.push_back(0);
+  L.rep_mol.push_back(0);
+  L.rep_atom.push_back(0);
+  L.rep_blk.push_back(0);
+  apad = 4;
+  int a0 = 0;
+  for (int r = 0; r < R; r++) {
+    std::vector<int> st;
+    int ap = 0;
+    int rc = mol_ranges(c[r].mol_id, c[r].n_atoms, st, ap);
+    if (rc) return rc;
+    apad = std::max(apad, ap);
+    for (size_t m = 1; m < st.size(); m++) {
+      L.mol_start.push_back(a0 + st[m]);
+      L.mol_rep.push_back(r);
+    }
+    for (int i = 0; i < c[r].n_atoms; i++) L.atom_rep.push_back(r);
+    for (int b = 0; b < c[r].n_atoms; b += 256) {
+      L.blk_a0.push_back(a0 + b);
+      L.blk_a1.push_back(a0 + std::min(b + 256, c[r].n_atoms));
+      L.blk_rep.push_back(r);
+    }
+    a0 += c[r].n_atoms;
+    L.rep_mol.push_back((int)L.mol_rep.size());
+    L.rep_atom.push_back(a0);
+    L.rep_blk.push_back((int)L.blk_rep.size());
+  }
+  return FMD_OK;
+}
+
+int upload_layout(AtomicDev &d, const Layout &L) {
+  auto up = [](int *dst, const std::vector<int> &v) {
+    return hipMemcpy(dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice);
+  };
+  hipError_t e = up(d.mol_start, L.mol_start);
+  if (e == hipSuccess) e = up(d.mol_rep, L.mol_rep);
+  if (e == hipSuccess) e = up(d.atom_rep, L.atom_rep);
+  if (e == hipSuccess) e = up(d.rep_mol, L.rep_mol);
+  if (e == hipSuccess) e = up(d.rep_atom, L.rep_atom);
+  if (e == hipSuccess) e = up(d.blk_a0, L.blk_a0);
+  if (e == hipSuccess) e = up(d.blk_a1, L.blk_a1);
+  if (e == hipSuccess) e = up(d.blk_rep, L.blk_rep);
+  if (e == hipSuccess) e = up(d.rep_blk, L.rep_blk);
+  if (e != hipSuccess) return hip_fail(e, "upload replica layout");
+  return FMD_OK;
+}
+
+int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   memset(&d, 0, sizeof d);
-  const size_t Na = p.Na, Nm = p.Nm, ns = Na * p.maxr, nt = Nm * p.apad;
-  d.nblk = (p.Na + 255) / 256;
+  const size_t Na = p.Na, Nm = p.Nm, ns = Na * p.maxr, nt = Nm * p.apad, R = p.R;
+  d.nblk = nblk;
   hipError_t e = hipSuccess;
 #define AL(ptr, n) \
   if (e == hipSuccess) e = dalloc(&(ptr), (n))
@@ -133,11 +186,19 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring) {
   AL(d.lj_part, Nm * kAtPart);
   AL(d.ke_part, d.nblk);
   AL(d.hh, 18);
-  AL(d.state, 1);
-  AL(d.mid, 1);
-  AL(d.ctrl, ring);
-  AL(d.thermo, ring);
+  AL(d.state, R);
+  AL(d.mid, R);
+  AL(d.ctrl, ring * R);
+  AL(d.thermo, ring * R);
   AL(d.flags, 4);
+  AL(d.mol_rep, Nm);
+  AL(d.atom_rep, Na);
+  AL(d.rep_mol, R + 1);
+  AL(d.rep_atom, R + 1);
+  AL(d.blk_a0, nblk);
+  AL(d.blk_a1, nblk);
+  AL(d.blk_rep, nblk);
+  AL(d.rep_blk, R + 1);
 #undef AL
   if (e != hipSuccess) {
     free_atomic_dev(d);
@@ -145,16 +206,18 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring) {
   }
   (void)hipMemset(d.flags, 0, 4 * sizeof(int));
   (void)hipMemset(d.F, 0, 3 * Na * sizeof(double));
-  (void)hipMemset(d.state, 0, sizeof(AtomicState));
+  (void)hipMemset(d.state, 0, R * sizeof(AtomicState));
   (void)hipMemset(d.nlc_r, 0, Na * sizeof(int));
   (void)hipMemset(d.mol_nl_count, 0, Nm * sizeof(int));
   return FMD_OK;
 }
 
-void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Nm, int apad) {
+void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Na, int Nm, int R,
+                 int apad) {
   memset(&p, 0, sizeof p);
-  p.Na = c.n_atoms;
+  p.Na = Na;
   p.Nm = Nm;
+  p.R = R;
   p.apad = apad;
   p.maxr = c.rebo_max_neigh > 0 ? c.rebo_max_neigh : 12;
   p.mol_cap = 160;
@@ -189,7 +252,7 @@ int engine_prepare_atomic(fmd_engine *e) {
   const AtomicDev &d = A.d;
   const AtomicParams &p = A.p;
   hipStream_t s = e->stream;
-  const double *hh = at_state_hh(d);
+  const HHRef hh = hh_state(d);
   HIPCHK(at_launch_apply_pbc(d, p, hh, s));  // AB:852
   HIPCHK(at_launch_centers(d, p, hh, s));
   HIPCHK(at_launch_lists(d, p, hh, 1, s));
@@ -210,59 +273,20 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
   AtomicEngine &A = *e->at;
   const AtomicDev &d = A.d;
   const AtomicParams &p = A.p;
-  hipStream_t s = e->stream;
   if (!e->prepared) {
     int rc = engine_prepare_atomic(e);
     if (rc) return rc;
   }
-  for (int c0 = 0; c0 < nsteps; c0 += e->ring) {
-    const int n = std::min(e->ring, nsteps - c0);
-    for (int k = 0; k < n; k++) {
-      StepCtrl &c = e->h_ctrl[k];
-      if (ctrl) {
-        c.Tt = ctrl[c0 + k].Tt;
-        c.rescale_T = ctrl[c0 + k].rescale_T;
-        c.flags = ctrl[c0 + k].flags;
-      } else {
-        c.Tt = e->tt;
-        c.rescale_T = 0;
-        c.flags = 0;
-      }
-      c.pad = 0;
-    }
-    if (ctrl) e->tt = ctrl[c0 + n - 1].Tt;
-    HIPCHK(hipMemcpyAsync(d.ctrl, e->h_ctrl, n * sizeof(StepCtrl), hipMemcpyHostToDevice, s));
-    while (e->timing && (int)e->ev.size() < 2 * n) {
-      hipEvent_t ev;
-      HIPCHK(hipEventCreate(&ev));
-      e->ev.push_back(ev);
-    }
-    for (int k = 0; k < n; k++) {
-      const long long g = (long long)first_step + c0 + k + 1;
-      const int fl = e->h_ctrl[k].flags;
-      const int rebuild = (g % e->nlup == 0) || (fl & FMD_CTRL_REBUILD);  // AB:881
-      const int rescale = (fl & FMD_CTRL_RESCALE) ? 1 : 0;
-      HIPCHK(at_launch_step(d, p, k, k, rebuild, rescale, e->timing ? e->ev[2 * k] : nullptr,
-                            e->timing ? e->ev[2 * k + 1] : nullptr, s));
-      e->rebuilds += rebuild;
-    }
-    if (thermo)
-      HIPCHK(hipMemcpyAsync(e->h_thermo, d.thermo, n * sizeof(Thermo), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (thermo) memcpy(thermo + c0, e->h_thermo, n * sizeof(Thermo));
-    if (e->timing) {
-      for (int k = 0; k < n; k++) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]));
-        e->t_sum += ms;
-        e->t_n++;
-      }
-    }
-    e->steps += n;
-  }
+  auto launch = [&](int slot, int rebuild, int rescale, hipEvent_t e0, hipEvent_t e1,
+                    unsigned evf) -> hipError_t {
+    return at_launch_step(d, p, slot, slot, rebuild, rescale, e0, e1, evf, e->stream);
+  };
+  int rc = run_steps(e, p.R, first_step, nsteps, ctrl, thermo, A.tt.data(), d.ctrl, d.thermo,
+                     launch);
+  if (rc) return rc;
   int f[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
   if (f[0] || f[1]) {
     e->overflow += f[0] + f[1];
     HIPCHK(hipMemset(d.flags, 0, 2 * sizeof(int)));
@@ -272,16 +296,23 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
 
 int engine_forces_atomic(fmd_engine *e, double *F, double *T, double *W9, double *energies) {
   (void)T;
+  return replica_forces(e, 0, F, W9, energies);
+}
+
+int replica_forces(fmd_engine *e, int r, double *F, double *W9, double *energies) {
   if (!e->at) return set_error(FMD_ERR_STATE, "engine has no atomic system");
+  if (r < 0 || r >= e->at->p.R) return set_error(FMD_ERR_ARG, "replica %d out of range", r);
   if (!e->prepared) {
     int rc = engine_prepare_atomic(e);
     if (rc) return rc;
   }
-  const AtomicDev &d = e->at->d;
+  const AtomicEngine &A = *e->at;
+  const AtomicDev &d = A.d;
   hipStream_t s = e->stream;
-  if (F) HIPCHK(hipMemcpyAsync(F, d.F, 3 * (size_t)e->at->p.Na * 8, hipMemcpyDeviceToHost, s));
+  const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
+  if (F) HIPCHK(hipMemcpyAsync(F, d.F + 3 * a0, 3 * na * 8, hipMemcpyDeviceToHost, s));
   AtomicState S;
-  HIPCHK(hipMemcpyAsync(&S, d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, d.state + r, sizeof S, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (W9)
     for (int k = 0; k < 9; k++) W9[k] = S.W9[k];
@@ -319,28 +350,54 @@ void fmd_atomic_config_default(fmd_atomic_config *c) {
 }
 
 int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
-  if (!c || !out || !c->mol_id || !c->mass || c->n_atoms < 1)
-    return set_error(FMD_ERR_ARG, "fmd_atomic_create: bad argument");
+  return fmd_batch_create(c, 1, out);
+}
+
+int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
+  if (!c || !out || R < 1) return set_error(FMD_ERR_ARG, "fmd_batch_create: bad argument");
+  for (int r = 0; r < R; r++) {
+    if (!c[r].mol_id || !c[r].mass || c[r].n_atoms < 1)
+      return set_error(FMD_ERR_ARG, "fmd_batch_create: replica %d: bad configuration", r);
+    const fmd_atomic_config &a = c[0], &b = c[r];
+    if (a.lj.sigma != b.lj.sigma || a.lj.eps != b.lj.eps || a.lj.rcut != b.lj.rcut ||
+        a.lj_skin != b.lj_skin || a.lj_max_neigh != b.lj_max_neigh ||
+        a.rebo_rlist != b.rebo_rlist || a.rebo_max_neigh != b.rebo_max_neigh ||
+        a.rebo_mode != b.rebo_mode || a.nlup != b.nlup || a.dt != b.dt ||
+        a.clamp_xi != b.clamp_xi || a.clamp_vg != b.clamp_vg)
+      return set_error(FMD_ERR_ARG,
+                       "fmd_batch_create: replica %d: potential, lists, dt, nlup and clamps "
+                       "must equal replica 0's",
+                       r);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return set_error(FMD_ERR_NODEVICE, "fmd_atomic_create: no HIP device");
-  std::vector<int> start;
+    return set_error(FMD_ERR_NODEVICE, "fmd_batch_create: no HIP device");
+  Layout L;
   int apad = 0;
-  int rc = mol_ranges(c->mol_id, c->n_atoms, start, apad);
+  int rc = build_layout(c, R, L, apad);
   if (rc) return rc;
   fmd_engine *e = new fmd_engine();
   e->kind = FMD_KIND_ATOMIC;
   e->at = new AtomicEngine();
+  AtomicEngine &A = *e->at;
   e->nlup = c->nlup > 0 ? c->nlup : 20;
-  fill_params(e->at->p, *c, (int)start.size() - 1, apad);
-  rc = engine_init_common(e, 2048);
-  if (rc == FMD_OK) rc = atomic_alloc(e->at->d, e->at->p, e->ring);
+  e->use_graph = R > 1;
+  fill_params(A.p, *c, L.rep_atom[R], (int)L.mol_rep.size(), R, apad);
+  A.rep_atom = L.rep_atom;
+  A.rep_mol = L.rep_mol;
+  A.mol_start = L.mol_start;
+  A.tt.assign(R, 0.0);
+  A.small.assign(R, 0);
+  const int ring = std::max(64, 2048 / R);
+  rc = engine_init_common(e, ring, R);
+  if (rc == FMD_OK) rc = atomic_alloc(A.d, A.p, e->ring, (int)L.blk_rep.size());
+  if (rc == FMD_OK) rc = upload_layout(A.d, L);
   if (rc == FMD_OK) {
-    hipError_t he = hipMemcpy(e->at->d.mol_start, start.data(), start.size() * sizeof(int),
-                              hipMemcpyHostToDevice);
-    if (he == hipSuccess)
-      he = hipMemcpy(e->at->d.mass, c->mass, c->n_atoms * sizeof(double), hipMemcpyHostToDevice);
-    if (he != hipSuccess) rc = hip_fail(he, "upload molecule layout");
+    hipError_t he = hipSuccess;
+    for (int r = 0; r < R && he == hipSuccess; r++)
+      he = hipMemcpy(A.d.mass + L.rep_atom[r], c[r].mass, c[r].n_atoms * sizeof(double),
+                     hipMemcpyHostToDevice);
+    if (he != hipSuccess) rc = hip_fail(he, "upload masses");
   }
   if (rc) {
     fmd_engine_destroy(e);
@@ -350,16 +407,23 @@ int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
   return FMD_OK;
 }
 
-int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel, const double *h,
-                         const fmd_npt_state *npt) {
+int fmd_batch_size(fmd_engine *e, int *R) {
+  if (!e || !R || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
+  *R = e->at->p.R;
+  return FMD_OK;
+}
+
+int fmd_batch_set_state(fmd_engine *e, int r, const double *pos, const double *vel,
+                        const double *h, const fmd_npt_state *npt) {
   if (!e || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
   AtomicEngine &A = *e->at;
+  if (r < 0 || r >= A.p.R) return set_error(FMD_ERR_ARG, "replica %d out of range", r);
   hipStream_t s = e->stream;
-  const size_t n3 = 3 * (size_t)A.p.Na * 8;
-  if (pos) HIPCHK(hipMemcpyAsync(A.d.pos, pos, n3, hipMemcpyHostToDevice, s));
-  if (vel) HIPCHK(hipMemcpyAsync(A.d.vel, vel, n3, hipMemcpyHostToDevice, s));
+  const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
+  if (pos) HIPCHK(hipMemcpyAsync(A.d.pos + 3 * a0, pos, 3 * na * 8, hipMemcpyHostToDevice, s));
+  if (vel) HIPCHK(hipMemcpyAsync(A.d.vel + 3 * a0, vel, 3 * na * 8, hipMemcpyHostToDevice, s));
   AtomicState S;
-  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, A.d.state + r, sizeof S, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (h) {
     for (int k = 0; k < 9; k++) S.h[k] = h[k];
@@ -373,31 +437,36 @@ int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel, co
     S.Pe = npt->Pe;
     S.Tt = npt->Tt;
     S.Nf = npt->Nf;
-    e->tt = npt->Tt;
+    A.tt[r] = npt->Tt;
+    if (A.p.R == 1) e->tt = npt->Tt;
   }
   S.scale = 1.0;
   if (pos) {
-    std::vector<int> start(A.p.Nm + 1);
-    HIPCHK(hipMemcpy(start.data(), A.d.mol_start, start.size() * sizeof(int),
-                     hipMemcpyDeviceToHost));
-    A.p.small_box = decide_small_box(A.p, S.h, max_radius(pos, start, S.h));
+    // this replica's molecule ranges, relative to its first atom
+    std::vector<int> start;
+    for (int m = A.rep_mol[r]; m <= A.rep_mol[r + 1]; m++) start.push_back(A.mol_start[m] - (int)a0);
+    A.small[r] = decide_small_box(A.p, S.h, max_radius(pos, start, S.h));
+    A.p.small_box = 0;
+    for (int x : A.small) A.p.small_box |= x;
   }
-  HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.state + r, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   e->prepared = false;
+  engine_drop_graphs(e);  // captured launches hold the old parameters
   return FMD_OK;
 }
 
-int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
-                         fmd_npt_state *npt) {
+int fmd_batch_get_state(fmd_engine *e, int r, double *pos, double *vel, double *h,
+                        fmd_npt_state *npt) {
   if (!e || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
   AtomicEngine &A = *e->at;
+  if (r < 0 || r >= A.p.R) return set_error(FMD_ERR_ARG, "replica %d out of range", r);
   hipStream_t s = e->stream;
-  const size_t n3 = 3 * (size_t)A.p.Na * 8;
-  if (pos) HIPCHK(hipMemcpyAsync(pos, A.d.pos, n3, hipMemcpyDeviceToHost, s));
-  if (vel) HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
+  const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
+  if (pos) HIPCHK(hipMemcpyAsync(pos, A.d.pos + 3 * a0, 3 * na * 8, hipMemcpyDeviceToHost, s));
+  if (vel) HIPCHK(hipMemcpyAsync(vel, A.d.vel + 3 * a0, 3 * na * 8, hipMemcpyDeviceToHost, s));
   AtomicState S;
-  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, A.d.state + r, sizeof S, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (h)
     for (int k = 0; k < 9; k++) h[k] = S.h[k];
@@ -411,6 +480,21 @@ int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
     npt->Nf = S.Nf;
   }
   return FMD_OK;
+}
+
+int fmd_batch_forces(fmd_engine *e, int r, double *F, double *W9, double *energies) {
+  if (!e || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
+  return replica_forces(e, r, F, W9, energies);
+}
+
+int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel, const double *h,
+                         const fmd_npt_state *npt) {
+  return fmd_batch_set_state(e, 0, pos, vel, h, npt);
+}
+
+int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
+                         fmd_npt_state *npt) {
+  return fmd_batch_get_state(e, 0, pos, vel, h, npt);
 }
 
 }  // extern "C"
@@ -460,10 +544,7 @@ int stage_positions(fmd_engine *e, const double *pos, const double *h, const dou
   }
   HIPCHK(hipMemcpyAsync(A.d.pos, pos, 3 * (size_t)A.p.Na * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(A.d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
-  std::vector<int> start(A.p.Nm + 1);
-  HIPCHK(hipMemcpy(start.data(), A.d.mol_start, start.size() * sizeof(int),
-                   hipMemcpyDeviceToHost));
-  A.p.small_box = decide_small_box(A.p, h, max_radius(pos, start, h));
+  A.p.small_box = decide_small_box(A.p, h, max_radius(pos, A.mol_start, h));
   return FMD_OK;
 }
 
@@ -502,8 +583,8 @@ int fmd_airebo_nlist_rebo(const double *pos, const double *h, const double *hi, 
   AtomicEngine &A = *e->at;
   hipStream_t s = e->stream;
   HIPCHK(hipMemsetAsync(A.d.nll_r, 0, (size_t)Na * kRefMaxRebo * sizeof(int), s));
-  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
-  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 1, s));
+  HIPCHK(at_launch_centers(A.d, A.p, hh_fixed(A.d.hh), s));
+  HIPCHK(at_launch_lists(A.d, A.p, hh_fixed(A.d.hh), 1, s));
   HIPCHK(hipMemcpyAsync(nlc, A.d.nlc_r, (size_t)Na * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(nll, A.d.nll_r, (size_t)Na * kRefMaxRebo * sizeof(int),
                         hipMemcpyDeviceToHost, s));
@@ -523,9 +604,9 @@ int fmd_airebo_nlist_lj(const double *pos, const double *h, const double *hi, in
   HIPCHK(hipMallocAsync((void **)&dl, (size_t)Na * kRefMaxLj * sizeof(int), s));
   HIPCHK(hipMallocAsync((void **)&dc, (size_t)Na * sizeof(int), s));
   HIPCHK(hipMemsetAsync(dl, 0, (size_t)Na * kRefMaxLj * sizeof(int), s));
-  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
-  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 0, s));
-  HIPCHK(at_launch_halflist(A.d, A.p, A.d.hh, dc, dl, kRefMaxLj, s));
+  HIPCHK(at_launch_centers(A.d, A.p, hh_fixed(A.d.hh), s));
+  HIPCHK(at_launch_lists(A.d, A.p, hh_fixed(A.d.hh), 0, s));
+  HIPCHK(at_launch_halflist(A.d, A.p, hh_fixed(A.d.hh), dc, dl, kRefMaxLj, s));
   HIPCHK(hipMemcpyAsync(nlc, dc, (size_t)Na * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(nll, dl, (size_t)Na * kRefMaxLj * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipFreeAsync(dl, s));
@@ -547,11 +628,11 @@ int fmd_airebo_compute_forces(double *F, double *vir9, const double *pos, const 
   hipStream_t s = e->stream;
   const int lj_compat = A.p.lj_compat;
   A.p.lj_compat = 0;  // the molecule list only; jmax comes from the caller's list
-  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
-  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 0, s));
+  HIPCHK(at_launch_centers(A.d, A.p, hh_fixed(A.d.hh), s));
+  HIPCHK(at_launch_lists(A.d, A.p, hh_fixed(A.d.hh), 0, s));
   A.p.lj_compat = lj_compat;
   if ((rc = stage_lists(e, nlc_r, nll_r, nlc_l, nll_l))) return rc;
-  HIPCHK(at_launch_forces(A.d, A.p, A.d.hh, s));
+  HIPCHK(at_launch_forces(A.d, A.p, hh_fixed(A.d.hh), s));
   HIPCHK(at_launch_observe(A.d, A.p, kAtObsForce, s));
   AtomicState S;
   HIPCHK(hipMemcpyAsync(F, A.d.F, 3 * (size_t)Na * 8, hipMemcpyDeviceToHost, s));
@@ -587,8 +668,8 @@ int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doubl
   // the molecule list from the current positions, then the caller's lists
   const int lj_compat = A.p.lj_compat;
   A.p.lj_compat = 0;
-  HIPCHK(at_launch_centers(A.d, A.p, A.d.hh, s));
-  HIPCHK(at_launch_lists(A.d, A.p, A.d.hh, 0, s));
+  HIPCHK(at_launch_centers(A.d, A.p, hh_fixed(A.d.hh), s));
+  HIPCHK(at_launch_lists(A.d, A.p, hh_fixed(A.d.hh), 0, s));
   A.p.lj_compat = lj_compat;
   if ((rc = stage_lists(e, nlc_r, nll_r, nlc_l, nll_l))) return rc;
   AtomicState S;
@@ -610,7 +691,7 @@ int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doubl
   HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(A.d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
   HIPCHK(at_launch_observe(A.d, A.p, kAtObsKE, s));
-  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, nullptr, s));
+  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, nullptr, 0, s));
   HIPCHK(hipMemcpyAsync(pos, A.d.pos, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(F, A.d.F, n3, hipMemcpyDeviceToHost, s));

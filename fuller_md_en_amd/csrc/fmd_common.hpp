@@ -180,7 +180,12 @@ FMD_HD float rcp32(float x) {
 // caller applies 48 eps once per partial sum); adds the reduced energy
 // sr6 (sr6 - 1) - vshift / (4 eps) into e.  One reciprocal, 10 VALU ops.
 FMD_HD float lj_fm(float r2, const LJ32 &p, float &e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // max(r2, rmin2) for finite r2 in one v_med3_f32 (fmaxf adds canonicalizes)
+  float ri2 = rcp32(__builtin_amdgcn_fmed3f(r2, p.rmin2, 3.0e38f));
+#else
   float ri2 = rcp32(fmaxf(r2, p.rmin2));
+#endif
   float sr2 = p.sig2 * ri2;
   float sr6 = sr2 * sr2 * sr2;
   e += sr6 * (sr6 - 1.0f) - p.vs4;

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <string>
 #include <vector>
 
 #include "../../include/fmd.h"
@@ -44,14 +46,34 @@ struct fmd_engine {
   std::vector<hipEvent_t> ev;
   double t_sum = 0;
   int t_n = 0;
+  // hipGraph replay of step sequences: executable graphs keyed by the step
+  // pattern they were captured from
+  bool use_graph = false;
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
 };
 
 namespace fmd {
-int engine_init_common(fmd_engine *e, int ring);
+int engine_init_common(fmd_engine *e, int ring, int R);
+// Launch of one step into control / record slot `slot`.
+using StepLaunch = std::function<hipError_t(int slot, int rebuild, int rescale, hipEvent_t ev0,
+                                            hipEvent_t ev1, unsigned ev_flags)>;
+// Steps first_step+1 .. first_step+nsteps of R replicas: controls
+// ctrl[k * R + r] (null: hold each replica's target tt[r]), records
+// thermo[k * R + r]; list rebuild every e->nlup global steps or on request.
+// Replays captured graphs when e->use_graph.
+int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
+              fmd_thermo *thermo, double *tt, StepCtrl *d_ctrl, Thermo *d_thermo,
+              const StepLaunch &launch);
+// The executable graph for `key`, captured from `launch` (which enqueues on
+// e->stream) the first time the key is seen.
+int engine_graph(fmd_engine *e, const std::string &key, const std::function<int()> &launch,
+                 hipGraphExec_t *out);
+void engine_drop_graphs(fmd_engine *e);
 // Atomic (AIREBO) engine, fmd_atomic_api.hip.
 int engine_prepare_atomic(fmd_engine *e);
 int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctrl *ctrl,
                       fmd_thermo *thermo);
 int engine_forces_atomic(fmd_engine *e, double *F, double *T, double *W9, double *energies);
+int replica_forces(fmd_engine *e, int r, double *F, double *W9, double *energies);
 void engine_free_atomic(fmd_engine *e);
 }  // namespace fmd

@@ -114,3 +114,23 @@ def rigid_system(nc: int = 3, spec: str = "C60", crystal: str = "FCC", T: float 
     pos, h = make_crystal(crystal, a0, nc)
     vel, omg, qv = rigid_init_state(len(pos), seed, T, mol)
     return dict(mol=mol, a0=a0, pos=pos, h=h, vel=vel, omg=omg, qv=qv)
+
+
+def airebo_system(nc: int = 3, spec: str = "C60", crystal: str = "FCC", T: float = 300.0,
+                  seed: int = 42, init_scale: float = 1.0, jitter: float = 0.0,
+                  jitter_seed: int = 12345, libdir: str = LIBDIR):
+    """The AIREBO program's initial system (AB:763-815): atoms at lattice
+    point + .cc1 coordinates (no rotation), mol_id = molecule + 1, carbon
+    masses, per-atom Maxwell velocities from `seed`; optional uniform jitter
+    of +-jitter A per coordinate from its own generator."""
+    mol = molecule(spec, libdir)
+    a0 = default_a0(mol.dmol, crystal, init_scale)
+    cen, h = make_crystal(crystal, a0, nc)
+    pos = (cen[:, None, :] + mol.coords[None]).reshape(-1, 3)
+    if jitter:
+        pos = pos + np.random.default_rng(jitter_seed).uniform(-jitter, jitter, pos.shape)
+    mol_id = np.repeat(np.arange(1, len(cen) + 1), mol.natom).astype(np.int32)
+    mass = np.full(len(pos), MASS_C)
+    vel = atomic_init_velocities(mass, seed, T)
+    return dict(mol=mol, a0=a0, pos=pos, h=h, mol_id=mol_id, mass=mass, vel=vel,
+                n_mol=len(cen))

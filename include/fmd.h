@@ -222,6 +222,24 @@ int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel,
 int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
                          fmd_npt_state *npt);
 
+/* ---------------------------------------------------- batched replicas
+ * R independent AIREBO NPT systems (sizes, species, T and P may differ) in
+ * one engine: every kernel launch of a step covers all replicas, and steps
+ * are replayed from captured HIP graphs.  No reference counterpart (the
+ * reference runs one system per process); replica r alone follows the
+ * AIREBO program's step_npt (AB:518-590).  cfg[r] gives replica r's atoms;
+ * potential, list, dt, nlup and clamp fields must equal cfg[0]'s.
+ * fmd_engine_run then takes ctrl and thermo arrays of nsteps x R entries,
+ * ctrl[k * R + r] / thermo[k * R + r] for step k of replica r. */
+int fmd_batch_create(const fmd_atomic_config *cfg, int R, fmd_engine **out);
+int fmd_batch_size(fmd_engine *e, int *R);
+int fmd_batch_set_state(fmd_engine *e, int r, const double *pos, const double *vel,
+                        const double *h, const fmd_npt_state *npt);
+int fmd_batch_get_state(fmd_engine *e, int r, double *pos, double *vel, double *h,
+                        fmd_npt_state *npt);
+/* Replica r's forces (its atoms x 3), W9 and {Ep, Ep_rebo, Ep_lj}. */
+int fmd_batch_forces(fmd_engine *e, int r, double *F, double *W9, double *energies);
+
 /* Per-step phase control (the COLD / WARM / PROD logic of LJ:848-894). */
 #define FMD_CTRL_ZERO_XI_VG 1  /* xi = 0, Vg = 0 before the step */
 #define FMD_CTRL_ZERO_VG 2     /* Vg = 0 before the step */
@@ -265,6 +283,10 @@ int fmd_engine_counters(fmd_engine *e, long long *out3);
  * fmd_engine_set_timing(e, 1)).  *n receives the number of launches. */
 int fmd_engine_set_timing(fmd_engine *e, int on);
 int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n);
+
+/* Replay step sequences from captured HIP graphs (1) or launch every kernel
+ * (0).  Default: on for batches of more than one replica, off otherwise. */
+int fmd_engine_set_graph(fmd_engine *e, int on);
 
 /* Wait for all work of the engine. */
 int fmd_engine_sync(fmd_engine *e);

@@ -1,0 +1,94 @@
+"""Batched AIREBO replicas (fmd_batch_*): R independent NPT systems of
+uneven size, species, temperature and pressure in one engine.
+
+* Each replica of a batch evolves bit for bit as the same system run alone
+  in its own engine (same kernels, replica-local reduction order), across a
+  list rebuild, with graph replay on and off.
+* Forces of every replica against the FP64 oracle (compat REBO), 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_max
+from fuller_md_en_amd import airebo, system
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+# (species, nc, T, P, jitter seed): C70 tiles are wider than C60's, so the
+# batch pads every molecule tile to 72 atoms (two atoms per lane)
+SPECS = [("C60", 3, 300.0, 0.0, 1), ("C70", 3, 900.0, 2.0, 2), ("C60", 3, 1500.0, 5.0, 3)]
+
+
+def _systems():
+    out = []
+    for spec, nc, T, P, js in SPECS:
+        s = system.airebo_system(nc, spec, T=T, seed=40 + js, jitter=0.01, jitter_seed=js)
+        s["T"], s["P"] = T, P
+        out.append(s)
+    return out
+
+
+def _npt(s):
+    return airebo.make_npt(s["T"], s["P"], len(s["pos"]))
+
+
+def _run_batch(systems, nsteps, graph):
+    b = airebo.ReplicaBatch([(s["mol_id"], s["mass"]) for s in systems], graph=graph)
+    for r, s in enumerate(systems):
+        b.set_state(r, pos=s["pos"], vel=s["vel"], h=s["h"], npt=_npt(s))
+    th = b.run(0, nsteps)
+    st = [b.get_state(r) for r in range(len(systems))]
+    ep = np.array([[[t.Ep, t.KE, t.P, t.V] for t in row] for row in th])
+    assert b.counters()[0] == 0
+    b.close()
+    return st, ep
+
+
+def test_batch_replicas_equal_single_runs():
+    systems = _systems()
+    nsteps = 25  # crosses the list rebuild at step 20
+    st_b, ep_b = _run_batch(systems, nsteps, graph=True)
+    for r, s in enumerate(systems):
+        print(f"replica {r} {SPECS[r]}", flush=True)
+        eng = airebo.AtomicEngine(s["mol_id"], s["mass"])
+        eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"], npt=_npt(s))
+        th = eng.run(0, nsteps)
+        st = eng.get_state()
+        eng.close()
+        ep = np.array([[t.Ep, t.KE, t.P, t.V] for t in th])
+        assert np.array_equal(ep, ep_b[:, r]), r
+        assert np.array_equal(st["pos"], st_b[r]["pos"]) and np.array_equal(st["vel"], st_b[r]["vel"])
+        assert np.array_equal(st["h"], st_b[r]["h"])
+
+
+def test_batch_graph_replay_equals_launches():
+    systems = _systems()[:2]
+    st1, ep1 = _run_batch(systems, 23, graph=True)
+    st2, ep2 = _run_batch(systems, 23, graph=False)
+    assert np.array_equal(ep1, ep2)
+    for a, b in zip(st1, st2):
+        assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
+
+
+def test_batch_replica_forces_match_oracle():
+    systems = _systems()
+    b = airebo.ReplicaBatch([(s["mol_id"], s["mass"]) for s in systems])
+    for r, s in enumerate(systems):
+        b.set_state(r, pos=s["pos"], vel=s["vel"], h=s["h"], npt=_npt(s))
+    b.prepare()
+    for r, s in enumerate(systems):
+        print(f"replica {r} {SPECS[r]}", flush=True)
+        pos = s["pos"].reshape(-1).copy()
+        oracle.lib().orc_apply_pbc(pos, s["h"], oracle.inv9(s["h"]), len(s["pos"]))
+        pos = pos.reshape(-1, 3)
+        lists = (*oracle.nlist_rebo(pos, s["h"], s["mol_id"]),
+                 *oracle.nlist_lj(pos, s["h"], s["mol_id"]))
+        E0, F0, W0 = oracle.airebo_forces(pos, s["h"], lists, oracle.REBO_COMPAT)
+        E, F, W = b.forces(r)
+        assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
+        for x, y in zip(E, E0):
+            assert abs(x - y) / abs(y) < TOL
+    b.close()
