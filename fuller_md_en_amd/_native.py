@@ -116,6 +116,20 @@ _SIGS = {
     "fmd_batch_get_state": (_i, [_vp, _i, _vp, _vp, _vp, C.POINTER(NPTState)]),
     "fmd_batch_forces": (_i, [_vp, _i, _vp, _vp, _vp]),
     "fmd_engine_set_graph": (_i, [_vp, _i]),
+    "fmd_sg_param_count": (_i, []),
+    "fmd_sg_feature_count": (_i, []),
+    "fmd_sg_attach": (_i, [_vp, _i]),
+    "fmd_sg_set_params": (_i, [_vp, _vp, _vp, _vp]),
+    "fmd_sg_get_params": (_i, [_vp, _vp, _vp, _vp]),
+    "fmd_sg_collect": (_i, [_vp]),
+    "fmd_sg_frame_count": (_i, [_vp, C.POINTER(_i)]),
+    "fmd_sg_fit_normalisation": (_i, [_vp]),
+    "fmd_sg_train": (_i, [_vp, _i, _d, _vp]),
+    "fmd_sg_loss_grad": (_i, [_vp, _i, _vp, _vp]),
+    "fmd_sg_descriptor": (_i, [_vp, _i, _vp]),
+    "fmd_sg_evaluate": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "fmd_sg_enable": (_i, [_vp, _i, _i, _d]),
+    "fmd_sg_stats": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "fmd_engine_prepare": (_i, [_vp]),
     "fmd_engine_run": (_i, [_vp, _i, _i, _vp, _vp]),
     "fmd_engine_forces": (_i, [_vp, _vp, _vp, _vp, _vp]),

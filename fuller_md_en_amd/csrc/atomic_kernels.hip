@@ -26,6 +26,7 @@
 #include <cstddef>
 
 #include "fmd_atomic.hpp"
+#include "fmd_surrogate.hpp"
 
 namespace fmd {
 
@@ -853,13 +854,28 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef h
   return hipGetLastError();
 }
 
-hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
-                            hipStream_t s) {
+hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
+                          hipStream_t s) {
   k_rebo_pass1<<<(p.Na * p.maxr + 255) / 256, 256, 0, s>>>(d, p, hh);
-  if (p.rebo_mode == kReboPhysical)
+  if (mode == kReboPhysical)
     k_rebo_pass2<kReboPhysical><<<d.nblk, 256, 0, s>>>(d, p);
   else
     k_rebo_pass2<kReboCompat><<<d.nblk, 256, 0, s>>>(d, p);
+  return hipGetLastError();
+}
+
+hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                            hipStream_t s) {
+  hipError_t e;
+  if (p.use_sg) {
+    // the learned surrogate in place of compute_rebo: Frebo and rebo_part
+    if ((e = sg_launch_descr(d, p, d.sgw, d.sgd, s)) != hipSuccess) return e;
+    if ((e = sg_launch_mlp(p, d.sgw, d.sgd, s)) != hipSuccess) return e;
+    if ((e = sg_launch_force(d, p, d.sgw, d.sgd, d.Frebo, s)) != hipSuccess) return e;
+    if ((e = sg_launch_parts(d, d.sgd, d.rebo_part, s)) != hipSuccess) return e;
+  } else if ((e = at_launch_rebo(d, p, hh, p.rebo_mode, s)) != hipSuccess) {
+    return e;
+  }
   const int per = (p.Nm + kXcd - 1) / kXcd;
   dim3 g(per * kXcd);
   const bool one = ns_of(p) == 1;

@@ -39,6 +39,7 @@
 #include "fmd_common.hpp"
 #include "fmd_rebo_passes.hpp"
 #include "fmd_rigid.hpp"  // StepCtrl, Thermo
+#include "fmd_sg_types.hpp"
 
 namespace fmd {
 
@@ -68,6 +69,7 @@ struct AtomicParams {
   int lj_compat;   // 1: pairs limited by the reference's truncated half list
   int rebo_mode;   // kReboCompat / kReboPhysical
   int small_box;   // 1: per-pair minimum image (a cell narrower than the list)
+  int use_sg;      // 1: the learned surrogate stands in for the REBO term
   double rebo_rlist2;  // REBO list radius^2 (2.3^2, AB:253)
   double lj_rlist2;    // LJ half-list radius^2 ((rc + 2)^2, AB:265)
   double mol_rlist;    // molecule list radius beyond the two molecule radii
@@ -96,6 +98,8 @@ struct AtomicDev {
   int *mol_rep, *atom_rep, *rep_mol, *rep_atom;
   int *blk_a0, *blk_a1, *blk_rep, *rep_blk;
   int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)
+  SgWeights sgw;  // learned surrogate (allocated when attached)
+  SgDev sgd;
 };
 
 // h and h^-1 of each replica: 18 doubles at p + r * stride (stride 0: one
@@ -141,6 +145,10 @@ hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
                           int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
                           hipEvent_t ev1, unsigned ev_flags, hipStream_t s);
+// REBO forces alone into Frebo and rebo_part, in the given mode (needs
+// centres and the REBO list).
+hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
+                          hipStream_t s);
 // jmax from a reference-layout half list (stride cap) for the drop-in path.
 hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
                                     const int *nlc_l, const int *nll_l, int cap,
@@ -149,9 +157,12 @@ hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
 hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                               int *nlc_l, int *nll_l, int cap, hipStream_t s);
 
+struct Surrogate;  // fmd_surrogate_api.hip
+
 struct AtomicEngine {
   AtomicParams p{};
   AtomicDev d{};
+  Surrogate *sg = nullptr;
   // host copies of the replica tables
   std::vector<int> rep_atom, rep_mol, mol_start;
   std::vector<double> tt;  // per replica: target temperature (host mirror)

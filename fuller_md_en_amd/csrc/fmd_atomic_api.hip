@@ -281,9 +281,21 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
                     unsigned evf) -> hipError_t {
     return at_launch_step(d, p, slot, slot, rebuild, rescale, e0, e1, evf, e->stream);
   };
-  int rc = run_steps(e, p.R, first_step, nsteps, ctrl, thermo, A.tt.data(), d.ctrl, d.thermo,
-                     launch);
-  if (rc) return rc;
+  // with the learned surrogate on, stop at every judge step (global step a
+  // multiple of judge_every) to compare it with full REBO
+  int done = 0, every = 0, rc = FMD_OK;
+  while (done < nsteps) {
+    int n = nsteps - done;
+    const bool judge = surrogate_judging(e, &every);
+    if (judge) n = std::min(n, every - (int)(((long long)first_step + done) % every));
+    rc = run_steps(e, p.R, first_step + done, n, ctrl ? ctrl + (size_t)done * p.R : nullptr,
+                   thermo ? thermo + (size_t)done * p.R : nullptr, A.tt.data(), d.ctrl,
+                   d.thermo, launch);
+    if (rc) return rc;
+    done += n;
+    if (judge && ((long long)first_step + done) % every == 0 && (rc = surrogate_judge(e)))
+      return rc;
+  }
   int f[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -326,6 +338,7 @@ int replica_forces(fmd_engine *e, int r, double *F, double *W9, double *energies
 
 void engine_free_atomic(fmd_engine *e) {
   if (!e->at) return;
+  engine_free_surrogate(*e->at);
   free_atomic_dev(e->at->d);
   delete e->at;
   e->at = nullptr;

@@ -75,5 +75,9 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
                       fmd_thermo *thermo);
 int engine_forces_atomic(fmd_engine *e, double *F, double *T, double *W9, double *energies);
 int replica_forces(fmd_engine *e, int r, double *F, double *W9, double *energies);
+// Learned surrogate (fmd_surrogate_api.hip).
+void engine_free_surrogate(AtomicEngine &A);
+bool surrogate_judging(fmd_engine *e, int *every);
+int surrogate_judge(fmd_engine *e);
 void engine_free_atomic(fmd_engine *e);
 }  // namespace fmd

@@ -240,6 +240,43 @@ int fmd_batch_get_state(fmd_engine *e, int r, double *pos, double *vel, double *
 /* Replica r's forces (its atoms x 3), W9 and {Ep, Ep_rebo, Ep_lj}. */
 int fmd_batch_forces(fmd_engine *e, int r, double *F, double *W9, double *energies);
 
+/* ---------------------------------------------- learned REBO surrogate
+ * No counterpart in the reference: a per-atom energy model
+ * E = sum_i MLP(D_i) (40 invariant same-molecule descriptors within 6 A,
+ * 3 tanh layers of width 64, BF16 MFMA) whose forces are exact gradients,
+ * standing in for the exact-gradient REBO term (compute_rebo AB:275-421,
+ * FMD_REBO_PHYSICAL).  Trained on the GPU on frames the engine generates;
+ * while enabled, full REBO re-runs every judge_every steps and the engine
+ * falls back to it for good when the force RMSE exceeds the threshold.
+ * Atomic engines only; frames and training need a single-system engine.
+ * theta: fmd_sg_param_count() floats (W1 [40][64], b1, W2 [64][64], b2,
+ * W3 [64][64], b3, w4 [64], b4); mu, sd: fmd_sg_feature_count() floats. */
+int fmd_sg_param_count(void);
+int fmd_sg_feature_count(void);
+int fmd_sg_attach(fmd_engine *e, int seed);
+int fmd_sg_set_params(fmd_engine *e, const float *theta, const float *mu, const float *sd);
+int fmd_sg_get_params(fmd_engine *e, float *theta, float *mu, float *sd);
+/* Store the current positions and their exact-gradient REBO forces as a
+ * training frame. */
+int fmd_sg_collect(fmd_engine *e);
+int fmd_sg_frame_count(fmd_engine *e, int *n);
+/* mu, sd from the descriptors of the stored frames. */
+int fmd_sg_fit_normalisation(fmd_engine *e);
+/* iters Adam steps (one frame each, cycling) on the mean squared force
+ * error; loss[it] = that error before step it (eV^2/A^2), or NULL. */
+int fmd_sg_train(fmd_engine *e, int iters, double lr, double *loss);
+/* Loss and its gradient over theta on one stored frame (no update). */
+int fmd_sg_loss_grad(fmd_engine *e, int frame, double *loss, float *grad);
+/* Normalised descriptors of a stored frame (Na x 40). */
+int fmd_sg_descriptor(fmd_engine *e, int frame, float *feat);
+/* Surrogate forces (Na x 3), virial, energy on a stored frame (frame >= 0,
+ * rmse against its REBO forces) or on the current state (frame < 0). */
+int fmd_sg_evaluate(fmd_engine *e, int frame, double *F, double *W9, double *E,
+                    double *rmse);
+int fmd_sg_enable(fmd_engine *e, int on, int judge_every, double threshold);
+int fmd_sg_stats(fmd_engine *e, double *last_rmse, int *judged, int *fallbacks,
+                 int *active);
+
 /* Per-step phase control (the COLD / WARM / PROD logic of LJ:848-894). */
 #define FMD_CTRL_ZERO_XI_VG 1  /* xi = 0, Vg = 0 before the step */
 #define FMD_CTRL_ZERO_VG 2     /* Vg = 0 before the step */
