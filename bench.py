@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=-1,
                     help="untimed steps first (default: 100 rigid / airebo, 20 batch)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="rigid workload: default parameters instead of the autotuner's")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the CPU baseline sample (default: auto, ~20 s)")
     ap.add_argument("--workload", choices=["rigid", "airebo", "batch", "surrogate"],
@@ -63,17 +65,37 @@ def parse():
     return ap.parse_args()
 
 
-def build_system():
+def build_system(params=None):
+    """The rigid workload's engine; params: autotuner parameter set (None:
+    the defaults)."""
     from fuller_md_en_amd import rigid, system
+    params = params or {k: v[0] for k, v in __import__(
+        "fuller_md_en_amd.autotune", fromlist=["RIGID_SPACE"]).RIGID_SPACE.items()}
     mol = system.molecule("C60")
     a0 = system.default_a0(mol.dmol, "FCC", 1.0)
     scale = A_LATTICE / a0
     sysd = system.rigid_system(nc=NC, spec="C60", T=T_K, seed=SEED, init_scale=scale)
     eng = rigid.RigidEngine(mol.coords, len(sysd["pos"]), mol.I0, mol.mass, rmol=mol.rmol,
-                            dt=1.0)
+                            dt=1.0, skin=params["skin"], nlup=params["nlup"],
+                            half_pairs=bool(params["half_pairs"]))
+    eng.set_graph(bool(params["graph"]))
     eng.set_state(pos=sysd["pos"], vel=sysd["vel"], qv=sysd["qv"], omg=sysd["omg"],
                   h=sysd["h"], npt=rigid.make_npt(T_K, 0.0, len(sysd["pos"])))
     return eng, mol, sysd, scale
+
+
+def tuned_params(args):
+    """Autotuner choice for the rigid workload (cached per device)."""
+    from fuller_md_en_amd import autotune
+    if args.no_tune:
+        return {k: v[0] for k, v in autotune.RIGID_SPACE.items()}, None
+    key = f"{autotune.device_name()}|rigid|C60-FCC{NC}|a={A_LATTICE}"
+
+    def build(p):
+        return build_system(p)[0]
+    entry = autotune.tune(key, autotune.rigid_runner(build, steps=200, warmup=20),
+                          autotune.RIGID_SPACE)
+    return entry["params"], entry
 
 
 def cpu_baseline(scale: float, steps: int):
@@ -353,7 +375,8 @@ def main():
         return main_batch(args)
     if args.workload == "surrogate":
         return main_surrogate(args)
-    eng, mol, sysd, scale = build_system()
+    params, tune_entry = tuned_params(args)
+    eng, mol, sysd, scale = build_system(params)
     n_sites = len(sysd["pos"]) * mol.natom
     eng.prepare()
     step0 = 0
@@ -377,11 +400,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "rigid-LJ C60 FCC 10x10x10 (4000 mol, 240000 sites), "
                                "a=14.17 A, 300 K, 0 GPa, dt=1 fs, NPT",
-                   "sites": n_sites, "precision": "fp32 pair forces, fp64 integrator/virial"},
+                   "sites": n_sites, "precision": "fp32 pair forces, fp64 integrator/virial",
+                   "tuned": params,
+                   "untuned_ms_per_step": (tune_entry["default_seconds"] * 1e3 / 200
+                                           if tune_entry else None)},
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_rigid_forces", "kernel_ms": kms, "launches": kn},
+                     "kernel": "k_rigid_forces_half" if params["half_pairs"] else "k_rigid_forces",
+                     "kernel_ms": kms, "launches": kn},
         "final": {"T": last.T, "P": last.P, "Ep_per_mol": last.Ep / len(sysd["pos"])},
     }
     if not args.no_cpu:

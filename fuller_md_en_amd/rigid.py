@@ -167,6 +167,9 @@ class RigidEngine:
     def set_timing(self, on=True):
         N.check(N.lib().fmd_engine_set_timing(self._h, 1 if on else 0))
 
+    def set_graph(self, on=True):
+        N.check(N.lib().fmd_engine_set_graph(self._h, 1 if on else 0))
+
     def kernel_time(self):
         ms, n = C.c_double(0), C.c_int(0)
         N.check(N.lib().fmd_engine_kernel_time(self._h, C.byref(ms), C.byref(n)))

@@ -65,7 +65,8 @@ def test_rebo_physical_forces_are_the_gradient():
 
 
 def test_rebo_compat_forces_are_not_the_gradient():
-    """SURVEY finding 1: the reference's REBO forces differ from -dE/dx."""
+    """The reference's REBO forces (compat mode) differ from -dE/dx: the angular term
+    carries the opposite sign and no d cos / d r_partner term (AB:385-386, 413-419)."""
     pos, h, mol = _sys()
     _, Fp, _ = _rebo_E(pos, h, mol, oracle.REBO_PHYSICAL)
     _, Fc, _ = _rebo_E(pos, h, mol, oracle.REBO_COMPAT)
