@@ -252,18 +252,60 @@ __global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams 
     jm[s] = INT_MAX;
     done[s] = lane + 64 * s >= n;
   }
+  // FP32 prefilter through the molecule tiles (when the cell is wide enough
+  // that the centres' minimum image is every listed atom pair's): only pairs
+  // within 1e-5 relative of the list radius take the reference's FP64 test.
+  const bool fast = p.scan_fast;
+  const float rl2 = (float)p.lj_rlist2, lo = rl2 * (1.0f - 1e-5f), hi2 = rl2 * (1.0f + 1e-5f);
+  const float rl = sqrtf(rl2);
+  const double cix = d.center[3 * m], ciy = d.center[3 * m + 1], ciz = d.center[3 * m + 2];
+  float oax[NS], oay[NS], oaz[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    const float4 o = a < n ? d.off[(size_t)m * p.apad + a] : make_float4(3e5f, 3e5f, 3e5f, 0.f);
+    oax[s] = o.x;
+    oay[s] = o.y;
+    oaz[s] = o.z;
+  }
   const int nq = d.mol_nl_count[m];
   for (int k = 0; k < nq; k++) {
     const int q = uniform(d.mol_nl[(size_t)m * p.mol_cap + k]);
     if (q < m) continue;  // molecule ranges ascend: its atoms are all < i
     const int t0 = d.mol_start[q], tn = d.mol_start[q + 1] - t0;
+    float cx[NS], cy[NS], cz[NS];
+    if (fast) {
+      double dx = d.center[3 * q] - cix, dy = d.center[3 * q + 1] - ciy,
+             dz = d.center[3 * q + 2] - ciz;
+      mimg(dx, dy, dz, hi, h);
+      const float reach = rl + d.radius[q] + 0.05f;
+      bool near = false;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        cx[s] = (float)(dx - (double)oax[s]);
+        cy[s] = (float)(dy - (double)oay[s]);
+        cz[s] = (float)(dz - (double)oaz[s]);
+        near |= !done[s] && cx[s] * cx[s] + cy[s] * cy[s] + cz[s] * cz[s] < reach * reach;
+      }
+      if (!__any(near)) continue;  // no atom of q within the list radius
+    }
+    const float4 *tq = d.off + (size_t)q * p.apad;
     for (int b = 0; b < tn; b++) {
       const int j = t0 + b;
+      const float4 rb = tq[b];
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         if (done[s]) continue;
         const int i = s0 + lane + 64 * s;
-        if (r2_ref(d.pos, i, j, h, hi) < p.lj_rlist2) {
+        bool in;
+        if (fast) {
+          const float ex = rb.x + cx[s], ey = rb.y + cy[s], ez = rb.z + cz[s];
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          in = r2 < lo || (r2 < hi2 && r2_ref(d.pos, i, j, h, hi) < p.lj_rlist2);
+        } else {
+          in = r2_ref(d.pos, i, j, h, hi) < p.lj_rlist2;
+        }
+        if (in) {
           if (WRITE) nll_l[(size_t)i * cap + cnt[s]] = j;
           cnt[s]++;
           if (cnt[s] == cap) {

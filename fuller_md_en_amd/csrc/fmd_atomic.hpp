@@ -70,6 +70,8 @@ struct AtomicParams {
   int rebo_mode;   // kReboCompat / kReboPhysical
   int small_box;   // 1: per-pair minimum image (a cell narrower than the list)
   int use_sg;      // 1: the learned surrogate stands in for the REBO term
+  int scan_fast;   // 1: list walk prefiltered in FP32 through the molecule
+                   // tiles (cell wide enough for the list radius + 2 R_max)
   double rebo_rlist2;  // REBO list radius^2 (2.3^2, AB:253)
   double lj_rlist2;    // LJ half-list radius^2 ((rc + 2)^2, AB:265)
   double mol_rlist;    // molecule list radius beyond the two molecule radii
@@ -167,6 +169,7 @@ struct AtomicEngine {
   std::vector<int> rep_atom, rep_mol, mol_start;
   std::vector<double> tt;  // per replica: target temperature (host mirror)
   std::vector<int> small;  // per replica: cell narrower than the LJ reach
+  std::vector<int> fast;   // per replica: FP32-prefiltered list walk allowed
 };
 
 }  // namespace fmd

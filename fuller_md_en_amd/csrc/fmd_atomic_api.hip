@@ -243,6 +243,12 @@ int decide_small_box(const AtomicParams &p, const double *h, double rmax) {
   return (rc + 2.0 * rmax + 1.0) >= 0.5 * min_width(h) ? 1 : 0;
 }
 
+// The list walk may go through the molecule tiles when every atom pair within
+// the list radius is the centres' minimum image plus the two offsets.
+int decide_scan_fast(const AtomicParams &p, const double *h, double rmax) {
+  return (std::sqrt(p.lj_rlist2) + 2.0 * rmax + 1.0) < 0.5 * min_width(h) ? 1 : 0;
+}
+
 }  // namespace
 
 namespace fmd {
@@ -401,6 +407,7 @@ int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
   A.mol_start = L.mol_start;
   A.tt.assign(R, 0.0);
   A.small.assign(R, 0);
+  A.fast.assign(R, 0);
   const int ring = std::max(64, 2048 / R);
   rc = engine_init_common(e, ring, R);
   if (rc == FMD_OK) rc = atomic_alloc(A.d, A.p, e->ring, (int)L.blk_rep.size());
@@ -458,9 +465,13 @@ int fmd_batch_set_state(fmd_engine *e, int r, const double *pos, const double *v
     // this replica's molecule ranges, relative to its first atom
     std::vector<int> start;
     for (int m = A.rep_mol[r]; m <= A.rep_mol[r + 1]; m++) start.push_back(A.mol_start[m] - (int)a0);
-    A.small[r] = decide_small_box(A.p, S.h, max_radius(pos, start, S.h));
+    const double rmax = max_radius(pos, start, S.h);
+    A.small[r] = decide_small_box(A.p, S.h, rmax);
+    A.fast[r] = decide_scan_fast(A.p, S.h, rmax);
     A.p.small_box = 0;
+    A.p.scan_fast = 1;
     for (int x : A.small) A.p.small_box |= x;
+    for (int x : A.fast) A.p.scan_fast &= x;
   }
   HIPCHK(hipMemcpyAsync(A.d.state + r, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -557,7 +568,9 @@ int stage_positions(fmd_engine *e, const double *pos, const double *h, const dou
   }
   HIPCHK(hipMemcpyAsync(A.d.pos, pos, 3 * (size_t)A.p.Na * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(A.d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
-  A.p.small_box = decide_small_box(A.p, h, max_radius(pos, A.mol_start, h));
+  const double rmax = max_radius(pos, A.mol_start, h);
+  A.p.small_box = decide_small_box(A.p, h, rmax);
+  A.p.scan_fast = decide_scan_fast(A.p, h, rmax);
   return FMD_OK;
 }
 

@@ -60,6 +60,19 @@ def test_lj_truncated_list_equals_reference():
     assert (ll == oll).all()
 
 
+def test_lj_half_list_wide_cell_equals_oracle():
+    """3x3x3 (cell wide enough for the FP32-prefiltered walk through the
+    molecule tiles): the truncated half list equals the oracle's, entry for
+    entry."""
+    pos, h, mol = crystal(3, 23, 0.1)
+    cl, ll = airebo.build_nlist_lj(pos, h, oracle.inv9(h), mol)
+    ocl, oll = oracle.nlist_lj(pos, h, mol)
+    assert (cl == ocl).all()
+    for i in range(len(cl)):
+        assert (ll[i, :cl[i]] == oll[i, :cl[i]]).all()
+    assert (cl == 400).any()  # truncation active
+
+
 @pytest.mark.parametrize("name", ["airebo_nc1.npz", "airebo_nc2.npz"])
 def test_compute_forces_compat_matches_reference(name):
     g = golden(name)
