@@ -157,6 +157,33 @@ __global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
     }
   }
   rmax = wave_max(rmax);
+  // groups of four consecutive tile slots: centre and radius of their atoms
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    const bool real = a < n;
+    float ox = 0.f, oy = 0.f, oz = 0.f, c = real ? 1.f : 0.f;
+    if (real) {
+      ox = (float)(da[s][0] - mx);
+      oy = (float)(da[s][1] - my);
+      oz = (float)(da[s][2] - mz);
+    }
+    float sx4 = ox, sy4 = oy, sz4 = oz, c4 = c;
+    for (int k = 1; k <= 2; k <<= 1) {
+      sx4 += __shfl_xor(sx4, k, 64);
+      sy4 += __shfl_xor(sy4, k, 64);
+      sz4 += __shfl_xor(sz4, k, 64);
+      c4 += __shfl_xor(c4, k, 64);
+    }
+    const float inv = c4 > 0.f ? 1.f / c4 : 0.f;
+    const float qx = sx4 * inv, qy = sy4 * inv, qz = sz4 * inv;
+    float rq = real ? sqrtf((ox - qx) * (ox - qx) + (oy - qy) * (oy - qy) + (oz - qz) * (oz - qz))
+                    : 0.f;
+    for (int k = 1; k <= 2; k <<= 1) rq = fmaxf(rq, __shfl_xor(rq, k, 64));
+    if ((a & 3) == 0 && a < p.apad)
+      d.qcen[(size_t)m * (p.apad >> 2) + (a >> 2)] =
+          c4 > 0.f ? make_float4(qx, qy, qz, rq + 1e-4f) : make_float4(1e6f, 1e6f, 1e6f, 0.f);
+  }
   if (lane == 0) {
     d.center[3 * m] = rx + mx;
     d.center[3 * m + 1] = ry + my;
@@ -586,6 +613,8 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
     EA[s] = 0;
   }
   const LJ32 lj = p.lj;
+  // quad cull radius: rc (with its FP32 band) + the radius of molecule i
+  const float reach_i = sqrtf(lj.rc2_hi) + d.radius[mi] + 0.05f;
   const int i_last = s0 + n - 1;
   const int jmin_i = d.jmax_min[mi];
   const int nq = d.mol_nl_count[mi];
@@ -620,7 +649,23 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
     }
     const float4 *sj = d.off + (size_t)mj * apad;
     const int *jj = d.jmax + (size_t)mj * apad;
-    for (int b = 0; b < apad; b += 4) {
+    // groups of four atoms of mj within reach of molecule i (all of them on
+    // the per-pair image path)
+    const int nq = apad >> 2;
+    unsigned long long qmask = nq == 64 ? ~0ull : ((1ull << nq) - 1ull);
+    if (!PERPAIR) {
+      bool near = false;
+      if (lane < nq) {
+        const float4 c = d.qcen[(size_t)mj * nq + lane];
+        const float qx = (float)dx + c.x, qy = (float)dy + c.y, qz = (float)dz + c.z;
+        const float rr = reach_i + c.w;
+        near = qx * qx + qy * qy + qz * qz < rr * rr;
+      }
+      qmask = __ballot(near);
+    }
+    while (qmask) {
+      const int b = 4 * __builtin_ctzll(qmask);
+      qmask &= qmask - 1ull;
       float4 rb[4];
       int jm[4];
 #pragma unroll

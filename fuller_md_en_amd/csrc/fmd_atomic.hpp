@@ -13,6 +13,8 @@
 //   off                    float4[Nm*apad] atom offsets from the centre, one
 //                                         padded tile per molecule (padding
 //                                         sites at +1e6 A)
+//   qcen                   float4[Nm*apad/4] groups of four tile slots:
+//                                         centre offset and radius
 //   jmax                   int[Nm*apad]   per atom: largest j kept in its
 //                                         400-entry half list (INT_MAX if
 //                                         the list is not truncated)
@@ -86,6 +88,7 @@ struct AtomicDev {
   double *center;
   float *radius;
   float4 *off;
+  float4 *qcen;  // Nm*apad/4: centre offset and radius of each 4-slot group
   int *jmax, *jmax_min;
   int *mol_nl_count, *mol_nl;
   int *nlc_r, *nll_r;

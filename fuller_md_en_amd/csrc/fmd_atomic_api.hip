@@ -82,7 +82,8 @@ double max_radius(const double *pos, const std::vector<int> &start, const double
 
 void free_atomic_dev(AtomicDev &d) {
   void *ptrs[] = {d.pos,       d.vel,       d.F,        d.Frebo,     d.mass,       d.mol_start,
-                  d.center,    d.radius,    d.off,      d.jmax,      d.jmax_min,   d.mol_nl_count,
+                  d.center,    d.radius,    d.off,      d.qcen,      d.jmax,       d.jmax_min,
+                  d.mol_nl_count,
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.lj_part, d.ke_part,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
@@ -170,6 +171,7 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   AL(d.center, 3 * Nm);
   AL(d.radius, Nm);
   AL(d.off, nt);
+  AL(d.qcen, nt / 4);
   AL(d.jmax, nt);
   AL(d.jmax_min, Nm);
   AL(d.mol_nl_count, Nm);
