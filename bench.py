@@ -37,6 +37,11 @@ T_K = 300.0
 # accumulation, 24 torque, 12 virial) x 163.2 unique pairs per site for the
 # C60 FCC crystal at a = 14.17 A.
 FLOP_PER_SITE_STEP = 69.0 * 163.2
+# HBM bytes per launch of the pair-once rigid kernel on this workload, from
+# rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors
+# being what the same counters read for an elementwise kernel moving 1 GiB
+# each way (profiles/pmc_traffic_rigid_c60_fcc10.md).
+TRAFFIC_RIGID_HALF = (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -407,7 +412,8 @@ def main():
                                            if tune_entry else None)},
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": TRAFFIC_RIGID_HALF if params["half_pairs"] else None,
                      "kernel": "k_rigid_forces_half" if params["half_pairs"] else "k_rigid_forces",
                      "kernel_ms": kms, "launches": kn},
         "final": {"T": last.T, "P": last.P, "Ep_per_mol": last.Ep / len(sysd["pos"])},
