@@ -567,7 +567,10 @@ __device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, i
 // force); virial 1/2 sum_j dm (x) G_aj - r_a (x) sum_j G_aj (the
 // reference's sum over pairs of d (x) f, AB:465-472, regrouped through the
 // symmetric pair set); energy 1/2 per ordered pair.
-template <int NS, bool PERPAIR>
+// TRACE (time steps on the molecule-image path): only the virial trace, the
+// part the barostat uses (AB:527, 583), one FP64 FMA per (lane, neighbour)
+// instead of nine; the diagonal then carries trace / 3.
+template <int NS, bool PERPAIR, bool TRACE>
 __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRef hh) {
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int bid = blockIdx.x;
@@ -603,12 +606,12 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
     ia[s] = a < n ? s0 + a : s0;  // idle lanes never pass the cutoff test
     lane_ok[s] = a < n;
   }
-  double GA[NS][3], WA[NS][9];
+  double GA[NS][3], WA[NS][TRACE ? 1 : 9];
   float EA[NS], WD[NS][6];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     for (int k = 0; k < 3; k++) GA[s][k] = 0;
-    for (int k = 0; k < 9; k++) WA[s][k] = 0;
+    for (int k = 0; k < (TRACE ? 1 : 9); k++) WA[s][k] = 0;
     for (int k = 0; k < 6; k++) WD[s][k] = 0;
     EA[s] = 0;
   }
@@ -688,7 +691,9 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
       GA[s][0] += g0;
       GA[s][1] += g1;
       GA[s][2] += g2;
-      if (!PERPAIR) {
+      if (TRACE) {
+        WA[s][0] += dx * g0 + dy * g1 + dz * g2;
+      } else if (!PERPAIR) {
         WA[s][0] += dx * g0;
         WA[s][1] += dx * g1;
         WA[s][2] += dx * g2;
@@ -714,6 +719,12 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
       // sum over ordered pairs of 1/2 fm d d^T (symmetric)
       const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
       for (int q = 0; q < 9; q++) acc[q] += 0.5 * c48 * (double)WD[s][map[q]];
+    } else if (TRACE) {
+      const double tr = c48 * (0.5 * WA[s][0] - (rax[s] * GA[s][0] + ray[s] * GA[s][1] +
+                                                 raz[s] * GA[s][2]));
+      acc[0] += tr / 3.0;
+      acc[4] += tr / 3.0;
+      acc[8] += tr / 3.0;
     } else {
       const double ra[3] = {rax[s], ray[s], raz[s]};
       for (int x = 0; x < 3; x++)
@@ -968,14 +979,19 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
   const bool one = ns_of(p) == 1;
   if (p.small_box) {
     if (one)
-      k_at_lj<1, true><<<g, 256, 0, s>>>(d, p, hh);
+      k_at_lj<1, true, false><<<g, 256, 0, s>>>(d, p, hh);
     else
-      k_at_lj<2, true><<<g, 256, 0, s>>>(d, p, hh);
+      k_at_lj<2, true, false><<<g, 256, 0, s>>>(d, p, hh);
+  } else if (p.lj_trace) {
+    if (one)
+      k_at_lj<1, false, true><<<g, 256, 0, s>>>(d, p, hh);
+    else
+      k_at_lj<2, false, true><<<g, 256, 0, s>>>(d, p, hh);
   } else {
     if (one)
-      k_at_lj<1, false><<<g, 256, 0, s>>>(d, p, hh);
+      k_at_lj<1, false, false><<<g, 256, 0, s>>>(d, p, hh);
     else
-      k_at_lj<2, false><<<g, 256, 0, s>>>(d, p, hh);
+      k_at_lj<2, false, false><<<g, 256, 0, s>>>(d, p, hh);
   }
   return hipGetLastError();
 }
@@ -1000,7 +1016,9 @@ hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_sl
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = at_launch_centers(d, p, hh_mid(d), s)) != hipSuccess) return e;
   if (ev0 && (e = hipEventRecordWithFlags(ev0, s, ev_flags)) != hipSuccess) return e;
-  if ((e = at_launch_forces(d, p, hh_mid(d), s)) != hipSuccess) return e;
+  AtomicParams ps = p;
+  ps.lj_trace = p.trace_virial;  // steps need the virial trace only
+  if ((e = at_launch_forces(d, ps, hh_mid(d), s)) != hipSuccess) return e;
   if (ev1 && (e = hipEventRecordWithFlags(ev1, s, ev_flags)) != hipSuccess) return e;
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
   k_at_finish<<<p.R, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);

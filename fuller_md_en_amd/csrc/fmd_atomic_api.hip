@@ -329,6 +329,12 @@ int replica_forces(fmd_engine *e, int r, double *F, double *W9, double *energies
   const AtomicEngine &A = *e->at;
   const AtomicDev &d = A.d;
   hipStream_t s = e->stream;
+  if (A.p.trace_virial && e->steps > 0) {
+    // steps kept the LJ virial trace only: the full tensor at the current state
+    HIPCHK(at_launch_centers(d, A.p, hh_state(d), s));
+    HIPCHK(at_launch_forces(d, A.p, hh_state(d), s));
+    HIPCHK(at_launch_observe(d, A.p, kAtObsForce, s));
+  }
   const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
   if (F) HIPCHK(hipMemcpyAsync(F, d.F + 3 * a0, 3 * na * 8, hipMemcpyDeviceToHost, s));
   AtomicState S;
@@ -404,6 +410,7 @@ int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
   e->nlup = c->nlup > 0 ? c->nlup : 20;
   e->use_graph = R > 1;
   fill_params(A.p, *c, L.rep_atom[R], (int)L.mol_rep.size(), R, apad);
+  A.p.trace_virial = 1;
   A.rep_atom = L.rep_atom;
   A.rep_mol = L.rep_mol;
   A.mol_start = L.mol_start;
@@ -555,6 +562,7 @@ int atomic_workspace(int na, const int *mol_id, int rebo_mode, fmd_engine **out)
     g_aws.mol_id.assign(mol_id, mol_id + na);
   }
   g_aws.e->at->p.rebo_mode = rebo_mode;
+  g_aws.e->at->p.trace_virial = 0;  // the drop-ins return the full virial tensor
   *out = g_aws.e;
   return FMD_OK;
 }
