@@ -37,11 +37,14 @@ T_K = 300.0
 # accumulation, 24 torque, 12 virial) x 163.2 unique pairs per site for the
 # C60 FCC crystal at a = 14.17 A.
 FLOP_PER_SITE_STEP = 69.0 * 163.2
-# HBM bytes per launch of the pair-once rigid kernel on this workload, from
-# rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors
-# being what the same counters read for an elementwise kernel moving 1 GiB
-# each way (profiles/pmc_traffic_rigid_c60_fcc10.md).
-TRAFFIC_RIGID_HALF = (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0
+# Force kernel of each rigid pair mode (half_pairs: 1 per owned pair, 2 per
+# molecule pair-once, 0 both sides).
+RIGID_KERNEL = {1: "k_rigid_pairs", 2: "k_rigid_forces_half", 0: "k_rigid_forces"}
+# HBM bytes per launch of the force kernel on this workload, from rocprofv3
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors being what
+# the same counters read for an elementwise kernel moving 1 GiB each way
+# (profiles/pmc_traffic_rigid_c60_fcc10.md).
+RIGID_TRAFFIC = {2: (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -82,7 +85,7 @@ def build_system(params=None):
     sysd = system.rigid_system(nc=NC, spec="C60", T=T_K, seed=SEED, init_scale=scale)
     eng = rigid.RigidEngine(mol.coords, len(sysd["pos"]), mol.I0, mol.mass, rmol=mol.rmol,
                             dt=1.0, skin=params["skin"], nlup=params["nlup"],
-                            half_pairs=bool(params["half_pairs"]))
+                            half_pairs=int(params["half_pairs"]))
     eng.set_graph(bool(params["graph"]))
     eng.set_state(pos=sysd["pos"], vel=sysd["vel"], qv=sysd["qv"], omg=sysd["omg"],
                   h=sysd["h"], npt=rigid.make_npt(T_K, 0.0, len(sysd["pos"])))
@@ -413,8 +416,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_RIGID_HALF if params["half_pairs"] else None,
-                     "kernel": "k_rigid_forces_half" if params["half_pairs"] else "k_rigid_forces",
+                     "traffic": RIGID_TRAFFIC.get(int(params["half_pairs"])),
+                     "kernel": RIGID_KERNEL[int(params["half_pairs"])],
                      "kernel_ms": kms, "launches": kn},
         "final": {"T": last.T, "P": last.P, "Ep_per_mol": last.Ep / len(sysd["pos"])},
     }

@@ -9,7 +9,8 @@ fastest under a key of device, engine kind and system shape.
 
 Tunables:
   rigid    skin (list radius beyond the force cutoff), nlup (rebuild
-           interval), half_pairs (pair-once kernel), graph (step replay)
+           interval), half_pairs (1 pair-once per owned pair, 2 pair-once
+           per molecule, 0 both sides), graph (step replay)
   atomic   graph (step replay); the list radius and rebuild interval are
            part of the reference's truncated-half-list semantics and stay
 """
@@ -50,7 +51,7 @@ def store(key: str, entry: dict, path: str | None = None) -> None:
     os.replace(tmp, path)
 
 
-RIGID_SPACE = {"skin": [3.0, 2.0, 1.5], "nlup": [25, 10, 50], "half_pairs": [1, 0],
+RIGID_SPACE = {"skin": [3.0, 2.0, 1.5], "nlup": [25, 10, 50], "half_pairs": [1, 2, 0],
                "graph": [0, 1]}
 ATOMIC_SPACE = {"graph": [0, 1]}
 

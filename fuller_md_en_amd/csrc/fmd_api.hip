@@ -126,7 +126,7 @@ static void free_rigid(RigidDev &d) {
                   d.sites,    d.sites_d,  d.body,     d.qbody,    d.qcen,     d.hh,
                   d.nl_count, d.nl_list,  d.hl_count, d.hl_list,  d.in_count, d.in_list,
                   d.pairout,  d.flags,    d.molpart,  d.blockpart, d.state,   d.mid,
-                  d.ctrl,     d.thermo};
+                  d.ctrl,     d.thermo,   d.hl_perm,  d.ord_ctl};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   memset(&d, 0, sizeof d);
@@ -163,7 +163,11 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
     AL(d.hl_list, N * p.max_neigh);
     AL(d.in_count, N);
     AL(d.in_list, N * p.max_neigh);
-    AL(d.pairout, 6 * N * p.max_neigh);
+    AL(d.pairout, kPairOut * N * p.max_neigh);
+  }
+  if (p.pair_kernel) {
+    AL(d.hl_perm, N * p.max_neigh);
+    AL(d.ord_ctl, 1);
   }
   AL(d.flags, 4);
   AL(d.molpart, N * kMolPart);
@@ -179,6 +183,7 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   }
   (void)hipMemset(d.flags, 0, 4 * sizeof(int));
   (void)hipMemset(d.nl_count, 0, N * sizeof(int));
+  if (d.ord_ctl) (void)hipMemset(d.ord_ctl, 0, sizeof(int));
   (void)hipMemset(d.Fv, 0, 3 * N * sizeof(double));
   (void)hipMemset(d.Tv, 0, 3 * N * sizeof(double));
   (void)hipMemset(d.state, 0, sizeof(RigidState));
@@ -252,6 +257,7 @@ int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0
   p.cull_r2 = cr * cr;
   if (qbody) {
     p.use_half = 1;
+    p.pair_kernel = natom <= 64;
     p.nq = p.natom_pad / 4;
     p.reach = (float)cr;
     qbody->assign(4 * (size_t)p.nq, 0.0);
@@ -297,6 +303,7 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
   int rc = rigid_params(e->rp, c->n_mol, c->natom, c->body, c->I0, c->Mmol, c->lj,
                         c->rmcut * c->rmcut, c->rmcut + c->skin, c->max_neigh, c->dt,
                         c->clamp_xi, c->clamp_vg, body4, c->half_pairs ? &qbody : nullptr);
+  if (c->half_pairs == 2) e->rp.pair_kernel = 0;  // the per-molecule pair-once kernel
   if (rc == FMD_OK) rc = engine_init_common(e, 4096, 1);
   if (rc == FMD_OK) rc = rigid_alloc(e->rd, e->rp, e->ring);
   if (rc != FMD_OK) {
@@ -523,7 +530,7 @@ static int rigid_prepare(fmd_engine *e) {
   const double *hh = state_hh(d);
   HIPCHK(rigid_launch_apply_pbc(d, p, hh, s));
   HIPCHK(rigid_launch_nlist(d, p, hh, s));
-  if (p.use_half) HIPCHK(rigid_launch_halflist(d, p, s));
+  if (p.use_half) HIPCHK(rigid_launch_halflist(d, p, hh, s));
   HIPCHK(rigid_launch_sites(d, p, s));
   HIPCHK(rigid_launch_forces(d, p, hh, s));
   HIPCHK(rigid_launch_observe(d, p, kObsKE | kObsForce, s));

@@ -10,6 +10,9 @@
 //                                                 R(q_i) body_a, FP32; padding
 //                                                 sites sit at +1e6 A
 //   nl_count / nl_list      int[N] / int[N*max_neigh]  symmetric full list
+//   hl_* / in_*             int, same shapes          owned pairs / incoming slots
+//   pairout                 double[N*max_neigh*16]    per owned pair, both sides
+//   hl_perm                 int[N*max_neigh]          owned pairs by centre distance
 //   molpart                 double[N*kMolPart]    per-molecule W9, E partials
 //   blockpart               double[nblk*kBlkPart] per-block KE / W / E sums
 //   RigidState              NPT + cell, FP64 (one per system)
@@ -24,6 +27,10 @@ namespace fmd {
 
 constexpr int kMolPart = 10;  // W9 + E
 constexpr int kBlkPart = 12;  // sum v^2, sum w^2, W9, E
+// Per owned molecule pair (slot owner * max_neigh + position in its half
+// list): [0..2] F_i [3..5] tau_i [6] energy [7] sum d.f  (owner side, written
+// by the pair kernel) and [8..10] F_j [11..13] tau_j (the other molecule).
+constexpr int kPairOut = 16;
 
 // Per-step control word (host-decided phase logic of LJ:848-894).
 enum : int {
@@ -65,6 +72,7 @@ struct RigidParams {
   int N, natom, natom_pad, max_neigh;
   int nq;             // site quads per molecule (natom_pad / 4)
   int use_half;       // time stepping with the half-pair kernel (trace virial)
+  int pair_kernel;    // ... as one wavefront per owned pair (natom <= 64)
   double rmcut2;      // molecule-pair cutoff^2 inside the force loop
   double rlist2;      // list radius^2 (rmcut + skin)^2
   double dt, I0, Mmol;
@@ -89,7 +97,10 @@ struct RigidDev {
   // pairs owned by each molecule (half list) and, per molecule, the slots of
   // the pairs owned by its neighbours that carry its force / torque
   int *hl_count, *hl_list, *in_count, *in_list;
-  double *pairout;  // N*max_neigh*6: force and torque on the non-owner
+  double *pairout;  // N*max_neigh*kPairOut: per owned pair, both sides
+  // pair kernel work order: hl_perm[i][r] = half-list position of i's r-th
+  // nearest owned pair; ord_ctl[0] = the largest owned-pair count
+  int *hl_perm, *ord_ctl;
   double *molpart, *blockpart;
   RigidState *state;
   RigidMid *mid;
@@ -106,8 +117,10 @@ hipError_t rigid_launch_apply_pbc(const RigidDev &d, const RigidParams &p, const
                                   hipStream_t s);
 hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const double *hh,
                               hipStream_t s);
-// Half list and incoming slots from the symmetric list.
-hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, hipStream_t s);
+// Half list and incoming slots from the symmetric list; with the pair kernel
+// also its work order (centre distances from the cell `hh`).
+hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, const double *hh,
+                                 hipStream_t s);
 hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const double *hh,
                                hipStream_t s);
 // Observables of the current state into RigidState: what = kObsKE | kObsForce.

@@ -20,6 +20,7 @@
 //
 // No floating-point atomics anywhere: every sum has a fixed order, so runs are
 // bitwise repeatable.
+#include <algorithm>
 #include <cstddef>
 
 #include "fmd_rigid.hpp"
@@ -63,6 +64,36 @@ __device__ __forceinline__ double wave_sum(double v) {
   v += dpp64<0x141>(v);  // row_half_mirror
   v += dpp64<0x140>(v);  // row_mirror
   return (readlane64(v, 0) + readlane64(v, 16)) + (readlane64(v, 32) + readlane64(v, 48));
+}
+
+// Sums of eight doubles over the 64 lanes, in a fixed order; every lane gets
+// all eight.  Transposed butterfly: the exchanges with lane ^ 32, ^ 16, ^ 8
+// each halve the values a lane carries (it keeps the half its side of the
+// split owns and adds the partner's copy of it), so lane L then holds a
+// partial of value (L >> 3) & 7; three DPP pairings inside each group of 8
+// complete it, and lane 8k holds sum k.  Whole wavefront active.
+__device__ __forceinline__ void wave_sum8(double *v) {
+  const int lane = __lane_id();
+  double k4[4], k2[2];
+  bool up = lane & 32;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double send = up ? v[c] : v[c + 4];
+    k4[c] = (up ? v[c + 4] : v[c]) + __shfl_xor(send, 32);
+  }
+  up = lane & 16;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const double send = up ? k4[c] : k4[c + 2];
+    k2[c] = (up ? k4[c + 2] : k4[c]) + __shfl_xor(send, 16);
+  }
+  up = lane & 8;
+  double k1 = (up ? k2[1] : k2[0]) + dpp64<0x128>(up ? k2[0] : k2[1]);  // row_ror:8 = lane ^ 8
+  k1 += dpp64<0xB1>(k1);   // quad_perm [1,0,3,2]
+  k1 += dpp64<0x4E>(k1);   // quad_perm [2,3,0,1]
+  k1 += dpp64<0x141>(k1);  // row_half_mirror
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = readlane64(k1, 8 * c);
 }
 
 // Lab-frame offsets of the sites of molecule i (lanes a and a+64): FP32 for
@@ -142,15 +173,37 @@ __device__ __forceinline__ bool owns(int i, int j, int N) {
 
 // From the symmetric list: the owned neighbours of i (hl, in list order) and
 // the pairout slots (owner * max_neigh + position in its half list) of the
-// pairs owned by i's neighbours, where i's force and torque arrive.
-__global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p) {
+// pairs owned by i's neighbours, where i's force and torque arrive.  For the
+// pair kernel also the owned pairs' order by centre distance (hl_perm).
+__global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p,
+                                                       const double *__restrict__ hh) {
   const int i = blockIdx.x, lane = threadIdx.x;
   const int cnt = d.nl_count[i], cap = p.max_neigh;
+  const double *h = hh, *hi = hh + 9;
   int nh = 0, nin = 0;
   for (int k0 = 0; k0 < cnt; k0 += 64) {
     const int k = k0 + lane;
     const int j = k < cnt ? d.nl_list[(size_t)i * cap + k] : -1;
     const bool mine = j >= 0 && owns(i, j, p.N);
+    if (p.pair_kernel) {
+      // rank of each owned pair of this chunk by centre distance (ties by
+      // list position): hl_perm[i][nh + rank] = its half-list position
+      float dist = 3.0e38f;
+      if (mine) {
+        double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
+               dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
+        mimg_exact(dx, dy, dz, hi, h);
+        dist = (float)(dx * dx + dy * dy + dz * dz);
+      }
+      const unsigned long long bm = __ballot(mine);
+      int rank = 0;
+      for (int s = 0; s < 64; s++) {
+        const float ds = __shfl(dist, s);
+        rank += ds < dist || (ds == dist && s < lane);
+      }
+      if (mine)
+        d.hl_perm[(size_t)i * cap + nh + rank] = nh + __popcll(bm & ((1ull << lane) - 1ull));
+    }
     bool in = false;
     int slot = 0;
     if (j >= 0 && !mine) {  // position of i among j's owned neighbours
@@ -177,6 +230,22 @@ __global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p
   if (lane == 0) {
     d.hl_count[i] = nh;
     d.in_count[i] = nin;
+  }
+}
+
+// Largest owned-pair count over the molecules into ord_ctl[0]: the pair
+// kernel's work items are (rank r, molecule i) for r below it.  One workgroup.
+__global__ __launch_bounds__(1024) void k_rigid_order_max(const int *__restrict__ hl_count,
+                                                          int N, int *__restrict__ ord_ctl) {
+  __shared__ int wmax[16];
+  int m = 0;
+  for (int i = threadIdx.x; i < N; i += 1024) m = max(m, hl_count[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; w++) m = max(m, wmax[w]);
+    ord_ctl[0] = m;
   }
 }
 
@@ -532,7 +601,7 @@ __global__ __launch_bounds__(256) void k_rigid_forces_half(
       }
       qmask = __ballot(near);
     }
-    double* out = pairout + ((size_t)i * p.max_neigh + k) * 6;
+    double *out = pairout + ((size_t)i * p.max_neigh + k) * kPairOut + 8;
     if (qmask == 0) {
       if (lane < 6) out[lane] = 0.0;
       continue;
@@ -646,6 +715,138 @@ __global__ __launch_bounds__(256) void k_rigid_forces_half(
   }
 }
 
+// Wavefront-scope ordering of LDS writes before reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Pair-once forces with one wavefront per owned molecule pair (natom <= 64).
+// Work item t = r N + i is molecule i's r-th nearest owned pair (order built
+// at the list rebuild), so the costly close pairs of all molecules start
+// first and the light far ones fill the tail.
+// Per pair (i, j): the sites of i within reach of j's centre (set A) and the
+// sites of j within reach of i's centre (set B) are compacted through LDS;
+// the lanes hold A in P groups of 64/P lanes (P the largest power of two
+// with P |A| <= 64, at most 16) and the groups take P sites of B per pass.
+// A lane's sums over its B sites (G, the force on j from its i site; energy;
+// sum d.f) are summed over the wavefront into the pair's slot of pairout:
+// F_i = -sum G, tau_i = -sum r_a x G, F_j = sum G, tau_j = sum r_a x G -
+// dm x F_j (every pair force lies along its pair vector).  The cutoff test
+// is the same as k_rigid_forces' (FP32 r^2, FP64 re-test inside the band).
+__global__ __launch_bounds__(256) void k_rigid_pairs(
+    const double *__restrict__ pos, const float4 *__restrict__ sites,
+    const double *__restrict__ sites_d, const int *__restrict__ hl_count,
+    const int *__restrict__ hl_list, const int *__restrict__ hl_perm,
+    const int *__restrict__ ord_ctl, const double *__restrict__ hh,
+    double *__restrict__ pairout, RigidParams p) {
+  __shared__ float4 sB[4][64];
+  __shared__ int sA[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  const int items = p.N * uniform(ord_ctl[0]);
+  const double *h = hh, *hi = hh + 9;
+  const int npad = p.natom_pad, cap = p.max_neigh;
+  const LJ32 lj = p.lj;
+  const double c48 = 48.0 * p.eps, e4 = 4.0 * p.eps;
+  const float reach2 = p.reach * p.reach;
+  for (int t = blockIdx.x * 4 + wv; t < items; t += gridDim.x * 4) {
+    const int r = uniform(t / p.N), i = t - r * p.N;
+    if (r >= hl_count[i]) continue;
+    const int slot = i * cap + uniform(hl_perm[(size_t)i * cap + r]);
+    const int j = uniform(hl_list[slot]);
+    double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+           dz = pos[3 * j + 2] - pos[3 * i + 2];
+    mimg_exact(dx, dy, dz, hi, h);
+    double *out = pairout + (size_t)slot * kPairOut;
+    bool inA = false, inB = false;
+    float4 sb = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (dx * dx + dy * dy + dz * dz <= p.rmcut2 && lane < p.natom) {
+      const float4 sa = sites[(size_t)i * npad + lane];
+      sb = sites[(size_t)j * npad + lane];
+      const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+      const float ax = sa.x - fx, ay = sa.y - fy, az = sa.z - fz;
+      const float bx = sb.x + fx, by = sb.y + fy, bz = sb.z + fz;
+      inA = ax * ax + ay * ay + az * az < reach2;
+      inB = bx * bx + by * by + bz * bz < reach2;
+    }
+    const unsigned long long mA = __ballot(inA), mB = __ballot(inB);
+    if (mA == 0 || mB == 0) {
+      if (lane < kPairOut) out[lane] = 0.0;
+      continue;
+    }
+    const int nA = __popcll(mA), nB = __popcll(mB);
+    if (inA) sA[wv][__popcll(mA & ((1ull << lane) - 1ull))] = lane;
+    if (inB) {
+      sb.w = __int_as_float(lane);
+      sB[wv][__popcll(mB & ((1ull << lane) - 1ull))] = sb;
+    }
+    wave_lds_sync();
+    int lgG = 6;  // log2 of the lanes per group
+    while (lgG > 2 && nA <= (1 << (lgG - 1))) lgG--;
+    const int P = 64 >> lgG;
+    const int g = lane >> lgG, ta = lane & ((1 << lgG) - 1);
+    const bool act = ta < nA;
+    double ra[3] = {0.0, 0.0, 0.0};
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    if (act) {
+      const double *q = sites_d + 3 * ((size_t)i * npad + sA[wv][ta]);
+      ra[0] = q[0];
+      ra[1] = q[1];
+      ra[2] = q[2];
+      cx = (float)(dx - ra[0]);
+      cy = (float)(dy - ra[1]);
+      cz = (float)(dz - ra[2]);
+    }
+    const double dm[3] = {dx, dy, dz};
+    float gx = 0.f, gy = 0.f, gz = 0.f, e = 0.f, tr = 0.f;
+    for (int b0 = 0; b0 < nB; b0 += P) {
+      const int bi = b0 + g;
+      if (act && bi < nB) {
+        const float4 rb = sB[wv][bi];
+        const float ex = rb.x + cx, ey = rb.y + cy, ez = rb.z + cz;
+        const float r2 = ex * ex + ey * ey + ez * ez;
+        if (r2 < lj.rc2_hi) {
+          bool in = r2 < lj.rc2_lo;
+          if (!in)
+            in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + __float_as_int(rb.w)), ra) <
+                 lj.rc2d;
+          if (in) {
+            const float fm = lj_fm(r2, lj, e);
+            tr += fm * r2;
+            gx += fm * ex;
+            gy += fm * ey;
+            gz += fm * ez;
+          }
+        }
+      }
+    }
+    // the next pair's compaction overwrites sA / sB
+    wave_lds_sync();
+    const double g0 = c48 * gx, g1 = c48 * gy, g2 = c48 * gz;
+    double v[8] = {g0, g1, g2, ra[1] * g2 - ra[2] * g1, ra[2] * g0 - ra[0] * g2,
+                   ra[0] * g1 - ra[1] * g0, e4 * e, c48 * tr};
+    wave_sum8(v);
+    if (lane == 0) {
+      out[0] = -v[0];
+      out[1] = -v[1];
+      out[2] = -v[2];
+      out[3] = -v[3];
+      out[4] = -v[4];
+      out[5] = -v[5];
+      out[6] = v[6];
+      out[7] = v[7];
+      out[8] = v[0];
+      out[9] = v[1];
+      out[10] = v[2];
+      out[11] = v[3] - (dy * v[2] - dz * v[1]);
+      out[12] = v[4] - (dz * v[0] - dx * v[2]);
+      out[13] = v[5] - (dx * v[1] - dy * v[0]);
+    }
+  }
+}
+
 // ------------------------------------------------------ kick 2 + partials
 // Block sums of 12 doubles in a fixed order.
 __device__ __forceinline__ void block_sum12(double *v, double *out) {
@@ -672,11 +873,32 @@ __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, 
   double v[kBlkPart];
   for (int k = 0; k < kBlkPart; k++) v[k] = 0;
   if (i < p.N) {
-    double f[6] = {0, 0, 0, 0, 0, 0};
-    if (kick && p.use_half) {
+    double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (kick && p.pair_kernel) {
+      // both sides from the pair slots: owned pairs (F, tau, energy, sum d.f)
+      // then the incoming ones (F, tau)
+      const int nh = d.hl_count[i], nin = d.in_count[i];
+      for (int m = lane; m < nh; m += 64) {
+        const double *o = d.pairout + ((size_t)i * p.max_neigh + m) * kPairOut;
+        for (int c = 0; c < 8; c++) f[c] += o[c];
+      }
+      for (int m = lane; m < nin; m += 64) {
+        const double *o =
+            d.pairout + (size_t)d.in_list[(size_t)i * p.max_neigh + m] * kPairOut + 8;
+        for (int c = 0; c < 6; c++) f[c] += o[c];
+      }
+      wave_sum8(f);
+      if (lane == 0) {
+        double *mp = d.molpart + (size_t)i * kMolPart;
+        mp[0] = mp[4] = mp[8] = f[7] / 3.0;
+        mp[1] = mp[2] = mp[3] = mp[5] = mp[6] = mp[7] = 0.0;
+        mp[9] = f[6];
+      }
+    } else if (kick && p.use_half) {
       const int nin = d.in_count[i];
       for (int m = lane; m < nin; m += 64) {
-        const double *o = d.pairout + (size_t)d.in_list[(size_t)i * p.max_neigh + m] * 6;
+        const double *o =
+            d.pairout + (size_t)d.in_list[(size_t)i * p.max_neigh + m] * kPairOut + 8;
         for (int c = 0; c < 6; c++) f[c] += o[c];
       }
       for (int c = 0; c < 6; c++) f[c] = wave_sum(f[c]);
@@ -686,9 +908,9 @@ __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, 
       double wx = d.omg[3 * i], wy = d.omg[3 * i + 1], wz = d.omg[3 * i + 2];
       if (kick) {
         double F[3], T[3];
-        for (int c = 0; c < 3; c++) {
-          F[c] = d.Fv[3 * i + c] + f[c];
-          T[c] = d.Tv[3 * i + c] + f[3 + c];
+        for (int c = 0; c < 3; c++) {  // the pair kernel leaves both sides in f
+          F[c] = (p.pair_kernel ? 0.0 : d.Fv[3 * i + c]) + f[c];
+          T[c] = (p.pair_kernel ? 0.0 : d.Tv[3 * i + c]) + f[3 + c];
         }
         if (p.use_half)
           for (int c = 0; c < 3; c++) {
@@ -874,13 +1096,26 @@ hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const do
   return hipGetLastError();
 }
 
-hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, hipStream_t s) {
-  k_rigid_halflist<<<p.N, 64, 0, s>>>(d, p);
+hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, const double *hh,
+                                 hipStream_t s) {
+  k_rigid_halflist<<<p.N, 64, 0, s>>>(d, p, hh);
+  if (p.pair_kernel) k_rigid_order_max<<<1, 1024, 0, s>>>(d.hl_count, p.N, d.ord_ctl);
   return hipGetLastError();
 }
 
 static hipError_t rigid_launch_forces_half(const RigidDev &d, const RigidParams &p,
                                            const double *hh, hipStream_t s) {
+  if (p.pair_kernel) {
+    // one wavefront per work item; the item count is on the device, so the
+    // grid covers up to 32 owned pairs per molecule (a C60 crystal owns at
+    // most about 30 within the list radius), waves past the count exit at
+    // once and the kernel's grid-stride loop takes any excess
+    const size_t items = (size_t)p.N * std::min(p.max_neigh, 32);
+    dim3 g((unsigned)std::min<size_t>((items + 3) / 4, 1u << 20));
+    k_rigid_pairs<<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.hl_count, d.hl_list,
+                                    d.hl_perm, d.ord_ctl, hh, d.pairout, p);
+    return hipGetLastError();
+  }
   const int per = (p.N + kXcd - 1) / kXcd;
   dim3 g(per * kXcd);
   if (ns_of(p) == 1)
@@ -909,7 +1144,7 @@ hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_s
       reinterpret_cast<const char *>(d.mid) + offsetof(RigidMid, h));
   if (rebuild) {
     if ((e = rigid_launch_nlist(d, p, state_hh, s)) != hipSuccess) return e;
-    if (p.use_half && (e = rigid_launch_halflist(d, p, s)) != hipSuccess) return e;
+    if (p.use_half && (e = rigid_launch_halflist(d, p, state_hh, s)) != hipSuccess) return e;
   }
   dim3 g((p.N + 3) / 4);
   if (ns_of(p) == 1)

@@ -93,7 +93,7 @@ class RigidEngine:
     """HBM-resident rigid-body NPT engine (fmd_rigid_create / fmd_engine_*)."""
 
     def __init__(self, body, n_mol, I0, Mmol, rmol, dt=1.0, lj=None, nlup=25,
-                 max_neigh=MAX_NEIGH, rmcut=None, skin=3.0, half_pairs=True):
+                 max_neigh=MAX_NEIGH, rmcut=None, skin=3.0, half_pairs=1):
         self.body = _c(body).reshape(-1, 3)
         cfg = N.RigidConfig()
         N.lib().fmd_rigid_config_default(C.byref(cfg), rmol)

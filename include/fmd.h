@@ -183,8 +183,10 @@ typedef struct {
   double clamp_xi, clamp_vg; /* 0.1, 0.01 (C1:520, 525) */
   int half_pairs;       /* 1 (default): time steps evaluate each molecule pair once
                            (Newton's third law) and keep only the virial trace, the
-                           part the pressure uses; 0: every pair from both sides and
-                           the full tensor, as the reference computes */
+                           part the pressure uses; with at most 64 sites one
+                           wavefront per pair, 2: the same per molecule (its owned
+                           pairs in one workgroup); 0: every pair from both sides
+                           and the full tensor, as the reference computes */
 } fmd_rigid_config;
 
 /* Defaults of the reference rigid programs for the given molecule radius. */

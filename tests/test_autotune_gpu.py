@@ -15,7 +15,7 @@ def _build(g):
     def build(params):
         eng = rigid.RigidEngine(g["body"], len(g["pos"]), float(g["I0"]), float(g["Mmol"]),
                                 rmol=3.55, rmcut=float(g["rmcut"]), skin=params["skin"],
-                                nlup=params["nlup"], half_pairs=bool(params["half_pairs"]))
+                                nlup=params["nlup"], half_pairs=int(params["half_pairs"]))
         eng.set_state(pos=g["pos"], vel=g["vel"], qv=g["qv"], omg=g["omg"], h=g["h"],
                       npt=rigid.make_npt(300.0, 0.0, len(g["pos"])))
         return eng

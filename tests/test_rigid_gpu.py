@@ -126,10 +126,12 @@ def test_engine_trajectory_matches_reference():
     eng.close()
 
 
-@pytest.mark.parametrize("half", [True, False])
+@pytest.mark.parametrize("half", [1, 2, 0])
 @pytest.mark.parametrize("tag", ["C60", "C70", "C84"])
 def test_engine_steps_match_reference(tag, half):
-    """Resident engine, both pair modes, against the reference's 3 NPT steps.
+    """Resident engine, every pair mode (1: wavefront per owned pair for up to
+    64 sites, else per molecule; 2: per molecule; 0: both sides), against the
+    reference's 3 NPT steps.
 
     C70 leaves a last site quad with two real sites; C84 (84 sites) runs the
     two-sites-per-lane kernels."""
@@ -155,12 +157,12 @@ def test_engine_steps_match_reference(tag, half):
 
 
 def test_engine_half_pairs_equal_full_pairs():
-    """108 molecules, 20 steps with a list rebuild: the pair-once kernel (own
-    share + gathered neighbour shares, trace virial) against the both-sides
-    kernel."""
+    """108 molecules, 20 steps with list rebuilds: the pair-once kernels (per
+    owned pair, and per molecule: own share + gathered neighbour shares, trace
+    virial) against the both-sides kernel."""
     g = golden("rigid_core_nc3.npz")
     out = []
-    for half in (True, False):
+    for half in (1, 2, 0):
         eng = rigid.RigidEngine(g["body"], len(g["pos"]), float(g["I0"]), float(g["Mmol"]),
                                 rmol=3.55, rmcut=float(g["rmcut"]), nlup=10, half_pairs=half)
         eng.set_state(pos=g["pos"], vel=g["vel"], qv=g["qv"], omg=g["omg"], h=g["h"],
@@ -169,11 +171,12 @@ def test_engine_half_pairs_equal_full_pairs():
         out.append((eng.get_state(), np.array([[t.Ep, t.KE, t.P] for t in th])))
         assert eng.counters()[0] == 0
         eng.close()
-    (s1, t1), (s2, t2) = out
-    for k in range(3):
-        assert rel_max(t1[:, k], t2[:, k]) < 1e-6
-    assert np.abs(s1["pos"] - s2["pos"]).max() < 1e-8
-    assert rel_max(s1["vel"], s2["vel"]) < 1e-6 and rel_max(s1["omg"], s2["omg"]) < 1e-6
+    s2, t2 = out[-1]
+    for s1, t1 in out[:-1]:
+        for k in range(3):
+            assert rel_max(t1[:, k], t2[:, k]) < 1e-6
+        assert np.abs(s1["pos"] - s2["pos"]).max() < 1e-8
+        assert rel_max(s1["vel"], s2["vel"]) < 1e-6 and rel_max(s1["omg"], s2["omg"]) < 1e-6
 
 
 def test_engine_deterministic():
