@@ -741,7 +741,7 @@ __global__ __launch_bounds__(256) void k_rigid_pairs(
     const int *__restrict__ hl_list, const int *__restrict__ hl_perm,
     const int *__restrict__ ord_ctl, const double *__restrict__ hh,
     double *__restrict__ pairout, RigidParams p) {
-  __shared__ float4 sB[4][64];
+  __shared__ float4 sB[4][64 + 16];
   __shared__ int sA[4][64];
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
@@ -782,6 +782,7 @@ __global__ __launch_bounds__(256) void k_rigid_pairs(
       sb.w = __int_as_float(lane);
       sB[wv][__popcll(mB & ((1ull << lane) - 1ull))] = sb;
     }
+    if (lane < 16) sB[wv][nB + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, 0.f);
     wave_lds_sync();
     int lgG = 6;  // log2 of the lanes per group
     while (lgG > 2 && nA <= (1 << (lgG - 1))) lgG--;
@@ -789,7 +790,7 @@ __global__ __launch_bounds__(256) void k_rigid_pairs(
     const int g = lane >> lgG, ta = lane & ((1 << lgG) - 1);
     const bool act = ta < nA;
     double ra[3] = {0.0, 0.0, 0.0};
-    float cx = 0.f, cy = 0.f, cz = 0.f;
+    float cx = 1.0e6f, cy = 1.0e6f, cz = 1.0e6f;
     if (act) {
       const double *q = sites_d + 3 * ((size_t)i * npad + sA[wv][ta]);
       ra[0] = q[0];
@@ -801,24 +802,22 @@ __global__ __launch_bounds__(256) void k_rigid_pairs(
     }
     const double dm[3] = {dx, dy, dz};
     float gx = 0.f, gy = 0.f, gz = 0.f, e = 0.f, tr = 0.f;
+#pragma unroll 2
     for (int b0 = 0; b0 < nB; b0 += P) {
-      const int bi = b0 + g;
-      if (act && bi < nB) {
-        const float4 rb = sB[wv][bi];
-        const float ex = rb.x + cx, ey = rb.y + cy, ez = rb.z + cz;
-        const float r2 = ex * ex + ey * ey + ez * ez;
-        if (r2 < lj.rc2_hi) {
-          bool in = r2 < lj.rc2_lo;
-          if (!in)
-            in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + __float_as_int(rb.w)), ra) <
-                 lj.rc2d;
-          if (in) {
-            const float fm = lj_fm(r2, lj, e);
-            tr += fm * r2;
-            gx += fm * ex;
-            gy += fm * ey;
-            gz += fm * ez;
-          }
+      const float4 rb = sB[wv][b0 + g];
+      const float ex = rb.x + cx, ey = rb.y + cy, ez = rb.z + cz;
+      const float r2 = ex * ex + ey * ey + ez * ez;
+      if (r2 < lj.rc2_hi) {
+        bool in = r2 < lj.rc2_lo;
+        if (!in)
+          in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + __float_as_int(rb.w)), ra) <
+               lj.rc2d;
+        if (in) {
+          const float fm = lj_fm(r2, lj, e);
+          tr += fm * r2;
+          gx += fm * ex;
+          gy += fm * ey;
+          gz += fm * ez;
         }
       }
     }
