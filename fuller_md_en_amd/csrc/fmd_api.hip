@@ -126,7 +126,7 @@ static void free_rigid(RigidDev &d) {
                   d.sites,    d.sites_d,  d.body,     d.qbody,    d.qcen,     d.hh,
                   d.nl_count, d.nl_list,  d.hl_count, d.hl_list,  d.in_count, d.in_list,
                   d.pairout,  d.flags,    d.molpart,  d.blockpart, d.state,   d.mid,
-                  d.ctrl,     d.thermo,   d.hl_perm,  d.ord_ctl};
+                  d.ctrl,     d.thermo,   d.hl_item,  d.ord_ctl};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   memset(&d, 0, sizeof d);
@@ -166,7 +166,7 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
     AL(d.pairout, kPairOut * N * p.max_neigh);
   }
   if (p.pair_kernel) {
-    AL(d.hl_perm, N * p.max_neigh);
+    AL(d.hl_item, N * p.max_neigh);
     AL(d.ord_ctl, 1);
   }
   AL(d.flags, 4);
@@ -239,6 +239,11 @@ int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0
   p.clamp_vg = cvg;
   p.lj = make_lj32(lj);
   p.eps = lj.eps;
+  p.isig = (float)(1.0 / std::sqrt((double)p.lj.sig2));
+  p.rc2_lo_s = p.lj.rc2_lo / p.lj.sig2;
+  p.rc2_hi_s = p.lj.rc2_hi / p.lj.sig2;
+  p.rmin2_s = p.lj.rmin2 / p.lj.sig2;
+  p.c48s = 48.0 * lj.eps / std::sqrt((double)p.lj.sig2);
   std::vector<int> perm(natom);
   for (int a = 0; a < natom; a++) perm[a] = a;
   if (qbody) quad_order(body, natom, perm);

@@ -12,7 +12,7 @@
 //   nl_count / nl_list      int[N] / int[N*max_neigh]  symmetric full list
 //   hl_* / in_*             int, same shapes          owned pairs / incoming slots
 //   pairout                 double[N*max_neigh*16]    per owned pair, both sides
-//   hl_perm                 int[N*max_neigh]          owned pairs by centre distance
+//   hl_item                 int2[max_neigh*N]         pair kernel work items, rank-major
 //   molpart                 double[N*kMolPart]    per-molecule W9, E partials
 //   blockpart               double[nblk*kBlkPart] per-block KE / W / E sums
 //   RigidState              NPT + cell, FP64 (one per system)
@@ -81,6 +81,10 @@ struct RigidParams {
   float reach;        // rc + max site radius + FP32 margin: quad cull radius
   LJ32 lj;
   double eps;         // LJ epsilon (energy scale of the reduced sums)
+  // the pair kernel's tests in units of sigma: 1/sigma, the cutoff band and
+  // the r^2 clamp over sigma^2, and 48 eps / sigma
+  float isig, rc2_lo_s, rc2_hi_s, rmin2_s;
+  double c48s;
 };
 
 // Device buffers of one rigid system.
@@ -98,9 +102,11 @@ struct RigidDev {
   // the pairs owned by its neighbours that carry its force / torque
   int *hl_count, *hl_list, *in_count, *in_list;
   double *pairout;  // N*max_neigh*kPairOut: per owned pair, both sides
-  // pair kernel work order: hl_perm[i][r] = half-list position of i's r-th
-  // nearest owned pair; ord_ctl[0] = the largest owned-pair count
-  int *hl_perm, *ord_ctl;
+  // pair kernel work items: hl_item[r * N + i] = (slot, j) of molecule i's
+  // r-th nearest owned pair, (-1, -1) past its count; ord_ctl[0] = the
+  // largest owned-pair count
+  int2 *hl_item;
+  int *ord_ctl;
   double *molpart, *blockpart;
   RigidState *state;
   RigidMid *mid;

@@ -36,6 +36,15 @@ __device__ __forceinline__ double clampd(double x, double lo, double hi) {
 }
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniform_f(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniform_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // One DPP lane permutation of a double (both halves), all rows and banks.
 template <int CTRL>
@@ -174,7 +183,8 @@ __device__ __forceinline__ bool owns(int i, int j, int N) {
 // From the symmetric list: the owned neighbours of i (hl, in list order) and
 // the pairout slots (owner * max_neigh + position in its half list) of the
 // pairs owned by i's neighbours, where i's force and torque arrive.  For the
-// pair kernel also the owned pairs' order by centre distance (hl_perm).
+// pair kernel also its work items: the owned pairs by centre distance
+// (hl_item).
 __global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p,
                                                        const double *__restrict__ hh) {
   const int i = blockIdx.x, lane = threadIdx.x;
@@ -187,7 +197,7 @@ __global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p
     const bool mine = j >= 0 && owns(i, j, p.N);
     if (p.pair_kernel) {
       // rank of each owned pair of this chunk by centre distance (ties by
-      // list position): hl_perm[i][nh + rank] = its half-list position
+      // list position): work item (nh + rank, i) is (slot, j)
       float dist = 3.0e38f;
       if (mine) {
         double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
@@ -202,7 +212,8 @@ __global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p
         rank += ds < dist || (ds == dist && s < lane);
       }
       if (mine)
-        d.hl_perm[(size_t)i * cap + nh + rank] = nh + __popcll(bm & ((1ull << lane) - 1ull));
+        d.hl_item[(size_t)(nh + rank) * p.N + i] =
+            make_int2(i * cap + nh + __popcll(bm & ((1ull << lane) - 1ull)), j);
     }
     bool in = false;
     int slot = 0;
@@ -227,6 +238,8 @@ __global__ __launch_bounds__(64) void k_rigid_halflist(RigidDev d, RigidParams p
     nin += __popcll(bi);
     if (j >= 0 && !mine && !in) atomicAdd(&d.flags[0], 1);  // asymmetric (overflowed) list
   }
+  if (p.pair_kernel)  // no pair past i's count
+    for (int r = nh + lane; r < cap; r += 64) d.hl_item[(size_t)r * p.N + i] = make_int2(-1, -1);
   if (lane == 0) {
     d.hl_count[i] = nh;
     d.in_count[i] = nin;
@@ -722,10 +735,15 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pair-once forces with one wavefront per owned molecule pair (natom <= 64).
-// Work item t = r N + i is molecule i's r-th nearest owned pair (order built
-// at the list rebuild), so the costly close pairs of all molecules start
-// first and the light far ones fill the tail.
+// Pair-once forces for natom <= 64.  Work item t = r N + i is molecule i's
+// r-th nearest owned pair (built at the list rebuild); each wavefront takes
+// four consecutive items (same rank, neighbouring molecules: similar work),
+// so the costly close pairs start first and the light far ones fill the
+// tail, and the hardware balances the wavefronts.
+// Screening, the four items at once: lanes 0-3 take the FP64 minimum image,
+// lanes 16k + q test quad q of item k -- some site quad of i within reach of
+// j's centre and some quad of j within reach of i's centre; items that fail
+// get zero shares, the others are computed one after the other.
 // Per pair (i, j): the sites of i within reach of j's centre (set A) and the
 // sites of j within reach of i's centre (set B) are compacted through LDS;
 // the lanes hold A in P groups of 64/P lanes (P the largest power of two
@@ -734,114 +752,172 @@ __device__ __forceinline__ void wave_lds_sync() {
 // sum d.f) are summed over the wavefront into the pair's slot of pairout:
 // F_i = -sum G, tau_i = -sum r_a x G, F_j = sum G, tau_j = sum r_a x G -
 // dm x F_j (every pair force lies along its pair vector).  The cutoff test
-// is the same as k_rigid_forces' (FP32 r^2, FP64 re-test inside the band).
-__global__ __launch_bounds__(256) void k_rigid_pairs(
+// is the same as k_rigid_forces' (FP32 r^2, FP64 re-test inside the band),
+// on coordinates in units of sigma.
+__global__ __launch_bounds__(256, 8) void k_rigid_pairs(
     const double *__restrict__ pos, const float4 *__restrict__ sites,
-    const double *__restrict__ sites_d, const int *__restrict__ hl_count,
-    const int *__restrict__ hl_list, const int *__restrict__ hl_perm,
-    const int *__restrict__ ord_ctl, const double *__restrict__ hh,
-    double *__restrict__ pairout, RigidParams p) {
+    const double *__restrict__ sites_d, const float4 *__restrict__ qcen,
+    const int2 *__restrict__ hl_item, const int *__restrict__ ord_ctl,
+    const double *__restrict__ hh, double *__restrict__ pairout, RigidParams p) {
   __shared__ float4 sB[4][64 + 16];
   __shared__ int sA[4][64];
+  __shared__ double sDm[4][4][3];  // per wavefront: the four items' dm
+  __shared__ int2 sIt[4][4];       // ... and (slot, j)
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
   const int items = p.N * uniform(ord_ctl[0]);
   const double *h = hh, *hi = hh + 9;
-  const int npad = p.natom_pad, cap = p.max_neigh;
+  const int npad = p.natom_pad, nq = p.nq, cap = p.max_neigh;
   const LJ32 lj = p.lj;
-  const double c48 = 48.0 * p.eps, e4 = 4.0 * p.eps;
-  const float reach2 = p.reach * p.reach;
-  for (int t = blockIdx.x * 4 + wv; t < items; t += gridDim.x * 4) {
-    const int r = uniform(t / p.N), i = t - r * p.N;
-    if (r >= hl_count[i]) continue;
-    const int slot = i * cap + uniform(hl_perm[(size_t)i * cap + r]);
-    const int j = uniform(hl_list[slot]);
-    double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
-           dz = pos[3 * j + 2] - pos[3 * i + 2];
-    mimg_exact(dx, dy, dz, hi, h);
-    double *out = pairout + (size_t)slot * kPairOut;
-    bool inA = false, inB = false;
-    float4 sb = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (dx * dx + dy * dy + dz * dz <= p.rmcut2 && lane < p.natom) {
-      const float4 sa = sites[(size_t)i * npad + lane];
-      sb = sites[(size_t)j * npad + lane];
-      const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-      const float ax = sa.x - fx, ay = sa.y - fy, az = sa.z - fz;
-      const float bx = sb.x + fx, by = sb.y + fy, bz = sb.z + fz;
-      inA = ax * ax + ay * ay + az * az < reach2;
-      inB = bx * bx + by * by + bz * bz < reach2;
+  // pair tests in units of sigma: (r / sigma)^2 gives (sigma / r)^2 by one
+  // reciprocal; the FP64 band re-test stays in A
+  const float isig = p.isig, rc2_lo = p.rc2_lo_s, rc2_hi = p.rc2_hi_s, rmin2 = p.rmin2_s;
+  const double c48s = p.c48s, c48 = 48.0 * p.eps, e4 = 4.0 * p.eps;
+  const float reach = p.reach, reach2 = p.reach * p.reach;
+  for (int tb = (blockIdx.x * 4 + wv) * 4; tb < items; tb += gridDim.x * 16) {
+    // lanes 0-3: item tb + lane and its minimum image
+    int2 item = make_int2(-1, -1);
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    bool near = false;
+    if (lane < 4 && tb + lane < items) {
+      item = hl_item[tb + lane];
+      if (item.x >= 0) {
+        const int i = item.x / cap, j = item.y;
+        mx = pos[3 * j] - pos[3 * i];
+        my = pos[3 * j + 1] - pos[3 * i + 1];
+        mz = pos[3 * j + 2] - pos[3 * i + 2];
+        mimg_exact(mx, my, mz, hi, h);
+        near = mx * mx + my * my + mz * mz <= p.rmcut2;
+      }
+      sDm[wv][lane][0] = mx;
+      sDm[wv][lane][1] = my;
+      sDm[wv][lane][2] = mz;
+      sIt[wv][lane] = item;
     }
-    const unsigned long long mA = __ballot(inA), mB = __ballot(inB);
-    if (mA == 0 || mB == 0) {
-      if (lane < kPairOut) out[lane] = 0.0;
-      continue;
+    // lane 16k + q: quad q of item k
+    const int k = lane >> 4, q = lane & 15;
+    const int slot_k = __shfl(item.x, k), j_k = __shfl(item.y, k);
+    const bool near_k = __shfl((int)near, k) != 0;
+    const float fx = __shfl((float)mx, k), fy = __shfl((float)my, k), fz = __shfl((float)mz, k);
+    bool qa = false, qb = false;
+    if (near_k && q < nq) {
+      const float4 ca = qcen[(size_t)(slot_k / cap) * nq + q];
+      const float4 cb = qcen[(size_t)j_k * nq + q];
+      const float ax = ca.x - fx, ay = ca.y - fy, az = ca.z - fz, ra = reach + ca.w;
+      const float bx = cb.x + fx, by = cb.y + fy, bz = cb.z + fz, rb = reach + cb.w;
+      qa = ax * ax + ay * ay + az * az < ra * ra;
+      qb = bx * bx + by * by + bz * bz < rb * rb;
     }
-    const int nA = __popcll(mA), nB = __popcll(mB);
-    if (inA) sA[wv][__popcll(mA & ((1ull << lane) - 1ull))] = lane;
-    if (inB) {
-      sb.w = __int_as_float(lane);
-      sB[wv][__popcll(mB & ((1ull << lane) - 1ull))] = sb;
-    }
-    if (lane < 16) sB[wv][nB + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, 0.f);
+    const unsigned long long qA = __ballot(qa), qB = __ballot(qb);
+    // items that cannot hold a site pair in range: zero shares (lane 16k + c
+    // writes double c of item k)
+    const bool live_k = ((qA >> (16 * k)) & 0xFFFFull) && ((qB >> (16 * k)) & 0xFFFFull);
+    if (slot_k >= 0 && !live_k) pairout[(size_t)slot_k * kPairOut + q] = 0.0;
+    const unsigned long long live = __ballot(live_k && q == 0);  // bit 16k: item k
     wave_lds_sync();
-    int lgG = 6;  // log2 of the lanes per group
-    while (lgG > 2 && nA <= (1 << (lgG - 1))) lgG--;
-    const int P = 64 >> lgG;
-    const int g = lane >> lgG, ta = lane & ((1 << lgG) - 1);
-    const bool act = ta < nA;
-    double ra[3] = {0.0, 0.0, 0.0};
-    float cx = 1.0e6f, cy = 1.0e6f, cz = 1.0e6f;
-    if (act) {
-      const double *q = sites_d + 3 * ((size_t)i * npad + sA[wv][ta]);
-      ra[0] = q[0];
-      ra[1] = q[1];
-      ra[2] = q[2];
-      cx = (float)(dx - ra[0]);
-      cy = (float)(dy - ra[1]);
-      cz = (float)(dz - ra[2]);
-    }
-    const double dm[3] = {dx, dy, dz};
-    float gx = 0.f, gy = 0.f, gz = 0.f, e = 0.f, tr = 0.f;
+#pragma unroll 1
+    for (int kk = 0; kk < 4; kk++) {
+      if (!((live >> (16 * kk)) & 1ull)) continue;
+      const int slot = uniform(sIt[wv][kk].x), j = uniform(sIt[wv][kk].y);
+      const int i = uniform(slot / cap);
+      const double dx = uniform_d(sDm[wv][kk][0]), dy = uniform_d(sDm[wv][kk][1]),
+                   dz = uniform_d(sDm[wv][kk][2]);
+      double *out = pairout + (size_t)slot * kPairOut;
+      bool inA = false, inB = false;
+      float4 sb = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane < p.natom) {
+        const float4 sa = sites[(size_t)i * npad + lane];
+        sb = sites[(size_t)j * npad + lane];
+        const float gx = (float)dx, gy = (float)dy, gz = (float)dz;
+        const float ax = sa.x - gx, ay = sa.y - gy, az = sa.z - gz;
+        const float bx = sb.x + gx, by = sb.y + gy, bz = sb.z + gz;
+        inA = ax * ax + ay * ay + az * az < reach2;
+        inB = bx * bx + by * by + bz * bz < reach2;
+      }
+      const unsigned long long mA = __ballot(inA), mB = __ballot(inB);
+      if (mA == 0 || mB == 0) {
+        if (lane < kPairOut) out[lane] = 0.0;
+        continue;
+      }
+      const int nA = __popcll(mA), nB = __popcll(mB);
+      if (inA) sA[wv][__popcll(mA & ((1ull << lane) - 1ull))] = lane;
+      if (inB)
+        sB[wv][__popcll(mB & ((1ull << lane) - 1ull))] =
+            make_float4(sb.x * isig, sb.y * isig, sb.z * isig, __int_as_float(lane));
+      if (lane < 16) sB[wv][nB + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, 0.f);
+      wave_lds_sync();
+      int lgG = 6;  // log2 of the lanes per group
+      while (lgG > 2 && nA <= (1 << (lgG - 1))) lgG--;
+      const int P = 64 >> lgG;
+      const int g = lane >> lgG, ta = lane & ((1 << lgG) - 1);
+      // lanes past |A| sit far away: every pair test of theirs fails
+      const bool act = ta < nA;
+      double ra[3] = {0.0, 0.0, 0.0};
+      float cx = 1.0e6f, cy = 1.0e6f, cz = 1.0e6f;
+      if (act) {
+        const double *qd = sites_d + 3 * ((size_t)i * npad + sA[wv][ta]);
+        ra[0] = qd[0];
+        ra[1] = qd[1];
+        ra[2] = qd[2];
+        cx = (float)(dx - ra[0]) * isig;
+        cy = (float)(dy - ra[1]) * isig;
+        cz = (float)(dz - ra[2]) * isig;
+      }
+      const double dm[3] = {dx, dy, dz};
+      // per lane: G / (48 eps / sigma) = sum t (sigma/r)^2 e', t = s6 (s6 - 1/2)
+      // with s6 = (sigma/r)^6 (so sum d.f = 48 eps sum t), sum s6 and the
+      // in-range pair count n: E = 4 eps (sum t - sum s6 / 2 - n vs4)
+      float gx = 0.f, gy = 0.f, gz = 0.f, tr = 0.f, s6 = 0.f;
+      int n = 0;
+      // no bounds tests: indices past |B| (up to |B| + P - 1) read the padding
 #pragma unroll 2
-    for (int b0 = 0; b0 < nB; b0 += P) {
-      const float4 rb = sB[wv][b0 + g];
-      const float ex = rb.x + cx, ey = rb.y + cy, ez = rb.z + cz;
-      const float r2 = ex * ex + ey * ey + ez * ez;
-      if (r2 < lj.rc2_hi) {
-        bool in = r2 < lj.rc2_lo;
-        if (!in)
-          in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + __float_as_int(rb.w)), ra) <
-               lj.rc2d;
-        if (in) {
-          const float fm = lj_fm(r2, lj, e);
-          tr += fm * r2;
-          gx += fm * ex;
-          gy += fm * ey;
-          gz += fm * ez;
+      for (int b0 = 0; b0 < nB; b0 += P) {
+        const float4 rb = sB[wv][b0 + g];
+        const float ex = rb.x + cx, ey = rb.y + cy, ez = rb.z + cz;
+        const float r2 = ex * ex + ey * ey + ez * ez;
+        if (r2 < rc2_hi) {
+          bool in = r2 < rc2_lo;
+          if (!in)
+            in = pair_r2_exact(dm, sites_d + 3 * ((size_t)j * npad + __float_as_int(rb.w)),
+                               ra) < lj.rc2d;
+          if (in) {
+            const float sr2 = rcp32(__builtin_amdgcn_fmed3f(r2, rmin2, 3.0e38f));
+            const float sr4 = sr2 * sr2;
+            const float sr6 = sr2 * sr4;
+            const float tt = sr6 * (sr6 - 0.5f);
+            const float fm = tt * sr2;
+            tr += tt;
+            s6 += sr6;
+            n++;
+            gx += fm * ex;
+            gy += fm * ey;
+            gz += fm * ez;
+          }
         }
       }
-    }
-    // the next pair's compaction overwrites sA / sB
-    wave_lds_sync();
-    const double g0 = c48 * gx, g1 = c48 * gy, g2 = c48 * gz;
-    double v[8] = {g0, g1, g2, ra[1] * g2 - ra[2] * g1, ra[2] * g0 - ra[0] * g2,
-                   ra[0] * g1 - ra[1] * g0, e4 * e, c48 * tr};
-    wave_sum8(v);
-    if (lane == 0) {
-      out[0] = -v[0];
-      out[1] = -v[1];
-      out[2] = -v[2];
-      out[3] = -v[3];
-      out[4] = -v[4];
-      out[5] = -v[5];
-      out[6] = v[6];
-      out[7] = v[7];
-      out[8] = v[0];
-      out[9] = v[1];
-      out[10] = v[2];
-      out[11] = v[3] - (dy * v[2] - dz * v[1]);
-      out[12] = v[4] - (dz * v[0] - dx * v[2]);
-      out[13] = v[5] - (dx * v[1] - dy * v[0]);
+      // the next pair's compaction overwrites sA / sB
+      wave_lds_sync();
+      const double g0 = c48s * gx, g1 = c48s * gy, g2 = c48s * gz;
+      double v[8] = {g0, g1, g2, ra[1] * g2 - ra[2] * g1, ra[2] * g0 - ra[0] * g2,
+                     ra[0] * g1 - ra[1] * g0,
+                     e4 * ((double)tr - 0.5 * (double)s6 - (double)lj.vs4 * n), c48 * tr};
+      wave_sum8(v);
+      if (lane == 0) {
+        out[0] = -v[0];
+        out[1] = -v[1];
+        out[2] = -v[2];
+        out[3] = -v[3];
+        out[4] = -v[4];
+        out[5] = -v[5];
+        out[6] = v[6];
+        out[7] = v[7];
+        out[8] = v[0];
+        out[9] = v[1];
+        out[10] = v[2];
+        out[11] = v[3] - (dy * v[2] - dz * v[1]);
+        out[12] = v[4] - (dz * v[0] - dx * v[2]);
+        out[13] = v[5] - (dx * v[1] - dy * v[0]);
+      }
     }
   }
 }
@@ -1105,14 +1181,14 @@ hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, const 
 static hipError_t rigid_launch_forces_half(const RigidDev &d, const RigidParams &p,
                                            const double *hh, hipStream_t s) {
   if (p.pair_kernel) {
-    // one wavefront per work item; the item count is on the device, so the
-    // grid covers up to 32 owned pairs per molecule (a C60 crystal owns at
-    // most about 30 within the list radius), waves past the count exit at
-    // once and the kernel's grid-stride loop takes any excess
+    // 4 work items per wavefront, 16 per workgroup; the item count is on the
+    // device, so the grid covers up to 32 owned pairs per molecule (a C60
+    // crystal owns at most about 30 within the list radius), items past the
+    // count are skipped and the kernel's grid-stride loop takes any excess
     const size_t items = (size_t)p.N * std::min(p.max_neigh, 32);
-    dim3 g((unsigned)std::min<size_t>((items + 3) / 4, 1u << 20));
-    k_rigid_pairs<<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.hl_count, d.hl_list,
-                                    d.hl_perm, d.ord_ctl, hh, d.pairout, p);
+    dim3 g((unsigned)std::min<size_t>((items + 15) / 16, 1u << 20));
+    k_rigid_pairs<<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.qcen, d.hl_item, d.ord_ctl,
+                                    hh, d.pairout, p);
     return hipGetLastError();
   }
   const int per = (p.N + kXcd - 1) / kXcd;
