@@ -735,11 +735,11 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pair-once forces for natom <= 64.  Work item t = r N + i is molecule i's
-// r-th nearest owned pair (built at the list rebuild); each wavefront takes
-// four consecutive items (same rank, neighbouring molecules: similar work),
-// so the costly close pairs start first and the light far ones fill the
-// tail, and the hardware balances the wavefronts.
+// Pair-once forces for natom <= 64.  Work item (r, i) is molecule i's r-th
+// nearest owned pair (hl_item[r N + i], built at the list rebuild); each
+// wavefront takes four items of one rank and neighbouring molecules (similar
+// work), the close pairs first, so the costly ones start early and the light
+// far ones fill the tail, and the hardware balances the wavefronts.
 // Screening, the four items at once: lanes 0-3 take the FP64 minimum image,
 // lanes 16k + q test quad q of item k -- some site quad of i within reach of
 // j's centre and some quad of j within reach of i's centre; items that fail
@@ -765,7 +765,6 @@ __global__ __launch_bounds__(256, 8) void k_rigid_pairs(
   __shared__ int2 sIt[4][4];       // ... and (slot, j)
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
-  const int items = p.N * uniform(ord_ctl[0]);
   const double *h = hh, *hi = hh + 9;
   const int npad = p.natom_pad, nq = p.nq, cap = p.max_neigh;
   const LJ32 lj = p.lj;
@@ -774,13 +773,22 @@ __global__ __launch_bounds__(256, 8) void k_rigid_pairs(
   const float isig = p.isig, rc2_lo = p.rc2_lo_s, rc2_hi = p.rc2_hi_s, rmin2 = p.rmin2_s;
   const double c48s = p.c48s, c48 = 48.0 * p.eps, e4 = 4.0 * p.eps;
   const float reach = p.reach, reach2 = p.reach * p.reach;
-  for (int tb = (blockIdx.x * 4 + wv) * 4; tb < items; tb += gridDim.x * 16) {
+  // XCD-local work: workgroups are dealt round-robin to the 8 XCDs, and the
+  // ones on XCD x take the molecules of slab x (an eighth of the index range,
+  // a slab of the crystal), rank by rank, so an XCD's L2 holds its slab's
+  // sites and their neighbours'
+  const int R = uniform(ord_ctl[0]);
+  const int ns = (p.N + kXcd - 1) / kXcd, xcd = blockIdx.x % kXcd;
+  const int lb = blockIdx.x / kXcd, nlb = (gridDim.x - xcd + kXcd - 1) / kXcd;
+  const int slab_items = ns * R;
+  for (int tb = (lb * 4 + wv) * 4; tb < slab_items; tb += nlb * 16) {
     // lanes 0-3: item tb + lane and its minimum image
     int2 item = make_int2(-1, -1);
     double mx = 0.0, my = 0.0, mz = 0.0;
     bool near = false;
-    if (lane < 4 && tb + lane < items) {
-      item = hl_item[tb + lane];
+    if (lane < 4 && tb + lane < slab_items) {
+      const int u = tb + lane, r = u / ns, i = xcd * ns + (u - r * ns);
+      if (i < p.N) item = hl_item[(size_t)r * p.N + i];
       if (item.x >= 0) {
         const int i = item.x / cap, j = item.y;
         mx = pos[3 * j] - pos[3 * i];
@@ -1181,12 +1189,13 @@ hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, const 
 static hipError_t rigid_launch_forces_half(const RigidDev &d, const RigidParams &p,
                                            const double *hh, hipStream_t s) {
   if (p.pair_kernel) {
-    // 4 work items per wavefront, 16 per workgroup; the item count is on the
-    // device, so the grid covers up to 32 owned pairs per molecule (a C60
-    // crystal owns at most about 30 within the list radius), items past the
-    // count are skipped and the kernel's grid-stride loop takes any excess
-    const size_t items = (size_t)p.N * std::min(p.max_neigh, 32);
-    dim3 g((unsigned)std::min<size_t>((items + 15) / 16, 1u << 20));
+    // 4 work items per wavefront, 16 per workgroup, 8 slabs; the item count
+    // is on the device, so the grid covers up to 32 owned pairs per molecule
+    // (a C60 crystal owns at most about 30 within the list radius), items
+    // past the count are skipped and the kernel's grid-stride loop takes any
+    // excess
+    const size_t slab = ((size_t)p.N + kXcd - 1) / kXcd * std::min(p.max_neigh, 32);
+    dim3 g((unsigned)(kXcd * std::min<size_t>((slab + 15) / 16, 1u << 17)));
     k_rigid_pairs<<<g, 256, 0, s>>>(d.pos, d.sites, d.sites_d, d.qcen, d.hl_item, d.ord_ctl,
                                     hh, d.pairout, p);
     return hipGetLastError();
