@@ -76,10 +76,10 @@ __device__ __noinline__ double r2_ref(const double *__restrict__ pos, int i, int
 
 __device__ __forceinline__ const double *hh_of(HHRef hh, int r) { return hh.p + r * hh.stride; }
 
-// Block sum of K doubles in a fixed order (256 threads).
-template <int K>
+// Block sum of K doubles in a fixed order (NW wavefronts of 64 threads).
+template <int K, int NW = 4>
 __device__ __forceinline__ void block_sum(const double *v, double *out) {
-  __shared__ double red[4][K];
+  __shared__ double red[NW][K];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int k = 0; k < K; k++) {
     double s = wave_sum(v[k]);
@@ -88,7 +88,9 @@ __device__ __forceinline__ void block_sum(const double *v, double *out) {
   __syncthreads();
   if (threadIdx.x < K) {
     const int k = threadIdx.x;
-    out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    double t = red[0][k];
+    for (int w = 1; w < NW; w++) t += red[w][k];
+    out[k] = t;
   }
   __syncthreads();
 }
@@ -480,28 +482,50 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 }
 
 // ------------------------------------------------------------------ REBO
+// Four threads per atom i: thread q takes i's list entries q, q + 4, ...
 __global__ __launch_bounds__(256) void k_rebo_pass1(AtomicDev d, AtomicParams p, HHRef hh) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= p.Na * p.maxr) return;
-  const int i = t / p.maxr, jn = t - i * p.maxr;
-  if (jn >= d.nlc_r[i]) return;
+  const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
+  if (i >= p.Na) return;
   const double *h = hh_of(hh, d.atom_rep[i]);
-  rebo_pass1<float>(i, jn, d.pos, h, h + 9, d.nlc_r, d.nll_r, p.maxr, d.bonds);
+  const int n = d.nlc_r[i];
+  for (int jn = threadIdx.x & 3; jn < n; jn += 4)
+    rebo_pass1<float>(i, jn, d.pos, h, h + 9, d.nlc_r, d.nll_r, p.maxr, d.bonds);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_rebo_pass2(AtomicDev d, AtomicParams p) {
-  const int x = d.blk_a0[blockIdx.x] + threadIdx.x;
+// Four threads per atom x (integrator block of up to 256 atoms -> 1024
+// threads): thread q takes x's neighbour entries q, q + 4, ...; the four
+// partial sums are added in a fixed order.  TRACE: the virial trace only (on
+// the diagonal, a third each), what the time steps use.
+template <int MODE, bool TRACE>
+__global__ __launch_bounds__(1024, 8) void k_rebo_pass2(AtomicDev d, AtomicParams p) {
+  const int x = d.blk_a0[blockIdx.x] + (threadIdx.x >> 2), q = threadIdx.x & 3;
+  const bool ok = x < d.blk_a1[blockIdx.x];
+  double F[3] = {0, 0, 0}, E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (ok)
+    for (int yn = q; yn < d.nlc_r[x]; yn += 4)
+      rebo_pass2_entry<float, MODE, TRACE>(x, yn, d.nlc_r, d.nll_r, p.maxr, d.bonds, F, E, W);
+  auto quad = [](double v) {
+    v += __shfl_xor(v, 1, 64);
+    return v + __shfl_xor(v, 2, 64);
+  };
+  for (int k = 0; k < 3; k++) F[k] = quad(F[k]);
+  E = quad(E);
+  if (TRACE)
+    W[0] = quad(W[0]);
+  else
+    for (int k = 0; k < 9; k++) W[k] = quad(W[k]);
   double part[kAtPart];
   for (int k = 0; k < kAtPart; k++) part[k] = 0;
-  if (x < d.blk_a1[blockIdx.x]) {
-    double o[13];
-    rebo_pass2<float, MODE>(x, d.nlc_r, d.nll_r, p.maxr, d.bonds, o);
-    for (int k = 0; k < 3; k++) d.Frebo[3 * x + k] = o[k];
-    for (int k = 0; k < 9; k++) part[k] = o[4 + k];
-    part[9] = o[3];
+  if (ok && q == 0) {
+    for (int k = 0; k < 3; k++) d.Frebo[3 * x + k] = F[k];
+    if (TRACE) {
+      part[0] = part[4] = part[8] = W[0] / 3.0;
+    } else {
+      for (int k = 0; k < 9; k++) part[k] = W[k];
+    }
+    part[9] = E;
   }
-  block_sum<kAtPart>(part, d.rebo_part + (size_t)blockIdx.x * kAtPart);
+  block_sum<kAtPart, 16>(part, d.rebo_part + (size_t)blockIdx.x * kAtPart);
 }
 
 // -------------------------------------------------------------------- LJ
@@ -954,11 +978,15 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef h
 
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
                           hipStream_t s) {
-  k_rebo_pass1<<<(p.Na * p.maxr + 255) / 256, 256, 0, s>>>(d, p, hh);
-  if (mode == kReboPhysical)
-    k_rebo_pass2<kReboPhysical><<<d.nblk, 256, 0, s>>>(d, p);
+  k_rebo_pass1<<<(p.Na + 63) / 64, 256, 0, s>>>(d, p, hh);
+  if (mode == kReboPhysical && p.lj_trace)
+    k_rebo_pass2<kReboPhysical, true><<<d.nblk, 1024, 0, s>>>(d, p);
+  else if (mode == kReboPhysical)
+    k_rebo_pass2<kReboPhysical, false><<<d.nblk, 1024, 0, s>>>(d, p);
+  else if (p.lj_trace)
+    k_rebo_pass2<kReboCompat, true><<<d.nblk, 1024, 0, s>>>(d, p);
   else
-    k_rebo_pass2<kReboCompat><<<d.nblk, 256, 0, s>>>(d, p);
+    k_rebo_pass2<kReboCompat, false><<<d.nblk, 1024, 0, s>>>(d, p);
   return hipGetLastError();
 }
 

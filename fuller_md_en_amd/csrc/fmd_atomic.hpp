@@ -73,7 +73,8 @@ struct AtomicParams {
   int small_box;   // 1: per-pair minimum image (a cell narrower than the list)
   int use_sg;      // 1: the learned surrogate stands in for the REBO term
   int trace_virial;  // 1: time steps keep only the LJ virial trace (engines)
-  int lj_trace;      // set per launch: this force evaluation keeps only the trace
+  int lj_trace;      // set per launch: this force evaluation keeps only the
+                     // virial trace (LJ and REBO)
   int scan_fast;   // 1: list walk prefiltered in FP32 through the molecule
                    // tiles (cell wide enough for the list radius + 2 R_max)
   double rebo_rlist2;  // REBO list radius^2 (2.3^2, AB:253)

@@ -5,11 +5,11 @@
 //           length, switch, pair terms, the bond-order sum Gs_ij over
 //           k in N(i)\{j}, b_ij = (1+Gs)^-1/2, the angular coefficient
 //           1/2 fc VA db/dGs, validity flags and the reverse entry (j -> i).
-//   pass 2, one thread per atom x: every force component that lands on x --
-//           the pair terms of its bonds (b-bar from both directions), the
-//           angular terms where x is the centre, the third atom (and, in
-//           the physical mode, the partner) -- plus x's share of energy and
-//           virial.
+//   pass 2, per atom x, one thread per neighbour entry: every force
+//           component that lands on x -- the pair terms of its bonds (b-bar
+//           from both directions), the angular terms where x is the centre,
+//           the third atom (and, in the physical mode, the partner) -- plus
+//           x's share of energy and virial, summed over the entries.
 //
 // Modes: kReboCompat reproduces the reference exactly (angular force on the
 // third atom with the reference's sign, no d cos / d r_partner term, pair-only
@@ -106,18 +106,21 @@ FMD_HD void rebo_pass1(int i, int jn, const double *pos, const double *h, const 
   B.bq[4 * s + 3] = b;
 }
 
-// Forces on atom x; out[0..2] force, out[3] energy share, out[4..12] virial
-// share (row-major).
-template <class R, int MODE>
-FMD_HD void rebo_pass2(int x, const int *nlc, const int *nll, int maxr, ReboBonds B,
-                       double *out) {
+// Contributions to atom x through its neighbour entry yn (neighbour y = c):
+// the pair term of bond (x, y) (energy and virial once, from the lower
+// index), the angular terms centred on x with partner y, and the angular
+// terms centred on y in which x is the third atom (physical mode: also those
+// with x as the partner).  Summed over yn they are every force component
+// that lands on x.  F += force, E += energy share, W += virial share
+// (row-major; TRACE: W[0] += its trace only).
+template <class R, int MODE, bool TRACE>
+FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, int maxr,
+                             ReboBonds B, double *F, double &E, double *W) {
   using namespace rebo;
-  double F[3] = {0, 0, 0}, E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int nx = nlc[x];
-  for (int yn = 0; yn < nx; yn++) {
-    const int s = x * maxr + yn;
-    const int fl = B.bflag[s];
-    if (!(fl & kBondPair)) continue;
+  const int s = x * maxr + yn;
+  const int fl = B.bflag[s];
+  if (fl & kBondPair) {
     const int y = nll[s];
     const R d[3] = {B.bd[4 * s], B.bd[4 * s + 1], B.bd[4 * s + 2]};
     const R r = B.bd[4 * s + 3];
@@ -131,8 +134,12 @@ FMD_HD void rebo_pass2(int x, const int *nlc, const int *nll, int maxr, ReboBond
     for (int k = 0; k < 3; k++) F[k] += fpair * (double)d[k];
     if (x < y) {
       E += f * (vr - bbar * va);
-      for (int a = 0; a < 3; a++)
-        for (int c = 0; c < 3; c++) W[3 * a + c] -= (double)d[a] * fpair * (double)d[c];
+      if (TRACE) {
+        W[0] -= fpair * ((double)d[0] * d[0] + (double)d[1] * d[1] + (double)d[2] * d[2]);
+      } else {
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) W[3 * a + c] -= (double)d[a] * fpair * (double)d[c];
+      }
     }
     // angular terms with centre x, partner y
     if (fl & kBondAngular) {
@@ -149,53 +156,53 @@ FMD_HD void rebo_pass2(int x, const int *nlc, const int *nll, int maxr, ReboBond
           for (int k = 0; k < 3; k++) F[k] += (double)ft[k];
         } else {
           for (int k = 0; k < 3; k++) F[k] -= (double)(ft[k] + fp[k]);
-          for (int a = 0; a < 3; a++)
-            for (int c = 0; c < 3; c++)
-              W[3 * a + c] += (double)(dt[a] * ft[c] + d[a] * fp[c]);
+          if (TRACE) {
+            W[0] += (double)(dt[0] * ft[0] + dt[1] * ft[1] + dt[2] * ft[2] + d[0] * fp[0] +
+                             d[1] * fp[1] + d[2] * fp[2]);
+          } else {
+            for (int a = 0; a < 3; a++)
+              for (int c = 0; c < 3; c++)
+                W[3 * a + c] += (double)(dt[a] * ft[c] + d[a] * fp[c]);
+          }
         }
       }
     }
   }
   // x as the third atom (and, physical, as the partner) of bonds centred on
-  // its neighbours c
-  for (int cn = 0; cn < nx; cn++) {
-    const int sx = B.brev[x * maxr + cn];  // entry c -> x
-    if (sx < 0) continue;
-    const int c = nll[x * maxr + cn];
-    const int fl_cx = B.bflag[sx];
-    const R dcx[3] = {B.bd[4 * sx], B.bd[4 * sx + 1], B.bd[4 * sx + 2]};
-    const R rcx = B.bd[4 * sx + 3];
-    const int nc = nlc[c];
-    if (fl_cx & kBondThird) {
-      const R fcx = B.ba[4 * sx], dfcx = B.ba[4 * sx + 1];
-      for (int pn = 0; pn < nc; pn++) {
-        const int sp = c * maxr + pn;
-        if (sp == sx || !(B.bflag[sp] & kBondAngular)) continue;
-        const R dcp[3] = {B.bd[4 * sp], B.bd[4 * sp + 1], B.bd[4 * sp + 2]};
-        R ft[3], fp[3];
-        angular(dcp, (R)B.bd[4 * sp + 3], dcx, rcx, fcx, dfcx, (R)B.ba[4 * sp + 2], ft, fp);
-        if (MODE == kReboCompat)
-          for (int k = 0; k < 3; k++) F[k] -= (double)ft[k];
-        else
-          for (int k = 0; k < 3; k++) F[k] += (double)ft[k];
-      }
-    }
-    if (MODE == kReboPhysical && (fl_cx & kBondAngular)) {
-      const R coeff = B.ba[4 * sx + 2];
-      for (int tn = 0; tn < nc; tn++) {
-        const int st = c * maxr + tn;
-        if (st == sx || !(B.bflag[st] & kBondThird)) continue;
-        const R dct[3] = {B.bd[4 * st], B.bd[4 * st + 1], B.bd[4 * st + 2]};
-        R ft[3], fp[3];
-        angular(dcx, rcx, dct, (R)B.bd[4 * st + 3], (R)B.ba[4 * st], (R)B.ba[4 * st + 1],
-                coeff, ft, fp);
-        for (int k = 0; k < 3; k++) F[k] += (double)fp[k];
-      }
+  // its neighbour c
+  const int sx = B.brev[s];  // entry c -> x
+  if (sx < 0) return;
+  const int c = nll[s];
+  const int fl_cx = B.bflag[sx];
+  const R dcx[3] = {B.bd[4 * sx], B.bd[4 * sx + 1], B.bd[4 * sx + 2]};
+  const R rcx = B.bd[4 * sx + 3];
+  const int nc = nlc[c];
+  if (fl_cx & kBondThird) {
+    const R fcx = B.ba[4 * sx], dfcx = B.ba[4 * sx + 1];
+    for (int pn = 0; pn < nc; pn++) {
+      const int sp = c * maxr + pn;
+      if (sp == sx || !(B.bflag[sp] & kBondAngular)) continue;
+      const R dcp[3] = {B.bd[4 * sp], B.bd[4 * sp + 1], B.bd[4 * sp + 2]};
+      R ft[3], fp[3];
+      angular(dcp, (R)B.bd[4 * sp + 3], dcx, rcx, fcx, dfcx, (R)B.ba[4 * sp + 2], ft, fp);
+      if (MODE == kReboCompat)
+        for (int k = 0; k < 3; k++) F[k] -= (double)ft[k];
+      else
+        for (int k = 0; k < 3; k++) F[k] += (double)ft[k];
     }
   }
-  for (int k = 0; k < 3; k++) out[k] = F[k];
-  out[3] = E;
-  for (int k = 0; k < 9; k++) out[4 + k] = W[k];
+  if (MODE == kReboPhysical && (fl_cx & kBondAngular)) {
+    const R coeff = B.ba[4 * sx + 2];
+    for (int tn = 0; tn < nc; tn++) {
+      const int st = c * maxr + tn;
+      if (st == sx || !(B.bflag[st] & kBondThird)) continue;
+      const R dct[3] = {B.bd[4 * st], B.bd[4 * st + 1], B.bd[4 * st + 2]};
+      R ft[3], fp[3];
+      angular(dcx, rcx, dct, (R)B.bd[4 * st + 3], (R)B.ba[4 * st], (R)B.ba[4 * st + 1], coeff,
+              ft, fp);
+      for (int k = 0; k < 3; k++) F[k] += (double)fp[k];
+    }
+  }
 }
 
 }  // namespace fmd
