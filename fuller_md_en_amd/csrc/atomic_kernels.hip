@@ -482,14 +482,29 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 }
 
 // ------------------------------------------------------------------ REBO
-// Four threads per atom i: thread q takes i's list entries q, q + 4, ...
+// Four threads per atom i (64 atoms per workgroup): thread q takes i's list
+// entries q, q + 4, ...; the bond vectors (FP64 minimum image) are formed
+// once per entry into LDS and read there by the bond-order sums of i's other
+// entries.
 __global__ __launch_bounds__(256) void k_rebo_pass1(AtomicDev d, AtomicParams p, HHRef hh) {
-  const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
-  if (i >= p.Na) return;
-  const double *h = hh_of(hh, d.atom_rep[i]);
-  const int n = d.nlc_r[i];
-  for (int jn = threadIdx.x & 3; jn < n; jn += 4)
-    rebo_pass1<float>(i, jn, d.pos, h, h + 9, d.nlc_r, d.nll_r, p.maxr, d.bonds);
+  extern __shared__ double4 vec[];  // [64 atoms][maxr]
+  const int al = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int i = blockIdx.x * 64 + al;
+  const int n = i < p.Na ? d.nlc_r[i] : 0;
+  double4 *vi = vec + (size_t)al * p.maxr;
+  if (i < p.Na) {
+    const double *h = hh_of(hh, d.atom_rep[i]), *hi = h + 9;
+    for (int jn = q; jn < n; jn += 4) {
+      const int j = d.nll_r[(size_t)i * p.maxr + jn];
+      double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
+             dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
+      mimg(dx, dy, dz, hi, h);
+      vi[jn] = make_double4(dx, dy, dz, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  }
+  __syncthreads();
+  for (int jn = q; jn < n; jn += 4)
+    rebo_pass1<float>(i, jn, vi, d.nlc_r, d.nll_r, p.maxr, d.bonds);
 }
 
 // Four threads per atom x (integrator block of up to 256 atoms -> 1024
@@ -978,7 +993,7 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef h
 
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
                           hipStream_t s) {
-  k_rebo_pass1<<<(p.Na + 63) / 64, 256, 0, s>>>(d, p, hh);
+  k_rebo_pass1<<<(p.Na + 63) / 64, 256, 64 * p.maxr * sizeof(double4), s>>>(d, p, hh);
   if (mode == kReboPhysical && p.lj_trace)
     k_rebo_pass2<kReboPhysical, true><<<d.nblk, 1024, 0, s>>>(d, p);
   else if (mode == kReboPhysical)

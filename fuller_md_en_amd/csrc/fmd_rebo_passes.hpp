@@ -41,17 +41,16 @@ struct ReboBonds {
   int *bflag, *brev;
 };
 
+// Entry (i -> j = nll[i maxr + jn]); vec[kn] = (dx, dy, dz, r) of i's entry
+// kn (minimum image r_k - r_i, FP64), so every bond vector of i is taken once.
 template <class R>
-FMD_HD void rebo_pass1(int i, int jn, const double *pos, const double *h, const double *hi,
-                       const int *nlc, const int *nll, int maxr, ReboBonds B) {
+FMD_HD void rebo_pass1(int i, int jn, const double4 *vec, const int *nlc, const int *nll,
+                       int maxr, ReboBonds B) {
   using namespace rebo;
   const int s = i * maxr + jn;
   const int j = nll[s];
   const int nni = nlc[i];
-  double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
-         dz = pos[3 * j + 2] - pos[3 * i + 2];
-  mimg(dx, dy, dz, hi, h);
-  const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+  const double dx = vec[jn].x, dy = vec[jn].y, dz = vec[jn].z, rd = vec[jn].w;
   const R r = (R)rd;
   int rev = -1;
   for (int ln = 0; ln < nlc[j]; ln++)
@@ -74,12 +73,8 @@ FMD_HD void rebo_pass1(int i, int jn, const double *pos, const double *h, const 
   const R ux = (R)dx, uy = (R)dy, uz = (R)dz;
   R Gs = 0;
   for (int kn = 0; kn < nni; kn++) {
-    const int k = nll[i * maxr + kn];
-    if (k == j) continue;
-    double ex = pos[3 * k] - pos[3 * i], ey = pos[3 * k + 1] - pos[3 * i + 1],
-           ez = pos[3 * k + 2] - pos[3 * i + 2];
-    mimg(ex, ey, ez, hi, h);
-    const double rkd = sqrt(ex * ex + ey * ey + ez * ez);
+    if (kn == jn) continue;
+    const double ex = vec[kn].x, ey = vec[kn].y, ez = vec[kn].z, rkd = vec[kn].w;
     if (rkd > kDmax) continue;
     const R rk = (R)rkd;
     const R fk = fc(rk);
