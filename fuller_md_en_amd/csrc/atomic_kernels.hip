@@ -99,6 +99,7 @@ __device__ __forceinline__ void block_sum(const double *v, double *out) {
 __global__ __launch_bounds__(256) void k_at_apply_pbc(double *__restrict__ pos,
                                                       const int *__restrict__ atom_rep,
                                                       HHRef hh, int n) {
+#pragma clang fp contract(off)  // the reference's apply_pbc bit for bit (see inv9)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const double *h = hh_of(hh, atom_rep[i]), *hi = h + 9;

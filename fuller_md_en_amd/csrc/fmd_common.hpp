@@ -27,12 +27,21 @@ constexpr double kEv2KcalMol = 23.06054783;
 constexpr double kMassC = 12.011;
 
 // ------------------------------------------------------------ cell algebra
+// The cell algebra has the reference's operation order and no FMA
+// contraction: h^-1 feeds the minimum image, where a pair exactly half a
+// cell apart is decided by the last bit.
 FMD_HD double det9(const double *h) {  // C1:137-141
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   return h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
          h[2] * (h[3] * h[7] - h[4] * h[6]);
 }
 
 FMD_HD void inv9(const double *h, double *hi) {  // C1:147-158
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   double d = det9(h), id = 1.0 / d;
   hi[0] = id * (h[4] * h[8] - h[5] * h[7]);
   hi[1] = id * (h[2] * h[7] - h[1] * h[8]);

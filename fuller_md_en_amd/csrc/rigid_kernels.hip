@@ -275,6 +275,7 @@ __global__ __launch_bounds__(256) void k_rigid_sites(RigidDev d, RigidParams p) 
 
 __global__ __launch_bounds__(256) void k_rigid_apply_pbc(double *__restrict__ pos,
                                                          const double *__restrict__ hh, int n) {
+#pragma clang fp contract(off)  // the reference's apply_pbc bit for bit (see inv9)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const double *h = hh, *hi = hh + 9;

@@ -62,6 +62,7 @@ def lj():
         L.reflj_default_a0.restype = _d
         L.reflj_default_a0.argtypes = [_d, C.c_char_p, _d]
         L.reflj_make_crystal.argtypes = [C.c_char_p, _d, _i, _dp, _dp]
+        L.reflj_apply_pbc.argtypes = [_dp, _dp, _dp, _i]
         L.reflj_nlist.argtypes = [_dp, _dp, _dp, _i, _d, _ip, _ip]
         L.reflj_forces.restype = _d
         L.reflj_forces.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i,

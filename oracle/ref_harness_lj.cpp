@@ -50,6 +50,10 @@ int reflj_make_crystal(const char *st, double a, int nc, double *pos, double *h)
   return make_bcc(a, nc, pos, h);
 }
 
+void reflj_apply_pbc(double *pos, const double *h, const double *hi, int n) {
+  apply_pbc(pos, h, hi, n);
+}
+
 void reflj_nlist(const double *pos, const double *h, const double *hi, int n, double rmcut,
                  int *nl_count, int *nl_list) {
   nlist_build_sym(pos, h, hi, n, rmcut, nl_count, nl_list);

@@ -20,8 +20,9 @@ Fixtures
   rigid_core_nc3.npz      C1 initial state at nc=3 (config 1, 108 molecules):
                           list and forces.
   rigid_cc1_<X>.npz       LJ program functions on .cc1 molecules (C60, C70,
-                          C84 No.01) with seeded random orientations: list,
-                          forces, 3 NPT steps.
+                          C84 No.01 in FCC; C72 and C84 No.20 in HCP, C76-Td
+                          and C60 in BCC) with seeded random orientations:
+                          list, forces, 3 NPT steps.
   airebo_nc1.npz          AB on a jittered 1x1x1 C60 crystal (240 atoms):
                           REBO / LJ lists, compute_rebo, compute_lj,
                           compute_forces and 5 NPT steps.
@@ -109,22 +110,31 @@ def random_quats(rng, n):
     return q / np.linalg.norm(q, axis=1, keepdims=True)
 
 
-def gen_cc1():
+CC1_CASES = [("C60", "C60-76/C60-Ih.cc1", "FCC"), ("C70", "C60-76/C70-D5h.cc1", "FCC"),
+             ("C84", "C84/C84-No.01-D2.cc1", "FCC")]
+# other lattices and species (written by `gen_golden.py lattices`)
+LATTICE_CASES = [("C72_HCP", "C60-76/C72-D6d.cc1", "HCP"),
+                 ("C76Td_BCC", "C60-76/C76-Td.cc1", "BCC"),
+                 ("C84Td_HCP", "C84/C84-No.20-Td.cc1", "HCP"),
+                 ("C60_BCC", "C60-76/C60-Ih.cc1", "BCC")]
+
+
+def gen_cc1(cases=CC1_CASES):
     L = ref.lj()
-    cases = [("C60", "C60-76/C60-Ih.cc1"), ("C70", "C60-76/C70-D5h.cc1"),
-             ("C84", "C84/C84-No.01-D2.cc1")]
-    for tag, rel in cases:
+    for tag, rel, st in cases:
         coords = np.zeros(84 * 3)
         props = np.zeros(5)
         natom = L.reflj_load_cc1(os.path.join(LIBDIR, rel).encode(), coords, props)
         body = coords[: natom * 3].reshape(natom, 3).copy()
         rmol, dmol, I0, Mmol = props[1], props[2], props[3], props[4]
-        a0 = L.reflj_default_a0(dmol, b"FCC", 1.0)
+        a0 = L.reflj_default_a0(dmol, st.encode(), 1.0)
         nc = 2
-        n = 4 * nc ** 3
-        pos = np.zeros(n * 3)
+        pos = np.zeros(8 * nc ** 3 * 3)
         h = np.zeros(9)
-        L.reflj_make_crystal(b"FCC", a0, nc, pos, h)
+        n = L.reflj_make_crystal(st.encode(), a0, nc, pos, h)
+        pos = pos[: 3 * n].copy()
+        if st != "FCC":  # the program wraps before its first forces (LJ:827)
+            L.reflj_apply_pbc(pos, h, oracle.inv9(h), n)
         rng = np.random.default_rng(2024 + natom)
         qv = random_quats(rng, n)
         sv = np.sqrt(oracle.KB * 300.0 * oracle.CONV / Mmol)
@@ -142,6 +152,7 @@ def gen_cc1():
         ep = L.reflj_forces(Fv, Tv, W, pos, qv.reshape(-1), body.reshape(-1), h, hi, cnt, lst,
                             n, natom, rmcut ** 2, lab)
         rec = dict(natom=natom, body=body, I0=I0, Mmol=Mmol, Rmol=rmol, Dmol=dmol, a0=a0, nc=nc,
+                   crystal=st,
                    pos=pos.reshape(n, 3), qv=qv, vel=vel, omg=omg, h=h, rmcut=rmcut,
                    nl_count=cnt, nl_list=lst.reshape(n, 80), Ep=ep, Fv=Fv.reshape(n, 3),
                    Tv=Tv.reshape(n, 3), Wm9=W)
@@ -236,6 +247,9 @@ def gen_airebo():
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference exists")
+    if sys.argv[1:] == ["lattices"]:
+        gen_cc1(LATTICE_CASES)
+        sys.exit(0)
     gen_rng()
     gen_core()
     gen_cc1()

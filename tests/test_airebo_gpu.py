@@ -142,6 +142,30 @@ def test_engine_forces_nc3_against_oracle(mode):
     eng.close()
 
 
+@pytest.mark.parametrize("spec,lattice,nc", [("C70", "HCP", 3), ("C60", "BCC", 3),
+                                             ("C84:20", "HCP", 2)])
+def test_engine_forces_other_lattices_against_oracle(spec, lattice, nc):
+    """HCP and BCC crystals of other species (the reference's lattice and .cc1
+    builders, +-0.02 A jitter), exact-gradient REBO + LJ with the truncated
+    half list, against the FP64 oracle on the same lists."""
+    from fuller_md_en_amd import system
+    print(f"case {spec} {lattice} nc={nc}", flush=True)
+    s = system.airebo_system(nc, spec, lattice, jitter=0.02, jitter_seed=5)
+    p = s["pos"].reshape(-1).copy()
+    oracle.lib().orc_apply_pbc(p, s["h"], oracle.inv9(s["h"]), len(s["pos"]))
+    pos, h, mol = p.reshape(-1, 3), s["h"], s["mol_id"]
+    lists = (*oracle.nlist_rebo(pos, h, mol), *oracle.nlist_lj(pos, h, mol))
+    E0, F0, W0 = oracle.airebo_forces(pos, h, lists, oracle.REBO_PHYSICAL)
+    eng = _engine(pos, h, mol, mode=airebo.REBO_PHYSICAL)
+    eng.prepare()
+    E, F, W = eng.forces()
+    assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
+    for a, b in zip(E, E0):
+        assert rel(a, b) < TOL
+    assert eng.counters()[0] == 0
+    eng.close()
+
+
 def test_engine_trajectory_matches_reference():
     g = golden("airebo_nc1.npz")
     eng = _engine(g["pos"], g["h"], g["mol_id"], vel=g["vel"])

@@ -18,9 +18,11 @@ def close(a, b, rt=RT):
     assert np.abs(a - b).max() <= rt * scale, (np.abs(a - b).max(), scale)
 
 
-@pytest.mark.parametrize("name", ["rigid_core_nc2.npz", "rigid_core_nc3.npz",
-                                  "rigid_cc1_C60.npz", "rigid_cc1_C70.npz",
-                                  "rigid_cc1_C84.npz"])
+CC1 = ["C60", "C70", "C84", "C72_HCP", "C76Td_BCC", "C84Td_HCP", "C60_BCC"]
+
+
+@pytest.mark.parametrize("name", ["rigid_core_nc2.npz", "rigid_core_nc3.npz"] +
+                         [f"rigid_cc1_{t}.npz" for t in CC1])
 def test_rigid_list_and_forces(name):
     g = golden(name)
     cnt, lst = oracle.rigid_nlist(g["pos"], g["h"], float(g["rmcut"]))
@@ -74,7 +76,7 @@ def test_rigid_core_trajectory():
     close(r["npt"].to_array(), g["npt_end"], 1e-9)
 
 
-@pytest.mark.parametrize("tag", ["C60", "C70", "C84"])
+@pytest.mark.parametrize("tag", CC1)
 def test_rigid_cc1_steps(tag):
     g = golden(f"rigid_cc1_{tag}.npz")
     r = _rigid_traj(g, 3, 0)
@@ -140,3 +142,24 @@ def test_airebo_nc2_truncated_lj():
     close(E, g["E"])
     close(F, g["F"])
     close(W, g["W"])
+
+
+@pytest.mark.parametrize("tag", ["C72_HCP", "C76Td_BCC", "C84Td_HCP", "C60_BCC"])
+def test_host_lattices_equal_reference(tag):
+    """The host crystal builders and lattice constant (LJ:229-254) against
+    the reference program's for HCP and BCC, and the .cc1 loader's body
+    frame for other species (LJ:178-207)."""
+    from fuller_md_en_amd import system
+    g = golden(f"rigid_cc1_{tag}.npz")
+    st = str(g["crystal"])
+    # the reference's --fullerene spellings (LJ:210-226)
+    mol = system.molecule({"C72_HCP": "C72:D6d", "C76Td_BCC": "C76:Td",
+                           "C84Td_HCP": "C84:20", "C60_BCC": "C60"}[tag])
+    assert mol.natom == int(g["natom"])
+    close(mol.coords, g["body"], 1e-14)
+    a0 = system.default_a0(mol.dmol, st, 1.0)
+    close(a0, float(g["a0"]), 1e-14)
+    pos, h = system.make_crystal(st, a0, int(g["nc"]))
+    assert pos.shape == g["pos"].shape
+    close(pos, g["pos"], 1e-14)
+    close(h, g["h"], 1e-14)
