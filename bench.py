@@ -44,7 +44,8 @@ RIGID_KERNEL = {1: "k_rigid_pairs", 2: "k_rigid_forces_half", 0: "k_rigid_forces
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors being what
 # the same counters read for an elementwise kernel moving 1 GiB each way
 # (profiles/pmc_traffic_rigid_c60_fcc10.md).
-RIGID_TRAFFIC = {2: (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0}
+RIGID_TRAFFIC = {1: (9710.9 * 2.0 + 23524.8 * 1.0) * 1024.0,
+                 2: (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
