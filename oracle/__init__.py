@@ -27,6 +27,11 @@ EV2GPA = 160.21766208
 
 REBO_COMPAT = 0
 REBO_PHYSICAL = 1
+MM_COMPAT = 0
+MM_PHYSICAL = 1
+KCAL2EV = 1.0 / 23.06054783
+MM_K4_DEFAULT = (469.0 * KCAL2EV, 63.0 * KCAL2EV, 14.5 * KCAL2EV, 15.0 * KCAL2EV)
+MM_RLIST = 3.0 * 3.431 + 2.0
 
 
 class LJParams(C.Structure):
@@ -53,6 +58,14 @@ class NPT(C.Structure):
             s.Vg[i] = a[2 + i]
         s.W, s.Pe, s.Tt, s.Nf = a[11], a[12], a[13], int(a[14])
         return s
+
+
+class MMTopo(C.Structure):
+    """orc_mm_topo: the flat topology arrays of build_flat_topology (MM:228-239)."""
+    _fields_ = [("Nb", C.c_int), ("Nang", C.c_int), ("Ndih", C.c_int), ("Nimp", C.c_int),
+                ("b", C.c_void_p), ("bp", C.c_void_p), ("a", C.c_void_p),
+                ("ap", C.c_void_p), ("d", C.c_void_p), ("dp", C.c_void_p),
+                ("dm", C.c_void_p), ("im", C.c_void_p), ("ip", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -106,6 +119,14 @@ def _load():
         "orc_airebo_step_npt": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _d,
                                        C.POINTER(NPT), _ip, _ip, _i, _ip, _ip, _i, _i,
                                        C.POINTER(LJParams), _dp]),
+        "orc_lj_params_mm": (None, [C.POINTER(LJParams)]),
+        "orc_mm_topology_counts": (None, [_i, _ip, _i, _ip]),
+        "orc_mm_topology": (None, [_dp, _i, _ip, _i, _i, _dp, _ip, _dp, _ip, _dp, _ip, _dp,
+                                   _ip, _ip, _dp, _ip]),
+        "orc_mm_forces": (None, [_dp, _dp, _dp, _dp, _dp, C.POINTER(MMTopo), _ip, _ip, _i,
+                                 _dp, _ip, _i, _i, _i, _dp]),
+        "orc_mm_step_npt": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _i, _d, C.POINTER(NPT),
+                                   C.POINTER(MMTopo), _ip, _ip, _i, _dp, _ip, _i, _i, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -242,3 +263,73 @@ def airebo_forces(pos, h, lists, mode=REBO_COMPAT, params=None):
                             nll_r.size // na, _c(nlc_l, np.int32), nll_l.reshape(-1),
                             nll_l.size // na, na, mode, C.byref(params), out)
     return tuple(out), F.reshape(na, 3), W
+
+
+# ------------------------------------------------------ molecular mechanics
+def mm_topology(coords, bonds, nmol, k4=MM_K4_DEFAULT):
+    """build_flat_topology (MM:254-334) of nmol copies: dict of flat arrays
+    b (Nb,2), bp (Nb,{kb,r0}), a (Nang,3), ap, d (Ndih,4), dp, dm, i (Nimp,4),
+    ip, mol_id."""
+    coords = _c(coords).reshape(-1)
+    bonds = _c(bonds, np.int32).reshape(-1)
+    natom, nbond = coords.size // 3, bonds.size // 2
+    cnt = np.zeros(4, np.int32)
+    lib().orc_mm_topology_counts(natom, bonds, nbond, cnt)
+    nb, na_, nd, ni = (int(x) * nmol for x in cnt)
+    t = dict(b=np.zeros((nb, 2), np.int32), bp=np.zeros((nb, 2)),
+             a=np.zeros((na_, 3), np.int32), ap=np.zeros((na_, 2)),
+             d=np.zeros((nd, 4), np.int32), dp=np.zeros((nd, 2)),
+             dm=np.zeros(nd, np.int32), i=np.zeros((ni, 4), np.int32),
+             ip=np.zeros((ni, 2)), mol_id=np.zeros(nmol * natom, np.int32))
+    lib().orc_mm_topology(coords, natom, bonds, nbond, nmol, _c(k4), t["b"].reshape(-1),
+                          t["bp"].reshape(-1), t["a"].reshape(-1), t["ap"].reshape(-1),
+                          t["d"].reshape(-1), t["dp"].reshape(-1), t["dm"],
+                          t["i"].reshape(-1), t["ip"].reshape(-1), t["mol_id"])
+    return t
+
+
+def _mm_struct(t):
+    arrs = {k: _c(t[k], np.int32 if k in ("b", "a", "d", "dm", "i") else np.float64)
+            for k in ("b", "bp", "a", "ap", "d", "dp", "dm", "i", "ip")}
+    s = MMTopo(len(arrs["b"]), len(arrs["a"]), len(arrs["d"]), len(arrs["i"]),
+               *(arrs[k].ctypes.data for k in ("b", "bp", "a", "ap", "d", "dp", "dm", "i",
+                                               "ip")))
+    return s, arrs  # keep arrs alive with s
+
+
+def mm_nlist(pos, h, mol_id, max_neigh=400):
+    """build_nlist_lj (MM:337-349): half list within rc + 2 A."""
+    return nlist_lj(pos, h, mol_id, rlist=MM_RLIST, max_neigh=max_neigh)
+
+
+def mm_forces(pos, h, topo, nlc, nll, charge=None, mode=MM_COMPAT):
+    """compute_forces (MM:351-789) -> (out7 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot},
+    F[Na,3], W[9])."""
+    pos, h = _c(pos).reshape(-1), _c(h)
+    na = pos.size // 3
+    nll = _c(nll, np.int32)
+    q = np.zeros(na) if charge is None else _c(charge)
+    has_q = int(bool(np.any(np.abs(q) > 1e-10)))
+    s, keep = _mm_struct(topo)
+    F, W, out = np.zeros(na * 3), np.zeros(9), np.zeros(7)
+    lib().orc_mm_forces(F, W, pos, h, inv9(h), C.byref(s), _c(nlc, np.int32),
+                        nll.reshape(-1), nll.size // na, q, _c(topo["mol_id"], np.int32),
+                        has_q, na, mode, out)
+    return out, F.reshape(na, 3), W
+
+
+def mm_step_npt(pos, vel, F, W, h, npt, dt, topo, nlc, nll, charge=None, mode=MM_COMPAT):
+    """step_npt (MM:811-890) in place on copies; returns
+    (pos, vel, F, W, h, npt, out8 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot, KE})."""
+    pos, vel, F, W, h = (_c(x).reshape(-1).copy() for x in (pos, vel, F, W, h))
+    na = pos.size // 3
+    hi = inv9(h)
+    nll = _c(nll, np.int32)
+    q = np.zeros(na) if charge is None else _c(charge)
+    has_q = int(bool(np.any(np.abs(q) > 1e-10)))
+    s, keep = _mm_struct(topo)
+    out = np.zeros(8)
+    lib().orc_mm_step_npt(pos, vel, F, W, h, hi, na, dt, C.byref(npt), C.byref(s),
+                          _c(nlc, np.int32), nll.reshape(-1), nll.size // na, q,
+                          _c(topo["mol_id"], np.int32), has_q, mode, out)
+    return pos.reshape(na, 3), vel.reshape(na, 3), F.reshape(na, 3), W, h, npt, out

@@ -10,6 +10,7 @@
  *   C1 = src/fuller_LJ_npt_md_core_serial.cpp
  *   LJ = src/fuller_LJ_npt_md_serial_omp_acc.cpp
  *   AB = src/fuller_airebo_npt_md_serial_omp_acc.cpp
+ *   MM = src/fuller_LJ_npt_mmmd_serial_omp_acc.cpp
  * The arithmetic order follows the reference so that the serial reference and
  * this restatement agree to a few ulps on identical inputs.
  *
@@ -24,6 +25,11 @@
  *   orc_compute_lj       golden vectors; FD gradient.
  *   orc_build_nlist_*    golden lists.
  *   orc_airebo_step_npt  golden multi-step trajectories (AB).
+ *   orc_mm_topology      golden per-molecule terms (MM), bitwise.
+ *   orc_mm_forces        golden vectors incl. Coulomb (compat); physical mode
+ *                        by FD gradient and strain-derivative virial; compat
+ *                        bond/angle forces equal +gradient term by term.
+ *   orc_mm_step_npt      golden multi-step trajectories (MM).
  */
 #include <math.h>
 #include <string.h>
@@ -815,4 +821,447 @@ void orc_airebo_step_npt(double *pos, double *vel, double *F, double *vir9,
   out[1] = fr[1];
   out[2] = fr[2];
   out[3] = KE;
+}
+
+/* --------------------------------------------------- molecular mechanics */
+/* MM = src/fuller_LJ_npt_mmmd_serial_omp_acc.cpp.
+ *
+ * Modes: ORC_MM_COMPAT reproduces the reference forces term by term.  They
+ * are not the gradient of the reference energy: the bond and angle forces
+ * carry the opposite sign (F = +dE/dr, MM:384-392 and MM:455-462), and the
+ * middle-atom dihedral/improper forces use the dot-product coefficients with
+ * the opposite sign (MM:548-549 and MM:663-685; b1 = r2 - r1 there, where the
+ * analytic formula it follows has r1 - r2).  ORC_MM_PHYSICAL uses the exact
+ * gradient F = -dE/dr of the same energy, and the full virial tensor
+ * W = sum over terms of (relative position) (x) force, bonded terms included.
+ *
+ * Pinned: golden vectors (compat), term-by-term finite differences
+ * (physical; compat bond/angle equal +gradient), strain derivative of the
+ * energy (physical virial), Newton's third law per term. */
+
+/* sigma 3.431 A, eps 2.635 meV, rc = 3 sigma, shifted (MM:111-117). */
+void orc_lj_params_mm(orc_lj_params *p) {
+  const double sigma = 3.431, eps = 2.635e-3;
+  const double rc = 3.0 * sigma;
+  const double sr = 1.0 / 3.0, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+  p->sig2 = sigma * sigma;
+  p->eps = eps;
+  p->rc2 = rc * rc;
+  p->vshift = 4.0 * eps * (sr6 * sr6 - sr6);
+}
+
+/* Adjacency of build_flat_topology (MM:256-257): bonds in the given order,
+ * each appending to both ends.  off[natom+1], adj[2 nbond]. */
+static void mm_adjacency(int natom, const int *bonds, int nbond, int *off, int *adj) {
+  int i;
+  for (i = 0; i <= natom; i++) off[i] = 0;
+  for (i = 0; i < nbond; i++) {
+    off[bonds[2 * i] + 1]++;
+    off[bonds[2 * i + 1] + 1]++;
+  }
+  for (i = 0; i < natom; i++) off[i + 1] += off[i];
+  {
+    int fill[4096];
+    for (i = 0; i < natom; i++) fill[i] = off[i];
+    for (i = 0; i < nbond; i++) {
+      int bi = bonds[2 * i], bj = bonds[2 * i + 1];
+      adj[fill[bi]++] = bj;
+      adj[fill[bj]++] = bi;
+    }
+  }
+}
+
+void orc_mm_topology_counts(int natom, const int *bonds, int nbond, int *counts4) {
+  int off[4097], adj[8192];
+  mm_adjacency(natom, bonds, nbond, off, adj);
+  int nang = 0, ndih = 0, nimp = 0;
+  for (int j = 0; j < natom; j++) {
+    int dg = off[j + 1] - off[j];
+    nang += dg * (dg - 1) / 2;
+    if (dg == 3) nimp++;
+  }
+  for (int b = 0; b < nbond; b++) {
+    int bj = bonds[2 * b], bk = bonds[2 * b + 1];
+    for (int x = off[bj]; x < off[bj + 1]; x++) {
+      int i = adj[x];
+      if (i == bk) continue;
+      for (int y = off[bk]; y < off[bk + 1]; y++) {
+        int l = adj[y];
+        if (l == bj || l == i) continue;
+        ndih++;
+      }
+    }
+  }
+  counts4[0] = nbond;
+  counts4[1] = nang;
+  counts4[2] = ndih;
+  counts4[3] = nimp;
+}
+
+/* compute_phi0 MM:240-252 */
+static double mm_phi0(const double *c, int i, int j, int k, int l) {
+  double b1x = c[j * 3] - c[i * 3], b1y = c[j * 3 + 1] - c[i * 3 + 1],
+         b1z = c[j * 3 + 2] - c[i * 3 + 2];
+  double b2x = c[k * 3] - c[j * 3], b2y = c[k * 3 + 1] - c[j * 3 + 1],
+         b2z = c[k * 3 + 2] - c[j * 3 + 2];
+  double b3x = c[l * 3] - c[k * 3], b3y = c[l * 3 + 1] - c[k * 3 + 1],
+         b3z = c[l * 3 + 2] - c[k * 3 + 2];
+  double mx = b1y * b2z - b1z * b2y, my = b1z * b2x - b1x * b2z, mz = b1x * b2y - b1y * b2x;
+  double nx = b2y * b3z - b2z * b3y, ny = b2z * b3x - b2x * b3z, nz = b2x * b3y - b2y * b3x;
+  double mm = mx * mx + my * my + mz * mz, nn = nx * nx + ny * ny + nz * nz;
+  if (mm < 1e-20 || nn < 1e-20) return 0;
+  double b2len = sqrt(b2x * b2x + b2y * b2y + b2z * b2z);
+  double cosphi = (mx * nx + my * ny + mz * nz) / sqrt(mm * nn);
+  if (cosphi > 1.0) cosphi = 1.0;
+  if (cosphi < -1.0) cosphi = -1.0;
+  double sinphi = (mx * b3x + my * b3y + mz * b3z) * b2len / sqrt(mm * nn);
+  return atan2(sinphi, cosphi);
+}
+
+/* build_flat_topology MM:254-334: per-molecule terms (bonds in the given
+ * order with r0 from the geometry; angles i-j-k over neighbour pairs of each
+ * centre j; dihedrals i-bj-bk-l over each bond; impropers centre-first for
+ * three-coordinated atoms, gamma = 2 phi0 + pi, multiplicity 2), replicated
+ * over nmol molecules of natom atoms.  k4 = {kb, kth, v2, kimp}. */
+void orc_mm_topology(const double *coords, int natom, const int *bonds, int nbond, int nmol,
+                     const double *k4, int *b, double *bp, int *a, double *ap, int *d,
+                     double *dp, int *dm, int *im, double *ip, int *mol_id) {
+  int off[4097], adj[8192], cnt[4];
+  mm_adjacency(natom, bonds, nbond, off, adj);
+  orc_mm_topology_counts(natom, bonds, nbond, cnt);
+  const double PI = 3.14159265358979323846;
+  for (int m = 0; m < nmol; m++) {
+    const int o = m * natom;
+    int nb = 0, na_ = 0, nd = 0, ni = 0;
+    for (int t = 0; t < nbond; t++) {
+      int bi = bonds[2 * t], bj = bonds[2 * t + 1];
+      double dx = coords[bi * 3] - coords[bj * 3], dy = coords[bi * 3 + 1] - coords[bj * 3 + 1],
+             dz = coords[bi * 3 + 2] - coords[bj * 3 + 2];
+      int g = m * cnt[0] + nb++;
+      b[2 * g] = bi + o;
+      b[2 * g + 1] = bj + o;
+      bp[2 * g] = k4[0];
+      bp[2 * g + 1] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    for (int j = 0; j < natom; j++) {
+      for (int ii = off[j]; ii < off[j + 1]; ii++)
+        for (int kk = ii + 1; kk < off[j + 1]; kk++) {
+          int i = adj[ii], k = adj[kk];
+          double rji[3], rjk[3];
+          for (int c = 0; c < 3; c++) {
+            rji[c] = coords[i * 3 + c] - coords[j * 3 + c];
+            rjk[c] = coords[k * 3 + c] - coords[j * 3 + c];
+          }
+          double dji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+          double djk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+          double cs = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (dji * djk);
+          cs = clampd(cs, -1.0, 1.0);
+          int g = m * cnt[1] + na_++;
+          a[3 * g] = i + o;
+          a[3 * g + 1] = j + o;
+          a[3 * g + 2] = k + o;
+          ap[2 * g] = k4[1];
+          ap[2 * g + 1] = acos(cs);
+        }
+    }
+    for (int t = 0; t < nbond; t++) {
+      int bj = bonds[2 * t], bk = bonds[2 * t + 1];
+      for (int x = off[bj]; x < off[bj + 1]; x++) {
+        int i = adj[x];
+        if (i == bk) continue;
+        for (int y = off[bk]; y < off[bk + 1]; y++) {
+          int l = adj[y];
+          if (l == bj || l == i) continue;
+          int g = m * cnt[2] + nd++;
+          d[4 * g] = i + o;
+          d[4 * g + 1] = bj + o;
+          d[4 * g + 2] = bk + o;
+          d[4 * g + 3] = l + o;
+          dp[2 * g] = k4[2];
+          dp[2 * g + 1] = 2 * mm_phi0(coords, i, bj, bk, l) + PI;
+          dm[g] = 2;
+        }
+      }
+    }
+    for (int i = 0; i < natom; i++) {
+      if (off[i + 1] - off[i] != 3) continue;
+      int j = adj[off[i]], k = adj[off[i] + 1], l = adj[off[i] + 2];
+      int g = m * cnt[3] + ni++;
+      im[4 * g] = i + o;
+      im[4 * g + 1] = j + o;
+      im[4 * g + 2] = k + o;
+      im[4 * g + 3] = l + o;
+      ip[2 * g] = k4[3];
+      ip[2 * g + 1] = 2 * mm_phi0(coords, j, i, k, l) + PI;
+    }
+    for (int x = 0; x < natom; x++) mol_id[o + x] = m;
+  }
+}
+
+/* One torsion (dihedral MM:508-549 / improper MM:627-685) over atoms
+ * p0-p1-p2-p3 with E = Vn/2 (1 + cos(mult phi - gamma)).  Adds the four
+ * forces into F and (physical) the virial; returns the energy.  Skipped
+ * (0) when either cross product is degenerate. */
+static double mm_torsion(double *F, double *W, const double *pos, const double *h,
+                         const double *hi, int p0, int p1, int p2, int p3, double Vn,
+                         int mult, double gamma, int mode) {
+  double b1[3], b2[3], b3[3];
+  for (int c = 0; c < 3; c++) {
+    b1[c] = pos[p1 * 3 + c] - pos[p0 * 3 + c];
+    b2[c] = pos[p2 * 3 + c] - pos[p1 * 3 + c];
+    b3[c] = pos[p3 * 3 + c] - pos[p2 * 3 + c];
+  }
+  orc_mimg(b1, hi, h);
+  orc_mimg(b2, hi, h);
+  orc_mimg(b3, hi, h);
+  double mx = b1[1] * b2[2] - b1[2] * b2[1], my = b1[2] * b2[0] - b1[0] * b2[2],
+         mz = b1[0] * b2[1] - b1[1] * b2[0];
+  double nx = b2[1] * b3[2] - b2[2] * b3[1], ny = b2[2] * b3[0] - b2[0] * b3[2],
+         nz = b2[0] * b3[1] - b2[1] * b3[0];
+  double mm = mx * mx + my * my + mz * mz, nn = nx * nx + ny * ny + nz * nz;
+  if (mm < 1e-20 || nn < 1e-20) return 0;
+  double imm = 1.0 / mm, inn = 1.0 / nn;
+  double b2len = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+  double cosphi = (mx * nx + my * ny + mz * nz) / sqrt(mm * nn);
+  if (cosphi > 1.0) cosphi = 1.0;
+  if (cosphi < -1.0) cosphi = -1.0;
+  double b1b2 = b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2];
+  double b2b3 = b2[0] * b3[0] + b2[1] * b3[1] + b2[2] * b3[2];
+  double sinphi = (mx * b3[0] + my * b3[1] + mz * b3[2]) * b2len / sqrt(mm * nn);
+  double phi = atan2(sinphi, cosphi);
+  double E = 0.5 * Vn * (1.0 + cos(mult * phi - gamma));
+  double dphi = -0.5 * Vn * mult * sin(mult * phi - gamma);
+  double f1[3] = {dphi * b2len * imm * mx, dphi * b2len * imm * my, dphi * b2len * imm * mz};
+  double f4[3] = {-dphi * b2len * inn * nx, -dphi * b2len * inn * ny,
+                  -dphi * b2len * inn * nz};
+  double ci = b1b2 / (b2len * b2len), ck = b2b3 / (b2len * b2len);
+  double f2[3], f3[3];
+  for (int c = 0; c < 3; c++) {
+    if (mode == ORC_MM_PHYSICAL) {
+      f2[c] = -f1[c] - ci * f1[c] + ck * f4[c];
+      f3[c] = -f4[c] - ck * f4[c] + ci * f1[c];
+    } else {
+      f2[c] = -f1[c] + ci * f1[c] - ck * f4[c];
+      f3[c] = -f4[c] - ci * f1[c] + ck * f4[c];
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    F[p0 * 3 + c] += f1[c];
+    F[p1 * 3 + c] += f2[c];
+    F[p2 * 3 + c] += f3[c];
+    F[p3 * 3 + c] += f4[c];
+  }
+  if (mode == ORC_MM_PHYSICAL) {
+    /* positions relative to p0: b1, b1 + b2, b1 + b2 + b3 */
+    for (int x = 0; x < 3; x++)
+      for (int y = 0; y < 3; y++)
+        W[3 * x + y] += b1[x] * f2[y] + (b1[x] + b2[x]) * f3[y] +
+                        (b1[x] + b2[x] + b3[x]) * f4[y];
+  }
+  return E;
+}
+
+/* compute_forces MM:351-789.  out7 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot}.
+ * The LJ half list has stride max_l (400 in the reference).  Compat: the
+ * virial holds only the diagonal bond and LJ parts, as the reference's. */
+void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
+                   const double *hi, const orc_mm_topo *t, const int *nlc, const int *nll,
+                   int max_l, const double *charge, const int *mol_id, int has_charge, int na,
+                   int mode, double *out7) {
+  const int phys = mode == ORC_MM_PHYSICAL;
+  orc_lj_params lj;
+  orc_lj_params_mm(&lj);
+  for (int i = 0; i < na * 3; i++) F[i] = 0;
+  for (int i = 0; i < 9; i++) vir9[i] = 0;
+  double Eb = 0, Ea = 0, Ed = 0, Ei = 0, Elj = 0, Ec = 0;
+  /* bonds MM:383-441 */
+  for (int q = 0; q < t->Nb; q++) {
+    int i = t->b[2 * q], j = t->b[2 * q + 1];
+    double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                   pos[j * 3 + 2] - pos[i * 3 + 2]};
+    orc_mimg(d, hi, h);
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (r < 1e-10) continue;
+    double kb = t->bp[2 * q], dr = r - t->bp[2 * q + 1];
+    Eb += 0.5 * kb * dr * dr;
+    double fm = phys ? kb * dr / r : -kb * dr / r; /* force on i along d */
+    double f[3] = {fm * d[0], fm * d[1], fm * d[2]};
+    for (int c = 0; c < 3; c++) {
+      F[i * 3 + c] += f[c];
+      F[j * 3 + c] -= f[c];
+    }
+    if (phys) {
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) vir9[3 * x + y] -= d[x] * f[y];
+    } else {
+      vir9[0] += d[0] * f[0];
+      vir9[4] += d[1] * f[1];
+      vir9[8] += d[2] * f[2];
+    }
+  }
+  /* angles MM:443-477 */
+  for (int q = 0; q < t->Nang; q++) {
+    int i = t->a[3 * q], j = t->a[3 * q + 1], k = t->a[3 * q + 2];
+    double rji[3], rjk[3];
+    for (int c = 0; c < 3; c++) {
+      rji[c] = pos[i * 3 + c] - pos[j * 3 + c];
+      rjk[c] = pos[k * 3 + c] - pos[j * 3 + c];
+    }
+    orc_mimg(rji, hi, h);
+    orc_mimg(rjk, hi, h);
+    double dji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    double djk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    if (dji < 1e-10 || djk < 1e-10) continue;
+    double cs = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (dji * djk);
+    if (cs > 0.999999) cs = 0.999999;
+    if (cs < -0.999999) cs = -0.999999;
+    double kth = t->ap[2 * q], dth = acos(cs) - t->ap[2 * q + 1];
+    Ea += 0.5 * kth * dth * dth;
+    double sn = sqrt(1.0 - cs * cs + 1e-30);
+    double dV = phys ? kth * dth / sn : -kth * dth / sn;
+    double fi[3], fk[3];
+    for (int c = 0; c < 3; c++) {
+      fi[c] = dV * (rjk[c] / (dji * djk) - cs * rji[c] / (dji * dji));
+      fk[c] = dV * (rji[c] / (dji * djk) - cs * rjk[c] / (djk * djk));
+      F[i * 3 + c] += fi[c];
+      F[k * 3 + c] += fk[c];
+      F[j * 3 + c] -= fi[c] + fk[c];
+    }
+    if (phys)
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) vir9[3 * x + y] += rji[x] * fi[y] + rjk[x] * fk[y];
+  }
+  /* dihedrals MM:484-598 */
+  double Wt[9] = {0};
+  for (int q = 0; q < t->Ndih; q++)
+    Ed += mm_torsion(F, Wt, pos, h, hi, t->d[4 * q], t->d[4 * q + 1], t->d[4 * q + 2],
+                     t->d[4 * q + 3], t->dp[2 * q], t->dm[q], t->dp[2 * q + 1], mode);
+  /* impropers MM:600-686: atom order c1-c0-c2-c3, multiplicity 2 */
+  for (int q = 0; q < t->Nimp; q++)
+    Ei += mm_torsion(F, Wt, pos, h, hi, t->im[4 * q + 1], t->im[4 * q], t->im[4 * q + 2],
+                     t->im[4 * q + 3], t->ip[2 * q], 2, t->ip[2 * q + 1], mode);
+  if (phys)
+    for (int x = 0; x < 9; x++) vir9[x] += Wt[x];
+  /* LJ MM:688-743 */
+  for (int i = 0; i < na; i++) {
+    for (int jn = 0; jn < nlc[i]; jn++) {
+      int j = nll[i * max_l + jn];
+      double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                     pos[j * 3 + 2] - pos[i * 3 + 2]};
+      orc_mimg(d, hi, h);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 > lj.rc2) continue;
+      if (r2 < 0.25) r2 = 0.25;
+      double ri2 = 1.0 / r2, sr2 = lj.sig2 * ri2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+      double fm = 24 * lj.eps * (2 * sr12 - sr6) * ri2;
+      Elj += 4 * lj.eps * (sr12 - sr6) - lj.vshift;
+      for (int c = 0; c < 3; c++) {
+        F[i * 3 + c] -= fm * d[c];
+        F[j * 3 + c] += fm * d[c];
+      }
+      if (phys) {
+        for (int x = 0; x < 3; x++)
+          for (int y = 0; y < 3; y++) vir9[3 * x + y] += d[x] * fm * d[y];
+      } else {
+        vir9[0] += d[0] * fm * d[0];
+        vir9[4] += d[1] * fm * d[1];
+        vir9[8] += d[2] * fm * d[2];
+      }
+    }
+  }
+  /* Coulomb MM:745-767 (all pairs of different molecules within 3 sigma) */
+  if (has_charge) {
+    const double KC = 14.3996, rc = 3.0 * 3.431;
+    for (int i = 0; i < na - 1; i++)
+      for (int j = i + 1; j < na; j++) {
+        if (mol_id[i] == mol_id[j]) continue;
+        double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
+                       pos[j * 3 + 2] - pos[i * 3 + 2]};
+        orc_mimg(d, hi, h);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 > lj.rc2) continue;
+        double r = sqrt(r2), qq = KC * charge[i] * charge[j];
+        Ec += qq / r - qq / rc;
+        double fm = qq / (r2 * r);
+        for (int c = 0; c < 3; c++) {
+          F[i * 3 + c] -= fm * d[c];
+          F[j * 3 + c] += fm * d[c];
+        }
+        if (phys)
+          for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++) vir9[3 * x + y] += d[x] * fm * d[y];
+      }
+  }
+  out7[0] = Eb;
+  out7[1] = Ea;
+  out7[2] = Ed;
+  out7[3] = Ei;
+  out7[4] = Elj;
+  out7[5] = Ec;
+  out7[6] = Eb + Ea + Ed + Ei + Elj + Ec;
+}
+
+/* step_npt MM:811-890 (carbon mass for every atom).  out8 = out7 + {KE}. */
+void orc_mm_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
+                     double *hi, int na, double dt, orc_npt *npt, const orc_mm_topo *t,
+                     const int *nlc, const int *nll, int max_l, const double *charge,
+                     const int *mol_id, int has_charge, int mode, double *out8) {
+  const double mC = 12.011, mi = CONV / mC;
+  double hdt = 0.5 * dt, V = fabs(orc_mat_det9(h));
+  double s = 0;
+  for (int i = 0; i < na; i++)
+    s += mC * (vel[i * 3] * vel[i * 3] + vel[i * 3 + 1] * vel[i * 3 + 1] +
+               vel[i * 3 + 2] * vel[i * 3 + 2]);
+  double KE = 0.5 * s / CONV;
+  npt->xi += hdt * (2 * KE - npt->Nf * KB * npt->Tt) / npt->Q;
+  npt->xi = clampd(npt->xi, -0.05, 0.05);
+  double dP = orc_inst_P(vir9, KE, V) - npt->Pe;
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] += hdt * V * dP / (npt->W * EV2GPA);
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] = clampd(npt->Vg[a * 4], -0.005, 0.005);
+  double eps = npt->Vg[0] * hi[0] + npt->Vg[4] * hi[4] + npt->Vg[8] * hi[8];
+  double sc_nh = exp(-hdt * npt->xi), sc_pr = exp(-hdt * eps / 3.0), sc_v = sc_nh * sc_pr;
+  for (int i = 0; i < na * 3; i++) vel[i] = vel[i] * sc_v + hdt * F[i] * mi;
+  for (int i = 0; i < na; i++) {
+    double px = pos[i * 3], py = pos[i * 3 + 1], pz = pos[i * 3 + 2];
+    double vx = vel[i * 3], vy = vel[i * 3 + 1], vz = vel[i * 3 + 2];
+    double sx = hi[0] * px + hi[1] * py + hi[2] * pz, sy = hi[3] * px + hi[4] * py + hi[5] * pz,
+           sz = hi[6] * px + hi[7] * py + hi[8] * pz;
+    double vsx = hi[0] * vx + hi[1] * vy + hi[2] * vz, vsy = hi[3] * vx + hi[4] * vy + hi[5] * vz,
+           vsz = hi[6] * vx + hi[7] * vy + hi[8] * vz;
+    sx += dt * vsx;
+    sy += dt * vsy;
+    sz += dt * vsz;
+    sx -= floor(sx);
+    sy -= floor(sy);
+    sz -= floor(sz);
+    pos[i * 3] = sx;
+    pos[i * 3 + 1] = sy;
+    pos[i * 3 + 2] = sz;
+  }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) h[a * 3 + b] += dt * npt->Vg[a * 3 + b];
+  orc_mat_inv9(h, hi);
+  for (int i = 0; i < na; i++) {
+    double sx = pos[i * 3], sy = pos[i * 3 + 1], sz = pos[i * 3 + 2];
+    pos[i * 3] = h[0] * sx + h[1] * sy + h[2] * sz;
+    pos[i * 3 + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
+    pos[i * 3 + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
+  }
+  orc_mm_forces(F, vir9, pos, h, hi, t, nlc, nll, max_l, charge, mol_id, has_charge, na, mode,
+                out8);
+  double eps2 = npt->Vg[0] * hi[0] + npt->Vg[4] * hi[4] + npt->Vg[8] * hi[8];
+  double sc_v2 = sc_nh * exp(-hdt * eps2 / 3.0);
+  for (int i = 0; i < na * 3; i++) vel[i] = (vel[i] + hdt * F[i] * mi) * sc_v2;
+  s = 0;
+  for (int i = 0; i < na; i++)
+    s += mC * (vel[i * 3] * vel[i * 3] + vel[i * 3 + 1] * vel[i * 3 + 1] +
+               vel[i * 3 + 2] * vel[i * 3 + 2]);
+  KE = 0.5 * s / CONV;
+  npt->xi += hdt * (2 * KE - npt->Nf * KB * npt->Tt) / npt->Q;
+  npt->xi = clampd(npt->xi, -0.05, 0.05);
+  double V2 = fabs(orc_mat_det9(h));
+  dP = orc_inst_P(vir9, KE, V2) - npt->Pe;
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] += hdt * V2 * dP / (npt->W * EV2GPA);
+  for (int a = 0; a < 3; a++) npt->Vg[a * 4] = clampd(npt->Vg[a * 4], -0.005, 0.005);
+  out8[7] = KE;
 }

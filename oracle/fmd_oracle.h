@@ -82,6 +82,37 @@ void orc_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doub
                          const int *nlc_l, const int *nll_l, int max_l, int rebo_mode,
                          const orc_lj_params *p, double *out);
 
+/* Molecular mechanics (MM).  Flat topology arrays of build_flat_topology
+ * (MM:228-239) gathered in one struct. */
+typedef struct {
+  int Nb, Nang, Ndih, Nimp;
+  const int *b;     /* Nb x 2 */
+  const double *bp; /* Nb x {kb, r0} */
+  const int *a;     /* Nang x 3 (i, centre j, k) */
+  const double *ap; /* Nang x {kth, th0} */
+  const int *d;     /* Ndih x 4 */
+  const double *dp; /* Ndih x {Vn, gamma} */
+  const int *dm;    /* Ndih multiplicities */
+  const int *im;    /* Nimp x 4 (centre first) */
+  const double *ip; /* Nimp x {ki, gamma} */
+} orc_mm_topo;
+
+enum { ORC_MM_COMPAT = 0, ORC_MM_PHYSICAL = 1 };
+
+void orc_lj_params_mm(orc_lj_params *p);
+void orc_mm_topology_counts(int natom, const int *bonds, int nbond, int *counts4);
+void orc_mm_topology(const double *coords, int natom, const int *bonds, int nbond, int nmol,
+                     const double *k4, int *b, double *bp, int *a, double *ap, int *d,
+                     double *dp, int *dm, int *im, double *ip, int *mol_id);
+void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
+                   const double *hi, const orc_mm_topo *t, const int *nlc, const int *nll,
+                   int max_l, const double *charge, const int *mol_id, int has_charge, int na,
+                   int mode, double *out7);
+void orc_mm_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
+                     double *hi, int na, double dt, orc_npt *npt, const orc_mm_topo *t,
+                     const int *nlc, const int *nll, int max_l, const double *charge,
+                     const int *mol_id, int has_charge, int mode, double *out8);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,7 @@ def path(name: str) -> str:
 
 def available(which: str = "all") -> bool:
     names = {"core": ["libref_core.so"], "lj": ["libref_lj.so"],
-             "airebo": ["libref_airebo.so"]}
+             "airebo": ["libref_airebo.so"], "mm": ["libref_mm.so"]}
     want = sum(names.values(), []) if which == "all" else names[which]
     return all(os.path.exists(path(n)) for n in want)
 
@@ -96,6 +96,24 @@ def airebo():
     return _libs["airebo"]
 
 
+def mm():
+    if "mm" not in _libs:
+        L = C.CDLL(path("libref_mm.so"))
+        L.refmm_load_cc1.argtypes = [C.c_char_p, _dp, _ip, _i, _dp]
+        L.refmm_build.argtypes = [C.c_char_p, _i, _dp, _ip]
+        L.refmm_get.argtypes = [_ip, _dp, _ip, _dp, _ip, _dp, _ip, _ip, _dp, _ip, _i]
+        L.refmm_nlist_lj.argtypes = [_dp, _dp, _dp, _i, _ip, _ip]
+        L.refmm_compute_forces.argtypes = [_dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _i, _i,
+                                           _dp]
+        L.refmm_make_npt.argtypes = [_d, _d, _i, _dp]
+        L.refmm_step_npt.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _i, _d, _dp, _ip, _ip,
+                                     _dp, _i, _dp]
+        L.refmm_apply_pbc.argtypes = [_dp, _dp, _dp, _i]
+        L.refmm_mat_inv9.argtypes = [_dp, _dp]
+        _libs["mm"] = L
+    return _libs["mm"]
+
+
 def _c(a, dt=np.float64):
     return np.ascontiguousarray(a, dtype=dt)
 
@@ -161,3 +179,53 @@ def ab_compute_forces(pos, h, lists):
     F, W, out = np.zeros(na * 3), np.zeros(9), np.zeros(3)
     airebo().refab_compute_forces(F, W, pos, h, inv9(h), nlc_r, nll_r, nlc_l, nll_l, na, out)
     return tuple(out), F.reshape(na, 3), W
+
+
+# ------------------------------------------------------------------ MM
+def mm_load_cc1(path_cc1):
+    coords = np.zeros(3 * 256)
+    bonds = np.zeros(2 * 1024, np.int32)
+    props = np.zeros(4)
+    n = mm().refmm_load_cc1(path_cc1.encode(), coords, bonds, 1024, props)
+    nb = int(props[1])
+    return coords[:3 * n].reshape(n, 3), bonds[:2 * nb].reshape(nb, 2), props
+
+
+def mm_build(path_cc1, nmol, k4):
+    """The reference flat topology of nmol copies (kept inside the library for
+    mm_compute_forces / mm_step_npt) as a dict of arrays."""
+    counts = np.zeros(4, np.int32)
+    mm().refmm_build(path_cc1.encode(), nmol, _c(k4), counts)
+    nb, na_, nd, ni = (int(x) for x in counts)
+    natom = mm_load_cc1(path_cc1)[0].shape[0]
+    t = dict(b=np.zeros((nb, 2), np.int32), bp=np.zeros((nb, 2)),
+             a=np.zeros((na_, 3), np.int32), ap=np.zeros((na_, 2)),
+             d=np.zeros((nd, 4), np.int32), dp=np.zeros((nd, 2)),
+             dm=np.zeros(nd, np.int32), i=np.zeros((ni, 4), np.int32),
+             ip=np.zeros((ni, 2)), mol_id=np.zeros(nmol * natom, np.int32))
+    mm().refmm_get(t["b"].reshape(-1), t["bp"].reshape(-1), t["a"].reshape(-1),
+                   t["ap"].reshape(-1), t["d"].reshape(-1), t["dp"].reshape(-1), t["dm"],
+                   t["i"].reshape(-1), t["ip"].reshape(-1), t["mol_id"], nmol * natom)
+    return t
+
+
+def mm_nlist(pos, h):
+    pos, h = _c(pos).reshape(-1), _c(h)
+    na = pos.size // 3
+    nlc = np.zeros(na, np.int32)
+    nll = np.zeros(na * 400, np.int32)
+    mm().refmm_nlist_lj(pos, h, inv9(h), na, nlc, nll)
+    return nlc, nll.reshape(na, 400)
+
+
+def mm_compute_forces(pos, h, nlc, nll, charge=None):
+    """Over the topology of the last mm_build: (out7, F, vir9)."""
+    pos, h = _c(pos).reshape(-1), _c(h)
+    na = pos.size // 3
+    q = np.zeros(na) if charge is None else _c(charge)
+    F, W, out = np.zeros(na * 3), np.zeros(9), np.zeros(7)
+    mm().refmm_compute_forces(F, W, pos, h, inv9(h), _c(nlc, np.int32),
+                              _c(nll, np.int32).reshape(-1), q,
+                              int(charge is not None and bool(np.any(np.abs(q) > 1e-10))),
+                              na, out)
+    return out, F.reshape(na, 3), W

@@ -28,6 +28,14 @@ Fixtures
                           compute_forces and 5 NPT steps.
   airebo_nc2.npz          AB forces at 2x2x2 (1920 atoms), where the 400-entry
                           LJ half list truncates (inputs and outputs only).
+  mm_c60_nc1.npz          MM on a jittered 1x1x1 C60 crystal (240 atoms): the
+                          one-molecule topology of build_flat_topology, LJ
+                          list, compute_forces without and with (random)
+                          charges, and 5 NPT steps at dt = 0.1 fs.
+  mm_c60_nc2.npz          MM forces at 2x2x2 (1920 atoms; the 400-entry half
+                          list truncates): inputs and outputs only.
+  mm_c70_hcp.npz          MM on C70 in a 2x2x2 HCP cell (1120 atoms): topology
+                          and forces.
 """
 from __future__ import annotations
 
@@ -244,13 +252,90 @@ def gen_airebo():
          E=np.array(E), F=F, W=W)
 
 
+MM_K4 = np.array([469.0, 63.0, 14.5, 15.0]) / 23.06054783  # MM:1044-1047
+
+
+def mm_system(spec, crystal, nc, seed, jitter=0.02):
+    path = os.path.join(LIBDIR, spec)
+    coords, bonds, props = ref.mm_load_cc1(path)
+    L = ref.lj()
+    a0 = L.reflj_default_a0(props[3], crystal.encode(), 1.0)
+    nmax = 4 * nc ** 3
+    cen, h = np.zeros(3 * nmax), np.zeros(9)
+    L.reflj_make_crystal(crystal.encode(), a0, nc, cen, h)
+    nm = (4 if crystal == "FCC" else 2) * nc ** 3
+    cen = cen[: 3 * nm].reshape(nm, 3)
+    pos = (cen[:, None, :] + coords[None]).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    pos = pos + rng.uniform(-jitter, jitter, pos.shape)
+    p = pos.reshape(-1).copy()
+    ref.mm().refmm_apply_pbc(p, h, oracle.inv9(h), len(pos))
+    topo = ref.mm_build(path, nm, MM_K4)
+    return path, coords, bonds, p.reshape(-1, 3), h, nm, topo, rng
+
+
+def mm_template(topo, nm):
+    """Terms of molecule 0 (build_flat_topology replicates them, MM:313-332)."""
+    out = {}
+    for k in ("b", "bp", "a", "ap", "d", "dp", "dm", "i", "ip"):
+        out["t_" + k] = topo[k][: len(topo[k]) // nm]
+    return out
+
+
+def gen_mm():
+    L = ref.mm()
+    # C60 FCC 1x1x1: everything
+    path, coords, bonds, pos, h, nm, topo, rng = mm_system("C60-76/C60-Ih.cc1", "FCC", 1, 2024)
+    na = len(pos)
+    nlc, nll = ref.mm_nlist(pos, h)
+    out, F, W = ref.mm_compute_forces(pos, h, nlc, nll)
+    q = rng.uniform(-0.1, 0.1, na)
+    out_q, F_q, W_q = ref.mm_compute_forces(pos, h, nlc, nll, charge=q)
+    vel = np.sqrt(oracle.KB * 300.0 * oracle.CONV / 12.011) * rng.normal(size=(na, 3))
+    vel -= vel.mean(axis=0)
+    state = np.zeros(15)
+    L.refmm_make_npt(300.0, 0.0, na, state)
+    npt0 = state.copy()
+    hi = oracle.inv9(h)
+    P, V, FF, WW, hh, hi2 = (pos.reshape(-1).copy(), vel.reshape(-1).copy(),
+                             F.reshape(-1).copy(), W.copy(), h.copy(), hi.copy())
+    traj = []
+    o8 = np.zeros(8)
+    for _ in range(5):
+        L.refmm_step_npt(P, V, FF, WW, hh, hi2, na, 0.1, state, nlc, nll.reshape(-1),
+                         np.zeros(na), 0, o8)
+        traj.append(o8.copy())
+    save("mm_c60_nc1.npz", coords=coords, bonds=bonds, k4=MM_K4, pos=pos, h=h,
+         mol_id=topo["mol_id"], nlc=nlc, nll=pack_list(nlc, nll), E=out, F=F, W=W, charge=q,
+         E_q=out_q, F_q=F_q, W_q=W_q, vel=vel, npt0=npt0, dt=0.1, traj=np.array(traj),
+         pos_end=P.reshape(na, 3), vel_end=V.reshape(na, 3), F_end=FF.reshape(na, 3),
+         W_end=WW, h_end=hh, npt_end=state, **mm_template(topo, nm))
+    # C60 FCC 2x2x2: the 400-entry truncation is active
+    path, coords, bonds, pos, h, nm, topo, rng = mm_system("C60-76/C60-Ih.cc1", "FCC", 2, 99)
+    nlc, nll = ref.mm_nlist(pos, h)
+    out, F, W = ref.mm_compute_forces(pos, h, nlc, nll)
+    save("mm_c60_nc2.npz", pos=pos, h=h, mol_id=topo["mol_id"], nlc=nlc,
+         lj_last=np.array([nll[i, nlc[i] - 1] if nlc[i] else -1 for i in range(len(pos))],
+                          np.int32), E=out, F=F, W=W)
+    # C70 HCP 2x2x2: another species, non-orthogonal cell
+    path, coords, bonds, pos, h, nm, topo, rng = mm_system("C60-76/C70-D5h.cc1", "HCP", 2, 7)
+    nlc, nll = ref.mm_nlist(pos, h)
+    out, F, W = ref.mm_compute_forces(pos, h, nlc, nll)
+    save("mm_c70_hcp.npz", coords=coords, bonds=bonds, k4=MM_K4, pos=pos, h=h,
+         mol_id=topo["mol_id"], nlc=nlc, E=out, F=F, W=W, **mm_template(topo, nm))
+
+
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference exists")
     if sys.argv[1:] == ["lattices"]:
         gen_cc1(LATTICE_CASES)
         sys.exit(0)
+    if sys.argv[1:] == ["mm"]:
+        gen_mm()
+        sys.exit(0)
     gen_rng()
     gen_core()
     gen_cc1()
     gen_airebo()
+    gen_mm()

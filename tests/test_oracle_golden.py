@@ -163,3 +163,74 @@ def test_host_lattices_equal_reference(tag):
     assert pos.shape == g["pos"].shape
     close(pos, g["pos"], 1e-14)
     close(h, g["h"], 1e-14)
+
+
+# ------------------------------------------------------ molecular mechanics
+def _mm_topo(g, nmol):
+    t = oracle.mm_topology(g["coords"], g["bonds"], nmol, g["k4"])
+    for k in ("b", "bp", "a", "ap", "d", "dp", "dm", "i", "ip"):
+        n = len(t[k]) // nmol
+        # molecule 0 bitwise as the reference builds it (MM:254-334) ...
+        assert np.array_equal(t[k][:n], g["t_" + k]), k
+        # ... and the replicas shifted by natom per molecule (MM:313-332)
+        if k in ("b", "a", "d", "i"):
+            natom = len(g["coords"])
+            shift = (np.arange(nmol) * natom).repeat(n)[:, None]
+            assert np.array_equal(t[k], np.tile(g["t_" + k], (nmol, 1)) + shift), k
+    return t
+
+
+@pytest.mark.parametrize("name", ["mm_c60_nc1.npz", "mm_c70_hcp.npz"])
+def test_mm_topology_list_and_forces(name):
+    g = golden(name)
+    nmol = int(g["mol_id"].max()) + 1
+    t = _mm_topo(g, nmol)
+    assert np.array_equal(t["mol_id"], g["mol_id"])
+    cnt, lst = oracle.mm_nlist(g["pos"], g["h"], g["mol_id"])
+    assert (cnt == g["nlc"]).all()
+    if "nll" in g:
+        assert (lst == unpack_list(g["nlc"], g["nll"], 400)).all()
+    E, F, W = oracle.mm_forces(g["pos"], g["h"], t, cnt, lst)
+    close(E, g["E"])
+    close(F, g["F"])
+    close(W, g["W"])
+    assert np.count_nonzero(W - np.diag(np.diag(W.reshape(3, 3))).reshape(-1)) == 0
+
+
+def test_mm_coulomb_and_steps():
+    g = golden("mm_c60_nc1.npz")
+    t = _mm_topo(g, 4)
+    cnt, lst = oracle.mm_nlist(g["pos"], g["h"], g["mol_id"])
+    E, F, W = oracle.mm_forces(g["pos"], g["h"], t, cnt, lst, charge=g["charge"])
+    assert abs(E[5]) > 1e-3  # the Coulomb term is exercised
+    close(E, g["E_q"])
+    close(F, g["F_q"])
+    close(W, g["W_q"])
+    P, V, F, W, h = g["pos"], g["vel"], g["F"], g["W"], g["h"]
+    npt = oracle.NPT.from_array(g["npt0"])
+    traj = []
+    for _ in range(5):
+        P, V, F, W, h, npt, out = oracle.mm_step_npt(P, V, F, W, h, npt, float(g["dt"]), t,
+                                                     cnt, lst)
+        traj.append(out)
+    close(np.array(traj), g["traj"], 1e-11)
+    close(P, g["pos_end"], 1e-11)
+    close(V, g["vel_end"], 1e-9)
+    close(h, g["h_end"], 1e-12)
+    close(npt.to_array(), g["npt_end"], 1e-9)
+
+
+def test_mm_nc2_truncated_lj():
+    g = golden("mm_c60_nc2.npz")
+    nmol = int(g["mol_id"].max()) + 1
+    coords = golden("mm_c60_nc1.npz")
+    t = oracle.mm_topology(coords["coords"], coords["bonds"], nmol, coords["k4"])
+    cl, ll = oracle.mm_nlist(g["pos"], g["h"], g["mol_id"])
+    assert (cl == g["nlc"]).all()
+    last = np.array([ll[i, cl[i] - 1] if cl[i] else -1 for i in range(len(cl))])
+    assert (last == g["lj_last"]).all()
+    assert (cl == 400).sum() > 0  # the truncation is exercised
+    E, F, W = oracle.mm_forces(g["pos"], g["h"], t, cl, ll)
+    close(E, g["E"])
+    close(F, g["F"])
+    close(W, g["W"])

@@ -155,3 +155,105 @@ def test_rigid_forces_are_the_gradient():
             q2[i] = rot(q2[i], -th)
             num = -(E(g["pos"], q1) - E(g["pos"], q2)) / (2 * th)
             assert abs(num - T[i, a]) < 1e-5 * max(1.0, np.abs(T).max())
+
+
+# ------------------------------------------------------ molecular mechanics
+# The reference's MM forces (compat) are not the gradient of its energy: bond
+# and angle forces are +dE/dr (MM:388-392, 455-457) and the middle-atom
+# torsion forces take the dot-product coefficients with the opposite sign
+# (MM:548-549, 663-685).  The physical mode is pinned here by the mathematics.
+def _mm_sys(k4=None, jit=0.05, seed=11, shear=True):
+    g = golden("mm_c60_nc1.npz")
+    rng = np.random.default_rng(seed)
+    pos = g["pos"] + rng.uniform(-jit, jit, g["pos"].shape)
+    h = g["h"].copy()
+    if shear:
+        h[1] = 0.4  # a non-orthogonal cell
+    t = oracle.mm_topology(g["coords"], g["bonds"], 4, g["k4"] if k4 is None else k4)
+    cnt, lst = oracle.mm_nlist(pos, h, t["mol_id"])
+    return pos, h, t, cnt, lst, rng
+
+
+def _mm_fd(pos, h, t, cnt, lst, mode, charge=None, atoms=range(0, 240, 13), d=1e-5):
+    G = np.zeros((len(atoms), 3))
+    for r, a in enumerate(atoms):
+        for x in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[a, x] += d
+            pm[a, x] -= d
+            G[r, x] = (oracle.mm_forces(pp, h, t, cnt, lst, charge, mode)[0][6] -
+                       oracle.mm_forces(pm, h, t, cnt, lst, charge, mode)[0][6]) / (2 * d)
+    return G
+
+
+def test_mm_physical_forces_are_the_gradient():
+    pos, h, t, cnt, lst, rng = _mm_sys()
+    q = rng.uniform(-0.05, 0.05, len(pos))
+    atoms = list(range(0, 240, 13))
+    for charge in (None, q):
+        E, F, W = oracle.mm_forces(pos, h, t, cnt, lst, charge, oracle.MM_PHYSICAL)
+        G = _mm_fd(pos, h, t, cnt, lst, oracle.MM_PHYSICAL, charge, atoms)
+        assert np.abs(F[atoms] + G).max() < 1e-7 * np.abs(F).max()
+
+
+@pytest.mark.parametrize("ab", [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)])
+def test_mm_physical_virial_is_strain_derivative(ab):
+    """W_ab = -dE/de_ab under r -> (1 + e) r, h -> (1 + e) h (lists fixed)."""
+    pos, h, t, cnt, lst, rng = _mm_sys()
+    q = rng.uniform(-0.05, 0.05, len(pos))
+    E, F, W = oracle.mm_forces(pos, h, t, cnt, lst, q, oracle.MM_PHYSICAL)
+    a, b = ab
+    d = 1e-5
+
+    def energy(e):
+        M = np.eye(3)
+        M[a, b] += e
+        return oracle.mm_forces(pos @ M.T, (M @ h.reshape(3, 3)).reshape(-1), t, cnt, lst, q,
+                                oracle.MM_PHYSICAL)[0][6]
+
+    wn = -(energy(d) - energy(-d)) / (2 * d)
+    assert abs(W[3 * a + b] - wn) < 1e-7 * np.abs(W).max()
+
+
+def test_mm_compat_forces_against_the_gradient():
+    """Term by term (one force constant at a time): bond and angle forces are
+    exactly +gradient; torsions are neither -gradient nor +gradient, while
+    the physical mode is -gradient for every term."""
+    k = np.array(oracle.MM_K4_DEFAULT)
+    for term in range(4):
+        k4 = np.zeros(4)
+        k4[term] = k[term]
+        pos, h, t, cnt, lst, _ = _mm_sys(k4=k4, shear=False)
+        # one molecule in a large cell: no LJ pairs
+        sel = t["mol_id"] == 0
+        p1 = pos[sel] - pos[sel].mean(axis=0) + 30.0
+        h1 = np.diag([60.0] * 3).reshape(-1)
+        t1 = oracle.mm_topology(golden("mm_c60_nc1.npz")["coords"],
+                                golden("mm_c60_nc1.npz")["bonds"], 1, k4)
+        c1, l1 = oracle.mm_nlist(p1, h1, t1["mol_id"])
+        assert c1.sum() == 0
+        atoms = list(range(0, 60, 7))
+        G = _mm_fd(p1, h1, t1, c1, l1, oracle.MM_COMPAT, None, atoms)
+        Fc = oracle.mm_forces(p1, h1, t1, c1, l1, None, oracle.MM_COMPAT)[1][atoms]
+        Fp = oracle.mm_forces(p1, h1, t1, c1, l1, None, oracle.MM_PHYSICAL)[1][atoms]
+        s = np.abs(G).max()
+        assert np.abs(Fp + G).max() < 1e-6 * s
+        if term < 2:
+            assert np.abs(Fc - G).max() < 1e-6 * s
+        else:
+            assert np.abs(Fc - G).max() > 0.1 * s and np.abs(Fc + G).max() > 0.1 * s
+
+
+def test_mm_newton_third_law_and_translation():
+    """Every term's forces sum to zero in both modes; energies equal between
+    modes and invariant under a rigid translation across the boundary."""
+    pos, h, t, cnt, lst, rng = _mm_sys()
+    q = rng.uniform(-0.05, 0.05, len(pos))
+    Ec, Fc, _ = oracle.mm_forces(pos, h, t, cnt, lst, q, oracle.MM_COMPAT)
+    Ep, Fp, _ = oracle.mm_forces(pos, h, t, cnt, lst, q, oracle.MM_PHYSICAL)
+    assert np.array_equal(Ec, Ep)
+    assert np.abs(Fc.sum(axis=0)).max() < 1e-10
+    assert np.abs(Fp.sum(axis=0)).max() < 1e-10
+    shift = np.array([3.3, -7.1, 5.2])
+    Es = oracle.mm_forces(pos + shift, h, t, cnt, lst, q, oracle.MM_PHYSICAL)[0]
+    assert np.abs(Es - Ep).max() < 1e-9 * np.abs(Ep).max()
