@@ -69,7 +69,29 @@ class Thermo(C.Structure):
     _fields_ = [("Ep", C.c_double), ("KE", C.c_double), ("KEt", C.c_double),
                 ("KEr", C.c_double), ("T", C.c_double), ("P", C.c_double), ("V", C.c_double),
                 ("xi", C.c_double), ("Ep_rebo", C.c_double), ("Ep_lj", C.c_double),
-                ("h", C.c_double * 9)]
+                ("h", C.c_double * 9), ("Eb", C.c_double), ("Ea", C.c_double),
+                ("Ed", C.c_double), ("Ei", C.c_double), ("Ecoul", C.c_double)]
+
+
+_IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+
+
+class MMTopology(C.Structure):
+    """fmd_mm_topology == the reference's FlatTopology (MM:224-239)."""
+    _fields_ = [("Nb", C.c_int), ("b_i0", _IP), ("b_i1", _IP), ("b_kb", _DP), ("b_r0", _DP),
+                ("Nang", C.c_int), ("ang_i0", _IP), ("ang_i1", _IP), ("ang_i2", _IP),
+                ("ang_kth", _DP), ("ang_th0", _DP),
+                ("Ndih", C.c_int), ("dih_i0", _IP), ("dih_i1", _IP), ("dih_i2", _IP),
+                ("dih_i3", _IP), ("dih_Vn", _DP), ("dih_mult", _IP), ("dih_gamma", _DP),
+                ("Nimp", C.c_int), ("imp_i0", _IP), ("imp_i1", _IP), ("imp_i2", _IP),
+                ("imp_i3", _IP), ("imp_ki", _DP), ("imp_gamma", _DP), ("mol_id", _IP)]
+
+
+class MMConfig(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("mol_id", _IP), ("ft", C.POINTER(MMTopology)),
+                ("charge", _DP), ("lj", LJParams), ("lj_skin", C.c_double),
+                ("lj_max_neigh", C.c_int), ("nlup", C.c_int), ("dt", C.c_double),
+                ("clamp_xi", C.c_double), ("clamp_vg", C.c_double), ("mm_mode", C.c_int)]
 
 
 CTRL_ZERO_XI_VG = 1
@@ -89,6 +111,19 @@ _SIGS = {
     "fmd_device_info": (_i, [C.c_char_p, _i, C.POINTER(_i), C.POINTER(_d)]),
     "fmd_lj_params_rigid": (None, [C.POINTER(LJParams)]),
     "fmd_lj_params_airebo": (None, [C.POINTER(LJParams)]),
+    "fmd_lj_params_mm": (None, [C.POINTER(LJParams)]),
+    "fmd_load_cc1_bonds": (_i, [C.c_char_p, _ip, _i, C.POINTER(_i)]),
+    "fmd_mm_topology_counts": (_i, [_i, _ip, _i, _ip]),
+    "fmd_mm_build_topology": (_i, [_dp, _i, _ip, _i, _i, _d, _d, _d, _d,
+                                   C.POINTER(MMTopology)]),
+    "fmd_mm_nlist_lj": (_i, [_dp, _dp, _dp, _i, _ip, _ip, _ip]),
+    "fmd_mm_compute_forces": (_i, [_dp, _dp, _dp, _dp, _dp, C.POINTER(MMTopology), _ip, _ip,
+                                   _vp, _ip, _i, _i, _i, _dp]),
+    "fmd_mm_step_npt": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _i, _d, C.POINTER(NPTState),
+                             C.POINTER(MMTopology), _ip, _ip, _vp, _ip, _i, _i, _dp]),
+    "fmd_mm_config_default": (None, [C.POINTER(MMConfig)]),
+    "fmd_mm_create": (_i, [C.POINTER(MMConfig), C.POINTER(_vp)]),
+    "fmd_mm_energies": (_i, [_vp, _dp]),
     "fmd_make_npt": (None, [C.POINTER(NPTState), _d, _d, _i]),
     "fmd_rigid_nlist": (_i, [_dp, _dp, _dp, _i, _d, _i, _ip, _ip, C.POINTER(_i)]),
     "fmd_rigid_forces": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i, _i,

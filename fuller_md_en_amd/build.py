@@ -23,7 +23,8 @@ LIB = os.path.join(PKG, "libfmd.so")
 
 ARCH = os.environ.get("FMD_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["rigid_kernels.hip", "fmd_api.hip", "fmd_atomic_api.hip", "atomic_kernels.hip",
-               "surrogate_kernels.hip", "fmd_surrogate_api.hip"]
+               "surrogate_kernels.hip", "fmd_surrogate_api.hip", "mm_kernels.hip",
+               "fmd_mm_api.hip"]
 HOST_SOURCES = ["fmd_host.cpp"]
 DRIVERS = {"fuller_LJ_npt_md": "driver_lj.cpp", "fuller_airebo_npt_md": "driver_airebo.cpp"}
 

@@ -818,42 +818,69 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
   block_sum<1>(v2, d.ke_part + blockIdx.x);
 }
 
-// Fixed-order sums of the three partial arrays: s[0] = sum m v^2,
-// s[1..9] = W9, s[10] = E_rebo, s[11] = E_lj.
+// Fixed-order sums of the partial arrays: s[0] = sum m v^2, s[1..9] = W9,
+// s[10] = E of the intramolecular term (REBO, or the MM bonded total),
+// s[11] = E_lj, s[12..15] = MM Eb, Ea, Ed, Ei, s[16] = E_coulomb.
 // Replica r's sums (one workgroup; the strided order is the same whatever
 // the replica's offset, so a replica sums bitwise as it would alone).
-__device__ __forceinline__ void at_reduce(const AtomicDev &d, int r, double *sums) {
-  double v[12];
-  for (int k = 0; k < 12; k++) v[k] = 0;
+constexpr int kAtSums = 17;
+__device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams &p, int r,
+                                          double *sums) {
+  double v[kAtSums];
+  for (int k = 0; k < kAtSums; k++) v[k] = 0;
+  const bool mm = p.ff == kFfMM;
+  // the reference's MM virial holds the diagonal only (MM:439-441, 735-742)
+  const bool diag = mm && p.mm_mode == kMmCompat;
   const int b0 = d.rep_blk[r], b1 = d.rep_blk[r + 1];
   for (int b = b0 + threadIdx.x; b < b1; b += 256) {
     v[0] += d.ke_part[b];
-    for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[(size_t)b * kAtPart + k];
-    v[10] += d.rebo_part[(size_t)b * kAtPart + 9];
+    if (!mm) {
+      for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[(size_t)b * kAtPart + k];
+      v[10] += d.rebo_part[(size_t)b * kAtPart + 9];
+    } else if (d.mm.charge) {
+      for (int k = 0; k < 9; k++) v[1 + k] += d.mm.coul_part[(size_t)b * kAtPart + k];
+      v[16] += d.mm.coul_part[(size_t)b * kAtPart + 9];
+    }
   }
   const int m0 = d.rep_mol[r], m1 = d.rep_mol[r + 1];
   for (int m = m0 + threadIdx.x; m < m1; m += 256) {
-    for (int k = 0; k < 9; k++) v[1 + k] += d.lj_part[(size_t)m * kAtPart + k];
+    for (int k = 0; k < 9; k++)
+      if (!diag || k % 4 == 0) v[1 + k] += d.lj_part[(size_t)m * kAtPart + k];
     v[11] += d.lj_part[(size_t)m * kAtPart + 9];
+    if (mm) {
+      const double *q = d.mm.part + (size_t)m * kMmPart;
+      for (int k = 0; k < 9; k++) v[1 + k] += q[k];
+      for (int k = 0; k < 4; k++) v[12 + k] += q[9 + k];
+    }
   }
-  __shared__ double tot[12];
-  block_sum<12>(v, tot);
-  for (int k = 0; k < 12; k++) sums[k] = tot[k];
+  __shared__ double tot[kAtSums];
+  block_sum<kAtSums>(v, tot);
+  for (int k = 0; k < kAtSums; k++) sums[k] = tot[k];
+  if (mm) sums[10] = ((sums[12] + sums[13]) + sums[14]) + sums[15];
+}
+
+__device__ __forceinline__ void at_set_energies(AtomicState &S, const double *s) {
+  S.Ep_rebo = s[10];
+  S.Ep_lj = s[11];
+  S.Eb = s[12];
+  S.Ea = s[13];
+  S.Ed = s[14];
+  S.Ei = s[15];
+  S.Ec = s[16];
+  S.Ep = (s[10] + s[11]) + s[16];
 }
 
 // One workgroup per replica.
 __global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p, int what) {
   const int r = blockIdx.x;
-  double s[12];
-  at_reduce(d, r, s);
+  double s[kAtSums];
+  at_reduce(d, p, r, s);
   if (threadIdx.x != 0) return;
   AtomicState &S = d.state[r];
   if (what & kAtObsKE) S.KE = 0.5 * s[0] / kConv;
   if (what & kAtObsForce) {
     for (int k = 0; k < 9; k++) S.W9[k] = s[1 + k];
-    S.Ep_rebo = s[10];
-    S.Ep_lj = s[11];
-    S.Ep = s[10] + s[11];
+    at_set_energies(S, s);
   }
 }
 
@@ -861,8 +888,8 @@ __global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p,
 __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
                                                    int thermo_slot) {
   const int r = blockIdx.x;
-  double s[12];
-  at_reduce(d, r, s);
+  double s[kAtSums];
+  at_reduce(d, p, r, s);
   if (threadIdx.x != 0) return;
   AtomicState &S = d.state[r];
   const AtomicMid &M = d.mid[r];
@@ -871,7 +898,6 @@ __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, 
   double KE = 0.5 * s[0] / kConv;
   double W9[9];
   for (int k = 0; k < 9; k++) W9[k] = s[1 + k];
-  const double Er = s[10], El = s[11];
   double xi = M.xi, Vg[9];
   for (int k = 0; k < 9; k++) Vg[k] = M.Vg[k];
   xi += hdt * (2 * KE - S.Nf * kB * c.Tt) / S.Q;
@@ -900,13 +926,11 @@ __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, 
   S.xi = xi;
   S.Tt = c.Tt;
   S.KE = KE;
-  S.Ep = Er + El;
-  S.Ep_rebo = Er;
-  S.Ep_lj = El;
+  at_set_energies(S, s);
   S.scale = scale;
   S.step += 1;
   Thermo &t = d.thermo[(size_t)thermo_slot * p.R + r];
-  t.Ep = Er + El;
+  t.Ep = S.Ep;
   t.KE = KE;
   t.KEt = KE;
   t.KEr = 0;
@@ -914,9 +938,14 @@ __global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, 
   t.P = P;
   t.V = V2;
   t.xi = xi;
-  t.Ep_rebo = Er;
-  t.Ep_lj = El;
+  t.Ep_rebo = S.Ep_rebo;
+  t.Ep_lj = S.Ep_lj;
   for (int k = 0; k < 9; k++) t.h[k] = M.h[k];
+  t.Eb = S.Eb;
+  t.Ea = S.Ea;
+  t.Ed = S.Ed;
+  t.Ei = S.Ei;
+  t.Ecoul = S.Ec;
 }
 
 // Velocity rescale by each replica's factor (1 where its control did not ask).
@@ -952,7 +981,7 @@ hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
   k_at_mol_nlist<<<p.Nm, 64, 0, s>>>(d, p, hh);
   dim3 g((p.Nm + 3) / 4);
   const bool one = ns_of(p) == 1;
-  if (rebo_list) {
+  if (rebo_list && p.ff != kFfMM) {
     if (one)
       k_at_rebo_nlist<1><<<g, 256, 0, s>>>(d, p, hh);
     else
@@ -1009,7 +1038,10 @@ hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, i
 hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                             hipStream_t s) {
   hipError_t e;
-  if (p.use_sg) {
+  if (p.ff == kFfMM) {
+    // bonded molecular-mechanics terms into Frebo and mm.part
+    if ((e = mm_launch_bonded(d, p, hh, s)) != hipSuccess) return e;
+  } else if (p.use_sg) {
     // the learned surrogate in place of compute_rebo: Frebo and rebo_part
     if ((e = sg_launch_descr(d, p, d.sgw, d.sgd, s)) != hipSuccess) return e;
     if ((e = sg_launch_mlp(p, d.sgw, d.sgd, s)) != hipSuccess) return e;
@@ -1037,7 +1069,9 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
     else
       k_at_lj<2, false, false><<<g, 256, 0, s>>>(d, p, hh);
   }
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (p.ff == kFfMM && d.mm.charge) return mm_launch_coulomb(d, p, hh, s);
+  return hipSuccess;
 }
 
 hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,

@@ -93,6 +93,16 @@ void fmd_lj_params_airebo(fmd_lj_params *p) {
   p->rmin2 = 0.25;
 }
 
+void fmd_lj_params_mm(fmd_lj_params *p) {
+  const double sigma = 3.431, eps = 2.635e-3;  // MM:111-117 (s / rc written as 1/3)
+  const double rc = 3.0 * sigma, sr = 1.0 / 3.0, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+  p->sigma = sigma;
+  p->eps = eps;
+  p->rcut = rc;
+  p->vshift = 4.0 * eps * (sr6 * sr6 - sr6);
+  p->rmin2 = 0.25;
+}
+
 void fmd_make_npt(fmd_npt_state *s, double T, double Pe, int Nf) {
   s->xi = 0;
   s->Q = std::max(Nf * kB * T * 100.0 * 100.0, 1e-20);

@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "fmd_common.hpp"
+#include "fmd_mm.hpp"
 #include "fmd_rebo_passes.hpp"
 #include "fmd_rigid.hpp"  // StepCtrl, Thermo
 #include "fmd_sg_types.hpp"
@@ -51,6 +52,7 @@ struct AtomicState {
   double h[9], hi[9];  // adjacent: kernels take them as one 18-double block
   double xi, Vg[9], Q, W, Pe, Tt;
   double KE, Ep, Ep_rebo, Ep_lj, W9[9];
+  double Eb, Ea, Ed, Ei, Ec;  // molecular-mechanics terms (Ep_rebo = Eb+Ea+Ed+Ei)
   double scale;
   int Nf, step;
 };
@@ -72,6 +74,8 @@ struct AtomicParams {
   int rebo_mode;   // kReboCompat / kReboPhysical
   int small_box;   // 1: per-pair minimum image (a cell narrower than the list)
   int use_sg;      // 1: the learned surrogate stands in for the REBO term
+  int ff;          // kFfAirebo / kFfMM: the intramolecular force field
+  int mm_mode;     // kMmCompat / kMmPhysical (ff == kFfMM)
   int trace_virial;  // 1: time steps keep only the LJ virial trace (engines)
   int lj_trace;      // set per launch: this force evaluation keeps only the
                      // virial trace (LJ and REBO)
@@ -81,6 +85,7 @@ struct AtomicParams {
   double lj_rlist2;    // LJ half-list radius^2 ((rc + 2)^2, AB:265)
   double mol_rlist;    // molecule list radius beyond the two molecule radii
   double dt, clamp_xi, clamp_vg;
+  double coul_rc;      // Coulomb cutoff (MM: the LJ cutoff, MM:119)
   LJ32 lj;
   double eps;
 };
@@ -108,6 +113,7 @@ struct AtomicDev {
   int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)
   SgWeights sgw;  // learned surrogate (allocated when attached)
   SgDev sgd;
+  MmDev mm;       // molecular-mechanics terms (ff == kFfMM)
 };
 
 // h and h^-1 of each replica: 18 doubles at p + r * stride (stride 0: one
@@ -157,6 +163,12 @@ hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_sl
 // centres and the REBO list).
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
                           hipStream_t s);
+// Molecular-mechanics bonded forces into Frebo and mm.part (mm_kernels.hip).
+hipError_t mm_launch_bonded(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                            hipStream_t s);
+// Coulomb forces added into F, partials into mm.coul_part (needs charges).
+hipError_t mm_launch_coulomb(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                             hipStream_t s);
 // jmax from a reference-layout half list (stride cap) for the drop-in path.
 hipError_t at_launch_jmax_from_list(const AtomicDev &d, const AtomicParams &p,
                                     const int *nlc_l, const int *nll_l, int cap,
@@ -166,11 +178,13 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef h
                               int *nlc_l, int *nll_l, int cap, hipStream_t s);
 
 struct Surrogate;  // fmd_surrogate_api.hip
+struct MmHost;     // fmd_mm_api.hip
 
 struct AtomicEngine {
   AtomicParams p{};
   AtomicDev d{};
   Surrogate *sg = nullptr;
+  MmHost *mmh = nullptr;  // host image of d.mm (molecular mechanics)
   // host copies of the replica tables
   std::vector<int> rep_atom, rep_mol, mol_start;
   std::vector<double> tt;  // per replica: target temperature (host mirror)

@@ -353,9 +353,95 @@ int replica_forces(fmd_engine *e, int r, double *F, double *W9, double *energies
 void engine_free_atomic(fmd_engine *e) {
   if (!e->at) return;
   engine_free_surrogate(*e->at);
+  engine_free_mm(*e->at);
   free_atomic_dev(e->at->d);
   delete e->at;
   e->at = nullptr;
+}
+
+// Per-pair or per-molecule LJ images for positions pos (host) in cell h.
+int atomic_image_mode(fmd_engine *e, const double *pos, const double *h) {
+  AtomicEngine &A = *e->at;
+  const double rmax = max_radius(pos, A.mol_start, h);
+  A.p.small_box = decide_small_box(A.p, h, rmax);
+  A.p.scan_fast = decide_scan_fast(A.p, h, rmax);
+  return FMD_OK;
+}
+
+// The reference-layout LJ half list (stride 400) of positions pos in the
+// single-system workspace e.
+int atomic_halflist(fmd_engine *e, const double *pos, const double *h, const double *hi,
+                    int *nlc, int *nll) {
+  constexpr int kCap = 400;
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const size_t Na = A.p.Na;
+  double hh[18];
+  for (int k = 0; k < 9; k++) {
+    hh[k] = h[k];
+    hh[9 + k] = hi[k];
+  }
+  HIPCHK(hipMemcpyAsync(A.d.pos, pos, 3 * Na * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
+  int rc = atomic_image_mode(e, pos, h);
+  if (rc) return rc;
+  int *dl = nullptr, *dc = nullptr;
+  HIPCHK(hipMallocAsync((void **)&dl, Na * kCap * sizeof(int), s));
+  HIPCHK(hipMallocAsync((void **)&dc, Na * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(dl, 0, Na * kCap * sizeof(int), s));
+  HIPCHK(at_launch_centers(A.d, A.p, hh_fixed(A.d.hh), s));
+  HIPCHK(at_launch_lists(A.d, A.p, hh_fixed(A.d.hh), 0, s));
+  HIPCHK(at_launch_halflist(A.d, A.p, hh_fixed(A.d.hh), dc, dl, kCap, s));
+  HIPCHK(hipMemcpyAsync(nlc, dc, Na * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(nll, dl, Na * kCap * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipFreeAsync(dl, s));
+  HIPCHK(hipFreeAsync(dc, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FMD_OK;
+}
+
+// One NPT step of the single-system workspace e whose positions, cell and
+// lists are staged: vel, F, vir9, h and *npt in and out; *Sout the state
+// after the step (h^-1 in Sout->hi).
+int atomic_step_once(fmd_engine *e, double *vel, double *F, double *vir9, double *h,
+                     fmd_npt_state *npt, AtomicState *Sout) {
+  AtomicEngine &A = *e->at;
+  hipStream_t s = e->stream;
+  const size_t n3 = 3 * (size_t)A.p.Na * 8;
+  HIPCHK(hipMemcpyAsync(A.d.vel, vel, n3, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.F, F, n3, hipMemcpyHostToDevice, s));
+  AtomicState S;
+  memset(&S, 0, sizeof S);
+  for (int k = 0; k < 9; k++) {
+    S.h[k] = h[k];
+    S.Vg[k] = npt->Vg[k];
+    S.W9[k] = vir9[k];
+  }
+  inv9(S.h, S.hi);
+  S.xi = npt->xi;
+  S.Q = npt->Q;
+  S.W = npt->W;
+  S.Pe = npt->Pe;
+  S.Tt = npt->Tt;
+  S.Nf = npt->Nf;
+  S.scale = 1.0;
+  StepCtrl c = {npt->Tt, 0.0, 0, 0};
+  HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A.d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
+  HIPCHK(at_launch_observe(A.d, A.p, kAtObsKE, s));
+  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, nullptr, 0, s));
+  HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(F, A.d.F, n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < 9; k++) {
+    h[k] = S.h[k];
+    vir9[k] = S.W9[k];
+    npt->Vg[k] = S.Vg[k];
+  }
+  npt->xi = S.xi;
+  *Sout = S;
+  return FMD_OK;
 }
 
 }  // namespace fmd
@@ -381,6 +467,13 @@ int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
 }
 
 int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
+  return atomic_create_ff(c, R, kFfAirebo, out);
+}
+
+}  // extern "C"
+
+// R replicas with the intramolecular force field ff (kFfAirebo / kFfMM).
+int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine **out) {
   if (!c || !out || R < 1) return set_error(FMD_ERR_ARG, "fmd_batch_create: bad argument");
   for (int r = 0; r < R; r++) {
     if (!c[r].mol_id || !c[r].mass || c[r].n_atoms < 1)
@@ -411,6 +504,9 @@ int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
   e->use_graph = R > 1;
   fill_params(A.p, *c, L.rep_atom[R], (int)L.mol_rep.size(), R, apad);
   A.p.trace_virial = 1;
+  A.p.ff = ff;
+  A.p.coul_rc = c->lj.rcut;
+  if (ff == kFfMM) A.p.maxr = 1;  // no REBO list
   A.rep_atom = L.rep_atom;
   A.rep_mol = L.rep_mol;
   A.mol_start = L.mol_start;
@@ -435,6 +531,8 @@ int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
   *out = e;
   return FMD_OK;
 }
+
+extern "C" {
 
 int fmd_batch_size(fmd_engine *e, int *R) {
   if (!e || !R || e->kind != FMD_KIND_ATOMIC) return set_error(FMD_ERR_ARG, "not an atomic engine");
@@ -578,10 +676,7 @@ int stage_positions(fmd_engine *e, const double *pos, const double *h, const dou
   }
   HIPCHK(hipMemcpyAsync(A.d.pos, pos, 3 * (size_t)A.p.Na * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(A.d.hh, hh, sizeof hh, hipMemcpyHostToDevice, s));
-  const double rmax = max_radius(pos, A.mol_start, h);
-  A.p.small_box = decide_small_box(A.p, h, rmax);
-  A.p.scan_fast = decide_scan_fast(A.p, h, rmax);
-  return FMD_OK;
+  return atomic_image_mode(e, pos, h);
 }
 
 // Caller's REBO list (stride 12) and the truncation points of its LJ half
@@ -633,22 +728,8 @@ int fmd_airebo_nlist_lj(const double *pos, const double *h, const double *hi, in
   if (!pos || !h || !hi || !nlc || !nll) return set_error(FMD_ERR_ARG, "nlist_lj: null argument");
   fmd_engine *e;
   int rc = atomic_workspace(Na, mol_id, FMD_REBO_COMPAT, &e);
-  if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
-  AtomicEngine &A = *e->at;
-  hipStream_t s = e->stream;
-  int *dl = nullptr, *dc = nullptr;
-  HIPCHK(hipMallocAsync((void **)&dl, (size_t)Na * kRefMaxLj * sizeof(int), s));
-  HIPCHK(hipMallocAsync((void **)&dc, (size_t)Na * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(dl, 0, (size_t)Na * kRefMaxLj * sizeof(int), s));
-  HIPCHK(at_launch_centers(A.d, A.p, hh_fixed(A.d.hh), s));
-  HIPCHK(at_launch_lists(A.d, A.p, hh_fixed(A.d.hh), 0, s));
-  HIPCHK(at_launch_halflist(A.d, A.p, hh_fixed(A.d.hh), dc, dl, kRefMaxLj, s));
-  HIPCHK(hipMemcpyAsync(nlc, dc, (size_t)Na * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(nll, dl, (size_t)Na * kRefMaxLj * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipFreeAsync(dl, s));
-  HIPCHK(hipFreeAsync(dc, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return FMD_OK;
+  if (rc) return rc;
+  return atomic_halflist(e, pos, h, hi, nlc, nll);
 }
 
 int fmd_airebo_compute_forces(double *F, double *vir9, const double *pos, const double *h,
@@ -696,10 +777,7 @@ int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doubl
   if (rc || (rc = stage_positions(e, pos, h, hi))) return rc;
   AtomicEngine &A = *e->at;
   hipStream_t s = e->stream;
-  const size_t n3 = 3 * (size_t)Na * 8;
   A.p.dt = dt;
-  HIPCHK(hipMemcpyAsync(A.d.vel, vel, n3, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(A.d.F, F, n3, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(A.d.mass, mass, (size_t)Na * 8, hipMemcpyHostToDevice, s));
   // the molecule list from the current positions, then the caller's lists
   const int lj_compat = A.p.lj_compat;
@@ -709,37 +787,10 @@ int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doubl
   A.p.lj_compat = lj_compat;
   if ((rc = stage_lists(e, nlc_r, nll_r, nlc_l, nll_l))) return rc;
   AtomicState S;
-  memset(&S, 0, sizeof S);
-  for (int k = 0; k < 9; k++) {
-    S.h[k] = h[k];
-    S.Vg[k] = npt->Vg[k];
-    S.W9[k] = vir9[k];
-  }
-  inv9(S.h, S.hi);
-  S.xi = npt->xi;
-  S.Q = npt->Q;
-  S.W = npt->W;
-  S.Pe = npt->Pe;
-  S.Tt = npt->Tt;
-  S.Nf = npt->Nf;
-  S.scale = 1.0;
-  StepCtrl c = {npt->Tt, 0.0, 0, 0};
-  HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(A.d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
-  HIPCHK(at_launch_observe(A.d, A.p, kAtObsKE, s));
-  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, nullptr, 0, s));
-  HIPCHK(hipMemcpyAsync(pos, A.d.pos, n3, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(F, A.d.F, n3, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));
+  if ((rc = atomic_step_once(e, vel, F, vir9, h, npt, &S))) return rc;
+  HIPCHK(hipMemcpyAsync(pos, A.d.pos, 3 * (size_t)Na * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (int k = 0; k < 9; k++) {
-    h[k] = S.h[k];
-    hi[k] = S.hi[k];
-    vir9[k] = S.W9[k];
-    npt->Vg[k] = S.Vg[k];
-  }
-  npt->xi = S.xi;
+  for (int k = 0; k < 9; k++) hi[k] = S.hi[k];
   if (out4) {
     out4[0] = S.Ep;
     out4[1] = S.Ep_rebo;

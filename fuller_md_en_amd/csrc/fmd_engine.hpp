@@ -80,4 +80,15 @@ void engine_free_surrogate(AtomicEngine &A);
 bool surrogate_judging(fmd_engine *e, int *every);
 int surrogate_judge(fmd_engine *e);
 void engine_free_atomic(fmd_engine *e);
+// Atomic engine creation with the intramolecular force field ff
+// (kFfAirebo: fmd_batch_create; kFfMM: fmd_mm_create).
+int atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine **out);
+// Drop-in helpers on a single-system workspace engine (fmd_atomic_api.hip).
+int atomic_image_mode(fmd_engine *e, const double *pos, const double *h);
+int atomic_halflist(fmd_engine *e, const double *pos, const double *h, const double *hi,
+                    int *nlc, int *nll);
+int atomic_step_once(fmd_engine *e, double *vel, double *F, double *vir9, double *h,
+                     fmd_npt_state *npt, AtomicState *Sout);
+// Molecular-mechanics device buffers (fmd_mm_api.hip).
+void engine_free_mm(AtomicEngine &A);
 }  // namespace fmd

@@ -264,3 +264,209 @@ int fmd_normal_stream(unsigned seed, int n, double *out) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------ molecular mechanics
+namespace {
+// Adjacency of build_flat_topology (MM:256-257): the bonds in order, each
+// appended to both of its atoms' lists.
+std::vector<std::vector<int>> mm_adjacency(int natom, const int *bonds, int nbond) {
+  std::vector<std::vector<int>> adj(natom);
+  for (int b = 0; b < nbond; b++) {
+    adj[bonds[2 * b]].push_back(bonds[2 * b + 1]);
+    adj[bonds[2 * b + 1]].push_back(bonds[2 * b]);
+  }
+  return adj;
+}
+
+bool mm_bonds_ok(int natom, const int *bonds, int nbond) {
+  if (natom < 1 || nbond < 0 || (nbond && !bonds)) return false;
+  for (int b = 0; b < 2 * nbond; b++)
+    if (bonds[b] < 0 || bonds[b] >= natom) return false;
+  return true;
+}
+
+// Torsion angle i-j-k-l of the reference geometry (compute_phi0 MM:240-252).
+double mm_phi0(const double *c, int i, int j, int k, int l) {
+  double b1x = c[j * 3] - c[i * 3], b1y = c[j * 3 + 1] - c[i * 3 + 1],
+         b1z = c[j * 3 + 2] - c[i * 3 + 2];
+  double b2x = c[k * 3] - c[j * 3], b2y = c[k * 3 + 1] - c[j * 3 + 1],
+         b2z = c[k * 3 + 2] - c[j * 3 + 2];
+  double b3x = c[l * 3] - c[k * 3], b3y = c[l * 3 + 1] - c[k * 3 + 1],
+         b3z = c[l * 3 + 2] - c[k * 3 + 2];
+  double mx = b1y * b2z - b1z * b2y, my = b1z * b2x - b1x * b2z, mz = b1x * b2y - b1y * b2x;
+  double nx = b2y * b3z - b2z * b3y, ny = b2z * b3x - b2x * b3z, nz = b2x * b3y - b2y * b3x;
+  double mm = mx * mx + my * my + mz * mz, nn = nx * nx + ny * ny + nz * nz;
+  if (mm < 1e-20 || nn < 1e-20) return 0;
+  double b2 = std::sqrt(b2x * b2x + b2y * b2y + b2z * b2z);
+  double c_ = (mx * nx + my * ny + mz * nz) / std::sqrt(mm * nn);
+  c_ = std::min(1.0, std::max(-1.0, c_));
+  double s_ = (mx * b3x + my * b3y + mz * b3z) * b2 / std::sqrt(mm * nn);
+  return std::atan2(s_, c_);
+}
+}  // namespace
+
+extern "C" {
+
+// The bond list of load_cc1 (MM:160-183): every adjacency entry (1-based in
+// the file) as the pair (min, max), unique, ascending.
+int fmd_load_cc1_bonds(const char *path, int *bonds, int cap, int *nbond) {
+  if (!path || !nbond) return FMD_ERR_ARG;
+  std::ifstream f(path);
+  if (!f) return FMD_ERR_ARG;
+  int n = 0;
+  f >> n;
+  if (n < 1 || n > kMaxNatom) return FMD_ERR_CAPACITY;
+  std::vector<std::pair<int, int>> pairs;
+  for (int i = 0; i < n; i++) {
+    std::string line;
+    std::getline(f, line);
+    while (line.empty() && f.good()) std::getline(f, line);
+    std::istringstream ss(line);
+    std::string el;
+    int idx, flag, b;
+    double x, y, z;
+    if (!(ss >> el >> idx >> x >> y >> z >> flag)) return FMD_ERR_ARG;
+    while (ss >> b) {
+      const int j = b - 1;
+      if (j < 0 || j >= n) return FMD_ERR_ARG;
+      pairs.push_back({std::min(i, j), std::max(i, j)});
+    }
+  }
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+  *nbond = (int)pairs.size();
+  if ((int)pairs.size() > cap) return FMD_ERR_CAPACITY;
+  for (size_t b = 0; b < pairs.size(); b++) {
+    bonds[2 * b] = pairs[b].first;
+    bonds[2 * b + 1] = pairs[b].second;
+  }
+  return FMD_OK;
+}
+
+int fmd_mm_topology_counts(int natom, const int *bonds, int nbond, int *counts4) {
+  if (!counts4 || !mm_bonds_ok(natom, bonds, nbond)) return FMD_ERR_ARG;
+  auto adj = mm_adjacency(natom, bonds, nbond);
+  int nang = 0, ndih = 0, nimp = 0;
+  for (int j = 0; j < natom; j++) {
+    const int d = (int)adj[j].size();
+    nang += d * (d - 1) / 2;
+    nimp += d == 3;
+  }
+  for (int b = 0; b < nbond; b++) {
+    const int bj = bonds[2 * b], bk = bonds[2 * b + 1];
+    for (int i : adj[bj]) {
+      if (i == bk) continue;
+      for (int l : adj[bk])
+        if (l != bj && l != i) ndih++;
+    }
+  }
+  counts4[0] = nbond;
+  counts4[1] = nang;
+  counts4[2] = ndih;
+  counts4[3] = nimp;
+  return FMD_OK;
+}
+
+// build_flat_topology (MM:254-334): bonds with their geometric length;
+// angles i-j-k over each pair of j's neighbours; dihedrals i-bj-bk-l over
+// each bond; impropers (centre, its three neighbours) of three-coordinated
+// atoms; gamma = 2 phi0 + pi and multiplicity 2 for the torsions; replicated
+// molecule by molecule.
+int fmd_mm_build_topology(const double *coords, int natom, const int *bonds, int nbond,
+                          int Nmol, double kb, double kth, double v2_dih, double k_imp,
+                          fmd_mm_topology *ft) {
+  int cnt[4];
+  if (!coords || !ft || Nmol < 1 || fmd_mm_topology_counts(natom, bonds, nbond, cnt))
+    return FMD_ERR_ARG;
+  auto adj = mm_adjacency(natom, bonds, nbond);
+  const double pi = 3.14159265358979323846;
+  struct Term {
+    int a[4];
+    double x0;
+  };
+  std::vector<Term> tb, ta, td, ti;
+  for (int b = 0; b < nbond; b++) {
+    const int bi = bonds[2 * b], bj = bonds[2 * b + 1];
+    double dx = coords[bi * 3] - coords[bj * 3], dy = coords[bi * 3 + 1] - coords[bj * 3 + 1],
+           dz = coords[bi * 3 + 2] - coords[bj * 3 + 2];
+    tb.push_back({{bi, bj, 0, 0}, std::sqrt(dx * dx + dy * dy + dz * dz)});
+  }
+  for (int j = 0; j < natom; j++) {
+    const auto &nj = adj[j];
+    for (size_t ii = 0; ii < nj.size(); ii++)
+      for (size_t kk = ii + 1; kk < nj.size(); kk++) {
+        const int i = nj[ii], k = nj[kk];
+        double rji[3], rjk[3];
+        for (int c = 0; c < 3; c++) {
+          rji[c] = coords[i * 3 + c] - coords[j * 3 + c];
+          rjk[c] = coords[k * 3 + c] - coords[j * 3 + c];
+        }
+        double dji = std::sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+        double djk = std::sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+        double cs = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (dji * djk);
+        cs = std::min(1.0, std::max(-1.0, cs));
+        ta.push_back({{i, j, k, 0}, std::acos(cs)});
+      }
+  }
+  for (int b = 0; b < nbond; b++) {
+    const int bj = bonds[2 * b], bk = bonds[2 * b + 1];
+    for (int i : adj[bj]) {
+      if (i == bk) continue;
+      for (int l : adj[bk]) {
+        if (l == bj || l == i) continue;
+        td.push_back({{i, bj, bk, l}, 2 * mm_phi0(coords, i, bj, bk, l) + pi});
+      }
+    }
+  }
+  for (int i = 0; i < natom; i++) {
+    if (adj[i].size() != 3) continue;
+    const int j = adj[i][0], k = adj[i][1], l = adj[i][2];
+    ti.push_back({{i, j, k, l}, 2 * mm_phi0(coords, j, i, k, l) + pi});
+  }
+  ft->Nb = cnt[0] * Nmol;
+  ft->Nang = cnt[1] * Nmol;
+  ft->Ndih = cnt[2] * Nmol;
+  ft->Nimp = cnt[3] * Nmol;
+  for (int m = 0; m < Nmol; m++) {
+    const int off = m * natom;
+    for (int t = 0; t < cnt[0]; t++) {
+      const int g = m * cnt[0] + t;
+      ft->b_i0[g] = tb[t].a[0] + off;
+      ft->b_i1[g] = tb[t].a[1] + off;
+      ft->b_kb[g] = kb;
+      ft->b_r0[g] = tb[t].x0;
+    }
+    for (int t = 0; t < cnt[1]; t++) {
+      const int g = m * cnt[1] + t;
+      ft->ang_i0[g] = ta[t].a[0] + off;
+      ft->ang_i1[g] = ta[t].a[1] + off;
+      ft->ang_i2[g] = ta[t].a[2] + off;
+      ft->ang_kth[g] = kth;
+      ft->ang_th0[g] = ta[t].x0;
+    }
+    for (int t = 0; t < cnt[2]; t++) {
+      const int g = m * cnt[2] + t;
+      ft->dih_i0[g] = td[t].a[0] + off;
+      ft->dih_i1[g] = td[t].a[1] + off;
+      ft->dih_i2[g] = td[t].a[2] + off;
+      ft->dih_i3[g] = td[t].a[3] + off;
+      ft->dih_Vn[g] = v2_dih;
+      ft->dih_mult[g] = 2;
+      ft->dih_gamma[g] = td[t].x0;
+    }
+    for (int t = 0; t < cnt[3]; t++) {
+      const int g = m * cnt[3] + t;
+      ft->imp_i0[g] = ti[t].a[0] + off;
+      ft->imp_i1[g] = ti[t].a[1] + off;
+      ft->imp_i2[g] = ti[t].a[2] + off;
+      ft->imp_i3[g] = ti[t].a[3] + off;
+      ft->imp_ki[g] = k_imp;
+      ft->imp_gamma[g] = ti[t].x0;
+    }
+    if (ft->mol_id)
+      for (int a = 0; a < natom; a++) ft->mol_id[off + a] = m;
+  }
+  return FMD_OK;
+}
+
+}  // extern "C"

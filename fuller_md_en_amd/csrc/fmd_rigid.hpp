@@ -66,6 +66,7 @@ struct RigidMid {
 struct Thermo {
   double Ep, KE, KEt, KEr, T, P, V, xi, Ep_rebo, Ep_lj;
   double h[9];
+  double Eb, Ea, Ed, Ei, Ecoul;  // molecular-mechanics terms (0 otherwise)
 };
 
 struct RigidParams {

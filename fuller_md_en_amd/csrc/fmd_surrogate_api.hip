@@ -200,6 +200,8 @@ int fmd_sg_feature_count(void) { return kSgNF; }
 int fmd_sg_attach(fmd_engine *e, int seed) {
   AtomicEngine *A = atomic_of(e);
   if (!A) return set_error(FMD_ERR_ARG, "fmd_sg_attach: not an atomic engine");
+  if (A->p.ff != kFfAirebo)
+    return set_error(FMD_ERR_ARG, "fmd_sg_attach: the surrogate stands in for REBO only");
   if (A->sg) return FMD_OK;
   A->sg = new Surrogate();
   Surrogate &S = *A->sg;

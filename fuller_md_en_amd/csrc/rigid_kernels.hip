@@ -1117,6 +1117,7 @@ __global__ __launch_bounds__(256) void k_rigid_finish(RigidDev d, RigidParams p,
   t.xi = xi;
   t.Ep_rebo = 0;
   t.Ep_lj = Ep;
+  t.Eb = t.Ea = t.Ed = t.Ei = t.Ecoul = 0;
   for (int k = 0; k < 9; k++) t.h[k] = M.h[k];
 }
 

@@ -61,6 +61,8 @@ typedef struct {
  * AIREBO sigma=3.40 eps=2.84e-3 rc=3 sigma (AB:119-124). */
 void fmd_lj_params_rigid(fmd_lj_params *p);
 void fmd_lj_params_airebo(fmd_lj_params *p);
+/* Molecular mechanics sigma=3.431 eps=2.635e-3 rc=3 sigma (MM:111-117). */
+void fmd_lj_params_mm(fmd_lj_params *p);
 
 /* NPTState of the reference (C1:478 / AB:511). */
 typedef struct {
@@ -162,6 +164,84 @@ int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doubl
                         const int *nlc_l, const int *nll_l, const int *mol_id,
                         int rebo_mode, double *out4);
 
+
+/* --------------------------------------------- molecular mechanics (MM)
+ * The bonded force field of src/fuller_LJ_npt_mmmd_serial_omp_acc.cpp:
+ * harmonic bonds and angles, cosine dihedrals and impropers within each
+ * molecule, shifted LJ between molecules over the reference's 400-entry half
+ * list, and a cut Coulomb term between charged atoms of different molecules.
+ *
+ * FMD_MM_COMPAT gives the reference's forces term by term.  They are not the
+ * gradient of its energy: bond and angle forces are +dE/dr (MM:388-392,
+ * 455-457) and the two middle atoms of each torsion take the dot-product
+ * coefficients with the opposite sign (MM:548-549, 663-685), so the
+ * reference trajectories heat up without bound.  The virial is the reference's: the
+ * diagonal of the bond and LJ parts.  FMD_MM_PHYSICAL: the exact gradient of
+ * the same energy and the full virial of every term. */
+#define FMD_MM_COMPAT 0
+#define FMD_MM_PHYSICAL 1
+
+/* FlatTopology (MM:224-239): the same members in the same order, so the
+ * reference's struct converts by pointer.  Terms list global atom indices. */
+typedef struct {
+  int Nb;
+  int *b_i0, *b_i1;
+  double *b_kb, *b_r0;
+  int Nang;
+  int *ang_i0, *ang_i1, *ang_i2;
+  double *ang_kth, *ang_th0;
+  int Ndih;
+  int *dih_i0, *dih_i1, *dih_i2, *dih_i3;
+  double *dih_Vn;
+  int *dih_mult;
+  double *dih_gamma;
+  int Nimp;
+  int *imp_i0, *imp_i1, *imp_i2, *imp_i3;
+  double *imp_ki, *imp_gamma;
+  int *mol_id;
+} fmd_mm_topology;
+
+/* The bonds of a .cc1 file (load_cc1 MM:160-183): each unique pair of its
+ * adjacency lists once, 0-based (i < j), in ascending order; bonds holds
+ * cap pairs. */
+int fmd_load_cc1_bonds(const char *path, int *bonds, int cap, int *nbond);
+/* Terms per molecule {bonds, angles, dihedrals, impropers} that
+ * build_flat_topology derives from a bond list. */
+int fmd_mm_topology_counts(int natom, const int *bonds, int nbond, int *counts4);
+/* Replaces build_flat_topology (MM:254-334): fills the caller's arrays of ft
+ * (counts4 x Nmol terms, mol_id of Nmol * natom atoms) for Nmol copies of the
+ * molecule; force constants in eV units (the reference's kcal/mol / 23.06).
+ * Sets the counts of ft. */
+int fmd_mm_build_topology(const double *coords, int natom, const int *bonds, int nbond,
+                          int Nmol, double kb, double kth, double v2_dih, double k_imp,
+                          fmd_mm_topology *ft);
+
+/* Replaces build_nlist_lj (MM:337-349): half list (j > i) of other-molecule
+ * atoms within rc + 2 A, ascending, first 400 kept, stride 400.  Atoms of a
+ * molecule contiguous. */
+int fmd_mm_nlist_lj(const double *pos, const double *h, const double *hi, int Na,
+                    const int *mol_id, int *nlc, int *nll);
+
+/* Replaces compute_forces (MM:351-789): F (Na x 3), vir9 and
+ * out7 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot}.  Every bonded term must lie
+ * within one molecule (as build_flat_topology makes them); molecules with
+ * identical terms share one device copy.  The LJ half list is used through
+ * its truncation points, as in fmd_airebo_compute_forces; charge is read
+ * only when has_charge. */
+int fmd_mm_compute_forces(double *F, double *vir9, const double *pos, const double *h,
+                          const double *hi, const fmd_mm_topology *ft, const int *nlc,
+                          const int *nll, const double *charge, const int *mol_id,
+                          int has_charge, int Na, int mm_mode, double *out7);
+
+/* Replaces step_npt (MM:811-890): one NH + PR step with carbon masses
+ * (clamps xi +-0.05, Vg +-0.005); updates pos, vel, F, vir9, h, hi, *npt;
+ * out8 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot, KE}. */
+int fmd_mm_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
+                    double *hi, int Na, double dt, fmd_npt_state *npt,
+                    const fmd_mm_topology *ft, const int *nlc, const int *nll,
+                    const double *charge, const int *mol_id, int has_charge, int mm_mode,
+                    double *out8);
+
 /* ------------------------------------------------------- resident engine */
 
 typedef struct fmd_engine fmd_engine;
@@ -223,6 +303,29 @@ int fmd_atomic_set_state(fmd_engine *e, const double *pos, const double *vel,
                          const double *h, const fmd_npt_state *npt);
 int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
                          fmd_npt_state *npt);
+
+/* Resident molecular-mechanics engine: an atomic engine (fmd_atomic_set_state,
+ * fmd_atomic_get_state, fmd_engine_*) whose intramolecular force is the MM
+ * bonded field.  Every atom has the carbon mass (MM:798, 831). */
+typedef struct {
+  int n_atoms;
+  const int *mol_id;          /* each molecule's atoms contiguous */
+  const fmd_mm_topology *ft;  /* terms over the n_atoms atoms */
+  const double *charge;       /* NULL: no Coulomb term (the reference's case) */
+  fmd_lj_params lj;           /* fmd_lj_params_mm */
+  double lj_skin;             /* 2.0 (MM:338) */
+  int lj_max_neigh;           /* 400 (MM:119); 0 = no truncation */
+  int nlup;                   /* 20 (MM:1175) */
+  double dt;                  /* 0.1 fs (MM:1025) */
+  double clamp_xi, clamp_vg;  /* 0.05, 0.005 (MM:818, 823) */
+  int mm_mode;                /* FMD_MM_COMPAT / FMD_MM_PHYSICAL */
+} fmd_mm_config;
+
+void fmd_mm_config_default(fmd_mm_config *c);
+int fmd_mm_create(const fmd_mm_config *c, fmd_engine **out);
+/* Energies of the engine's last force evaluation:
+ * {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot}. */
+int fmd_mm_energies(fmd_engine *e, double *out7);
 
 /* ---------------------------------------------------- batched replicas
  * R independent AIREBO NPT systems (sizes, species, T and P may differ) in
@@ -296,6 +399,9 @@ typedef struct {
 typedef struct {
   double Ep, KE, KEt, KEr, T, P, V, xi, Ep_rebo, Ep_lj;
   double h[9];
+  /* molecular-mechanics engines: the bonded terms and Coulomb (0 otherwise;
+   * Ep_rebo = Eb + Ea + Ed + Ei, the MM monitor's E_dih = Ed + Ei) */
+  double Eb, Ea, Ed, Ei, Ecoul;
 } fmd_thermo;
 
 /* Prepare the current state as the reference does before its loop
