@@ -62,11 +62,14 @@ def parse():
                     help="rigid workload: default parameters instead of the autotuner's")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the CPU baseline sample (default: auto, ~20 s)")
-    ap.add_argument("--workload", choices=["rigid", "airebo", "batch", "surrogate"],
+    ap.add_argument("--workload", choices=["rigid", "airebo", "batch", "surrogate", "mm"],
                     default="rigid",
                     help="rigid: C60 FCC 10^3 rigid LJ (default); airebo: C60 FCC 8^3 "
                          "flexible AIREBO (122,880 atoms); batch: 64 AIREBO NPT replicas")
     ap.add_argument("--nc", type=int, default=0, help="crystal repeats (default per workload)")
+    ap.add_argument("--mm", choices=["compat", "physical"], default="physical",
+                    help="mm workload: the reference's forces (which heat the crystal without "
+                         "bound) or the exact gradient")
     ap.add_argument("--rebo", choices=["compat", "physical"], default="",
                     help="REBO force mode (default: physical)")
     ap.add_argument("--sg-iters", type=int, default=30000,
@@ -208,6 +211,78 @@ def main_airebo(args):
     }
     if not args.no_cpu:
         res["cpu_baseline"] = airebo_cpu_baseline(args.cpu_steps or 20)
+    print(json.dumps(res))
+
+
+# Molecular mechanics: the intermolecular LJ of the force phase, 42 FLOP x
+# 169.6 unique in-cutoff pairs per atom (C60 FCC at the default a0, exact
+# list); the FP64 bonded terms come on top (their kernel time is reported
+# apart in the profile).
+FLOP_PER_ATOM_STEP_MM_LJ = 42.0 * 169.6
+
+
+def mm_cpu_baseline(steps: int):
+    """Reference serial MM build, bounded sample at 5x5x5 (30,000 atoms)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "fuller_LJ_npt_mmmd_serial")
+    if not os.path.exists(exe):
+        return None
+    libdir = os.path.join(ROOT, "fuller_md_en_amd", "FullereneLib")
+    cmd = [exe, "--cell=5", "--temp=300", "--pres=0", f"--step={steps}", "--seed=42",
+           f"--libdir={libdir}", f"--mon={steps}"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=900, cwd="/tmp").stdout
+    m = re.search(r"Done \(([\d.]+) sec\)", out)
+    if not m:
+        return None
+    secs = float(m.group(1))
+    return {"value": 30000 * steps / secs, "unit": "atom-steps/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"fuller_LJ_npt_mmmd_serial --cell=5 --step={steps} (30,000 atoms, "
+                      f"1 core; its O(N^2) list at every 20th step included), "
+                      f"'Done' timer {secs:.1f} s"}
+
+
+def main_mm(args):
+    from fuller_md_en_amd import mm
+    nc = args.nc or 8
+    s = mm.mm_system(nc, "C60", "FCC", T=T_K, seed=SEED)
+    na, nm = len(s["pos"]), s["n_mol"]
+    mode = mm.MM_COMPAT if args.mm == "compat" else mm.MM_PHYSICAL
+    eng = mm.MMEngine(s["ft"], dt=0.1, mode=mode)
+    from fuller_md_en_amd import airebo
+    eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"], npt=airebo.make_npt(T_K, 0.0, na))
+    eng.prepare()
+    step0 = 0
+    if args.warmup:
+        eng.run(step0, args.warmup, thermo=False)
+        step0 += args.warmup
+    eng.sync()
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    th = eng.run(step0, args.steps, thermo=True)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms, kn = eng.kernel_time()
+    value = na * args.steps / dt
+    achieved = FLOP_PER_ATOM_STEP_MM_LJ * na / (kms * 1e-3) / 1e12
+    last = th[args.steps - 1]
+    res = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"MM C60 FCC {nc}x{nc}x{nc} ({nm} mol, {na} atoms), bonded "
+                               f"{args.mm} (FP64) + LJ (400-entry half-list semantics), "
+                               "300 K, 0 GPa, dt=0.1 fs, NPT", "atoms": na},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_mm_bonded + k_at_lj (force phase)", "kernel_ms": kms,
+                     "launches": kn},
+        "final": {"T": last.T, "P": last.P, "E_bond_per_mol": last.Eb / nm,
+                  "E_angle_per_mol": last.Ea / nm, "E_dih_per_mol": (last.Ed + last.Ei) / nm,
+                  "E_lj_per_mol": last.Ep_lj / nm},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = mm_cpu_baseline(args.cpu_steps or 40)
     print(json.dumps(res))
 
 
@@ -370,7 +445,8 @@ def main():
               "decomposition is not built.", file=sys.stderr)
         sys.exit(2)
     if not args.steps:
-        args.steps = {"rigid": 10000, "airebo": 2000, "batch": 400, "surrogate": 1000}[args.workload]
+        args.steps = {"rigid": 10000, "airebo": 2000, "batch": 400, "surrogate": 1000,
+                      "mm": 2000}[args.workload]
     if args.warmup < 0:
         args.warmup = 20 if args.workload == "batch" else 100
     if not args.rebo:
@@ -383,6 +459,8 @@ def main():
         return main_airebo(args)
     if args.workload == "batch":
         return main_batch(args)
+    if args.workload == "mm":
+        return main_mm(args)
     if args.workload == "surrogate":
         return main_surrogate(args)
     params, tune_entry = tuned_params(args)

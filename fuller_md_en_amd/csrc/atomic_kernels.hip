@@ -27,6 +27,7 @@
 
 #include "fmd_atomic.hpp"
 #include "fmd_surrogate.hpp"
+#include "fmd_wave.hpp"
 
 namespace fmd {
 
@@ -38,11 +39,6 @@ __device__ __forceinline__ double clampd(double x, double lo, double hi) {
   return x < lo ? lo : (x > hi ? hi : x);
 }
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
@@ -76,9 +72,10 @@ __device__ __noinline__ double r2_ref(const double *__restrict__ pos, int i, int
 
 __device__ __forceinline__ const double *hh_of(HHRef hh, int r) { return hh.p + r * hh.stride; }
 
-// Block sum of K doubles in a fixed order (NW wavefronts of 64 threads).
+// Block sum of K doubles in a fixed order (NW wavefronts of 64 threads);
+// sum k goes to out[k * stride].
 template <int K, int NW = 4>
-__device__ __forceinline__ void block_sum(const double *v, double *out) {
+__device__ __forceinline__ void block_sum(const double *v, double *out, int stride = 1) {
   __shared__ double red[NW][K];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int k = 0; k < K; k++) {
@@ -90,7 +87,7 @@ __device__ __forceinline__ void block_sum(const double *v, double *out) {
     const int k = threadIdx.x;
     double t = red[0][k];
     for (int w = 1; w < NW; w++) t += red[w][k];
-    out[k] = t;
+    out[(size_t)k * stride] = t;
   }
   __syncthreads();
 }
@@ -521,8 +518,8 @@ __global__ __launch_bounds__(1024, 8) void k_rebo_pass2(AtomicDev d, AtomicParam
     for (int yn = q; yn < d.nlc_r[x]; yn += 4)
       rebo_pass2_entry<float, MODE, TRACE>(x, yn, d.nlc_r, d.nll_r, p.maxr, d.bonds, F, E, W);
   auto quad = [](double v) {
-    v += __shfl_xor(v, 1, 64);
-    return v + __shfl_xor(v, 2, 64);
+    v += dpp64<0xB1>(v);         // lane ^ 1
+    return v + dpp64<0x4E>(v);  // lane ^ 2
   };
   for (int k = 0; k < 3; k++) F[k] = quad(F[k]);
   E = quad(E);
@@ -541,7 +538,7 @@ __global__ __launch_bounds__(1024, 8) void k_rebo_pass2(AtomicDev d, AtomicParam
     }
     part[9] = E;
   }
-  block_sum<kAtPart, 16>(part, d.rebo_part + (size_t)blockIdx.x * kAtPart);
+  block_sum<kAtPart, 16>(part, d.rebo_part + blockIdx.x, d.nblk);
 }
 
 // -------------------------------------------------------------------- LJ
@@ -791,7 +788,7 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
     const int k = threadIdx.x;
     double v = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
     if (k == 9) v *= 2.0 * p.eps;  // 1/2 * 4 eps * sum of reduced energies
-    d.lj_part[(size_t)mi * kAtPart + k] = v;
+    d.lj_part[(size_t)k * p.Nm + mi] = v;
   }
 }
 
@@ -824,39 +821,51 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
 // Replica r's sums (one workgroup; the strided order is the same whatever
 // the replica's offset, so a replica sums bitwise as it would alone).
 constexpr int kAtSums = 17;
+constexpr int kFinThreads = 1024;  // one workgroup per replica: loads in flight
+// The partial arrays are component-major (component k of record r at
+// k * records + r), so the strided loops below read them coalesced.
+template <bool MM>
 __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams &p, int r,
                                           double *sums) {
   double v[kAtSums];
   for (int k = 0; k < kAtSums; k++) v[k] = 0;
-  const bool mm = p.ff == kFfMM;
-  // the reference's MM virial holds the diagonal only (MM:439-441, 735-742)
-  const bool diag = mm && p.mm_mode == kMmCompat;
+  const size_t nb = d.nblk, nm = p.Nm;
   const int b0 = d.rep_blk[r], b1 = d.rep_blk[r + 1];
-  for (int b = b0 + threadIdx.x; b < b1; b += 256) {
+  for (int b = b0 + threadIdx.x; b < b1; b += kFinThreads) {
     v[0] += d.ke_part[b];
-    if (!mm) {
-      for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[(size_t)b * kAtPart + k];
-      v[10] += d.rebo_part[(size_t)b * kAtPart + 9];
+    if (!MM) {
+      for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[k * nb + b];
+      v[10] += d.rebo_part[9 * nb + b];
     } else if (d.mm.charge) {
-      for (int k = 0; k < 9; k++) v[1 + k] += d.mm.coul_part[(size_t)b * kAtPart + k];
-      v[16] += d.mm.coul_part[(size_t)b * kAtPart + 9];
+      for (int k = 0; k < 9; k++) v[1 + k] += d.mm.coul_part[k * nb + b];
+      v[16] += d.mm.coul_part[9 * nb + b];
     }
   }
   const int m0 = d.rep_mol[r], m1 = d.rep_mol[r + 1];
-  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
-    for (int k = 0; k < 9; k++)
-      if (!diag || k % 4 == 0) v[1 + k] += d.lj_part[(size_t)m * kAtPart + k];
-    v[11] += d.lj_part[(size_t)m * kAtPart + 9];
-    if (mm) {
-      const double *q = d.mm.part + (size_t)m * kMmPart;
-      for (int k = 0; k < 9; k++) v[1 + k] += q[k];
-      for (int k = 0; k < 4; k++) v[12 + k] += q[9 + k];
+  for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
+    double x[MM ? 23 : 10];
+#pragma unroll
+    for (int k = 0; k < kAtPart; k++) x[k] = d.lj_part[k * nm + m];
+    if (MM) {
+#pragma unroll
+      for (int k = 0; k < 13; k++) x[kAtPart + k] = d.mm.part[k * nm + m];
+    }
+    for (int k = 0; k < 9; k++) v[1 + k] += x[k];
+    v[11] += x[9];
+    if (MM) {
+      for (int k = 0; k < 9; k++) v[1 + k] += x[kAtPart + k];
+      for (int k = 0; k < 4; k++) v[12 + k] += x[kAtPart + 9 + k];
     }
   }
   __shared__ double tot[kAtSums];
-  block_sum<kAtSums>(v, tot);
+  block_sum<kAtSums, kFinThreads / 64>(v, tot);
   for (int k = 0; k < kAtSums; k++) sums[k] = tot[k];
-  if (mm) sums[10] = ((sums[12] + sums[13]) + sums[14]) + sums[15];
+  if (MM) {
+    sums[10] = ((sums[12] + sums[13]) + sums[14]) + sums[15];
+    // the reference's MM virial holds the diagonal only (MM:439-441, 735-742)
+    if (p.mm_mode == kMmCompat)
+      for (int k : {1, 2, 3, 5, 6, 7}) sums[1 + k] = 0.0;
+  }
 }
 
 __device__ __forceinline__ void at_set_energies(AtomicState &S, const double *s) {
@@ -871,10 +880,12 @@ __device__ __forceinline__ void at_set_energies(AtomicState &S, const double *s)
 }
 
 // One workgroup per replica.
-__global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p, int what) {
+template <bool MM>
+__global__ __launch_bounds__(kFinThreads) void k_at_observe(AtomicDev d, AtomicParams p,
+                                                            int what) {
   const int r = blockIdx.x;
   double s[kAtSums];
-  at_reduce(d, p, r, s);
+  at_reduce<MM>(d, p, r, s);
   if (threadIdx.x != 0) return;
   AtomicState &S = d.state[r];
   if (what & kAtObsKE) S.KE = 0.5 * s[0] / kConv;
@@ -885,11 +896,12 @@ __global__ __launch_bounds__(256) void k_at_observe(AtomicDev d, AtomicParams p,
 }
 
 // One workgroup per replica.
-__global__ __launch_bounds__(256) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
+template <bool MM>
+__global__ __launch_bounds__(kFinThreads) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
                                                    int thermo_slot) {
   const int r = blockIdx.x;
   double s[kAtSums];
-  at_reduce(d, p, r, s);
+  at_reduce<MM>(d, p, r, s);
   if (threadIdx.x != 0) return;
   AtomicState &S = d.state[r];
   const AtomicMid &M = d.mid[r];
@@ -1077,7 +1089,10 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
 hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
                              hipStream_t s) {
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0);
-  k_at_observe<<<p.R, 256, 0, s>>>(d, p, what);
+  if (p.ff == kFfMM)
+    k_at_observe<true><<<p.R, kFinThreads, 0, s>>>(d, p, what);
+  else
+    k_at_observe<false><<<p.R, kFinThreads, 0, s>>>(d, p, what);
   return hipGetLastError();
 }
 
@@ -1099,7 +1114,10 @@ hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_sl
   if ((e = at_launch_forces(d, ps, hh_mid(d), s)) != hipSuccess) return e;
   if (ev1 && (e = hipEventRecordWithFlags(ev1, s, ev_flags)) != hipSuccess) return e;
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
-  k_at_finish<<<p.R, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+  if (p.ff == kFfMM)
+    k_at_finish<true><<<p.R, kFinThreads, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+  else
+    k_at_finish<false><<<p.R, kFinThreads, 0, s>>>(d, p, ctrl_slot, thermo_slot);
   if (rescale) k_at_rescale<<<d.nblk, 256, 0, s>>>(d, p);
   return hipGetLastError();
 }

@@ -22,7 +22,8 @@
 //   mol_nl_count / mol_nl  molecule neighbour list (ascending)
 //   nlc_r / nll_r          REBO list, reference layout (stride maxr = 12)
 //   bonds                  per directed REBO entry (fmd_rebo_passes.hpp)
-//   rebo_part, lj_part, ke_part  fixed-order partial sums
+//   rebo_part, lj_part, ke_part  fixed-order partial sums, component-major
+//                          (component k of block / molecule r at k * count + r)
 //
 // Replicas: R independent systems share every array, concatenated replica
 // after replica (atoms, molecules).  Molecule and atom ranges of replica r

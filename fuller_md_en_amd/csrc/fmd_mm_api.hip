@@ -20,9 +20,9 @@ struct MmHost {
   std::vector<MmTmpl> tmpl;
   std::vector<int> mol_tmpl;
   std::vector<int4> tq;
-  std::vector<double2> tp;
-  std::vector<int> tm, inc_off, inc, ch_u, ch_s;
-  int slots = 0, max_natom = 0;
+  std::vector<double2> tp, tg;
+  std::vector<int> tm, inc, inc_cut, ch_u, ch_s;
+  int slots = 0, max_natom = 0, max_inc = 0;
   std::vector<double> charge;  // empty: no Coulomb term
   std::vector<int> atom_mol;
   double *owned_charge = nullptr;  // device buffers behind MmDev::charge / atom_mol
@@ -34,10 +34,6 @@ struct MmHost {
 using namespace fmd;
 
 namespace {
-
-// LDS slots per chunk of the bonded kernel (24 B each): 1280 slots keep a C60
-// template (2310 slots) in two chunks and a workgroup under 32 KB of LDS.
-constexpr int kMmSlotCap = 1280;
 
 struct Term {
   int4 q;
@@ -71,9 +67,15 @@ bool same_terms(const MolTerms &a, const MolTerms &b) {
   return true;
 }
 
-// Appends a new template for terms T of a natom-atom molecule: its terms,
-// incidence lists (slot << 1 | negate, ascending per atom) and chunks.
-void add_template(MmHost &H, const MolTerms &T, int natom, int cap) {
+// What chunking needs of a template beyond MmTmpl.
+struct TmplBuild {
+  std::vector<int> nslot;       // slots of each unified term
+  std::vector<int> atom_start;  // natom + 1 starts of the atoms' entries (relative)
+};
+
+// Appends a new template for terms T of a natom-atom molecule: its terms and
+// incidence lists (slot << 1 | negate, ascending per atom); chunks later.
+void add_template(MmHost &H, const MolTerms &T, int natom, std::vector<TmplBuild> &tb) {
   MmTmpl t{};
   t.natom = natom;
   t.nb = (int)T[0].size();
@@ -81,63 +83,97 @@ void add_template(MmHost &H, const MolTerms &T, int natom, int cap) {
   t.nd = (int)T[2].size();
   t.ni = (int)T[3].size();
   t.u0 = (int)H.tq.size();
+  TmplBuild b;
   std::vector<std::vector<int>> lists(natom);
   int slot = 0;
-  std::vector<int> nslot;  // per unified term
   for (int k = 0; k < 4; k++) {
     for (const Term &x : T[k]) {
       H.tq.push_back(x.q);
       H.tp.push_back(x.p);
+      H.tg.push_back(k >= 2 ? make_double2(std::cos(x.p.y), std::sin(x.p.y))
+                            : make_double2(0.0, 0.0));
       H.tm.push_back(x.mult);
       if (k == 0) {  // force on q.x, its negative on q.y
         lists[x.q.x].push_back(slot << 1);
         lists[x.q.y].push_back(slot << 1 | 1);
-        nslot.push_back(1);
+        b.nslot.push_back(1);
       } else if (k == 1) {  // f_i, f_k, -(f_i + f_k) on the centre
         lists[x.q.x].push_back(slot << 1);
         lists[x.q.z].push_back((slot + 1) << 1);
         lists[x.q.y].push_back((slot + 2) << 1 | 1);
-        nslot.push_back(3);
+        b.nslot.push_back(3);
       } else {  // four forces in atom order
         lists[x.q.x].push_back(slot << 1);
         lists[x.q.y].push_back((slot + 1) << 1);
         lists[x.q.z].push_back((slot + 2) << 1);
         lists[x.q.w].push_back((slot + 3) << 1);
-        nslot.push_back(4);
+        b.nslot.push_back(4);
       }
-      slot += nslot.back();
+      slot += b.nslot.back();
     }
   }
-  t.inc0 = (int)H.inc_off.size();
+  t.inc0 = (int)H.inc.size();
   for (int a = 0; a < natom; a++) {
-    H.inc_off.push_back((int)H.inc.size());
+    b.atom_start.push_back((int)H.inc.size() - t.inc0);
     H.inc.insert(H.inc.end(), lists[a].begin(), lists[a].end());
   }
-  H.inc_off.push_back((int)H.inc.size());
-  // chunks at term boundaries, at most cap slots each
-  t.ch0 = (int)H.ch_u.size();
-  int u = 0, s = 0, nch = 0;
-  const int nu = (int)nslot.size();
-  H.ch_u.push_back(0);
-  H.ch_s.push_back(0);
-  while (u < nu) {
-    int s1 = s;
-    while (u < nu && s1 + nslot[u] - s <= cap) s1 += nslot[u++];
-    H.slots = std::max(H.slots, s1 - s);
-    s = s1;
-    H.ch_u.push_back(u);
-    H.ch_s.push_back(s);
-    nch++;
-  }
-  t.nch = nch;
+  t.ninc = (int)H.inc.size() - t.inc0;
+  b.atom_start.push_back(t.ninc);
   H.max_natom = std::max(H.max_natom, natom);
+  H.max_inc = std::max(H.max_inc, t.ninc);
   H.tmpl.push_back(t);
+  tb.push_back(std::move(b));
+}
+
+// Chunks of every template: at most cap slots each, cut at a multiple of
+// kMmThreads terms from the chunk's start where that is possible (so that no
+// pass over the terms runs with a nearly empty workgroup), and the per-atom
+// incidence cuts at the chunk boundaries.
+void chunk_templates(MmHost &H, const std::vector<TmplBuild> &tb, int cap) {
+  H.slots = 0;
+  for (size_t ti = 0; ti < H.tmpl.size(); ti++) {
+    MmTmpl &t = H.tmpl[ti];
+    const std::vector<int> &nslot = tb[ti].nslot;
+    const int nu = (int)nslot.size();
+    t.ch0 = (int)H.ch_u.size();
+    H.ch_u.push_back(0);
+    H.ch_s.push_back(0);
+    int u = 0, s = 0, nch = 0;
+    while (u < nu) {
+      int u1 = u, s1 = s;
+      while (u1 < nu && s1 + nslot[u1] - s <= cap) s1 += nslot[u1++];
+      if (u1 < nu && u1 - u >= kMmThreads) {
+        u1 = u + (u1 - u) / kMmThreads * kMmThreads;
+        s1 = s;
+        for (int x = u; x < u1; x++) s1 += nslot[x];
+      }
+      H.slots = std::max(H.slots, s1 - s);
+      u = u1;
+      s = s1;
+      H.ch_u.push_back(u);
+      H.ch_s.push_back(s);
+      nch++;
+    }
+    t.nch = nch;
+    // cut[c][a]: first entry of atom a whose slot is at or past chunk c's
+    // first slot (c = nch: the end of atom a's entries)
+    t.cut0 = (int)H.inc_cut.size();
+    for (int c = 0; c <= nch; c++) {
+      const int sc = H.ch_s[t.ch0 + c];
+      for (int a = 0; a < t.natom; a++) {
+        int e = tb[ti].atom_start[a];
+        const int e1 = tb[ti].atom_start[a + 1];
+        while (e < e1 && (H.inc[t.inc0 + e] >> 1) < sc) e++;
+        H.inc_cut.push_back(c == nch ? e1 : e);
+      }
+    }
+  }
 }
 
 // Templates of every molecule from the flat topology: each term goes to the
 // molecule of its first atom (all its atoms must belong to it), keeping the
 // terms' order; molecules with identical terms share a template.
-int mm_build_host(const fmd_mm_topology &ft, const std::vector<int> &mol_start, int Na, int cap,
+int mm_build_host(const fmd_mm_topology &ft, const std::vector<int> &mol_start, int Na,
                   MmHost &H) {
   H = MmHost();
   const int Nm = (int)mol_start.size() - 1;
@@ -207,8 +243,9 @@ int mm_build_host(const fmd_mm_topology &ft, const std::vector<int> &mol_start, 
       per[m][k].push_back(x);
     }
   }
-  std::unordered_map<uint64_t, std::vector<int>> seen;  // hash -> molecules with a template
+  std::unordered_map<uint64_t, std::vector<int>> seen;  // hash -> templates
   std::vector<int> first_of_tmpl;
+  std::vector<TmplBuild> tb;
   H.mol_tmpl.assign(Nm, -1);
   for (int m = 0; m < Nm; m++) {
     const int natom = mol_start[m + 1] - mol_start[m];
@@ -232,12 +269,16 @@ int mm_build_host(const fmd_mm_topology &ft, const std::vector<int> &mol_start, 
     }
     if (found < 0) {
       found = (int)H.tmpl.size();
-      add_template(H, per[m], natom, cap);
+      add_template(H, per[m], natom, tb);
       first_of_tmpl.push_back(m);
       seen[h].push_back(found);
     }
     H.mol_tmpl[m] = found;
   }
+  // LDS of one workgroup: atoms (24 B each), incidence entries (4 B), slots
+  const int cap = (kMmLds - 24 * H.max_natom - 4 * H.max_inc) / 24;
+  if (cap < 64) return set_error(FMD_ERR_CAPACITY, "MM: molecule topology too large for LDS");
+  chunk_templates(H, tb, cap);
   return FMD_OK;
 }
 
@@ -257,14 +298,14 @@ bool same_image(const MmHost &a, const MmHost &b) {
            (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
   };
   return eqv(a.tmpl, b.tmpl) && eqv(a.mol_tmpl, b.mol_tmpl) && eqv(a.tq, b.tq) &&
-         eqv(a.tp, b.tp) && eqv(a.tm, b.tm) && eqv(a.inc_off, b.inc_off) &&
-         eqv(a.inc, b.inc) && eqv(a.ch_u, b.ch_u) && eqv(a.ch_s, b.ch_s);
+         eqv(a.tp, b.tp) && eqv(a.tg, b.tg) && eqv(a.tm, b.tm) && eqv(a.inc, b.inc) &&
+         eqv(a.inc_cut, b.inc_cut) && eqv(a.ch_u, b.ch_u) && eqv(a.ch_s, b.ch_s);
 }
 
 // The templates of topology ft onto the device (skipped when unchanged).
 int mm_set_topology(AtomicEngine &A, const fmd_mm_topology &ft) {
   MmHost H;
-  int rc = mm_build_host(ft, A.mol_start, A.p.Na, kMmSlotCap, H);
+  int rc = mm_build_host(ft, A.mol_start, A.p.Na, H);
   if (rc) return rc;
   if (!A.mmh) A.mmh = new MmHost();
   MmHost &cur = *A.mmh;
@@ -274,9 +315,10 @@ int mm_set_topology(AtomicEngine &A, const fmd_mm_topology &ft) {
   if (e == hipSuccess) e = upload(&m.mol_tmpl, H.mol_tmpl);
   if (e == hipSuccess) e = upload(&m.tq, H.tq);
   if (e == hipSuccess) e = upload(&m.tp, H.tp);
+  if (e == hipSuccess) e = upload(&m.tg, H.tg);
   if (e == hipSuccess) e = upload(&m.tm, H.tm);
-  if (e == hipSuccess) e = upload(&m.inc_off, H.inc_off);
   if (e == hipSuccess) e = upload(&m.inc, H.inc);
+  if (e == hipSuccess) e = upload(&m.inc_cut, H.inc_cut);
   if (e == hipSuccess) e = upload(&m.ch_u, H.ch_u);
   if (e == hipSuccess) e = upload(&m.ch_s, H.ch_s);
   if (e == hipSuccess && !m.part)
@@ -285,6 +327,7 @@ int mm_set_topology(AtomicEngine &A, const fmd_mm_topology &ft) {
   m.ntmpl = (int)H.tmpl.size();
   m.slots = H.slots;
   m.max_natom = H.max_natom;
+  m.max_inc = H.max_inc;
   H.charge = std::move(cur.charge);
   H.atom_mol = std::move(cur.atom_mol);
   H.owned_charge = cur.owned_charge;
@@ -339,7 +382,7 @@ namespace fmd {
 
 void engine_free_mm(AtomicEngine &A) {
   MmDev &m = A.d.mm;
-  void *ptrs[] = {m.tmpl, m.mol_tmpl, m.tq, m.tp, m.tm, m.inc_off, m.inc, m.ch_u, m.ch_s,
+  void *ptrs[] = {m.tmpl, m.mol_tmpl, m.tq, m.tp, m.tg, m.tm, m.inc, m.inc_cut, m.ch_u, m.ch_s,
                   m.part, m.coul_part};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);

@@ -12,7 +12,8 @@
 //                dE_i/dd_ij per neighbour; the neighbour side is a
 //                fixed-order wavefront sum per neighbour (radial terms) or an
 //                LDS exchange over the bonded lists (angular terms)
-//   k_sg_parts   per-integrator-block energy / virial partials (REBO layout)
+//   k_sg_parts   per-integrator-block energy / virial partials (REBO layout,
+//                component-major)
 //   k_sg_u       training: u_i = sum_j dD_i/dd_ij . (e_j - e_i)
 //   k_sg_train   training: forward tangent along u and reverse sweep, weight
 //                gradients as MFMA products over the atoms of a workgroup
@@ -575,7 +576,7 @@ __global__ __launch_bounds__(256) void k_sg_force(AtomicDev d, AtomicParams p, S
 
 // ------------------------------------------------------------- partials
 template <int K>
-__device__ __forceinline__ void block_sum_d(const double *v, double *out) {
+__device__ __forceinline__ void block_sum_d(const double *v, double *out, int stride) {
   __shared__ double red[4][K];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int k = 0; k < K; k++) {
@@ -587,7 +588,7 @@ __device__ __forceinline__ void block_sum_d(const double *v, double *out) {
   __syncthreads();
   if (threadIdx.x < K) {
     const int k = threadIdx.x;
-    out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    out[(size_t)k * stride] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
   }
 }
 
@@ -599,7 +600,7 @@ __global__ __launch_bounds__(256) void k_sg_parts(AtomicDev d, SgDev s, double *
     for (int k = 0; k < 9; k++) v[k] = s.vir[9 * (size_t)i + k];
     v[9] = s.ei[i];
   }
-  block_sum_d<kAtPart>(v, part + (size_t)blockIdx.x * kAtPart);
+  block_sum_d<kAtPart>(v, part + blockIdx.x, d.nblk);  // component-major
 }
 
 // --------------------------------------------------------------- training
@@ -944,7 +945,7 @@ __global__ __launch_bounds__(256) void k_sg_sqerr(AtomicDev d, const double *F,
       const double e = F[3 * (size_t)i + x] - Fref[3 * (size_t)i + x];
       v[0] += e * e;
     }
-  block_sum_d<1>(v, part + blockIdx.x);
+  block_sum_d<1>(v, part + blockIdx.x, 1);
 }
 
 // grad[k] = scale * sum_g partials[g][k], g ascending.
