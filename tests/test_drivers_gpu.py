@@ -1,4 +1,5 @@
-"""The C++ drivers (fuller_LJ_npt_md, fuller_airebo_npt_md) end to end.
+"""The C++ drivers (fuller_LJ_npt_md, fuller_airebo_npt_md, fuller_LJ_npt_mmmd)
+end to end.
 
 * The reference's smoke script (src/Test_fuller.sh:197-279) runs each program
   with --step=200 --mon=100, plus the OVITO and restart variants, and passes
@@ -44,11 +45,13 @@ def rows(out):
 
 LJ = os.path.join(BIN, "fuller_LJ_npt_md")
 AB = os.path.join(BIN, "fuller_airebo_npt_md")
+MMD = os.path.join(BIN, "fuller_LJ_npt_mmmd")
 
 
 @pytest.mark.parametrize("exe,extra", [
     (LJ, []), (LJ, ["--ovito=100"]), (LJ, ["--restart=100"]),
-    (AB, []), (AB, ["--ovito=100"]), (AB, ["--restart=100"])])
+    (AB, []), (AB, ["--ovito=100"]), (AB, ["--restart=100"]),
+    (MMD, []), (MMD, ["--ovito=100"]), (MMD, ["--restart=100"])])
 def test_smoke_like_reference_script(tmp_path, exe, extra):
     rc, out = run(exe, ["--step=200", "--mon=100", *extra], tmp_path)
     assert rc == 0 and "Done" in out, out[-2000:]
@@ -162,3 +165,60 @@ def test_airebo_driver_rows_equal_reference(tmp_path):
     rc2, theirs = run(os.path.join(REF, "fuller_airebo_npt_md_serial"), args, tmp_path)
     assert rc1 == 0 and rc2 == 0
     _compare(rows(ours), rows(theirs), AB_TOL)
+
+
+# printed columns: T %7.1f, P %9.3f, a %8.3f, five energies %9.4f
+MM_TOL = {0: 0.11, 1: 0.0011, 2: 0.0011, 3: 1.1e-4, 4: 1.1e-4, 5: 1.1e-4, 6: 1.1e-4,
+          7: 1.1e-4}
+MM_REF = os.path.join(REF, "fuller_LJ_npt_mmmd_serial")
+needs_mm_ref = pytest.mark.skipif(not os.path.exists(MM_REF),
+                                  reason="reference MM build (oracle/_ref) not present")
+
+
+@needs_mm_ref
+def test_mm_driver_rows_equal_reference(tmp_path):
+    """The reference's (compat) bonded forces, 30 steps from the same seeded
+    state: the same monitor rows to their printed precision."""
+    args = ["--cell=2", "--temp=300", "--step=30", "--mon=5", "--seed=3"]
+    rc1, ours = run(MMD, args, tmp_path)
+    rc2, theirs = run(MM_REF, args, tmp_path)
+    assert rc1 == 0 and rc2 == 0
+    assert "Topology/mol: 90 bonds, 180 angles, 360 dihedrals, 60 impropers" in ours
+    _compare(rows(ours), rows(theirs), MM_TOL)
+
+
+@needs_mm_ref
+def test_mm_restart_files_cross_load(tmp_path):
+    """Our MM restart resumed by the reference, and the reference's by us."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    base = ["--cell=1", "--temp=300", "--step=20", "--mon=5", "--restart=10", "--seed=4",
+            "--fullerene=C70", "--crystal=hcp"]
+    assert run(MMD, base, a)[0] == 0
+    assert run(MM_REF, base, b)[0] == 0
+    ours = sorted(p for p in os.listdir(a) if p.endswith("_10.rst"))[0]
+    theirs = sorted(p for p in os.listdir(b) if p.endswith("_10.rst"))[0]
+    va = np.array([[float(x) for x in l.split()[2:]] for l in open(a / ours)
+                   if l.startswith("ATOM")])
+    vb = np.array([[float(x) for x in l.split()[2:]] for l in open(b / theirs)
+                   if l.startswith("ATOM")])
+    assert va.shape == vb.shape and np.abs(va[:, :3] - vb[:, :3]).max() < 1e-6
+    cont = ["--cell=1", "--temp=300", "--step=20", "--mon=5", "--seed=4", "--fullerene=C70",
+            "--crystal=hcp"]
+    _, o1 = run(MM_REF, [*cont, f"--resfile={a / ours}"], tmp_path)
+    _, o2 = run(MMD, [*cont, f"--resfile={b / theirs}"], tmp_path)
+    _, o3 = run(MMD, [*cont, f"--resfile={a / ours}"], tmp_path)
+    assert "Restarting from global step 10" in o1 and "Restarting from global step 10" in o2
+    _compare(rows(o3), rows(o1), MM_TOL)
+    _compare(rows(o2), rows(o1), MM_TOL)
+
+
+def test_mm_driver_physical_stays_bounded(tmp_path):
+    """--mm=physical: the exact-gradient field keeps the crystal near its
+    target temperature where the reference's forces heat it without bound."""
+    rc, out = run(MMD, ["--cell=2", "--temp=300", "--step=400", "--mon=100", "--mm=physical"],
+                  tmp_path)
+    assert rc == 0 and "Done" in out
+    r = rows(out)
+    assert len(r) == 4 and max(x[2][0] for x in r) < 600.0
