@@ -862,7 +862,7 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
   for (int k = 0; k < kAtSums; k++) sums[k] = tot[k];
   if (MM) {
     sums[10] = ((sums[12] + sums[13]) + sums[14]) + sums[15];
-    // the reference's MM virial holds the diagonal only (MM:439-441, 735-742)
+    // the reference's MM virial holds the diagonal only (MM:434-441, 749-757)
     if (p.mm_mode == kMmCompat)
       for (int k : {1, 2, 3, 5, 6, 7}) sums[1 + k] = 0.0;
   }

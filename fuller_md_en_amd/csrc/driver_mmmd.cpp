@@ -217,12 +217,12 @@ int main(int argc, char **argv) {
   const int Nmol = fmd_make_crystal(st.c_str(), a0, nc, centres.data(), h);
   const int Na = Nmol * natom;
   std::vector<double> pos(Na * 3), vel(Na * 3), mass(Na, 12.011);
-  for (int m = 0; m < Nmol; m++)  // MM:1091-1095: no rotation
+  for (int m = 0; m < Nmol; m++)  // MM:1100-1106: no rotation
     for (int a = 0; a < natom; a++) {
       const int idx = m * natom + a;
       for (int k = 0; k < 3; k++) pos[idx * 3 + k] = centres[m * 3 + k] + coords[a * 3 + k];
     }
-  // build_flat_topology (MM:1106): the caller's arrays, mol_id = molecule
+  // build_flat_topology (MM:1114): the caller's arrays, mol_id = molecule
   int cnt[4];
   if (fmd_mm_topology_counts(natom, bonds.data(), nbond, cnt) != FMD_OK) die("topology");
   const size_t nb = (size_t)cnt[0] * Nmol, nang = (size_t)cnt[1] * Nmol,

@@ -10,7 +10,7 @@
 //   [nb+na, nb+na+nd)          dihedrals   4 slots (atoms 0..3)
 //   [nb+na+nd, nb+na+nd+ni)    impropers   4 slots (c1, c0, c2, c3)
 // Slots are numbered in the same order (the reference's order of force
-// accumulation, MM:383-686).  Each atom's incidence list names its slots in
+// accumulation, MM:378-710).  Each atom's incidence list names its slots in
 // ascending order with a sign bit, so summing it reproduces the reference's
 // per-atom sum sequence.  Chunks cut the term range (at term boundaries, at a
 // multiple of the workgroup size where possible) so that one chunk's slots,
@@ -26,7 +26,7 @@ enum : int { kMmCompat = 0, kMmPhysical = 1 };
 constexpr int kMmPart = 13;        // per molecule: W9, Eb, Ea, Ed, Ei
 constexpr int kMmThreads = 256;    // workgroup of the bonded kernel
 constexpr int kMmMaxAtoms = 128;   // atoms of one molecule (a thread quad per atom, 2 rounds)
-constexpr double kCoulombK = 14.3996;  // eV A / e^2 (MM:118)
+constexpr double kCoulombK = 14.3996;  // eV A / e^2 (MM:117)
 constexpr int kMmLds = 53248;      // LDS budget of one bonded workgroup (bytes): three
                                    // workgroups per CU (160 KB of LDS)
 

@@ -219,14 +219,14 @@ int mm_build_host(const fmd_mm_topology &ft, const std::vector<int> &mol_start, 
       } else {
         if (!ft.imp_i0 || !ft.imp_i1 || !ft.imp_i2 || !ft.imp_i3 || !ft.imp_ki || !ft.imp_gamma)
           return set_error(FMD_ERR_ARG, "MM: null improper array");
-        // the improper's torsion runs c1-c0-c2-c3 (MM:604-611, 663-685)
+        // the improper's torsion runs c1-c0-c2-c3 (MM:612-615, 634)
         at[0] = ft.imp_i1[t];
         at[1] = ft.imp_i0[t];
         at[2] = ft.imp_i2[t];
         at[3] = ft.imp_i3[t];
         n = 4;
         x.p = make_double2(ft.imp_ki[t], ft.imp_gamma[t]);
-        x.mult = 2;  // MM:659
+        x.mult = 2;  // MM:628-629
       }
       for (int q = 0; q < n; q++)
         if (at[q] < 0 || at[q] >= Na)
@@ -336,8 +336,8 @@ int mm_set_topology(AtomicEngine &A, const fmd_mm_topology &ft) {
   return FMD_OK;
 }
 
-// Charges for the Coulomb term (MM:745-767); none when every |q| <= 1e-10,
-// as the reference decides (MM:1099-1101).
+// Charges for the Coulomb term (MM:766-787); none when every |q| <= 1e-10,
+// as the reference decides (MM:1111).
 int mm_set_charges(AtomicEngine &A, const double *charge) {
   MmDev &m = A.d.mm;
   bool any = false;

@@ -1,7 +1,7 @@
 // mm_kernels.hip -- gfx950 kernels of the molecular-mechanics force field
 // (compute_forces of src/fuller_LJ_npt_mmmd_serial_omp_acc.cpp, "MM").
 //
-//   k_mm_bonded   bonds, angles, dihedrals and impropers (MM:383-686): one
+//   k_mm_bonded   bonds, angles, dihedrals and impropers (MM:378-710): one
 //                 256-thread workgroup per molecule.  The molecule's atoms
 //                 and its incidence lists sit in LDS; threads evaluate terms
 //                 (one each, strided over a chunk) and write each force
@@ -9,14 +9,14 @@
 //                 sum the atom's slots.  No atomics: each atom's sum has a
 //                 fixed order, and every term is evaluated once.
 //   k_mm_coulomb  cut Coulomb between charged atoms of different molecules
-//                 (MM:745-767), both sides per atom, position tiles in LDS.
+//                 (MM:766-787), both sides per atom, position tiles in LDS.
 //
 // The terms are the reference's formulas with shared reciprocals in place of
 // repeated divisions, and the torsions evaluate cos / sin(mult phi - gamma)
 // algebraically instead of through atan2, cos and sin; each contribution
 // equals the reference's to a few ulps (FP64 throughout).
 //
-// Intermolecular LJ (MM:688-743) is the atomic engine's k_at_lj with the MM
+// Intermolecular LJ (MM:711-765) is the atomic engine's k_at_lj with the MM
 // parameters.
 #include <climits>
 #include <cstddef>
@@ -61,7 +61,7 @@ __device__ __forceinline__ void put3(double *s, double x, double y, double z) {
 // acc: [0] Eb, [1] Ea, [2] Ed, [3] Ei, then the virial: compat W00, W11, W22;
 // physical W9.
 
-// Bond i-j (MM:383-441): E = kb/2 (r - r0)^2.  Slot: the force on i
+// Bond i-j (MM:378-448): E = kb/2 (r - r0)^2.  Slot: the force on i
 // (compat: the reference's -kb dr/r d, d = r_j - r_i; physical: +kb dr/r d).
 template <int MODE>
 __device__ __forceinline__ void mm_bond(const double *xp, int i, int j, double2 pr,
@@ -92,7 +92,7 @@ __device__ __forceinline__ void mm_bond(const double *xp, int i, int j, double2 
   }
 }
 
-// Angle i-j-k centred on j (MM:443-477): E = kth/2 (theta - th0)^2, cos
+// Angle i-j-k centred on j (MM:449-493): E = kth/2 (theta - th0)^2, cos
 // clamped at +-0.999999.  Slots: f_i, f_k, f_i + f_k (j receives minus the
 // last).  Compat: the reference's dV = -kth dth / sin; physical: +.
 template <int MODE>
@@ -136,7 +136,7 @@ __device__ __forceinline__ void mm_angle(const double *xp, int i, int j, int k, 
   }
 }
 
-// Torsion p0-p1-p2-p3 (dihedral MM:484-598; improper MM:600-686 with its
+// Torsion p0-p1-p2-p3 (dihedral MM:494-603; improper MM:604-710 with its
 // atoms stored as c1, c0, c2, c3): E = Vn/2 (1 + cos(mult phi - gamma)).
 // Slots: the forces on p0..p3.  Compat: the reference's middle-atom
 // coefficients; physical: the exact gradient.  Returns the energy.

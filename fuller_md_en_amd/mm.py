@@ -23,7 +23,7 @@ MM_COMPAT = 0
 MM_PHYSICAL = 1
 MAX_LJ_NEIGH = 400  # MM:119
 KCAL2EV = 1.0 / 23.06054783  # MM:103
-# --ff_kb / --ff_kth / --ff_v2 / --ff_kimp defaults in kcal/mol (MM:1044-1047)
+# --ff_kb / --ff_kth / --ff_v2 / --ff_kimp defaults in kcal/mol (MM:1039-1042)
 FF_DEFAULTS_KCAL = (469.0, 63.0, 14.5, 15.0)
 
 
@@ -48,7 +48,7 @@ def load_cc1_bonds(path: str) -> np.ndarray:
 
 
 class FlatTopology:
-    """The flat topology arrays of build_flat_topology (MM:224-239), held as
+    """The flat topology arrays of build_flat_topology (MM:226-237), held as
     numpy arrays; ``struct`` is the fmd_mm_topology view of them."""
 
     KEYS = ("b_i0", "b_i1", "b_kb", "b_r0", "ang_i0", "ang_i1", "ang_i2", "ang_kth",
@@ -116,7 +116,7 @@ def _charge(charge, na):
 
 def compute_forces(pos, h, hi, ft: FlatTopology, nlc, nll, charge=None, mol_id=None,
                    mode=MM_COMPAT):
-    """compute_forces (MM:351-789) -> (out7 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot},
+    """compute_forces (MM:351-788) -> (out7 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot},
     F[Na,3], vir9[9])."""
     pos = _c(pos).reshape(-1)
     na = pos.size // 3
@@ -131,7 +131,7 @@ def compute_forces(pos, h, hi, ft: FlatTopology, nlc, nll, charge=None, mol_id=N
 
 def step_npt(pos, vel, F, vir9, h, hi, dt, npt, ft: FlatTopology, nlc, nll, charge=None,
              mol_id=None, mode=MM_COMPAT):
-    """step_npt (MM:811-890): in place like the reference; returns
+    """step_npt (MM:812-890): in place like the reference; returns
     out8 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot, KE}."""
     for a in (pos, vel, F, vir9, h, hi):
         if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
@@ -180,7 +180,7 @@ class MMEngine(AtomicEngine):
 def mm_system(nc: int = 3, spec: str = "C60", crystal: str = "FCC", T: float = 298.0,
               seed: int = 42, init_scale: float = 1.0, jitter: float = 0.0,
               jitter_seed: int = 12345, libdir: str = system.LIBDIR, k4=None):
-    """The MM program's initial system (MM:1062-1141): atoms at lattice point +
+    """The MM program's initial system (MM:1085-1141): atoms at lattice point +
     .cc1 coordinates, mol_id = molecule (0-based), its flat topology, carbon
     masses, per-atom Maxwell velocities from `seed`; optional uniform jitter
     of +-jitter A per coordinate from its own generator."""

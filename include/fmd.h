@@ -172,16 +172,16 @@ int fmd_airebo_step_npt(double *pos, double *vel, double *F, double *vir9, doubl
  * list, and a cut Coulomb term between charged atoms of different molecules.
  *
  * FMD_MM_COMPAT gives the reference's forces term by term.  They are not the
- * gradient of its energy: bond and angle forces are +dE/dr (MM:388-392,
- * 455-457) and the two middle atoms of each torsion take the dot-product
- * coefficients with the opposite sign (MM:548-549, 663-685), so the
+ * gradient of its energy: bond and angle forces are +dE/dr (MM:391,
+ * 468) and the two middle atoms of each torsion take the dot-product
+ * coefficients with the opposite sign (MM:528-529, 653, 672), so the
  * reference trajectories heat up without bound.  The virial is the reference's: the
  * diagonal of the bond and LJ parts.  FMD_MM_PHYSICAL: the exact gradient of
  * the same energy and the full virial of every term. */
 #define FMD_MM_COMPAT 0
 #define FMD_MM_PHYSICAL 1
 
-/* FlatTopology (MM:224-239): the same members in the same order, so the
+/* FlatTopology (MM:226-237): the same members in the same order, so the
  * reference's struct converts by pointer.  Terms list global atom indices. */
 typedef struct {
   int Nb;
@@ -222,7 +222,7 @@ int fmd_mm_build_topology(const double *coords, int natom, const int *bonds, int
 int fmd_mm_nlist_lj(const double *pos, const double *h, const double *hi, int Na,
                     const int *mol_id, int *nlc, int *nll);
 
-/* Replaces compute_forces (MM:351-789): F (Na x 3), vir9 and
+/* Replaces compute_forces (MM:351-788): F (Na x 3), vir9 and
  * out7 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot}.  Every bonded term must lie
  * within one molecule (as build_flat_topology makes them); molecules with
  * identical terms share one device copy.  The LJ half list is used through
@@ -233,7 +233,7 @@ int fmd_mm_compute_forces(double *F, double *vir9, const double *pos, const doub
                           const int *nll, const double *charge, const int *mol_id,
                           int has_charge, int Na, int mm_mode, double *out7);
 
-/* Replaces step_npt (MM:811-890): one NH + PR step with carbon masses
+/* Replaces step_npt (MM:812-890): one NH + PR step with carbon masses
  * (clamps xi +-0.05, Vg +-0.005); updates pos, vel, F, vir9, h, hi, *npt;
  * out8 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot, KE}. */
 int fmd_mm_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
@@ -306,18 +306,18 @@ int fmd_atomic_get_state(fmd_engine *e, double *pos, double *vel, double *h,
 
 /* Resident molecular-mechanics engine: an atomic engine (fmd_atomic_set_state,
  * fmd_atomic_get_state, fmd_engine_*) whose intramolecular force is the MM
- * bonded field.  Every atom has the carbon mass (MM:798, 831). */
+ * bonded field.  Every atom has the carbon mass (MM:798, 827). */
 typedef struct {
   int n_atoms;
   const int *mol_id;          /* each molecule's atoms contiguous */
   const fmd_mm_topology *ft;  /* terms over the n_atoms atoms */
   const double *charge;       /* NULL: no Coulomb term (the reference's case) */
   fmd_lj_params lj;           /* fmd_lj_params_mm */
-  double lj_skin;             /* 2.0 (MM:338) */
+  double lj_skin;             /* 2.0 (MM:339) */
   int lj_max_neigh;           /* 400 (MM:119); 0 = no truncation */
-  int nlup;                   /* 20 (MM:1175) */
-  double dt;                  /* 0.1 fs (MM:1025) */
-  double clamp_xi, clamp_vg;  /* 0.05, 0.005 (MM:818, 823) */
+  int nlup;                   /* 20 (MM:1179) */
+  double dt;                  /* 0.1 fs (MM:1023) */
+  double clamp_xi, clamp_vg;  /* 0.05, 0.005 (MM:818, 824) */
   int mm_mode;                /* FMD_MM_COMPAT / FMD_MM_PHYSICAL */
 } fmd_mm_config;
 

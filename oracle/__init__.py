@@ -303,7 +303,7 @@ def mm_nlist(pos, h, mol_id, max_neigh=400):
 
 
 def mm_forces(pos, h, topo, nlc, nll, charge=None, mode=MM_COMPAT):
-    """compute_forces (MM:351-789) -> (out7 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot},
+    """compute_forces (MM:351-788) -> (out7 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot},
     F[Na,3], W[9])."""
     pos, h = _c(pos).reshape(-1), _c(h)
     na = pos.size // 3
@@ -319,7 +319,7 @@ def mm_forces(pos, h, topo, nlc, nll, charge=None, mode=MM_COMPAT):
 
 
 def mm_step_npt(pos, vel, F, W, h, npt, dt, topo, nlc, nll, charge=None, mode=MM_COMPAT):
-    """step_npt (MM:811-890) in place on copies; returns
+    """step_npt (MM:812-890) in place on copies; returns
     (pos, vel, F, W, h, npt, out8 {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot, KE})."""
     pos, vel, F, W, h = (_c(x).reshape(-1).copy() for x in (pos, vel, F, W, h))
     na = pos.size // 3

@@ -828,9 +828,9 @@ void orc_airebo_step_npt(double *pos, double *vel, double *F, double *vir9,
  *
  * Modes: ORC_MM_COMPAT reproduces the reference forces term by term.  They
  * are not the gradient of the reference energy: the bond and angle forces
- * carry the opposite sign (F = +dE/dr, MM:384-392 and MM:455-462), and the
+ * carry the opposite sign (F = +dE/dr, MM:391 and MM:468), and the
  * middle-atom dihedral/improper forces use the dot-product coefficients with
- * the opposite sign (MM:548-549 and MM:663-685; b1 = r2 - r1 there, where the
+ * the opposite sign (MM:528-529 and MM:653, 672; b1 = r2 - r1 there, where the
  * analytic formula it follows has r1 - r2).  ORC_MM_PHYSICAL uses the exact
  * gradient F = -dE/dr of the same energy, and the full virial tensor
  * W = sum over terms of (relative position) (x) force, bonded terms included.
@@ -998,7 +998,7 @@ void orc_mm_topology(const double *coords, int natom, const int *bonds, int nbon
   }
 }
 
-/* One torsion (dihedral MM:508-549 / improper MM:627-685) over atoms
+/* One torsion (dihedral MM:494-603 / improper MM:604-710) over atoms
  * p0-p1-p2-p3 with E = Vn/2 (1 + cos(mult phi - gamma)).  Adds the four
  * forces into F and (physical) the virial; returns the energy.  Skipped
  * (0) when either cross product is degenerate. */
@@ -1061,7 +1061,7 @@ static double mm_torsion(double *F, double *W, const double *pos, const double *
   return E;
 }
 
-/* compute_forces MM:351-789.  out7 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot}.
+/* compute_forces MM:351-788.  out7 = {Eb, Ea, Ed, Ei, Elj, Ecoul, Etot}.
  * The LJ half list has stride max_l (400 in the reference).  Compat: the
  * virial holds only the diagonal bond and LJ parts, as the reference's. */
 void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
@@ -1074,7 +1074,7 @@ void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
   for (int i = 0; i < na * 3; i++) F[i] = 0;
   for (int i = 0; i < 9; i++) vir9[i] = 0;
   double Eb = 0, Ea = 0, Ed = 0, Ei = 0, Elj = 0, Ec = 0;
-  /* bonds MM:383-441 */
+  /* bonds MM:378-448 */
   for (int q = 0; q < t->Nb; q++) {
     int i = t->b[2 * q], j = t->b[2 * q + 1];
     double d[3] = {pos[j * 3] - pos[i * 3], pos[j * 3 + 1] - pos[i * 3 + 1],
@@ -1099,7 +1099,7 @@ void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
       vir9[8] += d[2] * f[2];
     }
   }
-  /* angles MM:443-477 */
+  /* angles MM:449-493 */
   for (int q = 0; q < t->Nang; q++) {
     int i = t->a[3 * q], j = t->a[3 * q + 1], k = t->a[3 * q + 2];
     double rji[3], rjk[3];
@@ -1131,18 +1131,18 @@ void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
       for (int x = 0; x < 3; x++)
         for (int y = 0; y < 3; y++) vir9[3 * x + y] += rji[x] * fi[y] + rjk[x] * fk[y];
   }
-  /* dihedrals MM:484-598 */
+  /* dihedrals MM:494-603 */
   double Wt[9] = {0};
   for (int q = 0; q < t->Ndih; q++)
     Ed += mm_torsion(F, Wt, pos, h, hi, t->d[4 * q], t->d[4 * q + 1], t->d[4 * q + 2],
                      t->d[4 * q + 3], t->dp[2 * q], t->dm[q], t->dp[2 * q + 1], mode);
-  /* impropers MM:600-686: atom order c1-c0-c2-c3, multiplicity 2 */
+  /* impropers MM:604-710: atom order c1-c0-c2-c3, multiplicity 2 */
   for (int q = 0; q < t->Nimp; q++)
     Ei += mm_torsion(F, Wt, pos, h, hi, t->im[4 * q + 1], t->im[4 * q], t->im[4 * q + 2],
                      t->im[4 * q + 3], t->ip[2 * q], 2, t->ip[2 * q + 1], mode);
   if (phys)
     for (int x = 0; x < 9; x++) vir9[x] += Wt[x];
-  /* LJ MM:688-743 */
+  /* LJ MM:711-765 */
   for (int i = 0; i < na; i++) {
     for (int jn = 0; jn < nlc[i]; jn++) {
       int j = nll[i * max_l + jn];
@@ -1169,7 +1169,7 @@ void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
       }
     }
   }
-  /* Coulomb MM:745-767 (all pairs of different molecules within 3 sigma) */
+  /* Coulomb MM:766-787 (all pairs of different molecules within 3 sigma) */
   if (has_charge) {
     const double KC = 14.3996, rc = 3.0 * 3.431;
     for (int i = 0; i < na - 1; i++)
@@ -1201,7 +1201,7 @@ void orc_mm_forces(double *F, double *vir9, const double *pos, const double *h,
   out7[6] = Eb + Ea + Ed + Ei + Elj + Ec;
 }
 
-/* step_npt MM:811-890 (carbon mass for every atom).  out8 = out7 + {KE}. */
+/* step_npt MM:812-890 (carbon mass for every atom).  out8 = out7 + {KE}. */
 void orc_mm_step_npt(double *pos, double *vel, double *F, double *vir9, double *h,
                      double *hi, int na, double dt, orc_npt *npt, const orc_mm_topo *t,
                      const int *nlc, const int *nll, int max_l, const double *charge,

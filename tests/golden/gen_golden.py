@@ -252,7 +252,7 @@ def gen_airebo():
          E=np.array(E), F=F, W=W)
 
 
-MM_K4 = np.array([469.0, 63.0, 14.5, 15.0]) / 23.06054783  # MM:1044-1047
+MM_K4 = np.array([469.0, 63.0, 14.5, 15.0]) / 23.06054783  # MM:1039-1042
 
 
 def mm_system(spec, crystal, nc, seed, jitter=0.02):
@@ -275,7 +275,7 @@ def mm_system(spec, crystal, nc, seed, jitter=0.02):
 
 
 def mm_template(topo, nm):
-    """Terms of molecule 0 (build_flat_topology replicates them, MM:313-332)."""
+    """Terms of molecule 0 (build_flat_topology replicates them, MM:314-332)."""
     out = {}
     for k in ("b", "bp", "a", "ap", "d", "dp", "dm", "i", "ip"):
         out["t_" + k] = topo[k][: len(topo[k]) // nm]

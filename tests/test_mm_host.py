@@ -21,7 +21,7 @@ def test_bonds_and_topology_equal_reference(name, spec):
     ft = mm.build_flat_topology(mol.coords, bonds, nmol)
     natom = len(mol.coords)
     nb = len(g["t_b"])
-    # molecule 0 bitwise, and every replica shifted by natom (MM:313-332)
+    # molecule 0 bitwise, and every replica shifted by natom (MM:314-332)
     assert np.array_equal(np.c_[ft["b_i0"], ft["b_i1"]][:nb], g["t_b"])
     assert np.array_equal(np.c_[ft["b_kb"], ft["b_r0"]][:nb], g["t_bp"])
     na = len(g["t_a"])

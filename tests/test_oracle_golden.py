@@ -172,7 +172,7 @@ def _mm_topo(g, nmol):
         n = len(t[k]) // nmol
         # molecule 0 bitwise as the reference builds it (MM:254-334) ...
         assert np.array_equal(t[k][:n], g["t_" + k]), k
-        # ... and the replicas shifted by natom per molecule (MM:313-332)
+        # ... and the replicas shifted by natom per molecule (MM:314-332)
         if k in ("b", "a", "d", "i"):
             natom = len(g["coords"])
             shift = (np.arange(nmol) * natom).repeat(n)[:, None]

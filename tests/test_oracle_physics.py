@@ -159,9 +159,9 @@ def test_rigid_forces_are_the_gradient():
 
 # ------------------------------------------------------ molecular mechanics
 # The reference's MM forces (compat) are not the gradient of its energy: bond
-# and angle forces are +dE/dr (MM:388-392, 455-457) and the middle-atom
+# and angle forces are +dE/dr (MM:391, 468) and the middle-atom
 # torsion forces take the dot-product coefficients with the opposite sign
-# (MM:548-549, 663-685).  The physical mode is pinned here by the mathematics.
+# (MM:528-529, 653, 672).  The physical mode is pinned here by the mathematics.
 def _mm_sys(k4=None, jit=0.05, seed=11, shear=True):
     g = golden("mm_c60_nc1.npz")
     rng = np.random.default_rng(seed)
