@@ -900,31 +900,34 @@ template <bool MM>
 __global__ __launch_bounds__(kFinThreads) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
                                                    int thermo_slot) {
   const int r = blockIdx.x;
+  // the state, midpoint and control before the reduction: their latency
+  // overlaps it
+  AtomicState &S = d.state[r];
+  const AtomicMid M = d.mid[r];
+  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + r];
+  const double Nf = S.Nf, Q = S.Q, Pe = S.Pe, Wb = S.W;
   double s[kAtSums];
   at_reduce<MM>(d, p, r, s);
   if (threadIdx.x != 0) return;
-  AtomicState &S = d.state[r];
-  const AtomicMid &M = d.mid[r];
-  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + r];
   const double hdt = 0.5 * p.dt;
   double KE = 0.5 * s[0] / kConv;
   double W9[9];
   for (int k = 0; k < 9; k++) W9[k] = s[1 + k];
   double xi = M.xi, Vg[9];
   for (int k = 0; k < 9; k++) Vg[k] = M.Vg[k];
-  xi += hdt * (2 * KE - S.Nf * kB * c.Tt) / S.Q;
+  xi += hdt * (2 * KE - Nf * kB * c.Tt) / Q;
   xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
   const double V2 = fabs(det9(M.h));
   const double P = (2 * KE + W9[0] + W9[4] + W9[8]) / (3 * V2) * kEv2GPa;
-  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * (P - S.Pe) / (S.W * kEv2GPa);
+  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * (P - Pe) / (Wb * kEv2GPa);
   for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
-  double T = 2 * KE / (S.Nf * kB);
+  double T = 2 * KE / (Nf * kB);
   double scale = 1.0;
   if ((c.flags & kCtrlRescale) && T > 0.1) {  // AB:898-908
     const double tgt = c.rescale_T > 0.1 ? c.rescale_T : 0.1;
     scale = sqrt(tgt / T);
     KE *= scale * scale;
-    T = 2 * KE / (S.Nf * kB);
+    T = 2 * KE / (Nf * kB);
     xi = 0;
     if (c.flags & kCtrlRescaleZeroVg)
       for (int k = 0; k < 9; k++) Vg[k] = 0;

@@ -14,7 +14,8 @@
 //   pairout                 double[N*max_neigh*16]    per owned pair, both sides
 //   hl_item                 int2[max_neigh*N]         pair kernel work items, rank-major
 //   molpart                 double[N*kMolPart]    per-molecule W9, E partials
-//   blockpart               double[nblk*kBlkPart] per-block KE / W / E sums
+//   blockpart               double[kBlkPart*nblk] per-block KE / W / E sums,
+//                           component-major
 //   RigidState              NPT + cell, FP64 (one per system)
 //   RigidMid                first-half results of the current step
 #pragma once

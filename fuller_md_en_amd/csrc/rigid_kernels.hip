@@ -47,28 +47,45 @@ __device__ __forceinline__ double uniform_d(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// The two halves-swaps of gfx950 on a pair of doubles: permlane32 exchanges
+// lanes 32-63 of a with lanes 0-31 of b, permlane16 rows 1 and 3 of a with
+// rows 0 and 2 of b.  r0 + r1 is then, in each lane, the lane's own value
+// of a (lower half / even rows) or b (upper half / odd rows) plus its
+// partner's (lane ^ 32 / lane ^ 16): one butterfly stage of two values.
+template <bool H32>
+__device__ __forceinline__ double swap_add(double a, double b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const int alo = (int)A, ahi = (int)(A >> 32), blo = (int)B, bhi = (int)(B >> 32);
+  int l0, l1, h0, h1;
+  if (H32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    l0 = lo[0], l1 = lo[1], h0 = hi[0], h1 = hi[1];
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    l0 = lo[0], l1 = lo[1], h0 = hi[0], h1 = hi[1];
+  }
+  const double r0 = __longlong_as_double(((long long)h0 << 32) | (unsigned int)l0);
+  const double r1 = __longlong_as_double(((long long)h1 << 32) | (unsigned int)l1);
+  return r0 + r1;
+}
+
 // Sums of eight doubles over the 64 lanes, in a fixed order; every lane gets
 // all eight.  Transposed butterfly: the exchanges with lane ^ 32, ^ 16, ^ 8
 // each halve the values a lane carries (it keeps the half its side of the
 // split owns and adds the partner's copy of it), so lane L then holds a
 // partial of value (L >> 3) & 7; three DPP pairings inside each group of 8
-// complete it, and lane 8k holds sum k.  Whole wavefront active.
+// complete it, and lane 8k holds sum k.  The first two exchanges are lane
+// swaps (no LDS traffic).  Whole wavefront active.
 __device__ __forceinline__ void wave_sum8(double *v) {
   const int lane = __lane_id();
   double k4[4], k2[2];
-  bool up = lane & 32;
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const double send = up ? v[c] : v[c + 4];
-    k4[c] = (up ? v[c + 4] : v[c]) + __shfl_xor(send, 32);
-  }
-  up = lane & 16;
+  for (int c = 0; c < 4; c++) k4[c] = swap_add<true>(v[c], v[c + 4]);  // c | c + 4
 #pragma unroll
-  for (int c = 0; c < 2; c++) {
-    const double send = up ? k4[c] : k4[c + 2];
-    k2[c] = (up ? k4[c + 2] : k4[c]) + __shfl_xor(send, 16);
-  }
-  up = lane & 8;
+  for (int c = 0; c < 2; c++) k2[c] = swap_add<false>(k4[c], k4[c + 2]);
+  const bool up = lane & 8;
   double k1 = (up ? k2[1] : k2[0]) + dpp64<0x128>(up ? k2[0] : k2[1]);  // row_ror:8 = lane ^ 8
   k1 += dpp64<0xB1>(k1);   // quad_perm [1,0,3,2]
   k1 += dpp64<0x4E>(k1);   // quad_perm [2,3,0,1]
@@ -77,8 +94,25 @@ __device__ __forceinline__ void wave_sum8(double *v) {
   for (int c = 0; c < 8; c++) v[c] = readlane64(k1, 8 * c);
 }
 
-// Lab-frame offsets of the sites of molecule i (lanes a and a+64): FP32 for
-// the pair loop and FP64 (bitwise the reference's lab array) for re-tests.
+// Lab-frame offset of site a of molecule i: FP32 for the pair loop and FP64
+// (bitwise the reference's lab array) for re-tests.
+__device__ __forceinline__ void write_site(int i, int a, const double *q, const double *body,
+                                           float4 *sites, double *sites_d, int natom_pad) {
+  const double *b = body + 4 * a;
+  double o[3];
+  if (b[3] != 0.0) {
+    o[0] = o[1] = o[2] = 1e6;
+  } else {
+    lab_exact(q, b, o);
+  }
+  const size_t t = (size_t)i * natom_pad + a;
+  sites[t] = make_float4((float)o[0], (float)o[1], (float)o[2], 0.f);
+  sites_d[3 * t] = o[0];
+  sites_d[3 * t + 1] = o[1];
+  sites_d[3 * t + 2] = o[2];
+}
+
+// The sites of molecule i on the lanes of a wavefront (lanes a and a + 64).
 template <int NS>
 __device__ __forceinline__ void write_sites(int i, const double *q, const double *body,
                                             float4 *sites, double *sites_d, int natom_pad,
@@ -86,32 +120,23 @@ __device__ __forceinline__ void write_sites(int i, const double *q, const double
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     int a = lane + 64 * s;
-    if (a < natom_pad) {
-      const double *b = body + 4 * a;
-      double o[3];
-      if (b[3] != 0.0) {
-        o[0] = o[1] = o[2] = 1e6;
-      } else {
-        lab_exact(q, b, o);
-      }
-      const size_t t = (size_t)i * natom_pad + a;
-      sites[t] = make_float4((float)o[0], (float)o[1], (float)o[2], 0.f);
-      sites_d[3 * t] = o[0];
-      sites_d[3 * t + 1] = o[1];
-      sites_d[3 * t + 2] = o[2];
-    }
+    if (a < natom_pad) write_site(i, a, q, body, sites, sites_d, natom_pad);
   }
 }
 
-// Lab-frame centres of the site quads (lanes q < nq), w = quad radius.
+// Lab-frame centre of site quad k of molecule i, w = the quad's radius.
+__device__ __forceinline__ void write_quad(int i, int k, const double *q, const double *qbody,
+                                           float4 *qcen, int nq) {
+  double o[3];
+  lab_exact(q, qbody + 4 * k, o);
+  qcen[(size_t)i * nq + k] =
+      make_float4((float)o[0], (float)o[1], (float)o[2], (float)qbody[4 * k + 3]);
+}
+
+// The quads of molecule i on lanes k < nq.
 __device__ __forceinline__ void write_quads(int i, const double *q, const double *qbody,
                                             float4 *qcen, int nq, int lane) {
-  if (lane < nq) {
-    double o[3];
-    lab_exact(q, qbody + 4 * lane, o);
-    qcen[(size_t)i * nq + lane] =
-        make_float4((float)o[0], (float)o[1], (float)o[2], (float)qbody[4 * lane + 3]);
-  }
+  if (lane < nq) write_quad(i, lane, q, qbody, qcen, nq);
 }
 
 // ------------------------------------------------------------------ list
@@ -301,63 +326,78 @@ __device__ __forceinline__ void npt_half1(const RigidState &S, const StepCtrl &c
   o.xi1 = xi;
 }
 
-template <int NS>
+// kKdMol molecules per workgroup: threads t < kKdMol integrate molecule t
+// (the NPT first-half scalars derived by each of them from the same inputs),
+// then the whole workgroup writes the molecules' sites and quads.  Every
+// phase is one dependent chain deep per thread.
+constexpr int kKdMol = 16;
 __global__ __launch_bounds__(256) void k_rigid_kick_drift(RigidDev d, RigidParams p,
                                                           int ctrl_slot) {
-  const RigidState &S = *d.state;
-  const StepCtrl c = d.ctrl[ctrl_slot];
-  Half1 H;
-  npt_half1(S, c, p, H);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    RigidMid &M = *d.mid;
-    for (int k = 0; k < 9; k++) {
-      M.h[k] = H.h1[k];
-      M.hi[k] = H.hi1[k];
-      M.Vg[k] = H.Vg1[k];
+  __shared__ double sq[kKdMol][4];
+  const int t = threadIdx.x, i0 = blockIdx.x * kKdMol;
+  if (t < kKdMol) {
+    const RigidState &S = *d.state;
+    const StepCtrl c = d.ctrl[ctrl_slot];
+    Half1 H;
+    npt_half1(S, c, p, H);
+    if (blockIdx.x == 0 && t == 0) {
+      RigidMid &M = *d.mid;
+      for (int k = 0; k < 9; k++) {
+        M.h[k] = H.h1[k];
+        M.hi[k] = H.hi1[k];
+        M.Vg[k] = H.Vg1[k];
+      }
+      M.xi = H.xi1;
+      M.sc_nh = H.sc_nh;
     }
-    M.xi = H.xi1;
-    M.sc_nh = H.sc_nh;
+    const int i = i0 + t;
+    if (i < p.N) {
+      const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
+      double v[3], w[3], r[3], q[4];
+      for (int a = 0; a < 3; a++) {
+        v[a] = d.vel[3 * i + a] * H.sc_v + hdt * d.Fv[3 * i + a] * cF;
+        w[a] = d.omg[3 * i + a] * H.sc_nh + hdt * d.Tv[3 * i + a] * cT;
+        r[a] = d.pos[3 * i + a];
+      }
+      const double *hi = H.hi;
+      double sx = hi[0] * r[0] + hi[1] * r[1] + hi[2] * r[2];
+      double sy = hi[3] * r[0] + hi[4] * r[1] + hi[5] * r[2];
+      double sz = hi[6] * r[0] + hi[7] * r[1] + hi[8] * r[2];
+      double vsx = hi[0] * v[0] + hi[1] * v[1] + hi[2] * v[2];
+      double vsy = hi[3] * v[0] + hi[4] * v[1] + hi[5] * v[2];
+      double vsz = hi[6] * v[0] + hi[7] * v[1] + hi[8] * v[2];
+      sx += p.dt * vsx;
+      sy += p.dt * vsy;
+      sz += p.dt * vsz;
+      sx -= floor(sx);
+      sy -= floor(sy);
+      sz -= floor(sz);
+      const double *h = H.h1;
+      d.pos[3 * i] = h[0] * sx + h[1] * sy + h[2] * sz;
+      d.pos[3 * i + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
+      d.pos[3 * i + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
+      for (int a = 0; a < 3; a++) {
+        d.vel[3 * i + a] = v[a];
+        d.omg[3 * i + a] = w[a];
+      }
+      for (int a = 0; a < 4; a++) q[a] = d.qv[4 * i + a];
+      quat_advance(q, w[0], w[1], w[2], p.dt);
+      for (int a = 0; a < 4; a++) {
+        d.qv[4 * i + a] = q[a];
+        sq[t][a] = q[a];
+      }
+    }
   }
-  const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= p.N) return;
-  double q[4] = {0, 0, 0, 0};
-  if (lane == 0) {
-    const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
-    double v[3], w[3], r[3];
-    for (int a = 0; a < 3; a++) {
-      v[a] = d.vel[3 * i + a] * H.sc_v + hdt * d.Fv[3 * i + a] * cF;
-      w[a] = d.omg[3 * i + a] * H.sc_nh + hdt * d.Tv[3 * i + a] * cT;
-      r[a] = d.pos[3 * i + a];
-    }
-    const double *hi = H.hi;
-    double sx = hi[0] * r[0] + hi[1] * r[1] + hi[2] * r[2];
-    double sy = hi[3] * r[0] + hi[4] * r[1] + hi[5] * r[2];
-    double sz = hi[6] * r[0] + hi[7] * r[1] + hi[8] * r[2];
-    double vsx = hi[0] * v[0] + hi[1] * v[1] + hi[2] * v[2];
-    double vsy = hi[3] * v[0] + hi[4] * v[1] + hi[5] * v[2];
-    double vsz = hi[6] * v[0] + hi[7] * v[1] + hi[8] * v[2];
-    sx += p.dt * vsx;
-    sy += p.dt * vsy;
-    sz += p.dt * vsz;
-    sx -= floor(sx);
-    sy -= floor(sy);
-    sz -= floor(sz);
-    const double *h = H.h1;
-    d.pos[3 * i] = h[0] * sx + h[1] * sy + h[2] * sz;
-    d.pos[3 * i + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
-    d.pos[3 * i + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
-    for (int a = 0; a < 3; a++) {
-      d.vel[3 * i + a] = v[a];
-      d.omg[3 * i + a] = w[a];
-    }
-    for (int a = 0; a < 4; a++) q[a] = d.qv[4 * i + a];
-    quat_advance(q, w[0], w[1], w[2], p.dt);
-    for (int a = 0; a < 4; a++) d.qv[4 * i + a] = q[a];
+  __syncthreads();
+  const int nm = min(kKdMol, p.N - i0), npad = p.natom_pad, nq = p.nq;
+  for (int k = t; k < nm * npad; k += 256) {
+    const int m = k / npad;
+    write_site(i0 + m, k - m * npad, sq[m], d.body, d.sites, d.sites_d, npad);
   }
-  for (int a = 0; a < 4; a++) q[a] = __shfl(q[a], 0, 64);
-  write_sites<NS>(i, q, d.body, d.sites, d.sites_d, p.natom_pad, lane);
-  write_quads(i, q, d.qbody, d.qcen, p.nq, lane);
+  for (int k = t; k < nm * nq; k += 256) {
+    const int m = k / nq;
+    write_quad(i0 + m, k - m * nq, sq[m], d.qbody, d.qcen, nq);
+  }
 }
 
 // ------------------------------------------------------------------ forces
@@ -929,6 +969,21 @@ __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, 
   double v[kBlkPart];
   for (int k = 0; k < kBlkPart; k++) v[k] = 0;
   if (i < p.N) {
+    // the kinematics and the second-half scale factors first: their latency
+    // overlaps the gather below
+    double vx = 0.0, vy = 0.0, vz = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+    if (lane == 0) {
+      vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
+      wx = d.omg[3 * i], wy = d.omg[3 * i + 1], wz = d.omg[3 * i + 2];
+    }
+    const double hdt = 0.5 * p.dt;
+    double sc_nh = 1.0, sc_v2 = 1.0;
+    if (kick) {
+      const RigidMid &M = *d.mid;
+      const double eps_tr2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
+      sc_nh = M.sc_nh;
+      sc_v2 = M.sc_nh * exp(-hdt * eps_tr2 / 3.0);
+    }
     double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (kick && p.pair_kernel) {
       // both sides from the pair slots: owned pairs (F, tau, energy, sum d.f)
@@ -960,8 +1015,6 @@ __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, 
       for (int c = 0; c < 6; c++) f[c] = wave_sum(f[c]);
     }
     if (lane == 0) {
-      double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
-      double wx = d.omg[3 * i], wy = d.omg[3 * i + 1], wz = d.omg[3 * i + 2];
       if (kick) {
         double F[3], T[3];
         for (int c = 0; c < 3; c++) {  // the pair kernel leaves both sides in f
@@ -973,16 +1026,13 @@ __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, 
             d.Fv[3 * i + c] = F[c];
             d.Tv[3 * i + c] = T[c];
           }
-        const RigidMid &M = *d.mid;
-        const double hdt = 0.5 * p.dt, cF = kConv / p.Mmol, cT = kConv / p.I0;
-        double eps_tr2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
-        double sc_v2 = M.sc_nh * exp(-hdt * eps_tr2 / 3.0);
+        const double cF = kConv / p.Mmol, cT = kConv / p.I0;
         vx = (vx + hdt * F[0] * cF) * sc_v2;
         vy = (vy + hdt * F[1] * cF) * sc_v2;
         vz = (vz + hdt * F[2] * cF) * sc_v2;
-        wx = (wx + hdt * T[0] * cT) * M.sc_nh;
-        wy = (wy + hdt * T[1] * cT) * M.sc_nh;
-        wz = (wz + hdt * T[2] * cT) * M.sc_nh;
+        wx = (wx + hdt * T[0] * cT) * sc_nh;
+        wy = (wy + hdt * T[1] * cT) * sc_nh;
+        wz = (wz + hdt * T[2] * cT) * sc_nh;
         d.vel[3 * i] = vx;
         d.vel[3 * i + 1] = vy;
         d.vel[3 * i + 2] = vz;
@@ -995,15 +1045,37 @@ __global__ __launch_bounds__(256) void k_rigid_kick2(RigidDev d, RigidParams p, 
       for (int k = 0; k < kMolPart; k++) v[2 + k] = d.molpart[(size_t)i * kMolPart + k];
     }
   }
-  block_sum12(v, d.blockpart + (size_t)blockIdx.x * kBlkPart);
+  // one molecule per wavefront, its values on lane 0: the block sum is the
+  // four wavefronts' lane-0 values in order (component-major blockpart)
+  __shared__ double red[4][kBlkPart];
+  const int wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < kBlkPart; k++) red[wv][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < kBlkPart) {
+    const int k = threadIdx.x;
+    d.blockpart[(size_t)k * d.nblk + blockIdx.x] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
 }
 
 // Fixed-order reduction of the block partials (one workgroup).
+// bp is component-major (component k of block b at k * nblk + b); every
+// thread issues the loads of four blocks before adding them.
 __device__ __forceinline__ void reduce_blocks(const double *bp, int nblk, double *sums) {
   double v[kBlkPart];
   for (int k = 0; k < kBlkPart; k++) v[k] = 0;
-  for (int b = threadIdx.x; b < nblk; b += 256)
-    for (int k = 0; k < kBlkPart; k++) v[k] += bp[(size_t)b * kBlkPart + k];
+  for (int b = threadIdx.x; b < nblk; b += 4 * 256) {
+    double x[4][kBlkPart];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int bu = b + 256 * u;
+#pragma unroll
+      for (int k = 0; k < kBlkPart; k++) x[u][k] = bu < nblk ? bp[(size_t)k * nblk + bu] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      for (int k = 0; k < kBlkPart; k++) v[k] += x[u][k];
+  }
   __shared__ double tot[kBlkPart];
   block_sum12(v, tot);
   __syncthreads();
@@ -1030,12 +1102,15 @@ __global__ __launch_bounds__(256) void k_rigid_observe(RigidDev d, RigidParams p
 // k_rigid_rescale; KE is rescaled by s^2 here).
 __global__ __launch_bounds__(256) void k_rigid_finish(RigidDev d, RigidParams p, int ctrl_slot,
                                                       int thermo_slot) {
+  // the state, midpoint and control are loaded before the reduction, so
+  // their latency overlaps it
+  RigidState &S = *d.state;
+  const RigidMid M = *d.mid;
+  const StepCtrl c = d.ctrl[ctrl_slot];
+  const double Nf = S.Nf, Q = S.Q, Pe = S.Pe, Wb = S.W;
   double sums[kBlkPart];
   reduce_blocks(d.blockpart, d.nblk, sums);
   if (threadIdx.x != 0) return;
-  RigidState &S = *d.state;
-  const RigidMid &M = *d.mid;
-  const StepCtrl c = d.ctrl[ctrl_slot];
   const double hdt = 0.5 * p.dt;
   double KEt = 0.5 * p.Mmol * sums[0] / kConv;
   double KEr = 0.5 * p.I0 * sums[1] / kConv;
@@ -1045,14 +1120,14 @@ __global__ __launch_bounds__(256) void k_rigid_finish(RigidDev d, RigidParams p,
   double Ep = sums[11];
   double xi = M.xi, Vg[9];
   for (int k = 0; k < 9; k++) Vg[k] = M.Vg[k];
-  xi += hdt * (2 * KE - S.Nf * kB * c.Tt) / S.Q;
+  xi += hdt * (2 * KE - Nf * kB * c.Tt) / Q;
   xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
   double V2 = fabs(det9(M.h));
   double P = (2 * KEt + W9[0] + W9[4] + W9[8]) / (3 * V2) * kEv2GPa;
-  double dP = P - S.Pe;
-  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * dP / (S.W * kEv2GPa);
+  double dP = P - Pe;
+  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * dP / (Wb * kEv2GPa);
   for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
-  double T = 2 * KE / (S.Nf * kB);
+  double T = 2 * KE / (Nf * kB);
   double scale = 1.0;
   if ((c.flags & kCtrlRescale) && T > 0.1) {
     double tgt = c.rescale_T > 0.1 ? c.rescale_T : 0.1;
@@ -1060,7 +1135,7 @@ __global__ __launch_bounds__(256) void k_rigid_finish(RigidDev d, RigidParams p,
     KEt *= scale * scale;
     KEr *= scale * scale;
     KE = KEt + KEr;
-    T = 2 * KE / (S.Nf * kB);
+    T = 2 * KE / (Nf * kB);
     xi = 0;
     if (c.flags & kCtrlRescaleZeroVg)
       for (int k = 0; k < 9; k++) Vg[k] = 0;
@@ -1204,11 +1279,7 @@ hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_s
     if ((e = rigid_launch_nlist(d, p, state_hh, s)) != hipSuccess) return e;
     if (p.use_half && (e = rigid_launch_halflist(d, p, state_hh, s)) != hipSuccess) return e;
   }
-  dim3 g((p.N + 3) / 4);
-  if (ns_of(p) == 1)
-    k_rigid_kick_drift<1><<<g, 256, 0, s>>>(d, p, ctrl_slot);
-  else
-    k_rigid_kick_drift<2><<<g, 256, 0, s>>>(d, p, ctrl_slot);
+  k_rigid_kick_drift<<<(p.N + kKdMol - 1) / kKdMol, 256, 0, s>>>(d, p, ctrl_slot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (ev0 && (e = hipEventRecordWithFlags(ev0, s, ev_flags)) != hipSuccess) return e;
   if (p.use_half)
