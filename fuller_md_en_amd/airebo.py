@@ -83,7 +83,7 @@ class AtomicEngine:
     """HBM-resident AIREBO NPT engine (fmd_atomic_create / fmd_engine_*)."""
 
     def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
-                 nlup=20, lj=None):
+                 nlup=20, lj=None, rebo_rlist=None):
         self._mol = _c(mol_id, np.int32)
         self._mass = _c(mass)
         cfg = N.AtomicConfig()
@@ -94,6 +94,8 @@ class AtomicEngine:
         cfg.dt, cfg.rebo_mode, cfg.lj_max_neigh, cfg.nlup = dt, mode, lj_max_neigh, nlup
         if lj is not None:
             cfg.lj = lj
+        if rebo_rlist is not None:
+            cfg.rebo_rlist = rebo_rlist
         self.cfg = cfg
         self.n = len(self._mol)
         h = C.c_void_p()

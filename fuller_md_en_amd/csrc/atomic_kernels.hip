@@ -12,8 +12,8 @@
 //                     (drop-in).  FP64 with the reference's operation order.
 //   k_at_kick_drift   NPT first half + half-kick + drift + cell update
 //                     (step_npt AB:522-568)
-//   k_rebo_pass1/2    REBO bond order (compute_rebo AB:275-421), see
-//                     fmd_rebo_passes.hpp
+//   k_rebo_mol        REBO bond order (compute_rebo AB:275-421), both passes
+//                     of fmd_rebo_passes.hpp, one workgroup per molecule
 //   k_at_lj           intermolecular LJ (compute_lj AB:424-473): one
 //                     256-thread workgroup per molecule, its 4 wavefronts
 //                     split the molecule list, lane = atom, neighbour atoms
@@ -480,65 +480,156 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 }
 
 // ------------------------------------------------------------------ REBO
-// Four threads per atom i (64 atoms per workgroup): thread q takes i's list
-// entries q, q + 4, ...; the bond vectors (FP64 minimum image) are formed
-// once per entry into LDS and read there by the bond-order sums of i's other
-// entries.
-__global__ __launch_bounds__(256) void k_rebo_pass1(AtomicDev d, AtomicParams p, HHRef hh) {
-  extern __shared__ double4 vec[];  // [64 atoms][maxr]
-  const int al = threadIdx.x >> 2, q = threadIdx.x & 3;
-  const int i = blockIdx.x * 64 + al;
-  const int n = i < p.Na ? d.nlc_r[i] : 0;
-  double4 *vi = vec + (size_t)al * p.maxr;
-  if (i < p.Na) {
-    const double *h = hh_of(hh, d.atom_rep[i]), *hi = h + 9;
-    for (int jn = q; jn < n; jn += 4) {
-      const int j = d.nll_r[(size_t)i * p.maxr + jn];
+// One 256-thread workgroup per molecule (the REBO list never leaves a
+// molecule), both passes of fmd_rebo_passes.hpp in one launch: pass 1 for
+// every directed entry of the molecule, a barrier, pass 2 one thread per
+// entry (entries packed atom after atom by prefix sums of the list counts,
+// entry e on thread e mod 256), a barrier, and each atom's force as the sum
+// of its entries' in list order.  Energy and virial: every thread's share,
+// one fixed-order workgroup sum.
+//
+// Fast path (the molecule has at most kReboSlots entries per atom on average
+// and the caller sized the LDS for it): every entry lives in LDS with local
+// atom indices, so the dependent lookups of pass 2 (reverse entry, the
+// centre's other bonds) are LDS reads.  Otherwise the entries go through the
+// global ReboBonds arrays in the reference list layout and the entry forces
+// through rebo_fe (the workgroup's barrier makes its own global writes
+// visible to it).  Both paths do the same arithmetic in the same order, so
+// the result does not depend on which one a molecule takes.
+constexpr int kReboSlots = 6;
+// LDS bytes per entry of the fast path: vec (double4), bp + bq (2 x double4),
+// bd + ba (2 x float4), flag, reverse entry, neighbour, owning atom
+constexpr int kReboSlotBytes = 32 + 64 + 32 + 16;
+constexpr size_t kReboFastLdsMax = 64 * 1024;
+
+__host__ __device__ inline size_t rebo_fast_lds(int apad) {
+  return (size_t)kReboSlots * apad * kReboSlotBytes + 4 * (2 * (size_t)apad + 1);
+}
+// global path: bond vectors of 64 atoms, prefix sums and owning atoms
+__host__ __device__ inline size_t rebo_slow_lds(int apad, int maxr) {
+  return (size_t)64 * maxr * sizeof(double4) + 4 * ((size_t)apad + 1 + (size_t)apad * maxr);
+}
+
+template <int MODE, bool TRACE>
+__global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, HHRef hh,
+                                                   int fast_ok) {
+  extern __shared__ double4 lds[];
+  __shared__ int wtot[4];
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int m = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+  if (m >= p.Nm) return;
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;  // n <= 256 (host check)
+  const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  // exclusive prefix sums of the list counts (one atom per thread)
+  const int c = t < n ? d.nlc_r[s0 + t] : 0;
+  int v = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) wtot[wv] = v;
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < wv; w++) off += wtot[w];
+  const int total = ((wtot[0] + wtot[1]) + wtot[2]) + wtot[3];
+  // the choice depends on the molecule alone (not on the tile width apad)
+  const bool fast = fast_ok && total <= kReboSlots * n;
+  double E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (fast) {
+    const int S = kReboSlots * p.apad;
+    double4 *vec = lds;
+    double *bp = (double *)(vec + S), *bq = bp + 4 * S;
+    float *bd = (float *)(bq + 4 * S), *ba = bd + 4 * S;
+    int *flag = (int *)(ba + 4 * S), *rev = flag + S, *nll = rev + S, *eat = nll + S;
+    int *nlc = eat + S, *pre = nlc + p.apad;
+    if (t < n) {
+      nlc[t] = c;
+      pre[t] = off + v - c;
+      for (int jn = 0; jn < c; jn++) eat[off + v - c + jn] = t;
+    }
+    __syncthreads();
+    const ReboBonds B{bd, ba, bp, bq, flag, rev};
+    const SlotPrefix slot{pre};
+    // local neighbours and bond vectors (FP64 minimum image)
+    for (int e = t; e < total; e += 256) {
+      const int xl = eat[e], i = s0 + xl;
+      const int j = d.nll_r[(size_t)i * p.maxr + (e - pre[xl])];
+      nll[e] = j - s0;
       double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
              dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
       mimg(dx, dy, dz, hi, h);
-      vi[jn] = make_double4(dx, dy, dz, sqrt(dx * dx + dy * dy + dz * dz));
+      vec[e] = make_double4(dx, dy, dz, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    __syncthreads();
+    for (int e = t; e < total; e += 256) {
+      const int xl = eat[e];
+      rebo_pass1<float>(xl, e - pre[xl], vec + pre[xl], nlc, nll, slot, B);
+    }
+    __syncthreads();
+    double *Fe = (double *)vec;  // pass 1 is done with vec
+    for (int e = t; e < total; e += 256) {
+      const int xl = eat[e];
+      double F[3] = {0, 0, 0};
+      rebo_pass2_entry<float, MODE, TRACE>(xl, e - pre[xl], nlc, nll, slot, B, F, E, W);
+      for (int k = 0; k < 3; k++) Fe[3 * e + k] = F[k];
+    }
+    __syncthreads();
+    if (t < n) {  // atom t, its c entries
+      double F[3] = {0, 0, 0};
+      for (int e = off + v - c; e < off + v; e++)
+        for (int k = 0; k < 3; k++) F[k] += Fe[3 * e + k];
+      for (int k = 0; k < 3; k++) d.Frebo[3 * ((size_t)s0 + t) + k] = F[k];
+    }
+  } else {
+    double4 *vg = lds;  // [64 atoms][maxr]
+    int *pre = (int *)(vg + 64 * p.maxr), *eat = pre + p.apad + 1;
+    if (t < n) {
+      pre[t] = off + v - c;
+      for (int jn = 0; jn < c; jn++) eat[off + v - c + jn] = t;
+    }
+    const SlotStride slot{p.maxr};
+    const int q = t & 3;
+    for (int base = 0; base < n; base += 64) {
+      const int xl = base + (t >> 2), i = s0 + xl;
+      const int cx = xl < n ? d.nlc_r[i] : 0;
+      double4 *vi = vg + (size_t)(t >> 2) * p.maxr;
+      for (int jn = q; jn < cx; jn += 4) {
+        const int j = d.nll_r[(size_t)i * p.maxr + jn];
+        double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
+               dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
+        mimg(dx, dy, dz, hi, h);
+        vi[jn] = make_double4(dx, dy, dz, sqrt(dx * dx + dy * dy + dz * dz));
+      }
+      __syncthreads();
+      for (int jn = q; jn < cx; jn += 4)
+        rebo_pass1<float>(i, jn, vi, d.nlc_r, d.nll_r, slot, d.bonds);
+      __syncthreads();
+    }
+    for (int e = t; e < total; e += 256) {
+      const int x = s0 + eat[e], yn = e - pre[eat[e]];
+      double F[3] = {0, 0, 0};
+      rebo_pass2_entry<float, MODE, TRACE>(x, yn, d.nlc_r, d.nll_r, slot, d.bonds, F, E, W);
+      for (int k = 0; k < 3; k++) d.rebo_fe[3 * slot(x, yn) + k] = F[k];
+    }
+    __syncthreads();
+    if (t < n) {  // atom t, its c entries
+      const int x = s0 + t;
+      double F[3] = {0, 0, 0};
+      for (int yn = 0; yn < c; yn++)
+        for (int k = 0; k < 3; k++) F[k] += d.rebo_fe[3 * slot(x, yn) + k];
+      for (int k = 0; k < 3; k++) d.Frebo[3 * (size_t)x + k] = F[k];
     }
   }
-  __syncthreads();
-  for (int jn = q; jn < n; jn += 4)
-    rebo_pass1<float>(i, jn, vi, d.nlc_r, d.nll_r, p.maxr, d.bonds);
-}
-
-// Four threads per atom x (integrator block of up to 256 atoms -> 1024
-// threads): thread q takes x's neighbour entries q, q + 4, ...; the four
-// partial sums are added in a fixed order.  TRACE: the virial trace only (on
-// the diagonal, a third each), what the time steps use.
-template <int MODE, bool TRACE>
-__global__ __launch_bounds__(1024, 8) void k_rebo_pass2(AtomicDev d, AtomicParams p) {
-  const int x = d.blk_a0[blockIdx.x] + (threadIdx.x >> 2), q = threadIdx.x & 3;
-  const bool ok = x < d.blk_a1[blockIdx.x];
-  double F[3] = {0, 0, 0}, E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (ok)
-    for (int yn = q; yn < d.nlc_r[x]; yn += 4)
-      rebo_pass2_entry<float, MODE, TRACE>(x, yn, d.nlc_r, d.nll_r, p.maxr, d.bonds, F, E, W);
-  auto quad = [](double v) {
-    v += dpp64<0xB1>(v);         // lane ^ 1
-    return v + dpp64<0x4E>(v);  // lane ^ 2
-  };
-  for (int k = 0; k < 3; k++) F[k] = quad(F[k]);
-  E = quad(E);
-  if (TRACE)
-    W[0] = quad(W[0]);
-  else
-    for (int k = 0; k < 9; k++) W[k] = quad(W[k]);
   double part[kAtPart];
-  for (int k = 0; k < kAtPart; k++) part[k] = 0;
-  if (ok && q == 0) {
-    for (int k = 0; k < 3; k++) d.Frebo[3 * x + k] = F[k];
-    if (TRACE) {
-      part[0] = part[4] = part[8] = W[0] / 3.0;
-    } else {
-      for (int k = 0; k < 9; k++) part[k] = W[k];
-    }
-    part[9] = E;
+  if (TRACE) {
+    for (int k = 0; k < 9; k++) part[k] = 0;
+    part[0] = part[4] = part[8] = W[0] / 3.0;
+  } else {
+    for (int k = 0; k < 9; k++) part[k] = W[k];
   }
-  block_sum<kAtPart, 16>(part, d.rebo_part + blockIdx.x, d.nblk);
+  part[9] = E;
+  block_sum<kAtPart>(part, d.rebo_mpart + m, p.Nm);
 }
 
 // -------------------------------------------------------------------- LJ
@@ -833,7 +924,7 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
   const int b0 = d.rep_blk[r], b1 = d.rep_blk[r + 1];
   for (int b = b0 + threadIdx.x; b < b1; b += kFinThreads) {
     v[0] += d.ke_part[b];
-    if (!MM) {
+    if (!MM && p.use_sg) {
       for (int k = 0; k < 9; k++) v[1 + k] += d.rebo_part[k * nb + b];
       v[10] += d.rebo_part[9 * nb + b];
     } else if (d.mm.charge) {
@@ -843,18 +934,24 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
   }
   const int m0 = d.rep_mol[r], m1 = d.rep_mol[r + 1];
   for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
-    double x[MM ? 23 : 10];
+    double x[MM ? kAtPart + 13 : 2 * kAtPart];
 #pragma unroll
     for (int k = 0; k < kAtPart; k++) x[k] = d.lj_part[k * nm + m];
     if (MM) {
 #pragma unroll
       for (int k = 0; k < 13; k++) x[kAtPart + k] = d.mm.part[k * nm + m];
+    } else if (!p.use_sg) {
+#pragma unroll
+      for (int k = 0; k < kAtPart; k++) x[kAtPart + k] = d.rebo_mpart[k * nm + m];
     }
     for (int k = 0; k < 9; k++) v[1 + k] += x[k];
     v[11] += x[9];
     if (MM) {
       for (int k = 0; k < 9; k++) v[1 + k] += x[kAtPart + k];
       for (int k = 0; k < 4; k++) v[12 + k] += x[kAtPart + 9 + k];
+    } else if (!p.use_sg) {
+      for (int k = 0; k < 9; k++) v[1 + k] += x[kAtPart + k];
+      v[10] += x[kAtPart + 9];
     }
   }
   __shared__ double tot[kAtSums];
@@ -1038,15 +1135,20 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef h
 
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
                           hipStream_t s) {
-  k_rebo_pass1<<<(p.Na + 63) / 64, 256, 64 * p.maxr * sizeof(double4), s>>>(d, p, hh);
+  if (p.apad > 256) return hipErrorInvalidValue;  // one thread per atom in the prefix sums
+  const size_t fast = rebo_fast_lds(p.apad), slow = rebo_slow_lds(p.apad, p.maxr);
+  const int fast_ok = fast <= kReboFastLdsMax;
+  const size_t shm = fast_ok && fast > slow ? fast : slow;
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  dim3 g(per * kXcd);
   if (mode == kReboPhysical && p.lj_trace)
-    k_rebo_pass2<kReboPhysical, true><<<d.nblk, 1024, 0, s>>>(d, p);
+    k_rebo_mol<kReboPhysical, true><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
   else if (mode == kReboPhysical)
-    k_rebo_pass2<kReboPhysical, false><<<d.nblk, 1024, 0, s>>>(d, p);
+    k_rebo_mol<kReboPhysical, false><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
   else if (p.lj_trace)
-    k_rebo_pass2<kReboCompat, true><<<d.nblk, 1024, 0, s>>>(d, p);
+    k_rebo_mol<kReboCompat, true><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
   else
-    k_rebo_pass2<kReboCompat, false><<<d.nblk, 1024, 0, s>>>(d, p);
+    k_rebo_mol<kReboCompat, false><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
   return hipGetLastError();
 }
 

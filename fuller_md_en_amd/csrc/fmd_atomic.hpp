@@ -21,9 +21,12 @@
 //   jmax_min               int[Nm]
 //   mol_nl_count / mol_nl  molecule neighbour list (ascending)
 //   nlc_r / nll_r          REBO list, reference layout (stride maxr = 12)
-//   bonds                  per directed REBO entry (fmd_rebo_passes.hpp)
-//   rebo_part, lj_part, ke_part  fixed-order partial sums, component-major
-//                          (component k of block / molecule r at k * count + r)
+//   bonds, rebo_fe         per directed REBO entry (fmd_rebo_passes.hpp), used
+//                          for molecules whose entries do not fit the LDS
+//   rebo_mpart, lj_part    fixed-order partial sums per molecule, and
+//   rebo_part, ke_part     per integrator block (rebo_part: the surrogate's),
+//                          component-major (component k of record r at
+//                          k * count + r)
 //
 // Replicas: R independent systems share every array, concatenated replica
 // after replica (atoms, molecules).  Molecule and atom ranges of replica r
@@ -102,7 +105,8 @@ struct AtomicDev {
   int *mol_nl_count, *mol_nl;
   int *nlc_r, *nll_r;
   ReboBonds bonds;
-  double *rebo_part, *lj_part, *ke_part;
+  double *rebo_part, *rebo_mpart, *lj_part, *ke_part;
+  double *rebo_fe;  // 3 x Na*maxr: per-entry REBO forces (global-memory REBO path)
   double *hh;  // 18 doubles for the stateless entry points
   AtomicState *state;
   AtomicMid *mid;
@@ -160,7 +164,7 @@ hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
                           int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
                           hipEvent_t ev1, unsigned ev_flags, hipStream_t s);
-// REBO forces alone into Frebo and rebo_part, in the given mode (needs
+// REBO forces alone into Frebo and rebo_mpart, in the given mode (needs
 // centres and the REBO list).
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
                           hipStream_t s);

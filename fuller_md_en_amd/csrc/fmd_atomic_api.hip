@@ -85,7 +85,8 @@ void free_atomic_dev(AtomicDev &d) {
                   d.center,    d.radius,    d.off,      d.qcen,      d.jmax,       d.jmax_min,
                   d.mol_nl_count,
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
-                  d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.lj_part, d.ke_part,
+                  d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.rebo_mpart,
+                  d.lj_part,   d.ke_part,   d.rebo_fe,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
                   d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
                   d.blk_rep,   d.rep_blk};
@@ -184,7 +185,9 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   AL(d.bonds.bq, 4 * ns);
   AL(d.bonds.bflag, ns);
   AL(d.bonds.brev, ns);
+  AL(d.rebo_fe, 3 * ns);
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
+  AL(d.rebo_mpart, Nm * kAtPart);
   AL(d.lj_part, Nm * kAtPart);
   AL(d.ke_part, d.nblk);
   AL(d.hh, 18);

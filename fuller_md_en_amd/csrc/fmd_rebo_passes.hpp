@@ -24,7 +24,8 @@ namespace fmd {
 enum : int { kReboCompat = 0, kReboPhysical = 1 };
 enum : int { kBondPair = 1, kBondThird = 2, kBondAngular = 4 };
 
-// Per directed-entry storage (slot = i * maxr + jn):
+// Per directed-entry storage (slot of entry jn of atom i, see SlotStride /
+// SlotPrefix):
 //   bd[4*slot]  dx dy dz r      FP32 (r_j - r_i, minimum image): angular terms
 //   ba[4*slot]  fc dfc coeff -  FP32 (coeff = 1/2 fc VA db/dGs)
 //   bp[4*slot]  vr dvr va dva   FP64 pair terms
@@ -41,20 +42,31 @@ struct ReboBonds {
   int *bflag, *brev;
 };
 
-// Entry (i -> j = nll[i maxr + jn]); vec[kn] = (dx, dy, dz, r) of i's entry
+// Where entry jn of atom i is stored: i * maxr + jn (the reference's list
+// layout) or, for a molecule staged in LDS, a prefix-sum offset + jn.
+struct SlotStride {
+  int maxr;
+  FMD_HD int operator()(int i, int jn) const { return i * maxr + jn; }
+};
+struct SlotPrefix {
+  const int *pre;
+  FMD_HD int operator()(int i, int jn) const { return pre[i] + jn; }
+};
+
+// Entry (i -> j = nll[slot(i, jn)]); vec[kn] = (dx, dy, dz, r) of i's entry
 // kn (minimum image r_k - r_i, FP64), so every bond vector of i is taken once.
-template <class R>
+template <class R, class Slot>
 FMD_HD void rebo_pass1(int i, int jn, const double4 *vec, const int *nlc, const int *nll,
-                       int maxr, ReboBonds B) {
+                       Slot slot, ReboBonds B) {
   using namespace rebo;
-  const int s = i * maxr + jn;
+  const int s = slot(i, jn);
   const int j = nll[s];
   const int nni = nlc[i];
   const double dx = vec[jn].x, dy = vec[jn].y, dz = vec[jn].z, rd = vec[jn].w;
   const R r = (R)rd;
   int rev = -1;
   for (int ln = 0; ln < nlc[j]; ln++)
-    if (nll[j * maxr + ln] == i) rev = j * maxr + ln;
+    if (nll[slot(j, ln)] == i) rev = slot(j, ln);
   B.brev[s] = rev;
   B.bd[4 * s] = (float)dx;
   B.bd[4 * s + 1] = (float)dy;
@@ -108,12 +120,12 @@ FMD_HD void rebo_pass1(int i, int jn, const double4 *vec, const int *nlc, const 
 // with x as the partner).  Summed over yn they are every force component
 // that lands on x.  F += force, E += energy share, W += virial share
 // (row-major; TRACE: W[0] += its trace only).
-template <class R, int MODE, bool TRACE>
-FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, int maxr,
+template <class R, int MODE, bool TRACE, class Slot>
+FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, Slot slot,
                              ReboBonds B, double *F, double &E, double *W) {
   using namespace rebo;
   const int nx = nlc[x];
-  const int s = x * maxr + yn;
+  const int s = slot(x, yn);
   const int fl = B.bflag[s];
   if (fl & kBondPair) {
     const int y = nll[s];
@@ -141,7 +153,7 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, int 
       const R coeff = B.ba[4 * s + 2];
       for (int tn = 0; tn < nx; tn++) {
         if (tn == yn) continue;
-        const int t = x * maxr + tn;
+        const int t = slot(x, tn);
         if (!(B.bflag[t] & kBondThird)) continue;
         const R dt[3] = {B.bd[4 * t], B.bd[4 * t + 1], B.bd[4 * t + 2]};
         R ft[3], fp[3];
@@ -175,7 +187,7 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, int 
   if (fl_cx & kBondThird) {
     const R fcx = B.ba[4 * sx], dfcx = B.ba[4 * sx + 1];
     for (int pn = 0; pn < nc; pn++) {
-      const int sp = c * maxr + pn;
+      const int sp = slot(c, pn);
       if (sp == sx || !(B.bflag[sp] & kBondAngular)) continue;
       const R dcp[3] = {B.bd[4 * sp], B.bd[4 * sp + 1], B.bd[4 * sp + 2]};
       R ft[3], fp[3];
@@ -189,7 +201,7 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, int 
   if (MODE == kReboPhysical && (fl_cx & kBondAngular)) {
     const R coeff = B.ba[4 * sx + 2];
     for (int tn = 0; tn < nc; tn++) {
-      const int st = c * maxr + tn;
+      const int st = slot(c, tn);
       if (st == sx || !(B.bflag[st] & kBondThird)) continue;
       const R dct[3] = {B.bd[4 * st], B.bd[4 * st + 1], B.bd[4 * st + 2]};
       R ft[3], fp[3];

@@ -166,6 +166,34 @@ def test_engine_forces_other_lattices_against_oracle(spec, lattice, nc):
     eng.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_rebo_lds_and_global_paths_agree(mode):
+    """The REBO kernel stages a molecule's list entries in LDS when they fit
+    six per atom on average, and works from the global entry arrays
+    otherwise.  A 2.6 A REBO list (nine entries per C60 atom: the extra ones
+    beyond the 2.0 A switch carry no force) sends every molecule down the
+    global path; forces, virial and energies must equal those of the 2.3 A
+    list (LDS path) up to summation order, and both the oracle's."""
+    pos, h, mol = crystal(2, 11, 0.02)
+    lists = (*oracle.nlist_rebo(pos, h, mol), *oracle.nlist_lj(pos, h, mol))
+    E0, F0, W0 = oracle.airebo_forces(pos, h, lists, mode)
+    out = []
+    for rl in (2.3, 2.6):
+        print(f"case mode={mode} rebo_rlist={rl}", flush=True)
+        eng = airebo.AtomicEngine(mol, np.full(len(pos), 12.011), mode=mode, rebo_rlist=rl)
+        eng.set_state(pos=pos, vel=np.zeros_like(pos), h=h,
+                      npt=airebo.make_npt(300.0, 0.0, len(pos)))
+        eng.prepare()
+        out.append(eng.forces())
+        assert eng.counters()[0] == 0
+        eng.close()
+    (Ea, Fa, Wa), (Eb, Fb, Wb) = out
+    assert rel_max(Fa, Fb) < 1e-12 and rel_max(Wa, Wb) < 1e-12
+    for a, b in zip(Ea, Eb):
+        assert rel(a, b) < 1e-12
+    assert rel_max(Fa, F0) < TOL and rel_max(Wa, W0) < TOL
+
+
 def test_engine_trajectory_matches_reference():
     g = golden("airebo_nc1.npz")
     eng = _engine(g["pos"], g["h"], g["mol_id"], vel=g["vel"])
