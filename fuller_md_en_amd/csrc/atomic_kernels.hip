@@ -7,9 +7,11 @@
 //                     AB:251-261; rebuild)
 //   k_at_scan_half    the reference's LJ half list walk (build_nlist_lj
 //                     AB:263-272): per atom, ascending j > i of other
-//                     molecules within rc + 2, first 400 kept.  Writes the
-//                     truncation point jmax (engine) or the list itself
-//                     (drop-in).  FP64 with the reference's operation order.
+//                     molecules within rc + 2, first 400 kept; writes the
+//                     list itself (drop-in).  FP64 with the reference's
+//                     operation order.
+//   k_at_scan_jmax    the same walk's truncation point jmax (engine): counts
+//                     per neighbour molecule over a workgroup, then one walk
 //   k_at_kick_drift   NPT first half + half-kick + drift + cell update
 //                     (step_npt AB:522-568)
 //   k_rebo_mol        REBO bond order (compute_rebo AB:275-421), both passes
@@ -260,6 +262,20 @@ __global__ __launch_bounds__(256) void k_at_rebo_nlist(AtomicDev d, AtomicParams
 }
 
 // ------------------------------------------------- LJ half-list walk
+// Atom j (tile offset rb) inside atom i's list radius: with the FP32 tiles
+// (c = the centres' minimum image minus i's offset) and the reference's FP64
+// test within 1e-5 relative of the radius, or the FP64 test alone.
+__device__ __forceinline__ bool scan_in(bool fast, float4 rb, float cx, float cy, float cz,
+                                        float lo, float hi2, const double *pos, int i, int j,
+                                        const double *h, const double *hi, double rl2) {
+  if (fast) {
+    const float ex = rb.x + cx, ey = rb.y + cy, ez = rb.z + cz;
+    const float r2 = ex * ex + ey * ey + ez * ez;
+    return r2 < lo || (r2 < hi2 && r2_ref(pos, i, j, h, hi) < rl2);
+  }
+  return r2_ref(pos, i, j, h, hi) < rl2;
+}
+
 // WRITE = 0: truncation point jmax per atom (tile layout) and jmax_min;
 // WRITE = 1: the reference-layout half list itself (nlc_l / nll_l).
 template <int NS, int WRITE>
@@ -324,15 +340,7 @@ __global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams 
       for (int s = 0; s < NS; s++) {
         if (done[s]) continue;
         const int i = s0 + lane + 64 * s;
-        bool in;
-        if (fast) {
-          const float ex = rb.x + cx[s], ey = rb.y + cy[s], ez = rb.z + cz[s];
-          const float r2 = ex * ex + ey * ey + ez * ez;
-          in = r2 < lo || (r2 < hi2 && r2_ref(d.pos, i, j, h, hi) < p.lj_rlist2);
-        } else {
-          in = r2_ref(d.pos, i, j, h, hi) < p.lj_rlist2;
-        }
-        if (in) {
+        if (scan_in(fast, rb, cx[s], cy[s], cz[s], lo, hi2, d.pos, i, j, h, hi, p.lj_rlist2)) {
           if (WRITE) nll_l[(size_t)i * cap + cnt[s]] = j;
           cnt[s]++;
           if (cnt[s] == cap) {
@@ -363,6 +371,162 @@ __global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams 
   }
   mn = wave_min(mn);
   if (!WRITE && lane == 0) d.jmax_min[m] = mn;
+}
+
+// Truncation points jmax (engine path), the walk split in three phases so that
+// a molecule's work spreads over a workgroup instead of one wavefront:
+//  1. the four wavefronts take the molecule list's entries round robin and
+//     count, per atom i of molecule m, the atoms of neighbour molecule q > m
+//     inside the list radius (counts in LDS, one byte each);
+//  2. per atom, the counts in ascending q give the molecule that holds its
+//     cap-th entry;
+//  3. that molecule's atoms are walked in ascending order to the cap-th one.
+// The radius test is scan_in, the one k_at_scan_half uses, so the decisions
+// are the same.
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_scan_jmax(AtomicDev d, AtomicParams p, HHRef hh) {
+  extern __shared__ unsigned char cnt_l[];  // [mol_cap][NS * 64]
+  __shared__ int mnw[NS];
+  constexpr int NA = NS * 64;
+  const int m = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = uniform(threadIdx.x >> 6);
+  const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
+  const bool fast = p.scan_fast;
+  const float rl2 = (float)p.lj_rlist2, lo = rl2 * (1.0f - 1e-5f), hi2 = rl2 * (1.0f + 1e-5f);
+  const float rl = sqrtf(rl2);
+  const float reach_m = sqrtf(hi2) + d.radius[m] + 0.05f;
+  const double cix = d.center[3 * m], ciy = d.center[3 * m + 1], ciz = d.center[3 * m + 2];
+  float oax[NS], oay[NS], oaz[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    const float4 o = a < n ? d.off[(size_t)m * p.apad + a] : make_float4(3e5f, 3e5f, 3e5f, 0.f);
+    oax[s] = o.x;
+    oay[s] = o.y;
+    oaz[s] = o.z;
+  }
+  const int nq = d.mol_nl_count[m];
+  const int *lq = d.mol_nl + (size_t)m * p.mol_cap;
+  for (int k = wv; k < nq; k += 4) {
+    const int q = uniform(lq[k]);
+    int c[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) c[s] = 0;
+    float cx[NS], cy[NS], cz[NS];
+    bool near = q > m;  // molecule ranges ascend: q < m holds only atoms < i
+    const int nq4 = p.apad >> 2;
+    unsigned long long qm = nq4 == 64 ? ~0ull : (1ull << nq4) - 1ull;
+    if (near && fast) {
+      near = false;
+      double dx = d.center[3 * q] - cix, dy = d.center[3 * q + 1] - ciy,
+             dz = d.center[3 * q + 2] - ciz;
+      mimg(dx, dy, dz, hi, h);
+      const float reach = rl + d.radius[q] + 0.05f;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        cx[s] = (float)(dx - (double)oax[s]);
+        cy[s] = (float)(dy - (double)oay[s]);
+        cz[s] = (float)(dz - (double)oaz[s]);
+        near |= lane + 64 * s < n && cx[s] * cx[s] + cy[s] * cy[s] + cz[s] * cz[s] < reach * reach;
+      }
+      near = __any(near);
+      // groups of four atoms of q within the list radius of some atom of m
+      // (radius with its FP32 band, plus m's radius and the group's)
+      bool qn = false;
+      if (lane < nq4) {
+        const float4 g = d.qcen[(size_t)q * nq4 + lane];
+        const float gx = (float)dx + g.x, gy = (float)dy + g.y, gz = (float)dz + g.z;
+        const float rr = reach_m + g.w;
+        qn = gx * gx + gy * gy + gz * gz < rr * rr;
+      }
+      qm = __ballot(qn);
+    }
+    if (near) {
+      const int t0 = d.mol_start[q], tn = d.mol_start[q + 1] - t0;
+      const float4 *tq = d.off + (size_t)q * p.apad;
+      while (qm) {
+        const int b4 = 4 * __builtin_ctzll(qm);
+        qm &= qm - 1ull;
+        float4 rb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) rb[u] = tq[b4 + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (b4 + u >= tn) break;
+#pragma unroll
+          for (int s = 0; s < NS; s++)
+            if (lane + 64 * s < n)
+              c[s] += scan_in(fast, rb[u], cx[s], cy[s], cz[s], lo, hi2, d.pos,
+                              s0 + lane + 64 * s, t0 + b4 + u, h, hi, p.lj_rlist2);
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) cnt_l[(size_t)k * NA + lane + 64 * s] = (unsigned char)c[s];
+  }
+  __syncthreads();
+  if (wv < NS) {
+    const int a = lane + 64 * wv;
+    int jm = INT_MAX;
+    if (a < n) {
+      const int cap = p.lj_cap;
+      int cum = 0, ks = -1;
+      // eight counts per round: their LDS reads are issued together
+      for (int k0 = 0; k0 < nq && ks < 0; k0 += 8) {
+        int ck[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) ck[u] = k0 + u < nq ? cnt_l[(size_t)(k0 + u) * NA + a] : 0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (ks < 0) {
+            if (cum + ck[u] >= cap)
+              ks = k0 + u;
+            else
+              cum += ck[u];
+          }
+        }
+      }
+      if (ks >= 0) {
+        const int q = lq[ks];
+        const int t0 = d.mol_start[q], tn = d.mol_start[q + 1] - t0;
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        if (fast) {
+          double dx = d.center[3 * q] - cix, dy = d.center[3 * q + 1] - ciy,
+                 dz = d.center[3 * q + 2] - ciz;
+          mimg(dx, dy, dz, hi, h);
+          const float ox = wv == 0 ? oax[0] : oax[NS - 1], oy = wv == 0 ? oay[0] : oay[NS - 1],
+                      oz = wv == 0 ? oaz[0] : oaz[NS - 1];
+          cx = (float)(dx - (double)ox);
+          cy = (float)(dy - (double)oy);
+          cz = (float)(dz - (double)oz);
+        }
+        const float4 *tq = d.off + (size_t)q * p.apad;
+        // four tile loads per round, then the tests in ascending order
+        for (int b0 = 0; b0 < tn && jm == INT_MAX; b0 += 4) {
+          float4 rb[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) rb[u] = tq[min(b0 + u, tn - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (jm == INT_MAX && b0 + u < tn &&
+                scan_in(fast, rb[u], cx, cy, cz, lo, hi2, d.pos, s0 + a, t0 + b0 + u, h, hi,
+                        p.lj_rlist2) &&
+                ++cum == cap)
+              jm = t0 + b0 + u;
+        }
+      }
+    }
+    if (a < p.apad) d.jmax[(size_t)m * p.apad + a] = jm;
+    const int mn = wave_min(jm);
+    if (lane == 0) mnw[wv] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int mn = mnw[0];
+    for (int s = 1; s < NS; s++) mn = min(mn, mnw[s]);
+    d.jmax_min[m] = mn;
+  }
 }
 
 // jmax from a reference-layout half list (the drop-in path).
@@ -1101,9 +1265,9 @@ hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
   }
   if (p.lj_compat) {
     if (one)
-      k_at_scan_half<1, 0><<<g, 256, 0, s>>>(d, p, hh, nullptr, nullptr, p.lj_cap);
+      k_at_scan_jmax<1><<<p.Nm, 256, (size_t)p.mol_cap * 64, s>>>(d, p, hh);
     else
-      k_at_scan_half<2, 0><<<g, 256, 0, s>>>(d, p, hh, nullptr, nullptr, p.lj_cap);
+      k_at_scan_jmax<2><<<p.Nm, 256, (size_t)p.mol_cap * 128, s>>>(d, p, hh);
   } else {
     const size_t n = (size_t)p.Nm * p.apad;
     k_at_fill_int<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d.jmax, INT_MAX, n);
