@@ -1047,6 +1047,347 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
   }
 }
 
+// ------------------------------------------------- LJ per-atom lists
+// Engines on cells with one image per molecule pair keep, per tile slot a of
+// molecule i, a list of the atoms within rc + 1 A at the list rebuild, so the
+// step evaluates about 1.4 candidates per pair in range instead of the 3.9 of
+// the tile loops of k_at_lj.  The list carries the reference's half-list
+// truncation (the PRED test of lj_group, applied once at the rebuild).  The
+// build stages each neighbour's set-up in LDS and its four wavefronts take
+// every neighbour's live four-atom groups by rank (k + r = w mod 4), each
+// writing its own segment of the atom's list, which the same wavefront of the
+// step then walks.  (Splitting the molecule list itself round robin is
+// unbalanced: lattice order can put most near neighbours in one residue of
+// k mod 4.)
+constexpr int kAlWaves = 4;
+constexpr int kAlMolCap = 160;  // the molecule list stride these kernels stage in LDS
+// padding entry: neighbour slot 255 (a far-away centre offset in the step's
+// table, tile slot 0 of molecule 0), never within the cutoff
+constexpr unsigned kAlPad = 255u << 8;
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p, HHRef hh) {
+  // per neighbour k: centre offset (FP64 minimum image), live four-atom
+  // groups (0: none within reach), molecule, first atom, atom count, PRED
+  __shared__ double dmk[kAlMolCap][3];
+  __shared__ unsigned long long qmk[kAlMolCap];
+  __shared__ int mjk[kAlMolCap], t0k[kAlMolCap], tnk[kAlMolCap], prk[kAlMolCap];
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+  if (mi >= p.Nm) return;
+  constexpr int NA = NS * 64;
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  const double *h = hh_of(hh, d.mol_rep[mi]), *hi = h + 9;
+  const int apad = p.apad;
+  const int s0 = d.mol_start[mi], n = d.mol_start[mi + 1] - s0;
+  const double cix = d.center[3 * mi], ciy = d.center[3 * mi + 1], ciz = d.center[3 * mi + 2];
+  float rax[NS], ray[NS], raz[NS];
+  int ia[NS], iam[NS], cnt[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < n) {
+      const float4 r = d.off[(size_t)mi * apad + a];
+      rax[s] = r.x;
+      ray[s] = r.y;
+      raz[s] = r.z;
+      iam[s] = d.jmax[(size_t)mi * apad + a];
+    } else {
+      rax[s] = ray[s] = raz[s] = -3.0e5f;  // never within reach
+      iam[s] = INT_MAX;
+    }
+    ia[s] = s0 + a;
+  }
+  const float rl2 = p.al_r2, rl = sqrtf(rl2);
+  const float reach_i = rl + d.radius[mi] + 0.05f;
+  const int i_last = s0 + n - 1;
+  const int jmin_i = d.jmax_min[mi];
+  const int nq = d.mol_nl_count[mi];
+  const int *lq = d.mol_nl + (size_t)mi * p.mol_cap;
+  const int cap = p.al_cap;
+  const int nq4 = apad >> 2;
+  // neighbour set-up, once per neighbour (wavefront w takes k = w, w + 4, ...)
+  for (int k = wv; k < nq; k += kAlWaves) {
+    const int mj = uniform(lq[k]);
+    double dx = d.center[3 * mj] - cix, dy = d.center[3 * mj + 1] - ciy,
+           dz = d.center[3 * mj + 2] - ciz;
+    mimg(dx, dy, dz, hi, h);
+    const float rcut = rl + d.radius[mj] + 0.05f;
+    bool act = false;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const float cx = (float)(dx - (double)rax[s]), cy = (float)(dy - (double)ray[s]),
+                  cz = (float)(dz - (double)raz[s]);
+      act |= cx * cx + cy * cy + cz * cz < rcut * rcut;
+    }
+    bool near = false;
+    if (lane < nq4) {
+      const float4 c = d.qcen[(size_t)mj * nq4 + lane];
+      const float qx = (float)dx + c.x, qy = (float)dy + c.y, qz = (float)dz + c.z;
+      const float rr = reach_i + c.w;
+      near = qx * qx + qy * qy + qz * qz < rr * rr;
+    }
+    const unsigned long long qm = __ballot(near);
+    const bool any_act = __any(act);
+    if (lane == 0) {
+      const int t0 = d.mol_start[mj], tn = d.mol_start[mj + 1] - t0;
+      dmk[k][0] = dx;
+      dmk[k][1] = dy;
+      dmk[k][2] = dz;
+      qmk[k] = any_act ? qm : 0ull;
+      mjk[k] = mj;
+      t0k[k] = t0;
+      tnk[k] = tn;
+      const bool j_above = mj > mi;
+      prk[k] = p.lj_compat && (j_above ? (t0 + tn - 1 > jmin_i) : (i_last > d.jmax_min[mj]));
+    }
+  }
+  __syncthreads();
+  // every pair kept by the list among this wavefront's share: of each
+  // neighbour's live groups, those of rank w, w + 4, ...
+  auto walk = [&](auto emit) {
+    for (int k = 0; k < nq; k++) {
+      unsigned long long qm = qmk[k];
+      if (!qm) continue;
+      const int mj = mjk[k], t0 = t0k[k], tn = tnk[k];
+      const bool pred = prk[k], j_above = mj > mi;
+      float cx[NS], cy[NS], cz[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        cx[s] = (float)(dmk[k][0] - (double)rax[s]);
+        cy[s] = (float)(dmk[k][1] - (double)ray[s]);
+        cz[s] = (float)(dmk[k][2] - (double)raz[s]);
+      }
+      const float4 *sj = d.off + (size_t)mj * apad;
+      const int *jj = d.jmax + (size_t)mj * apad;
+      // rank offset by k: the wavefront that takes a neighbour's extra group
+      // rotates
+      for (int r = k; qm; r++) {
+        const int b4 = 4 * __builtin_ctzll(qm);
+        qm &= qm - 1ull;
+        if ((r & (kAlWaves - 1)) != wv) continue;
+        float4 rb[4];
+        int jm[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          rb[u] = sj[b4 + u];  // slots past the atoms hold far-away sentinels
+          jm[u] = pred ? jj[b4 + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int b = b4 + u;
+          if (b >= tn) break;
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            const float ex = rb[u].x + cx[s], ey = rb[u].y + cy[s], ez = rb[u].z + cz[s];
+            bool in = ex * ex + ey * ey + ez * ez <= rl2;
+            if (pred) in = in && (j_above ? (t0 + b <= iam[s]) : (ia[s] <= jm[u]));
+            if (in) emit(s, k << 8 | b);
+          }
+        }
+      }
+    }
+  };
+  // the entries go out eight at a time (one 16-byte block per lane) into
+  // this wavefront's segment of the atom's list, padded to whole blocks
+  const int nbk = cap >> 3;  // blocks per segment
+  uint4 *out = reinterpret_cast<uint4 *>(d.al_e) + ((size_t)mi * kAlWaves + wv) * nbk * NA;
+  unsigned bw[NS][4];
+  int fill[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    cnt[s] = 0;
+    fill[s] = 0;
+    for (int k = 0; k < 4; k++) bw[s][k] = 0;
+  }
+  auto push = [&](int s, unsigned en) {
+    // shift the block right by one entry, the new one entering at the top
+    bw[s][0] = (bw[s][0] >> 16) | (bw[s][1] << 16);
+    bw[s][1] = (bw[s][1] >> 16) | (bw[s][2] << 16);
+    bw[s][2] = (bw[s][2] >> 16) | (bw[s][3] << 16);
+    bw[s][3] = (bw[s][3] >> 16) | (en << 16);
+    if (++fill[s] == 8) {
+      if (cnt[s] < nbk)
+        out[(size_t)cnt[s] * NA + lane + 64 * s] = make_uint4(bw[s][0], bw[s][1], bw[s][2], bw[s][3]);
+      cnt[s]++;
+      fill[s] = 0;
+    }
+  };
+  walk([&](int s, int en) { push(s, (unsigned)en); });
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    while (fill[s] != 0) push(s, kAlPad);
+    d.al_n[((size_t)mi * kAlWaves + wv) * NA + lane + 64 * s] =
+        (unsigned short)(cnt[s] < nbk ? cnt[s] : nbk);
+    if (cnt[s] > nbk) atomicAdd(&d.flags[2], 1);
+  }
+}
+
+// LJ from the per-atom lists: lane = atom slot of molecule i, wavefront w
+// walks segment w of the atom's list; each block is one 16-byte load of eight
+// entries, taken four at a time (the neighbours' centre offsets from LDS,
+// then their tile slots, then the pairs).  Pair vector
+// r_b - r_a = (off_b - off_a) + (centre_j - centre_i) in FP32 with the
+// centres' minimum image from FP64; the cutoff is the reference's (FP64
+// re-test within 1e-5 of it).  Sums: FP32 over 32 entries, then FP64.
+// Virial: 1/2 sum over ordered pairs of fm d (x) d (trace only on time steps).
+template <int NS, bool TRACE>
+__global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, HHRef hh) {
+  __shared__ float4 dmk[256];  // centre offset of neighbour k, its tile's byte offset
+  __shared__ int j0k[256];      // its first atom
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+  if (mi >= p.Nm) return;
+  constexpr int NA = NS * 64;
+  constexpr int NW = TRACE ? 1 : 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  const double *h = hh_of(hh, d.mol_rep[mi]), *hi = h + 9;
+  const int apad = p.apad;
+  const int s0 = d.mol_start[mi], n = d.mol_start[mi + 1] - s0;
+  const int nq = d.mol_nl_count[mi];
+  {
+    const int *lq = d.mol_nl + (size_t)mi * p.mol_cap;
+    const double cix = d.center[3 * mi], ciy = d.center[3 * mi + 1], ciz = d.center[3 * mi + 2];
+    for (int t = threadIdx.x; t < nq; t += 256) {
+      const int q = lq[t];
+      double dx = d.center[3 * q] - cix, dy = d.center[3 * q + 1] - ciy,
+             dz = d.center[3 * q + 2] - ciz;
+      mimg(dx, dy, dz, hi, h);
+      dmk[t] = make_float4((float)dx, (float)dy, (float)dz,
+                           __uint_as_float((unsigned)(q * apad) * (unsigned)sizeof(float4)));
+      j0k[t] = d.mol_start[q];
+    }
+    if (threadIdx.x == 0) {
+      dmk[kAlPad >> 8] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, __uint_as_float(0u));
+      j0k[kAlPad >> 8] = 0;
+    }
+  }
+  __syncthreads();
+  const LJ32 lj = p.lj;
+  const float4 *off = d.off;
+  double GA[NS][3], WA[NS][NW], EA[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    for (int k = 0; k < 3; k++) GA[s][k] = 0;
+    for (int k = 0; k < NW; k++) WA[s][k] = 0;
+    EA[s] = 0;
+    // the segment of the atom's list this wavefront's build counterpart wrote
+    const int cnt = a < n ? d.al_n[((size_t)mi * kAlWaves + wv) * NA + a] : 0;
+    const uint4 *lst = reinterpret_cast<const uint4 *>(d.al_e) +
+                       ((size_t)mi * kAlWaves + wv) * (p.al_cap >> 3) * NA + a;
+    float rx = 0.f, ry = 0.f, rz = 0.f;
+    if (a < n) {
+      const float4 r = off[(size_t)mi * apad + a];
+      rx = r.x;
+      ry = r.y;
+      rz = r.z;
+    }
+    const int ia = s0 + a;
+    float g[3] = {0.f, 0.f, 0.f}, e = 0.f, w[NW];
+    for (int k = 0; k < NW; k++) w[k] = 0.f;
+    auto flush = [&]() {
+      for (int k = 0; k < 3; k++) {
+        GA[s][k] += (double)g[k];
+        g[k] = 0.f;
+      }
+      for (int k = 0; k < NW; k++) {
+        WA[s][k] += (double)w[k];
+        w[k] = 0.f;
+      }
+      EA[s] += (double)e;
+      e = 0.f;
+    };
+    const char *offb = reinterpret_cast<const char *>(off);
+    for (int ib = 0; ib < cnt; ib++) {
+      const uint4 blk = lst[(size_t)ib * NA];
+      const unsigned wd[4] = {blk.x, blk.y, blk.z, blk.w};
+#pragma unroll
+      for (int h4 = 0; h4 < 2; h4++) {
+        // four entries, their centre offsets, then their tile slots: each
+        // group of loads in flight
+        unsigned en[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) en[u] = (wd[2 * h4 + (u >> 1)] >> (16 * (u & 1))) & 0xffffu;
+        float4 dm[4], rb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) dm[u] = dmk[en[u] >> 8];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          rb[u] = *reinterpret_cast<const float4 *>(
+              offb + (__float_as_uint(dm[u].w) + ((en[u] & 255u) << 4)));
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const float ex = (rb[u].x - rx) + dm[u].x, ey = (rb[u].y - ry) + dm[u].y,
+                      ez = (rb[u].z - rz) + dm[u].z;
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          bool in = r2 <= lj.rc2_hi;
+          // within FP32 rounding of the cutoff: the reference's own FP64 test
+          if (in && r2 > lj.rc2_lo)
+            in = r2_ref(d.pos, ia, j0k[en[u] >> 8] + (int)(en[u] & 255u), h, hi) <= lj.rc2d;
+          if (in) {
+            const float fm = lj_fm(r2, lj, e);
+            g[0] += fm * ex;
+            g[1] += fm * ey;
+            g[2] += fm * ez;
+            if (TRACE) {
+              w[0] += fm * r2;
+            } else {
+              w[0] += fm * ex * ex;
+              w[1] += fm * ex * ey;
+              w[2] += fm * ex * ez;
+              w[3] += fm * ey * ey;
+              w[4] += fm * ey * ez;
+              w[5] += fm * ez * ez;
+            }
+          }
+        }
+      }
+      if ((ib & 3) == 3) flush();  // every 32 entries
+    }
+    flush();
+  }
+  __shared__ double fsum[4][2 * 64][3];
+  double acc[10];
+  for (int k = 0; k < 10; k++) acc[k] = 0;
+  const double c48 = 48.0 * p.eps;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    for (int k = 0; k < 3; k++) fsum[wv][lane + 64 * s][k] = -c48 * GA[s][k];
+    if (TRACE) {
+      const double tr = 0.5 * c48 * WA[s][0];
+      acc[0] += tr / 3.0;
+      acc[4] += tr / 3.0;
+      acc[8] += tr / 3.0;
+    } else {
+      const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+      for (int q = 0; q < 9; q++) acc[q] += 0.5 * c48 * WA[s][map[q]];
+    }
+    acc[9] += EA[s];
+  }
+  __shared__ double red[4][10];
+  for (int k = 0; k < 10; k++) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = v;
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < n; a += 256) {
+    const int x = s0 + a;
+    for (int k = 0; k < 3; k++) {
+      const double f = ((fsum[0][a][k] + fsum[1][a][k]) + fsum[2][a][k]) + fsum[3][a][k];
+      d.F[3 * x + k] = d.Frebo[3 * x + k] + f;
+    }
+  }
+  if (threadIdx.x < 10) {
+    const int k = threadIdx.x;
+    double v = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    if (k == 9) v *= 2.0 * p.eps;  // 1/2 * 4 eps * sum of reduced energies
+    d.lj_part[(size_t)k * p.Nm + mi] = v;
+  }
+}
+
 // ------------------------------------------------------- kick 2 + partials
 __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, int kick) {
   const int i = d.blk_a0[blockIdx.x] + threadIdx.x;
@@ -1273,6 +1614,14 @@ hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
     k_at_fill_int<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d.jmax, INT_MAX, n);
     k_at_fill_int<<<(p.Nm + 255) / 256, 256, 0, s>>>(d.jmax_min, INT_MAX, p.Nm);
   }
+  if (p.lj_alist && !p.small_box) {
+    if (p.mol_cap > kAlMolCap || p.mol_cap > 256 || p.apad > 256) return hipErrorInvalidValue;
+    const int per = (p.Nm + kXcd - 1) / kXcd;
+    if (one)
+      k_at_lj_alist<1><<<per * kXcd, 256, 0, s>>>(d, p, hh);
+    else
+      k_at_lj_alist<2><<<per * kXcd, 256, 0, s>>>(d, p, hh);
+  }
   return hipGetLastError();
 }
 
@@ -1339,6 +1688,18 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
       k_at_lj<1, true, false><<<g, 256, 0, s>>>(d, p, hh);
     else
       k_at_lj<2, true, false><<<g, 256, 0, s>>>(d, p, hh);
+  } else if (p.lj_alist) {
+    if (p.lj_trace) {
+      if (one)
+        k_at_lj_al<1, true><<<g, 256, 0, s>>>(d, p, hh);
+      else
+        k_at_lj_al<2, true><<<g, 256, 0, s>>>(d, p, hh);
+    } else {
+      if (one)
+        k_at_lj_al<1, false><<<g, 256, 0, s>>>(d, p, hh);
+      else
+        k_at_lj_al<2, false><<<g, 256, 0, s>>>(d, p, hh);
+    }
   } else if (p.lj_trace) {
     if (one)
       k_at_lj<1, false, true><<<g, 256, 0, s>>>(d, p, hh);

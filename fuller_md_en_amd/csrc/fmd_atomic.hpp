@@ -21,6 +21,15 @@
 //   jmax_min               int[Nm]
 //   mol_nl_count / mol_nl  molecule neighbour list (ascending)
 //   nlc_r / nll_r          REBO list, reference layout (stride maxr = 12)
+//   al_e / al_n            per-atom LJ lists (engines): for tile slot a of
+//                          molecule i, 16-bit entries k << 8 | b (atom slot b
+//                          of the k-th neighbour in mol_nl) of the pairs
+//                          within rc + 1 A that the reference's truncated half
+//                          list keeps, in four segments (one per wavefront) of
+//                          blocks of eight (16 bytes, padded with a never-near
+//                          entry): block c of segment w of slot a at
+//                          ((i * 4 + w) * al_cap / 8 + c) * NA + a (NA = 64 NS);
+//                          al_n counts blocks per segment
 //   bonds, rebo_fe         per directed REBO entry (fmd_rebo_passes.hpp), used
 //                          for molecules whose entries do not fit the LDS
 //   rebo_mpart, lj_part    fixed-order partial sums per molecule, and
@@ -83,6 +92,10 @@ struct AtomicParams {
   int trace_virial;  // 1: time steps keep only the LJ virial trace (engines)
   int lj_trace;      // set per launch: this force evaluation keeps only the
                      // virial trace (LJ and REBO)
+  int lj_alist;    // 1: LJ from per-atom lists (engines; cells with one image
+                   // per molecule pair); 0: every tile pair each step (drop-ins)
+  int al_cap;      // per-atom list capacity per wavefront segment (entries)
+  float al_r2;     // per-atom list radius^2: (rc + 1 A)^2
   int scan_fast;   // 1: list walk prefiltered in FP32 through the molecule
                    // tiles (cell wide enough for the list radius + 2 R_max)
   double rebo_rlist2;  // REBO list radius^2 (2.3^2, AB:253)
@@ -107,12 +120,15 @@ struct AtomicDev {
   ReboBonds bonds;
   double *rebo_part, *rebo_mpart, *lj_part, *ke_part;
   double *rebo_fe;  // 3 x Na*maxr: per-entry REBO forces (global-memory REBO path)
+  unsigned short *al_e;  // per-atom LJ lists: [Nm][4][al_cap / 8][NA] blocks of 8 entries
+  unsigned short *al_n;  // segment lengths in blocks: [Nm][4][NA]  (NA = 64 * NS)
   double *hh;  // 18 doubles for the stateless entry points
   AtomicState *state;
   AtomicMid *mid;
   StepCtrl *ctrl;
   Thermo *thermo;
-  int *flags;  // [0] REBO list overflow, [1] molecule list overflow
+  int *flags;  // [0] REBO list overflow, [1] molecule list overflow,
+               // [2] per-atom LJ list overflow
   int *mol_rep, *atom_rep, *rep_mol, *rep_atom;
   int *blk_a0, *blk_a1, *blk_rep, *rep_blk;
   int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)

@@ -86,7 +86,7 @@ void free_atomic_dev(AtomicDev &d) {
                   d.mol_nl_count,
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.rebo_mpart,
-                  d.lj_part,   d.ke_part,   d.rebo_fe,
+                  d.lj_part,   d.ke_part,   d.rebo_fe,   d.al_e,       d.al_n,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
                   d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
                   d.blk_rep,   d.rep_blk};
@@ -186,6 +186,11 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   AL(d.bonds.bflag, ns);
   AL(d.bonds.brev, ns);
   AL(d.rebo_fe, 3 * ns);
+  {
+    const size_t na = p.apad <= 64 ? 64 : 128;  // lanes of a molecule tile
+    AL(d.al_e, Nm * 4 * (size_t)p.al_cap * na);
+    AL(d.al_n, Nm * 4 * na);
+  }
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
   AL(d.rebo_mpart, Nm * kAtPart);
   AL(d.lj_part, Nm * kAtPart);
@@ -233,6 +238,8 @@ void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Na, int Nm, in
   const double rl = c.lj.rcut + c.lj_skin;
   p.lj_rlist2 = rl * rl;  // (LJ_RCUT + 2.0)^2, AB:265
   p.mol_rlist = rl + 1.0;  // + 1 A for centre drift between rebuilds
+  p.al_r2 = (float)((c.lj.rcut + 1.0) * (c.lj.rcut + 1.0));  // per-atom lists: 1 A skin
+  p.al_cap = 384;  // per wavefront segment: about 3.4 times a C60 crystal's share
   p.dt = c.dt;
   p.clamp_xi = c.clamp_xi;
   p.clamp_vg = c.clamp_vg;
@@ -271,11 +278,11 @@ int engine_prepare_atomic(fmd_engine *e) {
   HIPCHK(at_launch_observe(d, p, kAtObsKE | kAtObsForce, s));
   e->rebuilds++;
   e->prepared = true;
-  int f[2] = {0, 0};
+  int f[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  e->overflow += f[0] + f[1];
-  if (f[0] || f[1]) HIPCHK(hipMemset(d.flags, 0, 2 * sizeof(int)));
+  e->overflow += f[0] + f[1] + f[2];
+  if (f[0] || f[1] || f[2]) HIPCHK(hipMemset(d.flags, 0, 3 * sizeof(int)));
   return FMD_OK;
 }
 
@@ -307,12 +314,12 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
     if (judge && ((long long)first_step + done) % every == 0 && (rc = surrogate_judge(e)))
       return rc;
   }
-  int f[2] = {0, 0};
+  int f[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
-  if (f[0] || f[1]) {
-    e->overflow += f[0] + f[1];
-    HIPCHK(hipMemset(d.flags, 0, 2 * sizeof(int)));
+  if (f[0] || f[1] || f[2]) {
+    e->overflow += f[0] + f[1] + f[2];
+    HIPCHK(hipMemset(d.flags, 0, 3 * sizeof(int)));
   }
   return FMD_OK;
 }
@@ -507,6 +514,7 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
   e->use_graph = R > 1;
   fill_params(A.p, *c, L.rep_atom[R], (int)L.mol_rep.size(), R, apad);
   A.p.trace_virial = 1;
+  A.p.lj_alist = 1;
   A.p.ff = ff;
   A.p.coul_rc = c->lj.rcut;
   if (ff == kFfMM) A.p.maxr = 1;  // no REBO list
@@ -664,6 +672,7 @@ int atomic_workspace(int na, const int *mol_id, int rebo_mode, fmd_engine **out)
   }
   g_aws.e->at->p.rebo_mode = rebo_mode;
   g_aws.e->at->p.trace_virial = 0;  // the drop-ins return the full virial tensor
+  g_aws.e->at->p.lj_alist = 0;      // and evaluate the given lists' pairs each call
   *out = g_aws.e;
   return FMD_OK;
 }

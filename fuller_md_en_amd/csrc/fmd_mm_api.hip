@@ -452,6 +452,7 @@ int mm_workspace(int na, const int *mol_id, int mode, fmd_engine **out) {
   }
   g_mws.e->at->p.mm_mode = mode;
   g_mws.e->at->p.trace_virial = 0;  // the drop-ins return the full virial
+  g_mws.e->at->p.lj_alist = 0;      // and evaluate the given lists' pairs each call
   *out = g_mws.e;
   return FMD_OK;
 }
