@@ -1100,6 +1100,26 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
     ia[s] = s0 + a;
   }
   const float rl2 = p.al_r2, rl = sqrtf(rl2);
+  // the lists just used were valid if no atom has moved half the skin since
+  // they were built (a pair closes by at most twice that)
+  if (wv == 0) {
+    const double half_skin = 0.5 * (sqrt((double)rl2) - (double)sqrtf(p.lj.rc2));
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int a = lane + 64 * s;
+      if (a < n) {
+        const int x = s0 + a;
+        const double px = d.pos[3 * x], py = d.pos[3 * x + 1], pz = d.pos[3 * x + 2];
+        const float4 q = d.al_pos[x];
+        if (q.w != 0.f) {
+          double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+          mimg(dx, dy, dz, hi, h);
+          if (dx * dx + dy * dy + dz * dz > half_skin * half_skin) atomicAdd(&d.flags[2], 1);
+        }
+        d.al_pos[x] = make_float4((float)px, (float)py, (float)pz, 1.f);
+      }
+    }
+  }
   const float reach_i = rl + d.radius[mi] + 0.05f;
   const int i_last = s0 + n - 1;
   const int jmin_i = d.jmax_min[mi];

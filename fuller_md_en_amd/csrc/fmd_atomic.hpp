@@ -122,13 +122,15 @@ struct AtomicDev {
   double *rebo_fe;  // 3 x Na*maxr: per-entry REBO forces (global-memory REBO path)
   unsigned short *al_e;  // per-atom LJ lists: [Nm][4][al_cap / 8][NA] blocks of 8 entries
   unsigned short *al_n;  // segment lengths in blocks: [Nm][4][NA]  (NA = 64 * NS)
+  float4 *al_pos;        // per atom: position at the last list build (w = 1 once set)
   double *hh;  // 18 doubles for the stateless entry points
   AtomicState *state;
   AtomicMid *mid;
   StepCtrl *ctrl;
   Thermo *thermo;
   int *flags;  // [0] REBO list overflow, [1] molecule list overflow,
-               // [2] per-atom LJ list overflow
+               // [2] per-atom LJ list overflow or an atom moved half the
+               // list skin between rebuilds
   int *mol_rep, *atom_rep, *rep_mol, *rep_atom;
   int *blk_a0, *blk_a1, *blk_rep, *rep_blk;
   int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)

@@ -87,6 +87,7 @@ void free_atomic_dev(AtomicDev &d) {
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.rebo_mpart,
                   d.lj_part,   d.ke_part,   d.rebo_fe,   d.al_e,       d.al_n,
+                  d.al_pos,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
                   d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
                   d.blk_rep,   d.rep_blk};
@@ -190,6 +191,7 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
     const size_t na = p.apad <= 64 ? 64 : 128;  // lanes of a molecule tile
     AL(d.al_e, Nm * 4 * (size_t)p.al_cap * na);
     AL(d.al_n, Nm * 4 * na);
+    AL(d.al_pos, Na);
   }
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
   AL(d.rebo_mpart, Nm * kAtPart);
@@ -215,6 +217,7 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
     return hip_fail(e, "hipMalloc (atomic engine)");
   }
   (void)hipMemset(d.flags, 0, 4 * sizeof(int));
+  (void)hipMemset(d.al_pos, 0, Na * sizeof(float4));
   (void)hipMemset(d.F, 0, 3 * Na * sizeof(double));
   (void)hipMemset(d.state, 0, R * sizeof(AtomicState));
   (void)hipMemset(d.nlc_r, 0, Na * sizeof(int));
@@ -558,7 +561,11 @@ int fmd_batch_set_state(fmd_engine *e, int r, const double *pos, const double *v
   if (r < 0 || r >= A.p.R) return set_error(FMD_ERR_ARG, "replica %d out of range", r);
   hipStream_t s = e->stream;
   const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
-  if (pos) HIPCHK(hipMemcpyAsync(A.d.pos + 3 * a0, pos, 3 * na * 8, hipMemcpyHostToDevice, s));
+  if (pos) {
+    HIPCHK(hipMemcpyAsync(A.d.pos + 3 * a0, pos, 3 * na * 8, hipMemcpyHostToDevice, s));
+    // new positions: the next list build has no earlier one to check against
+    HIPCHK(hipMemsetAsync(A.d.al_pos + a0, 0, na * sizeof(float4), s));
+  }
   if (vel) HIPCHK(hipMemcpyAsync(A.d.vel + 3 * a0, vel, 3 * na * 8, hipMemcpyHostToDevice, s));
   AtomicState S;
   HIPCHK(hipMemcpyAsync(&S, A.d.state + r, sizeof S, hipMemcpyDeviceToHost, s));

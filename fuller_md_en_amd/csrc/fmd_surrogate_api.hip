@@ -93,6 +93,7 @@ int restore_pos(fmd_engine *e) {
   AtomicEngine &A = *e->at;
   HIPCHK(hipMemcpyAsync(A.d.pos, A.sg->pos_save, 3 * (size_t)A.p.Na * 8,
                         hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemsetAsync(A.d.al_pos, 0, (size_t)A.p.Na * sizeof(float4), e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   e->prepared = false;  // tiles, lists and forces are recomputed before the next step
   return FMD_OK;
