@@ -236,16 +236,30 @@ __global__ __launch_bounds__(256) void k_at_rebo_nlist(AtomicDev d, AtomicParams
   if (m >= p.Nm) return;
   const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
+  // FP32 prefilter through the molecule's tile offsets (its atoms' minimum
+  // image about the centre): only pairs within 1e-5 relative of the list
+  // radius take the reference's FP64 test
+  const float rl2 = (float)p.rebo_rlist2, lo = rl2 * (1.0f - 1e-5f), hi2 = rl2 * (1.0f + 1e-5f);
+  const float4 *tile = d.off + (size_t)m * p.apad;
+  float4 oa[NS];
   int cnt[NS];
 #pragma unroll
-  for (int s = 0; s < NS; s++) cnt[s] = 0;
+  for (int s = 0; s < NS; s++) {
+    cnt[s] = 0;
+    const int a = lane + 64 * s;
+    oa[s] = a < n ? tile[a] : make_float4(3e5f, 3e5f, 3e5f, 0.f);
+  }
   for (int b = 0; b < n; b++) {
     const int j = s0 + b;
+    const float4 ob = tile[b];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const int a = lane + 64 * s;
       const int i = s0 + a;
-      if (a < n && j != i && r2_ref(d.pos, i, j, h, hi) < p.rebo_rlist2) {
+      const float dx = ob.x - oa[s].x, dy = ob.y - oa[s].y, dz = ob.z - oa[s].z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (a < n && j != i &&
+          (r2 < lo || (r2 < hi2 && r2_ref(d.pos, i, j, h, hi) < p.rebo_rlist2))) {
         if (cnt[s] < p.maxr) d.nll_r[(size_t)i * p.maxr + cnt[s]] = j;
         cnt[s]++;
       }
@@ -1321,8 +1335,11 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
       e = 0.f;
     };
     const char *offb = reinterpret_cast<const char *>(off);
+    uint4 nxt = cnt > 0 ? lst[0] : make_uint4(0u, 0u, 0u, 0u);
     for (int ib = 0; ib < cnt; ib++) {
-      const uint4 blk = lst[(size_t)ib * NA];
+      // the next block's load is in flight while this one is walked
+      const uint4 blk = nxt;
+      if (ib + 1 < cnt) nxt = lst[(size_t)(ib + 1) * NA];
       const unsigned wd[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
       for (int h4 = 0; h4 < 2; h4++) {
