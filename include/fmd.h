@@ -420,7 +420,9 @@ int fmd_engine_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctr
  * F (Na x 3), T unused.  W9, energies {Ep, Ep_rebo, Ep_lj}.  Any may be NULL. */
 int fmd_engine_forces(fmd_engine *e, double *F, double *T, double *W9, double *energies);
 
-/* Counters: [0] list overflows since creation, [1] list rebuilds, [2] steps. */
+/* Counters: [0] list faults since creation (a list overflow, or an atom of an
+ * atomic engine moving half its per-atom LJ list skin, 0.5 A, between
+ * rebuilds), [1] list rebuilds, [2] steps. */
 int fmd_engine_counters(fmd_engine *e, long long *out3);
 
 /* Mean duration (ms) of the force kernel over the launches timed since the
