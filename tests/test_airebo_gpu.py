@@ -194,6 +194,23 @@ def test_rebo_lds_and_global_paths_agree(mode):
     assert rel_max(Fa, F0) < TOL and rel_max(Wa, W0) < TOL
 
 
+def test_engine_counts_a_stale_lj_list():
+    """The engine's per-atom LJ lists carry a 1 A skin; an atom moving more
+    than half of it between two list rebuilds is counted as a list fault.
+    A uniform drift of 0.15 A/fs (twenty times the thermal speed at 300 K,
+    every bond unchanged) covers about 1 A in the 20 steps of 0.5 fs between
+    rebuilds; the same crystal at rest counts none."""
+    pos, h, mol = crystal(3, 7, 0.01)
+    for drift, faults in ((0.0, False), (0.15, True)):
+        print(f"case drift={drift}", flush=True)
+        vel = np.zeros_like(pos)
+        vel[:, 0] = drift
+        eng = _engine(pos, h, mol, vel=vel, mode=airebo.REBO_PHYSICAL)
+        eng.run(0, 21)
+        assert (eng.counters()[0] > 0) == faults
+        eng.close()
+
+
 def test_engine_trajectory_matches_reference():
     g = golden("airebo_nc1.npz")
     eng = _engine(g["pos"], g["h"], g["mol_id"], vel=g["vel"])
