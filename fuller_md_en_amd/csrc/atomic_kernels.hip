@@ -1195,12 +1195,18 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
       }
       const float4 *sj = d.off + (size_t)mj * apad;
       const int *jj = d.jmax + (size_t)mj * apad;
-      // rank offset by k: the wavefront that takes a neighbour's extra group
-      // rotates
-      for (int r = k; qm; r++) {
-        const int b4 = 4 * __builtin_ctzll(qm);
-        qm &= qm - 1ull;
-        if ((r & (kAlWaves - 1)) != wv) continue;
+      // this wavefront's groups: live groups whose rank r among the live ones
+      // (ascending) has r + k = w mod 4 (the wavefront that takes a
+      // neighbour's extra group rotates with k), picked by one ballot
+      bool mine = false;
+      if (lane < 64 && ((qm >> lane) & 1ull)) {
+        const int r = __popcll(qm & ((1ull << lane) - 1ull));
+        mine = ((r + k) & (kAlWaves - 1)) == wv;
+      }
+      unsigned long long my = __ballot(mine);
+      while (my) {
+        const int b4 = 4 * __builtin_ctzll(my);
+        my &= my - 1ull;
         float4 rb[4];
         int jm[4];
 #pragma unroll
