@@ -465,14 +465,40 @@ __global__ __launch_bounds__(256) void k_at_scan_jmax(AtomicDev d, AtomicParams 
         float4 rb[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) rb[u] = tq[b4 + u];
+        if (fast) {
+          // the group's four FP32 tests, then the FP64 re-test of those
+          // within the band (one branch per group; tile slots past the atoms
+          // hold far-away sentinels, never in the band)
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          if (b4 + u >= tn) break;
+          for (int s = 0; s < NS; s++) {
+            const bool real = lane + 64 * s < n;
+            bool in[4], band = false;
+            float r2[4];
 #pragma unroll
-          for (int s = 0; s < NS; s++)
-            if (lane + 64 * s < n)
-              c[s] += scan_in(fast, rb[u], cx[s], cy[s], cz[s], lo, hi2, d.pos,
-                              s0 + lane + 64 * s, t0 + b4 + u, h, hi, p.lj_rlist2);
+            for (int u = 0; u < 4; u++) {
+              const float ex = rb[u].x + cx[s], ey = rb[u].y + cy[s], ez = rb[u].z + cz[s];
+              r2[u] = ex * ex + ey * ey + ez * ez;
+              in[u] = real && r2[u] < lo;
+              band |= real && !(r2[u] < lo) && r2[u] < hi2;
+            }
+            if (__any(band)) {
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                if (real && !(r2[u] < lo) && r2[u] < hi2)
+                  in[u] = r2_ref(d.pos, s0 + lane + 64 * s, t0 + b4 + u, h, hi) < p.lj_rlist2;
+            }
+            c[s] += (int)in[0] + (int)in[1] + (int)in[2] + (int)in[3];
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (b4 + u >= tn) break;
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+              if (lane + 64 * s < n)
+                c[s] += scan_in(fast, rb[u], cx[s], cy[s], cz[s], lo, hi2, d.pos,
+                                s0 + lane + 64 * s, t0 + b4 + u, h, hi, p.lj_rlist2);
+          }
         }
       }
     }
