@@ -128,6 +128,8 @@ _SIGS = {
     "fmd_rigid_nlist": (_i, [_dp, _dp, _dp, _i, _d, _i, _ip, _ip, C.POINTER(_i)]),
     "fmd_rigid_forces": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i, _i,
                               _d, C.POINTER(LJParams), _vp, C.POINTER(_d)]),
+    "fmd_rigid_forces_noalloc": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _i,
+                                      _i, _i, _d, C.POINTER(LJParams), _vp, C.POINTER(_d)]),
     "fmd_rigid_step_npt": (_i, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _d, _d, _i,
                                 _i, _d, _d, C.POINTER(NPTState), _ip, _ip, _i,
                                 C.POINTER(LJParams), C.POINTER(_d), C.POINTER(_d)]),

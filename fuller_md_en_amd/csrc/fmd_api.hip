@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -204,10 +205,10 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
 // device sites) for the half-pair kernel's per-quad cull.  Greedy: the site
 // farthest from the centroid of those left, then its 3 nearest left.  The
 // order is internal to the device: sums over sites do not depend on it.
-static void quad_order(const double *body, int natom, std::vector<int> &perm) {
+static void quad_order(const double *body, int natom, int *perm) {
   std::vector<int> left(natom);
   for (int a = 0; a < natom; a++) left[a] = a;
-  perm.clear();
+  int np = 0;
   auto d2 = [&](int a, const double *c) {
     double x = body[3 * a] - c[0], y = body[3 * a + 1] - c[1], z = body[3 * a + 2] - c[2];
     return x * x + y * y + z * z;
@@ -224,14 +225,16 @@ static void quad_order(const double *body, int natom, std::vector<int> &perm) {
       return d2(x, body + 3 * s0) < d2(y, body + 3 * s0);
     });
     const size_t take = std::min<size_t>(4, left.size());
-    for (size_t t = 0; t < take; t++) perm.push_back(left[t]);
+    for (size_t t = 0; t < take; t++) perm[np++] = left[t];
     left.erase(left.begin(), left.begin() + take);
   }
 }
 
+// body4 receives 4 * natom_pad doubles (at most 4 * 128); no heap memory is
+// used unless qbody is asked for.
 int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0, double Mmol,
                  const fmd_lj_params &lj, double rmcut2, double rlist, int max_neigh,
-                 double dt, double cxi, double cvg, std::vector<double> &body4,
+                 double dt, double cxi, double cvg, double *body4,
                  std::vector<double> *qbody) {
   if (N < 1 || natom < 1 || natom > 128 || max_neigh < 1)
     return set_error(FMD_ERR_ARG, "rigid: need N >= 1, 1 <= natom <= 128, max_neigh >= 1");
@@ -254,11 +257,11 @@ int rigid_params(RigidParams &p, int N, int natom, const double *body, double I0
   p.rc2_hi_s = p.lj.rc2_hi / p.lj.sig2;
   p.rmin2_s = p.lj.rmin2 / p.lj.sig2;
   p.c48s = 48.0 * lj.eps / std::sqrt((double)p.lj.sig2);
-  std::vector<int> perm(natom);
+  int perm[128];
   for (int a = 0; a < natom; a++) perm[a] = a;
   if (qbody) quad_order(body, natom, perm);
   double rb = 0;
-  body4.assign(4 * (size_t)p.natom_pad, 0.0);
+  for (int k = 0; k < 4 * p.natom_pad; k++) body4[k] = 0.0;
   for (int a = 0; a < p.natom_pad; a++) {
     if (a < natom) {
       const double *b = body + 3 * perm[a];
@@ -314,10 +317,11 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
     return set_error(FMD_ERR_NODEVICE, "fmd_rigid_create: no HIP device");
   fmd_engine *e = new fmd_engine();
   e->kind = FMD_KIND_RIGID;
-  std::vector<double> body4, qbody;
+  std::vector<double> body4(4 * 128), qbody;
   int rc = rigid_params(e->rp, c->n_mol, c->natom, c->body, c->I0, c->Mmol, c->lj,
                         c->rmcut * c->rmcut, c->rmcut + c->skin, c->max_neigh, c->dt,
-                        c->clamp_xi, c->clamp_vg, body4, c->half_pairs ? &qbody : nullptr);
+                        c->clamp_xi, c->clamp_vg, body4.data(),
+                        c->half_pairs ? &qbody : nullptr);
   if (c->half_pairs == 2) e->rp.pair_kernel = 0;  // the per-molecule pair-once kernel
   if (rc == FMD_OK) rc = engine_init_common(e, 4096, 1);
   if (rc == FMD_OK) rc = rigid_alloc(e->rd, e->rp, e->ring);
@@ -326,7 +330,7 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
     return rc;
   }
   e->nlup = c->nlup > 0 ? c->nlup : 25;
-  hipError_t he = hipMemcpy(e->rd.body, body4.data(), body4.size() * sizeof(double),
+  hipError_t he = hipMemcpy(e->rd.body, body4.data(), 4 * e->rp.natom_pad * sizeof(double),
                             hipMemcpyHostToDevice);
   if (he == hipSuccess && e->rp.use_half)
     he = hipMemcpy(e->rd.qbody, qbody.data(), qbody.size() * sizeof(double),
@@ -737,11 +741,11 @@ int rigid_workspace(int N, int natom, int max_neigh, const double *body, double 
     g_rws.max_neigh = max_neigh;
   }
   fmd_engine *e = g_rws.e;
-  std::vector<double> body4;
+  std::vector<double> body4(4 * 128);
   int rc = rigid_params(e->rp, N, natom, body, I0, Mmol, *lj, rmcut2, 0.0, max_neigh, dt, 0.1,
-                        0.01, body4, nullptr);
+                        0.01, body4.data(), nullptr);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(e->rd.body, body4.data(), body4.size() * sizeof(double),
+  HIPCHK(hipMemcpyAsync(e->rd.body, body4.data(), 4 * e->rp.natom_pad * sizeof(double),
                         hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   *out = e;
@@ -898,6 +902,133 @@ int fmd_rigid_step_npt(double *pos, double *vel, double *qv, double *omg, double
   npt->xi = S.xi;
   if (Ep) *Ep = S.Ep;
   if (KE) *KE = S.KEt + S.KEr;
+  return FMD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- rigid forces, no allocation
+// fmd_rigid_forces_noalloc keeps every device buffer it needs in `scratch`,
+// one file-scope array per device, instead of the workspace engine.  The
+// array's name is a host-side stand-in: host code uses only the device
+// address that hipGetSymbolAddress returns for the current device.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;  // 16 MiB per device
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kNoAllocMaxDevices];  // scratch's address on each device
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_address(unsigned char **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return set_error(FMD_ERR_ARG, "fmd_rigid_forces_noalloc: device %d beyond %d", dev,
+                     kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FMD_OK;
+}
+
+// Carves consecutive 256-byte aligned pieces out of a byte range; with a
+// null base it only counts.
+struct Carve {
+  unsigned char *base;
+  size_t used = 0;
+  template <class T>
+  T *take(size_t n) {
+    T *p = base ? reinterpret_cast<T *>(base + used) : nullptr;
+    used += (n * sizeof(T) + 255) & ~size_t(255);
+    return p;
+  }
+};
+
+void carve_rigid(Carve &c, RigidDev &d, double *&hh, double *&out10, int N, int natom_pad,
+                 int max_neigh) {
+  const size_t n = (size_t)N;
+  hh = c.take<double>(18);
+  out10 = c.take<double>(kMolPart);
+  d.body = c.take<double>(4 * (size_t)natom_pad);
+  d.pos = c.take<double>(3 * n);
+  d.qv = c.take<double>(4 * n);
+  d.nl_count = c.take<int>(n);
+  d.nl_list = c.take<int>(n * max_neigh);
+  d.sites = c.take<float4>(n * natom_pad);
+  d.sites_d = c.take<double>(3 * n * natom_pad);
+  d.Fv = c.take<double>(3 * n);
+  d.Tv = c.take<double>(3 * n);
+  d.molpart = c.take<double>(n * kMolPart);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmd_rigid_forces_noalloc(double *Fv, double *Tv, double *Wm9, const double *pos,
+                             const double *qv, const double *body, const double *h,
+                             const double *hi, const int *nl_count, const int *nl_list,
+                             int max_neigh, int N, int natom, double rmcut2,
+                             const fmd_lj_params *lj, double *lab, double *Ep) {
+  if (!Fv || !Tv || !Wm9 || !pos || !qv || !body || !h || !hi || !nl_count || !nl_list)
+    return set_error(FMD_ERR_ARG, "fmd_rigid_forces_noalloc: null argument");
+  fmd_lj_params ljd;
+  if (!lj) {
+    fmd_lj_params_rigid(&ljd);
+    lj = &ljd;
+  }
+  RigidParams p;
+  double body4[4 * 128];
+  int rc = rigid_params(p, N, natom, body, 1.0, 1.0, *lj, rmcut2, 0.0, max_neigh, 1.0, 0.1,
+                        0.01, body4, nullptr);
+  if (rc) return rc;
+  RigidDev d;
+  memset(&d, 0, sizeof d);
+  double *hh_d = nullptr, *out10_d = nullptr;
+  Carve count{nullptr};
+  carve_rigid(count, d, hh_d, out10_d, N, p.natom_pad, max_neigh);
+  if (count.used > kNoAllocScratchBytes)
+    return set_error(FMD_ERR_CAPACITY,
+                     "fmd_rigid_forces_noalloc: N=%d natom=%d max_neigh=%d needs %zu bytes of "
+                     "scratch, more than its %zu",
+                     N, natom, max_neigh, count.used, kNoAllocScratchBytes);
+  unsigned char *base = nullptr;
+  if ((rc = scratch_address(&base))) return rc;
+  Carve carve{base};
+  carve_rigid(carve, d, hh_d, out10_d, N, p.natom_pad, max_neigh);
+  double hh[18];
+  for (int k = 0; k < 9; k++) {
+    hh[k] = h[k];
+    hh[9 + k] = hi[k];
+  }
+  hipStream_t s = 0;
+  const size_t n = (size_t)N;
+  HIPCHK(hipMemcpyAsync(hh_d, hh, sizeof hh, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.body, body4, 4 * (size_t)p.natom_pad * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.pos, pos, 3 * n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.qv, qv, 4 * n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.nl_count, nl_count, n * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.nl_list, nl_list, n * max_neigh * sizeof(int), hipMemcpyHostToDevice,
+                        s));
+  HIPCHK(rigid_launch_sites(d, p, s));
+  HIPCHK(rigid_launch_forces(d, p, hh_d, s));
+  HIPCHK(rigid_launch_reduce_molpart(d, p, out10_d, s));
+  double out10[kMolPart];
+  HIPCHK(hipMemcpyAsync(Fv, d.Fv, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(Tv, d.Tv, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(out10, out10_d, sizeof out10, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int k = 0; k < 9; k++) Wm9[k] = out10[k];
+  if (Ep) *Ep = out10[9];
+  if (lab) host_lab(qv, body, N, natom, lab);
   return FMD_OK;
 }
 

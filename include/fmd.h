@@ -115,6 +115,22 @@ int fmd_rigid_forces(double *Fv, double *Tv, double *Wm9, const double *pos,
                      int max_neigh, int N, int natom, double rmcut2,
                      const fmd_lj_params *lj, double *lab, double *Ep);
 
+/* No counterpart in the reference: fmd_rigid_forces for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * fmd_rigid_forces, but every device buffer lives in one fixed 16 MiB
+ * per-device scratch array of the library, set up on the first call on each
+ * device: no hipMalloc, hipFree or other allocating call.  Work runs on the
+ * null stream.  Inputs needing more than the scratch (about
+ * N * (40 natom_pad + 4 max_neigh + 190) bytes) return FMD_ERR_CAPACITY.
+ * Calls on one device share that scratch, so calls that overlap on one GPU
+ * (from several threads) race. */
+int fmd_rigid_forces_noalloc(double *Fv, double *Tv, double *Wm9, const double *pos,
+                             const double *qv, const double *body, const double *h,
+                             const double *hi, const int *nl_count, const int *nl_list,
+                             int max_neigh, int N, int natom, double rmcut2,
+                             const fmd_lj_params *lj, double *lab, double *Ep);
+
 /* Replaces step_npt (C1:504-510, LJ:434-439): one NH + PR velocity-Verlet
  * step of rigid molecules (clamps xi +-0.1, Vg +-0.01).  Updates pos, vel,
  * qv, omg, Fv, Tv, Wm9, h, hi and *npt in place; *Ep, *KE as the reference
