@@ -78,6 +78,58 @@ int fmd_load_cc1(const char *path, double *coords, int *natom, double *props) {
   return FMD_OK;
 }
 
+// generate_c60 (C1:228-252): the truncated icosahedron of the core program.
+// Its 60 vertices are the cyclic permutations of three sign-varied triples
+// in units where the edge is 2, (0, 1, 3 phi), (2, 1 + 2 phi, phi) and
+// (1, 2 + phi, 2 phi) with phi the golden ratio, listed family by family, then
+// permutation, then signs of the components in order (the first family has
+// no sign on its zero component); centred and scaled by 0.72 A (edge 1.44 A,
+// radius 3.568 A).  The vertex order is the core program's, so site sums run
+// in the same order.  props = {Rmol, Dmol, I0, Mmol} as fmd_load_cc1 gives
+// them, with I0 = (2/3) sum m r^2 (equal to the trace form for this shape).
+int fmd_generate_c60(double *coords, double *props) {
+  if (!coords || !props) return FMD_ERR_ARG;
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double fam[3][3] = {{0.0, 1.0, 3.0 * phi},
+                            {2.0, 1.0 + 2.0 * phi, phi},
+                            {1.0, 2.0 + phi, 2.0 * phi}};
+  double t[60][3];
+  int n = 0;
+  for (int f = 0; f < 3; f++)
+    for (int p = 0; p < 3; p++)             // component k of the triple goes to axis (p+k) mod 3
+      for (int s0 = (f == 0 ? 1 : -1); s0 <= 1; s0 += 2)
+        for (int s1 = -1; s1 <= 1; s1 += 2)
+          for (int s2 = -1; s2 <= 1; s2 += 2) {
+            const int sg[3] = {s0, s1, s2};
+            for (int k = 0; k < 3; k++) t[n][(p + k) % 3] = sg[k] * fam[f][k];
+            n++;
+          }
+  double cm[3] = {0, 0, 0};
+  for (int i = 0; i < 60; i++)
+    for (int a = 0; a < 3; a++) cm[a] += t[i][a];
+  for (int a = 0; a < 3; a++) cm[a] /= 60.0;
+  double rmol = 0, isum = 0;
+  for (int i = 0; i < 60; i++) {
+    for (int a = 0; a < 3; a++) t[i][a] = (t[i][a] - cm[a]) * 0.72;
+    const double r2 = t[i][0] * t[i][0] + t[i][1] * t[i][1] + t[i][2] * t[i][2];
+    rmol = std::max(rmol, std::sqrt(r2));
+    isum += kMassC * r2;
+  }
+  double dmol = 0;
+  for (int i = 0; i < 60; i++)
+    for (int j = i + 1; j < 60; j++) {
+      const double dx = t[i][0] - t[j][0], dy = t[i][1] - t[j][1], dz = t[i][2] - t[j][2];
+      dmol = std::max(dmol, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  for (int i = 0; i < 60; i++)
+    for (int a = 0; a < 3; a++) coords[3 * i + a] = t[i][a];
+  props[0] = rmol;
+  props[1] = dmol;
+  props[2] = isum * 2.0 / 3.0;
+  props[3] = 60 * kMassC;
+  return FMD_OK;
+}
+
 // resolve_fullerene (LJ:210-226): C60 | C70 | C72 | C74 | C76:<sym> |
 // C84:<n>[:<sym>] (case-insensitive) -> library path and display label.
 int fmd_resolve_fullerene(const char *spec_c, const char *lib_c, char *path, int pcap,

@@ -23,6 +23,7 @@ def _sig():
     L = N.lib()
     _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
     L.fmd_load_cc1.argtypes = [C.c_char_p, _dp, C.POINTER(C.c_int), _dp]
+    L.fmd_generate_c60.argtypes = [_dp, _dp]
     L.fmd_resolve_fullerene.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int,
                                         C.c_char_p, C.c_int]
     L.fmd_default_a0.restype = C.c_double
@@ -62,6 +63,13 @@ def load_cc1(path: str, label: str = "") -> Molecule:
     n = C.c_int(0)
     N.check(_sig().fmd_load_cc1(path.encode(), coords, C.byref(n), props))
     return Molecule(label, coords[: n.value * 3].reshape(n.value, 3).copy(), *props)
+
+
+def c60_analytic() -> Molecule:
+    """generate_c60 (C1:228-252): the core program's analytic C60."""
+    coords, props = np.zeros(180), np.zeros(4)
+    N.check(_sig().fmd_generate_c60(coords, props))
+    return Molecule("C60 (analytic)", coords.reshape(60, 3), *props)
 
 
 def molecule(spec: str = "C60", libdir: str = LIBDIR) -> Molecule:

@@ -78,6 +78,10 @@ void fmd_make_npt(fmd_npt_state *s, double T, double Pe, int Nf);
 /* load_cc1 (LJ:178-207, AB:195-210): coords (>= 84 x 3) centred on the
  * geometric centre; props = {Rmol, Dmol, I0, Mmol}. */
 int fmd_load_cc1(const char *path, double *coords, int *natom, double *props);
+/* generate_c60 (C1:228-252): the core program's analytic C60 (truncated
+ * icosahedron, edge 1.44 A, radius 3.568 A), coords 60 x 3 in its vertex
+ * order; props = {Rmol, Dmol, I0, Mmol} as fmd_load_cc1 gives them. */
+int fmd_generate_c60(double *coords, double *props);
 /* resolve_fullerene (LJ:210-226): species spec -> .cc1 path and label. */
 int fmd_resolve_fullerene(const char *spec, const char *libdir, char *path, int pcap,
                           char *label, int lcap);

@@ -49,6 +49,8 @@ def core():
         L.refc_init.argtypes = [_i, C.c_uint, _d, _dp, _dp, _dp, _dp, _dp]
         L.refc_init.restype = _i
         L.refc_normal_stream.argtypes = [C.c_uint, _i, _dp]
+        L.refc_main_loop.argtypes = [_i, C.c_uint, _i, _dp]
+        L.refc_main_loop.restype = _i
         L.refc_apply_pbc.argtypes = [_dp, _dp, _dp, _i]
         L.refc_mat_inv9.argtypes = [_dp, _dp]
         _libs["core"] = L

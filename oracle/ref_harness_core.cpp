@@ -10,6 +10,8 @@
 #include "fuller_LJ_npt_md_core_serial.cpp"
 #undef main
 
+#include <vector>
+
 static void npt_from(const double *s, NPTState &n) {
   n.xi = s[0];
   n.Q = s[1];
@@ -102,6 +104,43 @@ int refc_init(int nc, unsigned seed, double T, double *pos, double *vel, double 
   for (int a = 0; a < 3; a++) vcm[a] /= N;
   for (int i = 0; i < N; i++)
     for (int a = 0; a < 3; a++) vel[i * 3 + a] -= vcm[a];
+  return N;
+}
+
+// The main loop of the core program (C1:683-733) from the refc_init state of
+// `seed` at nc, nsteps steps: list rebuilt every 25 steps, the reference's
+// observables per step into series[4 * (g - 1) + {0, 1, 2, 3}] =
+// {T, P, a, Ecoh} (C1:716-720).  Returns N.
+int refc_main_loop(int nc, unsigned seed, int nsteps, double *series) {
+  C60Data c60 = generate_c60();
+  const int natom = C60_NATOM, Nmax = 4 * nc * nc * nc;
+  std::vector<double> pos(Nmax * 3), vel(Nmax * 3), omg(Nmax * 3), qv(Nmax * 4),
+      Fv(Nmax * 3), Tv(Nmax * 3), lab((size_t)Nmax * natom * 3);
+  std::vector<int> nl_count(Nmax), nl_list((size_t)Nmax * MAX_NEIGH);
+  double h[9] = {}, hi[9] = {}, Wm9[9] = {};
+  const int N = refc_init(nc, seed, 300.0, pos.data(), vel.data(), omg.data(), qv.data(), h);
+  mat_inv9(h, hi);
+  NPTState npt = make_npt(300.0, 0.0, N);
+  nlist_build_sym(pos.data(), h, hi, N, RMCUT, nl_count.data(), nl_list.data());
+  apply_pbc(pos.data(), h, hi, N);
+  forces(Fv.data(), Tv.data(), Wm9, pos.data(), qv.data(), c60.coords, h, hi, nl_count.data(),
+         nl_list.data(), N, natom, RMCUT2, lab.data());
+  for (int g = 1; g <= nsteps; g++) {
+    if (g % 25 == 0) {
+      mat_inv9(h, hi);
+      nlist_build_sym(pos.data(), h, hi, N, RMCUT, nl_count.data(), nl_list.data());
+    }
+    auto [Ep, KE] = step_npt(pos.data(), vel.data(), qv.data(), omg.data(), Fv.data(), Tv.data(),
+                             Wm9, h, hi, c60.coords, c60.I0, c60.Mmol, N, natom, RMCUT2, 1.0,
+                             npt, nl_count.data(), nl_list.data(), lab.data());
+    double kt = ke_trans(vel.data(), N, c60.Mmol);
+    double V = fabs(mat_det9(h));
+    double *o = series + 4 * (size_t)(g - 1);
+    o[0] = inst_T(KE, npt.Nf);
+    o[1] = inst_P(Wm9, kt, V);
+    o[2] = h[0] / nc;
+    o[3] = Ep / N;
+  }
   return N;
 }
 

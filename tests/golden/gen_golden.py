@@ -36,6 +36,14 @@ Fixtures
                           list truncates): inputs and outputs only.
   mm_c70_hcp.npz          MM on C70 in a 2x2x2 HCP cell (1120 atoms): topology
                           and forces.
+  core_ensemble_nc3.npz   the core program's main loop (C1:683-733) at nc=3
+                          (config 1: 108 molecules, 1000 steps) for seeds
+                          42..49: per-step T, P, a, Ecoh of seed 42 (the
+                          program's own run, checked against the printed rows
+                          and Avg line of the reference binary) and the
+                          averages over steps 750..1000 (C1:624, 720) of every
+                          seed, whose spread is the statistical error of one
+                          run's average.
 """
 from __future__ import annotations
 
@@ -325,11 +333,47 @@ def gen_mm():
          mol_id=topo["mol_id"], nlc=nlc, E=out, F=F, W=W, **mm_template(topo, nm))
 
 
+def _core_run(seed, nc=3, nsteps=1000):
+    series = np.zeros(4 * nsteps)
+    ref.core().refc_main_loop(nc, seed, nsteps, series)
+    return series.reshape(nsteps, 4)
+
+
+def gen_core_ensemble():
+    import re
+    import subprocess
+    from multiprocessing import Pool
+    seeds = list(range(42, 50))
+    with Pool(8) as pool:
+        runs = pool.map(_core_run, seeds)
+    avg_from = 1000 - 1000 // 4  # C1:624
+    avgs = np.array([r[avg_from - 1:].mean(axis=0) for r in runs])
+    # the harness loop is the program's: its rows and Avg line equal the binary's
+    out = subprocess.run([ref.path("fuller_LJ_core_omp"), "3"], capture_output=True, text=True,
+                         check=True, env=dict(os.environ, OMP_NUM_THREADS="8")).stdout
+    rows = [l.split() for l in out.splitlines() if re.match(r"\s+\d+\s+[-\d.]+\s", l)]
+    for r in rows:
+        g = int(r[0])
+        want = [f"{runs[0][g - 1, 0]:.1f}", f"{runs[0][g - 1, 1]:.3f}", f"{runs[0][g - 1, 2]:.3f}",
+                f"{runs[0][g - 1, 3]:.5f}"]
+        assert r[1:5] == want, (r, want)
+    m = re.search(r"Avg\((\d+)\): T=(\S+) P=(\S+) a=(\S+) Ecoh=(\S+)", out)
+    assert int(m.group(1)) == 1000 - avg_from + 1
+    assert [float(x) for x in m.group(2, 3, 4, 5)] == [
+        float(f"{avgs[0, 0]:.2f}"), float(f"{avgs[0, 1]:.4f}"), float(f"{avgs[0, 2]:.4f}"),
+        float(f"{avgs[0, 3]:.5f}")]
+    save("core_ensemble_nc3.npz", seeds=np.array(seeds), avg_from=avg_from,
+         series42=runs[0], avgs=avgs, columns=np.array(["T", "P", "a", "Ecoh"]))
+
+
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference exists")
     if sys.argv[1:] == ["lattices"]:
         gen_cc1(LATTICE_CASES)
+        sys.exit(0)
+    if sys.argv[1:] == ["core_ensemble"]:
+        gen_core_ensemble()
         sys.exit(0)
     if sys.argv[1:] == ["mm"]:
         gen_mm()
