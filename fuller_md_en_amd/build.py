@@ -27,7 +27,7 @@ HIP_SOURCES = ["rigid_kernels.hip", "fmd_api.hip", "fmd_atomic_api.hip", "atomic
                "fmd_mm_api.hip"]
 HOST_SOURCES = ["fmd_host.cpp"]
 DRIVERS = {"fuller_LJ_npt_md": "driver_lj.cpp", "fuller_airebo_npt_md": "driver_airebo.cpp",
-           "fuller_LJ_npt_mmmd": "driver_mmmd.cpp"}
+           "fuller_LJ_npt_mmmd": "driver_mmmd.cpp", "fuller_LJ_core": "driver_core.cpp"}
 
 
 def _hipcc() -> str:

@@ -124,6 +124,15 @@ def rigid_system(nc: int = 3, spec: str = "C60", crystal: str = "FCC", T: float 
     return dict(mol=mol, a0=a0, pos=pos, h=h, vel=vel, omg=omg, qv=qv)
 
 
+def core_system(nc: int = 3, seed: int = 42, T: float = 300.0, a0: float = 14.17):
+    """The core program's initial system (C1:640-680): analytic C60, FCC at
+    a0 = 14.17 A, seeded velocities / orientations at 300 K."""
+    mol = c60_analytic()
+    pos, h = make_crystal("FCC", a0, nc)
+    vel, omg, qv = rigid_init_state(len(pos), seed, T, mol)
+    return dict(mol=mol, a0=a0, pos=pos, h=h, vel=vel, omg=omg, qv=qv)
+
+
 def airebo_system(nc: int = 3, spec: str = "C60", crystal: str = "FCC", T: float = 300.0,
                   seed: int = 42, init_scale: float = 1.0, jitter: float = 0.0,
                   jitter_seed: int = 12345, libdir: str = LIBDIR):
