@@ -173,6 +173,7 @@ _SIGS = {
     "fmd_engine_counters": (_i, [_vp, C.POINTER(C.c_longlong)]),
     "fmd_engine_set_timing": (_i, [_vp, _i]),
     "fmd_engine_kernel_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
+    "fmd_engine_phase_times": (_i, [_vp, _dp, _ip]),
     "fmd_engine_sync": (_i, [_vp]),
     "fmd_engine_destroy": (None, [_vp]),
 }

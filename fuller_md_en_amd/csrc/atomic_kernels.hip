@@ -1735,7 +1735,7 @@ hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, i
 }
 
 hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
-                            hipStream_t s) {
+                            hipStream_t s, hipEvent_t ev_mid, unsigned ev_flags) {
   hipError_t e;
   if (p.ff == kFfMM) {
     // bonded molecular-mechanics terms into Frebo and mm.part
@@ -1749,6 +1749,7 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
   } else if ((e = at_launch_rebo(d, p, hh, p.rebo_mode, s)) != hipSuccess) {
     return e;
   }
+  if (ev_mid && (e = hipEventRecordWithFlags(ev_mid, s, ev_flags)) != hipSuccess) return e;
   const int per = (p.Nm + kXcd - 1) / kXcd;
   dim3 g(per * kXcd);
   const bool one = ns_of(p) == 1;
@@ -1796,22 +1797,26 @@ hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what
 }
 
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
-                          int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
-                          hipEvent_t ev1, unsigned ev_flags, hipStream_t s) {
+                          int thermo_slot, int rebuild, int rescale, const StepEvents *ev,
+                          hipStream_t s) {
   hipError_t e;
+  const unsigned evf = ev ? ev->flags : 0;
   if (rebuild) {
     // AB:881-887: both lists from the current positions and cell
+    if (ev && (e = hipEventRecordWithFlags(ev->l0, s, evf)) != hipSuccess) return e;
     if ((e = at_launch_centers(d, p, hh_state(d), s)) != hipSuccess) return e;
     if ((e = at_launch_lists(d, p, hh_state(d), 1, s)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecordWithFlags(ev->l1, s, evf)) != hipSuccess) return e;
   }
   k_at_kick_drift<<<d.nblk, 256, 0, s>>>(d, p, ctrl_slot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = at_launch_centers(d, p, hh_mid(d), s)) != hipSuccess) return e;
-  if (ev0 && (e = hipEventRecordWithFlags(ev0, s, ev_flags)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecordWithFlags(ev->f0, s, evf)) != hipSuccess) return e;
   AtomicParams ps = p;
   ps.lj_trace = p.trace_virial;  // steps need the virial trace only
-  if ((e = at_launch_forces(d, ps, hh_mid(d), s)) != hipSuccess) return e;
-  if (ev1 && (e = hipEventRecordWithFlags(ev1, s, ev_flags)) != hipSuccess) return e;
+  if ((e = at_launch_forces(d, ps, hh_mid(d), s, ev ? ev->fm : nullptr, evf)) != hipSuccess)
+    return e;
+  if (ev && (e = hipEventRecordWithFlags(ev->f1, s, evf)) != hipSuccess) return e;
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
   if (p.ff == kFfMM)
     k_at_finish<true><<<p.R, kFinThreads, 0, s>>>(d, p, ctrl_slot, thermo_slot);

@@ -472,26 +472,42 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
     if (ctrl)
       for (int r = 0; r < R; r++) tt[r] = ctrl[(size_t)(c0 + n - 1) * R + r].Tt;
     const size_t cb = (size_t)R * sizeof(StepCtrl), tb = (size_t)R * sizeof(Thermo);
-    while (e->timing && (int)e->ev.size() < 2 * n) {
+    while (e->timing && (int)e->ev.size() < kEvPerStep * n) {
       hipEvent_t ev;
       HIPCHK(hipEventCreate(&ev));
       e->ev.push_back(ev);
     }
+    // events of step slot k (graph replays reuse slots 0 .. K-1)
+    auto step_events = [&](int k, unsigned flags, StepEvents *out) -> const StepEvents * {
+      if (!e->timing) return nullptr;
+      const hipEvent_t *v = &e->ev[(size_t)kEvPerStep * k];
+      *out = StepEvents{v[0], v[1], v[2], v[3], v[4], flags};
+      return out;
+    };
+    auto elapsed = [&](hipEvent_t a, hipEvent_t b, int phase) -> int {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      e->t_sum[phase] += ms;
+      e->t_n[phase]++;
+      return FMD_OK;
+    };
+    // durations of slots 0 .. m-1, whose steps were global steps k0 .. k0+m-1
+    // of this chunk
     auto take_times = [&](int k0, int m) -> int {
       for (int k = 0; k < m; k++) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e->ev[2 * (k0 + k)], e->ev[2 * (k0 + k) + 1]));
-        e->t_sum += ms;
-        e->t_n++;
+        const hipEvent_t *v = &e->ev[(size_t)kEvPerStep * k];
+        if (elapsed(v[0], v[2], kPhaseForce) || elapsed(v[0], v[1], kPhaseIntra) ||
+            elapsed(v[1], v[2], kPhaseLJ))
+          return FMD_ERR_HIP;
+        if (rebuild[k0 + k] && elapsed(v[3], v[4], kPhaseList)) return FMD_ERR_HIP;
       }
       return FMD_OK;
     };
     if (!e->use_graph) {
       HIPCHK(hipMemcpyAsync(d_ctrl, e->h_ctrl, n * cb, hipMemcpyHostToDevice, s));
       for (int k = 0; k < n; k++) {
-        hipEvent_t e0 = e->timing ? e->ev[2 * k] : nullptr;
-        hipEvent_t e1 = e->timing ? e->ev[2 * k + 1] : nullptr;
-        HIPCHK(launch(k, rebuild[k], rescale[k], e0, e1, hipEventRecordDefault));
+        StepEvents sev;
+        HIPCHK(launch(k, rebuild[k], rescale[k], step_events(k, hipEventRecordDefault, &sev)));
       }
       if (thermo)
         HIPCHK(hipMemcpyAsync(e->h_thermo, d_thermo, n * tb, hipMemcpyDeviceToHost, s));
@@ -511,10 +527,9 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
         hipGraphExec_t exec;
         int rc = engine_graph(e, key, [&]() -> int {
           for (int j = 0; j < K; j++) {
-            hipEvent_t e0 = e->timing ? e->ev[2 * j] : nullptr;
-            hipEvent_t e1 = e->timing ? e->ev[2 * j + 1] : nullptr;
-            HIPCHK(launch(j, j == 0 ? rebuild[k] : 0, rescale[k + j], e0, e1,
-                          hipEventRecordExternal));
+            StepEvents sev;
+            HIPCHK(launch(j, j == 0 ? rebuild[k] : 0, rescale[k + j],
+                          step_events(j, hipEventRecordExternal, &sev)));
           }
           return FMD_OK;
         }, &exec);
@@ -527,7 +542,7 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
                                 hipMemcpyDeviceToHost, s));
         if (e->timing) {
           HIPCHK(hipStreamSynchronize(s));
-          if (take_times(0, K)) return FMD_ERR_HIP;
+          if (take_times(k, K)) return FMD_ERR_HIP;
         }
         k += K;
       }
@@ -577,9 +592,8 @@ static int rigid_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_c
     int rc = rigid_prepare(e);
     if (rc) return rc;
   }
-  auto launch = [&](int slot, int rebuild, int rescale, hipEvent_t e0, hipEvent_t e1,
-                    unsigned evf) -> hipError_t {
-    return rigid_launch_step(d, p, slot, slot, rebuild, rescale, e0, e1, evf, e->stream);
+  auto launch = [&](int slot, int rebuild, int rescale, const StepEvents *ev) -> hipError_t {
+    return rigid_launch_step(d, p, slot, slot, rebuild, rescale, ev, e->stream);
   };
   int rc = run_steps(e, 1, first_step, nsteps, ctrl, thermo, &e->tt, d.ctrl, d.thermo, launch);
   if (rc) return rc;
@@ -663,8 +677,10 @@ int fmd_engine_counters(fmd_engine *e, long long *out3) {
 int fmd_engine_set_timing(fmd_engine *e, int on) {
   if (!e) return set_error(FMD_ERR_ARG, "null engine");
   e->timing = on != 0;
-  e->t_sum = 0;
-  e->t_n = 0;
+  for (int k = 0; k < 4; k++) {
+    e->t_sum[k] = 0;
+    e->t_n[k] = 0;
+  }
   return FMD_OK;
 }
 
@@ -676,10 +692,23 @@ int fmd_engine_set_graph(fmd_engine *e, int on) {
 
 int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n) {
   if (!e) return set_error(FMD_ERR_ARG, "null engine");
-  if (mean_ms) *mean_ms = e->t_n ? e->t_sum / e->t_n : 0.0;
-  if (n) *n = e->t_n;
-  e->t_sum = 0;
-  e->t_n = 0;
+  if (mean_ms) *mean_ms = e->t_n[kPhaseForce] ? e->t_sum[kPhaseForce] / e->t_n[kPhaseForce] : 0.0;
+  if (n) *n = e->t_n[kPhaseForce];
+  for (int k = 0; k < 4; k++) {
+    e->t_sum[k] = 0;
+    e->t_n[k] = 0;
+  }
+  return FMD_OK;
+}
+
+int fmd_engine_phase_times(fmd_engine *e, double *mean_ms4, int *n4) {
+  if (!e) return set_error(FMD_ERR_ARG, "null engine");
+  for (int k = 0; k < 4; k++) {
+    if (mean_ms4) mean_ms4[k] = e->t_n[k] ? e->t_sum[k] / e->t_n[k] : 0.0;
+    if (n4) n4[k] = e->t_n[k];
+    e->t_sum[k] = 0;
+    e->t_n[k] = 0;
+  }
   return FMD_OK;
 }
 
@@ -884,7 +913,7 @@ int fmd_rigid_step_npt(double *pos, double *vel, double *qv, double *omg, double
   HIPCHK(hipMemcpyAsync(d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
   HIPCHK(rigid_launch_observe(d, e->rp, kObsKE, s));
-  HIPCHK(rigid_launch_step(d, e->rp, 0, 0, 0, 0, nullptr, nullptr, 0, s));
+  HIPCHK(rigid_launch_step(d, e->rp, 0, 0, 0, 0, nullptr, s));
   HIPCHK(hipMemcpyAsync(pos, d.pos, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(vel, d.vel, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(qv, d.qv, 4 * (size_t)N * 8, hipMemcpyDeviceToHost, s));

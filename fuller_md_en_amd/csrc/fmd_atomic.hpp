@@ -169,19 +169,22 @@ hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, HHRef hh
 hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                            int rebo_list, hipStream_t s);
 // REBO + LJ forces into F (and the partial sums); needs centres and lists.
+// ev_mid (may be null) is recorded between the intramolecular part (REBO,
+// surrogate or bonded terms) and the intermolecular LJ, with ev_flags.
 hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
-                            hipStream_t s);
+                            hipStream_t s, hipEvent_t ev_mid = nullptr,
+                            unsigned ev_flags = 0);
 enum : int { kAtObsKE = 1, kAtObsForce = 2 };
 hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
                              hipStream_t s);
 // One NPT step of every replica; controls from ctrl[ctrl_slot * R + r],
 // records to thermo[thermo_slot * R + r].  rescale: launch the velocity
 // rescale (it acts on the replicas whose control asks for it).
-// ev0 / ev1 (may be null) bracket the force phase; ev_flags is passed to
-// hipEventRecordWithFlags (hipEventRecordExternal inside a graph capture).
+// ev (may be null): events around the force phase, its LJ part and the list
+// rebuild.
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
-                          int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
-                          hipEvent_t ev1, unsigned ev_flags, hipStream_t s);
+                          int thermo_slot, int rebuild, int rescale, const StepEvents *ev,
+                          hipStream_t s);
 // REBO forces alone into Frebo and rebo_mpart, in the given mode (needs
 // centres and the REBO list).
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,

@@ -298,9 +298,8 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
     int rc = engine_prepare_atomic(e);
     if (rc) return rc;
   }
-  auto launch = [&](int slot, int rebuild, int rescale, hipEvent_t e0, hipEvent_t e1,
-                    unsigned evf) -> hipError_t {
-    return at_launch_step(d, p, slot, slot, rebuild, rescale, e0, e1, evf, e->stream);
+  auto launch = [&](int slot, int rebuild, int rescale, const StepEvents *ev) -> hipError_t {
+    return at_launch_step(d, p, slot, slot, rebuild, rescale, ev, e->stream);
   };
   // with the learned surrogate on, stop at every judge step (global step a
   // multiple of judge_every) to compare it with full REBO
@@ -442,7 +441,7 @@ int atomic_step_once(fmd_engine *e, double *vel, double *F, double *vir9, double
   HIPCHK(hipMemcpyAsync(A.d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(A.d.ctrl, &c, sizeof c, hipMemcpyHostToDevice, s));
   HIPCHK(at_launch_observe(A.d, A.p, kAtObsKE, s));
-  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, nullptr, 0, s));
+  HIPCHK(at_launch_step(A.d, A.p, 0, 0, 0, 0, nullptr, s));
   HIPCHK(hipMemcpyAsync(vel, A.d.vel, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(F, A.d.F, n3, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(&S, A.d.state, sizeof S, hipMemcpyDeviceToHost, s));

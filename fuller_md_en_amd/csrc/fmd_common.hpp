@@ -164,6 +164,17 @@ FMD_HD void quat_advance(double *q, double wx, double wy, double wz, double dt) 
 // Shifted 12-6 in FP32, with the energy kept in units of 4 eps and the shift
 // folded in (e = sr6 (sr6 - 1) - vshift / (4 eps)), so one pair costs one
 // reciprocal and 14 FMA-class operations.
+#if defined(__HIPCC__)
+// HIP events of one step on the engine's stream, bracketing its phases; a
+// null pointer to this struct means "not timed".  flags goes to
+// hipEventRecordWithFlags (hipEventRecordExternal inside a graph capture).
+struct StepEvents {
+  hipEvent_t f0, fm, f1;  // force phase: start, after the intramolecular part, end
+  hipEvent_t l0, l1;      // list rebuild (recorded on rebuild steps only)
+  unsigned flags;
+};
+#endif
+
 struct LJ32 {
   float sig2;   // sigma^2
   float rc2;    // cutoff^2 (pairs with r^2 < rc2, or <= for AIREBO)

@@ -41,11 +41,12 @@ struct fmd_engine {
   fmd::RigidDev rd{};
   // atomic
   fmd::AtomicEngine *at = nullptr;
-  // timing of the force kernel (HIP events on `stream`)
+  // timing of the step phases (HIP events on `stream`): kPhase* sums of the
+  // durations (ms) and their counts since the last read
   bool timing = false;
-  std::vector<hipEvent_t> ev;
-  double t_sum = 0;
-  int t_n = 0;
+  std::vector<hipEvent_t> ev;  // kEvPerStep per step of a run chunk
+  double t_sum[4] = {0, 0, 0, 0};
+  int t_n[4] = {0, 0, 0, 0};
   // hipGraph replay of step sequences: executable graphs keyed by the step
   // pattern they were captured from
   bool use_graph = false;
@@ -53,10 +54,16 @@ struct fmd_engine {
 };
 
 namespace fmd {
+// Timed phases: the force phase, its intramolecular part (REBO, surrogate or
+// bonded terms; none for rigid molecules), its intermolecular LJ part, and a
+// list rebuild.
+enum : int { kPhaseForce = 0, kPhaseIntra = 1, kPhaseLJ = 2, kPhaseList = 3 };
+constexpr int kEvPerStep = 5;  // StepEvents f0, fm, f1, l0, l1
 int engine_init_common(fmd_engine *e, int ring, int R);
-// Launch of one step into control / record slot `slot`.
-using StepLaunch = std::function<hipError_t(int slot, int rebuild, int rescale, hipEvent_t ev0,
-                                            hipEvent_t ev1, unsigned ev_flags)>;
+// Launch of one step into control / record slot `slot`; ev: the step's
+// events, or null when not timed.
+using StepLaunch =
+    std::function<hipError_t(int slot, int rebuild, int rescale, const StepEvents *ev)>;
 // Steps first_step+1 .. first_step+nsteps of R replicas: controls
 // ctrl[k * R + r] (null: hold each replica's target tt[r]), records
 // thermo[k * R + r]; list rebuild every e->nlup global steps or on request.

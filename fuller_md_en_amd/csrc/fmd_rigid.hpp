@@ -139,11 +139,11 @@ hipError_t rigid_launch_observe(const RigidDev &d, const RigidParams &p, int wha
 // kick-drift + sites, forces, kick + reductions + NPT second half, optional
 // velocity rescale.  Controls come from ctrl[ctrl_slot]; the record goes to
 // thermo[thermo_slot].
-// ev0 / ev1 (may be null) are recorded around the force kernel, with
-// ev_flags (hipEventRecordExternal inside a graph capture).
+// ev (may be null): events around the force kernel (f0 and fm before it, f1
+// after) and the list rebuild.
 hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_slot,
-                             int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
-                             hipEvent_t ev1, unsigned ev_flags, hipStream_t s);
+                             int thermo_slot, int rebuild, int rescale, const StepEvents *ev,
+                             hipStream_t s);
 // Sum of molpart (W9, E) into out10 on the device.
 hipError_t rigid_launch_reduce_molpart(const RigidDev &d, const RigidParams &p, double *out10,
                                        hipStream_t s);

@@ -1268,26 +1268,30 @@ hipError_t rigid_launch_observe(const RigidDev &d, const RigidParams &p, int wha
 }
 
 hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_slot,
-                             int thermo_slot, int rebuild, int rescale, hipEvent_t ev0,
-                             hipEvent_t ev1, unsigned ev_flags, hipStream_t s) {
+                             int thermo_slot, int rebuild, int rescale, const StepEvents *ev,
+                             hipStream_t s) {
   hipError_t e;
+  const unsigned evf = ev ? ev->flags : 0;
   const double *state_hh = reinterpret_cast<const double *>(
       reinterpret_cast<const char *>(d.state) + offsetof(RigidState, h));
   const double *mid_hh = reinterpret_cast<const double *>(
       reinterpret_cast<const char *>(d.mid) + offsetof(RigidMid, h));
   if (rebuild) {
+    if (ev && (e = hipEventRecordWithFlags(ev->l0, s, evf)) != hipSuccess) return e;
     if ((e = rigid_launch_nlist(d, p, state_hh, s)) != hipSuccess) return e;
     if (p.use_half && (e = rigid_launch_halflist(d, p, state_hh, s)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecordWithFlags(ev->l1, s, evf)) != hipSuccess) return e;
   }
   k_rigid_kick_drift<<<(p.N + kKdMol - 1) / kKdMol, 256, 0, s>>>(d, p, ctrl_slot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (ev0 && (e = hipEventRecordWithFlags(ev0, s, ev_flags)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecordWithFlags(ev->f0, s, evf)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecordWithFlags(ev->fm, s, evf)) != hipSuccess) return e;
   if (p.use_half)
     e = rigid_launch_forces_half(d, p, mid_hh, s);
   else
     e = rigid_launch_forces(d, p, mid_hh, s);
   if (e != hipSuccess) return e;
-  if (ev1 && (e = hipEventRecordWithFlags(ev1, s, ev_flags)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecordWithFlags(ev->f1, s, evf)) != hipSuccess) return e;
   k_rigid_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
   k_rigid_finish<<<1, 256, 0, s>>>(d, p, ctrl_slot, thermo_slot);
   if (rescale) k_rigid_rescale<<<(p.N + 255) / 256, 256, 0, s>>>(d, p);

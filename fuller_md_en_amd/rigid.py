@@ -175,5 +175,13 @@ class RigidEngine:
         N.check(N.lib().fmd_engine_kernel_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def phase_times(self):
+        """Mean ms and counts of the timed phases since the last read:
+        {"force", "intra", "lj", "list"} -> (ms, n)."""
+        ms, n = np.zeros(4), np.zeros(4, np.int32)
+        N.check(N.lib().fmd_engine_phase_times(self._h, ms, n))
+        return {k: (float(ms[i]), int(n[i])) for i, k in
+                enumerate(("force", "intra", "lj", "list"))}
+
     def sync(self):
         N.check(N.lib().fmd_engine_sync(self._h))

@@ -18,6 +18,11 @@ from . import _native as N
 NPARAM = None
 NFEAT = None
 H = 64
+# Matrix-core work per atom-step: the network's forward products
+# 2 (40*64 + 64*64 + 64*64 + 64) and the same again for the input gradient
+# that the forces need (the descriptor and force projections run on the
+# vector units and are not counted).
+FLOP_PER_ATOM = 2 * 2 * (40 * H + H * H + H * H + H)
 # parameter layout of theta (offsets into the flat FP32 vector)
 _SHAPES = None
 

@@ -450,6 +450,12 @@ int fmd_engine_counters(fmd_engine *e, long long *out3);
  * fmd_engine_set_timing(e, 1)).  *n receives the number of launches. */
 int fmd_engine_set_timing(fmd_engine *e, int on);
 int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n);
+/* Mean durations (ms) and counts of the timed step phases since the last
+ * read: [0] the force phase, [1] its intramolecular part (REBO, surrogate or
+ * bonded terms; 0 for rigid molecules), [2] its intermolecular LJ part, [3] a
+ * list rebuild (rebuild steps only).  Resets the sums, as does
+ * fmd_engine_kernel_time. */
+int fmd_engine_phase_times(fmd_engine *e, double *mean_ms4, int *n4);
 
 /* Replay step sequences from captured HIP graphs (1) or launch every kernel
  * (0).  Default: on for batches of more than one replica, off otherwise. */

@@ -77,9 +77,11 @@ def lj():
     return _libs["lj"]
 
 
-def airebo():
-    if "airebo" not in _libs:
-        L = C.CDLL(path("libref_airebo.so"))
+def airebo(omp: bool = False):
+    """The AIREBO harness: the serial build, or (omp) the OpenMP build."""
+    key = "airebo_omp" if omp else "airebo"
+    if key not in _libs:
+        L = C.CDLL(path("libref_airebo_omp.so" if omp else "libref_airebo.so"))
         L.refab_load_cc1.argtypes = [C.c_char_p, _dp, _dp]
         L.refab_nlist_rebo.argtypes = [_dp, _dp, _dp, _i, _ip, _ip, _ip]
         L.refab_nlist_lj.argtypes = [_dp, _dp, _dp, _i, _ip, _ip, _ip]
@@ -94,8 +96,8 @@ def airebo():
                                      _ip, _ip, _ip, _dp]
         L.refab_apply_pbc.argtypes = [_dp, _dp, _dp, _i]
         L.refab_mat_inv9.argtypes = [_dp, _dp]
-        _libs["airebo"] = L
-    return _libs["airebo"]
+        _libs[key] = L
+    return _libs[key]
 
 
 def mm():
