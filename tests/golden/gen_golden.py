@@ -36,6 +36,10 @@ Fixtures
                           list truncates): inputs and outputs only.
   mm_c70_hcp.npz          MM on C70 in a 2x2x2 HCP cell (1120 atoms): topology
                           and forces.
+  airebo_cfg3_nc5.npz     AB at config 3's size, C60 FCC 5x5x5 (30,000 atoms,
+                          +-0.01 A jitter): list counts and last LJ entries,
+                          compute_forces, and 5 NPT steps (float32 inputs
+                          and forces, stored exactly).
   core_ensemble_nc3.npz   the core program's main loop (C1:683-733) at nc=3
                           (config 1: 108 molecules, 1000 steps) for seeds
                           42..49: per-step T, P, a, Ecoh of seed 42 (the
@@ -260,6 +264,44 @@ def gen_airebo():
          E=np.array(E), F=F, W=W)
 
 
+def gen_airebo_cfg3():
+    """Config 3 at its size: C60 FCC 5x5x5 (30,000 atoms), +-0.01 A jitter
+    (numpy PCG64 seed 12345); positions and velocities rounded to float32 so
+    that the fixture stores them exactly in half the bytes."""
+    L = ref.airebo()
+    pos, h, mol, a0 = airebo_system(5)
+    pos = pos.astype(np.float32).astype(np.float64)
+    na = len(pos)
+    lists = ref.ab_nlists(pos, h, mol)
+    nlc_r, nll_r, nlc_l, nll_l = lists
+    E, F, W = ref.ab_compute_forces(pos, h, lists)
+    rng = np.random.default_rng(99)
+    vel = np.sqrt(oracle.KB * 300.0 * oracle.CONV / 12.011) * rng.normal(size=(na, 3))
+    vel = (vel - vel.mean(axis=0)).astype(np.float32).astype(np.float64)
+    mass = np.full(na, 12.011)
+    state = np.zeros(15)
+    L.refab_make_npt(300.0, 0.0, na, state)
+    npt0 = state.copy()
+    P, V, FF, WW, hh, hi = (pos.reshape(-1).copy(), vel.reshape(-1).copy(), F.reshape(-1).copy(),
+                            W.copy(), h.copy(), oracle.inv9(h))
+    out, traj = np.zeros(4), []
+    r_cnt, r_lst, l_cnt, l_lst = (x.reshape(-1).copy() for x in lists)
+    for _ in range(5):
+        L.refab_step_npt(P, V, FF, WW, hh, hi, mass, na, 0.5, state, r_cnt, r_lst, l_cnt, l_lst,
+                         out)
+        traj.append(out.copy())
+    # displacements by the minimum image of the initial cubic cell
+    L0 = np.diag(h.reshape(3, 3))
+    dpos = P.reshape(na, 3) - pos
+    dpos -= np.round(dpos / L0) * L0
+    save("airebo_cfg3_nc5.npz", pos=pos.astype(np.float32), vel=vel.astype(np.float32), h=h,
+         a0=a0, nlc_r=nlc_r.astype(np.int8), nlc_l=nlc_l.astype(np.int16),
+         lj_last=np.array([nll_l[i, nlc_l[i] - 1] if nlc_l[i] else -1 for i in range(na)],
+                          np.int32),
+         E=np.array(E), F=F.astype(np.float32), W=W, npt0=npt0, traj=np.array(traj),
+         dpos_end=dpos.astype(np.float32), h_end=hh, npt_end=state)
+
+
 MM_K4 = np.array([469.0, 63.0, 14.5, 15.0]) / 23.06054783  # MM:1039-1042
 
 
@@ -371,6 +413,9 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference exists")
     if sys.argv[1:] == ["lattices"]:
         gen_cc1(LATTICE_CASES)
+        sys.exit(0)
+    if sys.argv[1:] == ["airebo_cfg3"]:
+        gen_airebo_cfg3()
         sys.exit(0)
     if sys.argv[1:] == ["core_ensemble"]:
         gen_core_ensemble()
