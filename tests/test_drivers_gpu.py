@@ -222,3 +222,112 @@ def test_mm_driver_physical_stays_bounded(tmp_path):
     assert rc == 0 and "Done" in out
     r = rows(out)
     assert len(r) == 4 and max(x[2][0] for x in r) < 600.0
+
+
+AB_REF = os.path.join(REF, "fuller_airebo_npt_md_serial")
+needs_ab_ref = pytest.mark.skipif(not os.path.exists(AB_REF),
+                                  reason="reference AIREBO build (oracle/_ref) not present")
+
+
+@needs_ab_ref
+def test_airebo_restart_files_cross_load(tmp_path):
+    """write_restart_airebo / read_restart_airebo (AB:629-677): our AIREBO
+    restart resumed by the reference, and the reference's by us, give the
+    same next rows; the two step-10 files hold the same atoms."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    base = ["--cell=2", "--temp=300", "--step=20", "--mon=5", "--restart=10", "--seed=4"]
+    assert run(AB, base, a)[0] == 0
+    assert run(AB_REF, base, b)[0] == 0
+    ours = sorted(p for p in os.listdir(a) if p.endswith("_10.rst"))[0]
+    theirs = sorted(p for p in os.listdir(b) if p.endswith("_10.rst"))[0]
+    la = [l.split() for l in open(a / ours) if l.startswith("ATOM")]
+    lb = [l.split() for l in open(b / theirs) if l.startswith("ATOM")]
+    assert len(la) == len(lb) == 1920
+    # "ATOM index mol_id mass x y z vx vy vz": tag, index, molecule and mass
+    # text equal, then positions and velocities
+    assert [x[:4] for x in la] == [x[:4] for x in lb] and {len(x) for x in la} == {10}
+    va = np.array([[float(v) for v in x[4:]] for x in la])
+    vb = np.array([[float(v) for v in x[4:]] for x in lb])
+    assert np.abs(va[:, :3] - vb[:, :3]).max() < 1e-6  # positions, A
+    assert np.abs(va[:, 3:] - vb[:, 3:]).max() < 1e-5 * np.abs(vb[:, 3:]).max()
+    tags = lambda p: [l.split()[0] for l in open(p) if l.strip() and not l.startswith("#")]
+    assert tags(a / ours) == tags(b / theirs)
+    cont = ["--cell=2", "--temp=300", "--step=20", "--mon=5", "--seed=4"]
+    _, o1 = run(AB_REF, [*cont, f"--resfile={a / ours}"], tmp_path)
+    _, o2 = run(AB, [*cont, f"--resfile={b / theirs}"], tmp_path)
+    _, o3 = run(AB, [*cont, f"--resfile={a / ours}"], tmp_path)
+    assert "Restarting from global step 10" in o1 and "Restarting from global step 10" in o2
+    _compare(rows(o3), rows(o1), AB_TOL)
+    _compare(rows(o2), rows(o1), AB_TOL)
+
+
+def _xyz_frames(path):
+    lines = open(path).read().splitlines()
+    frames, i = [], 0
+    while i < len(lines):
+        n = int(lines[i])
+        frames.append((n, lines[i + 1], lines[i + 2:i + 2 + n]))
+        i += 2 + n
+    return frames
+
+
+ATOM_LINE = re.compile(r"^C +-?\d+\.\d{8} +-?\d+\.\d{8} +-?\d+\.\d{8} +\d+ +"
+                       r"-?\d\.\d{8}e[-+]\d\d +-?\d\.\d{8}e[-+]\d\d +-?\d\.\d{8}e[-+]\d\d$")
+
+
+def _compare_ovito(ours, theirs, pos_tol, vel_rel):
+    """Field by field: atom count, header text apart from the lattice
+    numbers, lattice, species, positions (minimum image), molecule ids and
+    velocities, each line in the reference's format."""
+    fa, fb = _xyz_frames(ours), _xyz_frames(theirs)
+    assert len(fa) == len(fb) > 0
+    for (na, ha, ba), (nb, hb, bb) in zip(fa, fb):
+        assert na == nb == len(ba) == len(bb)
+        ma = re.match(r'Lattice="([^"]+)" (.*)$', ha)
+        mb = re.match(r'Lattice="([^"]+)" (.*)$', hb)
+        assert ma and mb and ma.group(2) == mb.group(2)  # Properties, Time, Step, pbc
+        lat_a = np.array([float(x) for x in ma.group(1).split()])
+        lat_b = np.array([float(x) for x in mb.group(1).split()])
+        assert np.abs(lat_a - lat_b).max() < 1e-6
+        assert all(ATOM_LINE.match(l) for l in ba), ba[0]
+        A = np.array([[float(x) for x in l.split()[1:]] for l in ba])
+        B = np.array([[float(x) for x in l.split()[1:]] for l in bb])
+        assert np.array_equal(A[:, 3], B[:, 3])  # molecule ids
+        box = lat_b.reshape(3, 3)  # rows: the cell vectors
+        d = A[:, :3] - B[:, :3]
+        s = d @ np.linalg.inv(box)
+        d = (s - np.round(s)) @ box
+        assert np.abs(d).max() < pos_tol
+        assert np.abs(A[:, 4:] - B[:, 4:]).max() < vel_rel * np.abs(B[:, 4:]).max()
+
+
+@needs_ref
+def test_lj_ovito_file_equals_reference(tmp_path):
+    """write_ovito (LJ:541-555): the same frames as the reference's file."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    args = ["--cell=2", "--temp=300", "--step=40", "--mon=20", "--ovito=20", "--seed=9"]
+    assert run(LJ, args, a)[0] == 0
+    assert run(os.path.join(REF, "fuller_LJ_npt_md_serial"), args, b)[0] == 0
+    fa = [p for p in os.listdir(a) if p.startswith("ovito_traj")]
+    fb = [p for p in os.listdir(b) if p.startswith("ovito_traj")]
+    assert len(fa) == len(fb) == 1
+    _compare_ovito(a / fa[0], b / fb[0], 1e-6, 1e-5)
+
+
+@needs_ab_ref
+def test_airebo_ovito_file_equals_reference(tmp_path):
+    """write_ovito (AB:593-601): the same frames as the reference's file."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    args = ["--cell=2", "--temp=300", "--step=20", "--mon=10", "--ovito=10", "--seed=6"]
+    assert run(AB, args, a)[0] == 0
+    assert run(AB_REF, args, b)[0] == 0
+    fa = [p for p in os.listdir(a) if p.startswith("ovito_traj")]
+    fb = [p for p in os.listdir(b) if p.startswith("ovito_traj")]
+    assert len(fa) == len(fb) == 1
+    _compare_ovito(a / fa[0], b / fb[0], 1e-6, 1e-5)
