@@ -82,6 +82,7 @@ RIGID_TRAFFIC = {1: (9710.9 * 2.0 + 23524.8 * 1.0) * 1024.0,
                  2: (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0}
 AIREBO_LJ_TRAFFIC = (66058.9 * 2.0 + 3040.2 * 1.0) * 1024.0
 RIGID_KERNEL = {1: "k_rigid_pairs", 2: "k_rigid_forces_half", 0: "k_rigid_forces"}
+TIMING_STRIDE = 10
 
 
 def parse():
@@ -155,7 +156,9 @@ def timed_run(eng, warmup, steps, nlup):
     if warmup:
         eng.run(0, warmup, thermo=False)
     eng.sync()
-    eng.set_timing(True)
+    # HIP events on every 10th step and on every list rebuild: each event
+    # writes back the caches, so timing every step would slow the run itself
+    eng.set_timing(True, stride=TIMING_STRIDE)
     t0 = time.perf_counter()
     th = eng.run(warmup, steps, thermo=True)
     eng.sync()

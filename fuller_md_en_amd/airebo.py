@@ -146,8 +146,10 @@ class AtomicEngine:
         N.check(N.lib().fmd_engine_counters(self._h, out))
         return list(out)
 
-    def set_timing(self, on=True):
-        N.check(N.lib().fmd_engine_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on=True, stride=1):
+        """Event timing of the step phases on every `stride`-th step (and on
+        list rebuilds)."""
+        N.check(N.lib().fmd_engine_set_timing(self._h, stride if on else 0))
 
     def kernel_time(self):
         ms, n = C.c_double(0), C.c_int(0)
@@ -248,8 +250,10 @@ class ReplicaBatch:
         N.check(N.lib().fmd_engine_counters(self._h, out))
         return list(out)
 
-    def set_timing(self, on=True):
-        N.check(N.lib().fmd_engine_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on=True, stride=1):
+        """Event timing of the step phases on every `stride`-th step (and on
+        list rebuilds)."""
+        N.check(N.lib().fmd_engine_set_timing(self._h, stride if on else 0))
 
     def kernel_time(self):
         ms, n = C.c_double(0), C.c_int(0)

@@ -94,6 +94,19 @@ __device__ __forceinline__ void block_sum(const double *v, double *out, int stri
   __syncthreads();
 }
 
+// Per-record partials of a trace-only evaluation: the block sums of the
+// virial trace w and the energy e give W = (w/3) 1 and E, written as the
+// kAtPart components (W9 row-major, E) at out[k * stride].
+__device__ __forceinline__ void block_sum_trace(double w, double e, double *out, int stride) {
+  const double v[2] = {w, e};
+  __shared__ double tot[2];
+  block_sum<2>(v, tot);
+  if (threadIdx.x < kAtPart) {
+    const int k = threadIdx.x;
+    out[(size_t)k * stride] = k == 9 ? tot[1] : (k == 0 || k == 4 || k == 8) ? tot[0] / 3.0 : 0.0;
+  }
+}
+
 // ----------------------------------------------------------------- pbc
 __global__ __launch_bounds__(256) void k_at_apply_pbc(double *__restrict__ pos,
                                                       const int *__restrict__ atom_rep,
@@ -685,29 +698,51 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 
 // ------------------------------------------------------------------ REBO
 // One 256-thread workgroup per molecule (the REBO list never leaves a
-// molecule), both passes of fmd_rebo_passes.hpp in one launch: pass 1 for
-// every directed entry of the molecule, a barrier, pass 2 one thread per
-// entry (entries packed atom after atom by prefix sums of the list counts,
-// entry e on thread e mod 256), a barrier, and each atom's force as the sum
-// of its entries' in list order.  Energy and virial: every thread's share,
-// one fixed-order workgroup sum.
+// molecule).  Energy and virial: every thread's share, one fixed-order
+// workgroup sum.
 //
-// Fast path (the molecule has at most kReboSlots entries per atom on average
-// and the caller sized the LDS for it): every entry lives in LDS with local
-// atom indices, so the dependent lookups of pass 2 (reverse entry, the
-// centre's other bonds) are LDS reads.  Otherwise the entries go through the
-// global ReboBonds arrays in the reference list layout and the entry forces
-// through rebo_fe (the workgroup's barrier makes its own global writes
-// visible to it).  Both paths do the same arithmetic in the same order, so
-// the result does not depend on which one a molecule takes.
+// Fast path (the molecule has at most kReboSlots list entries and
+// kReboTriSlots angular triples per atom on average, and the caller sized the
+// LDS for them): the molecule's entries are packed atom after atom in LDS by
+// prefix sums of the list counts, with local atom indices, and
+//   B  one thread per entry: the FP64 minimum-image bond vector, kept as FP32
+//      (x, y, z, r) and the FP64 length;
+//   C  one thread per entry: the reverse entry, the entry's place among its
+//      atom's bonds (entries shorter than the 2.0 A switch, the only ones that
+//      take part in angular terms), the FP32 bond-order sum Gs and
+//      b = (1 + Gs)^-1/2; the bond's owner (lower atom index) computes the
+//      FP64 switch and pair quantities once per bond (bond_terms);
+//   D  one thread per bond entry c -> p: every angular triple (c, p, t) over
+//      c's other bonds t, stored once (force ft on t, fp on p);
+//   E  one thread per entry x -> y: every force component landing on x
+//      through it, in a fixed order -- the pair term, the triples centred on
+//      x with partner y, those centred on y with x as the third atom and (in
+//      the physical mode) with x as the partner;
+// then each atom's force is the sum of its entries' in list order.  Each
+// angular triple is evaluated once (the entry-based form evaluates it from
+// all three of its atoms).
+//
+// Otherwise the entries go through the global ReboBonds arrays in the
+// reference list layout and the entry forces through rebo_fe, evaluating
+// the triples where they are used (rebo_pass2_entry); both paths compute the
+// same FP32 triple terms and FP64 pair terms and add them in the same order,
+// and the path depends on the molecule alone.
 constexpr int kReboSlots = 6;
-// LDS bytes per entry of the fast path: vec (double4), bp + bq (2 x double4),
-// bd + ba (2 x float4), flag, reverse entry, neighbour, owning atom
-constexpr int kReboSlotBytes = 32 + 64 + 32 + 16;
+constexpr int kReboTriSlots = 7;
+// fast-path LDS bytes per entry: P (double4), FP64 length and b (double
+// each), bond vector (float4), fc / dfc (float2), reverse entry and owner
+// list (u16 each), local neighbour, owning atom, bond index, bond flags,
+// entry flags (u8 each)
+constexpr int kReboSlotBytes = 32 + 16 + 16 + 8 + 4 + 5;
+constexpr int kReboTriBytes = 24;  // ft, fp (float3 each)
 constexpr size_t kReboFastLdsMax = 64 * 1024;
+constexpr int kNoRev = 0xffff;  // no reverse entry (an asymmetric, overflowed list)
+constexpr int kNoBond = 0x7f;   // entry beyond the switch: not one of the atom's bonds
+enum : int { kEntryBond = 1, kBondVaPos = 8 };
 
 __host__ __device__ inline size_t rebo_fast_lds(int apad) {
-  return (size_t)kReboSlots * apad * kReboSlotBytes + 4 * (2 * (size_t)apad + 1);
+  return (size_t)kReboSlots * apad * kReboSlotBytes + (size_t)kReboTriSlots * apad * kReboTriBytes +
+         4 * (2 * (size_t)apad + 1) + (size_t)apad + 16;
 }
 // global path: bond vectors of 64 atoms, prefix sums and owning atoms
 __host__ __device__ inline size_t rebo_slow_lds(int apad, int maxr) {
@@ -718,7 +753,7 @@ template <int MODE, bool TRACE>
 __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, HHRef hh,
                                                    int fast_ok) {
   extern __shared__ double4 lds[];
-  __shared__ int wtot[4];
+  __shared__ int wtot[4], wtri[4], nown;
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int m = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
   if (m >= p.Nm) return;
@@ -737,55 +772,251 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
   int off = 0;
   for (int w = 0; w < wv; w++) off += wtot[w];
   const int total = ((wtot[0] + wtot[1]) + wtot[2]) + wtot[3];
-  // the choice depends on the molecule alone (not on the tile width apad)
-  const bool fast = fast_ok && total <= kReboSlots * n;
+  // the choice depends on the molecule alone (not on the tile width apad);
+  // the gather keeps kReboPer entries per thread in registers
+  constexpr int kReboPer = 2;
+  bool fast = fast_ok && total <= kReboSlots * n && total <= 256 * kReboPer;
   double E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (fast) {
-    const int S = kReboSlots * p.apad;
-    double4 *vec = lds;
-    double *bp = (double *)(vec + S), *bq = bp + 4 * S;
-    float *bd = (float *)(bq + 4 * S), *ba = bd + 4 * S;
-    int *flag = (int *)(ba + 4 * S), *rev = flag + S, *nll = rev + S, *eat = nll + S;
-    int *nlc = eat + S, *pre = nlc + p.apad;
+    const int S = kReboSlots * p.apad, TS = kReboTriSlots * p.apad;
+    char *base = reinterpret_cast<char *>(lds);
+    double4 *P = reinterpret_cast<double4 *>(base);
+    double *rdv = reinterpret_cast<double *>(P + S);  // FP64 length
+    double *bv = rdv + S;                              // b of the direction
+    float4 *bd = reinterpret_cast<float4 *>(bv + S);
+    float2 *fcd = reinterpret_cast<float2 *>(bd + S);
+    float *tri = reinterpret_cast<float *>(fcd + S);  // 6 floats per triple
+    int *pre = reinterpret_cast<int *>(tri + 6 * TS);
+    int *tpre = pre + p.apad + 1;  // triple offsets, exclusive within each wavefront
+    unsigned short *rev = reinterpret_cast<unsigned short *>(tpre + p.apad);
+    unsigned short *ownl = rev + S;  // the entries that own their bond, packed
+    unsigned char *nloc = reinterpret_cast<unsigned char *>(ownl + S);
+    unsigned char *eat = nloc + S, *bidx = eat + S, *bflg = bidx + S, *eflg = bflg + S;
+    unsigned char *nb = eflg + S;
     if (t < n) {
-      nlc[t] = c;
       pre[t] = off + v - c;
-      for (int jn = 0; jn < c; jn++) eat[off + v - c + jn] = t;
+      for (int jn = 0; jn < c; jn++) eat[off + v - c + jn] = (unsigned char)t;
+    }
+    if (t == 0) {
+      pre[n] = total;
+      nown = 0;
     }
     __syncthreads();
-    const ReboBonds B{bd, ba, bp, bq, flag, rev};
-    const SlotPrefix slot{pre};
-    // local neighbours and bond vectors (FP64 minimum image)
+    // B: local neighbours and bond vectors (FP64 minimum image); the entries
+    // that own their bond (lower atom index first) are listed in any order --
+    // which thread takes a bond does not change its result -- so that the
+    // FP64 bond terms run on as few wavefronts as possible
     for (int e = t; e < total; e += 256) {
       const int xl = eat[e], i = s0 + xl;
       const int j = d.nll_r[(size_t)i * p.maxr + (e - pre[xl])];
-      nll[e] = j - s0;
+      nloc[e] = (unsigned char)(j - s0);
       double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
              dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
       mimg(dx, dy, dz, hi, h);
-      vec[e] = make_double4(dx, dy, dz, sqrt(dx * dx + dy * dy + dz * dz));
+      const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+      bd[e] = make_float4((float)dx, (float)dy, (float)dz, (float)rd);
+      rdv[e] = rd;
+      eflg[e] = rd < rebo::kDmax ? kEntryBond : 0;
+      if (xl < j - s0) ownl[atomicAdd(&nown, 1)] = (unsigned short)e;
     }
     __syncthreads();
+    // C: atoms -- bond counts and the triple offsets (within each wavefront)
+    int ntri = 0;
+    if (t < n) {
+      int k = 0;
+      for (int e = pre[t]; e < pre[t + 1]; e++) k += eflg[e] & kEntryBond;
+      nb[t] = (unsigned char)k;
+      ntri = k * (k - 1);
+    }
+    {
+      int w = ntri;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(w, o, 64);
+        if (lane >= o) w += u;
+      }
+      if (t < n) tpre[t] = w - ntri;
+      if (lane == 63) wtri[wv] = w;
+    }
+    //    owners: the bond's FP64 terms, into its own slot and its reverse's
+    auto rev_of = [&](int e) {
+      const int xl = eat[e], yl = nloc[e];
+      int rv = kNoRev;
+      for (int f = pre[yl]; f < pre[yl + 1]; f++)
+        if (nloc[f] == xl) rv = f;
+      return rv;
+    };
+    auto owner_terms = [&](int e, int rv) {
+      double fd, dfd;
+      double4 P4;
+      int bf = bond_terms(rdv[e], fd, dfd, P4);
+      if (P4.y > 0.0) bf |= kBondVaPos;
+      P[e] = P4;
+      const float2 fc2 = make_float2((float)fd, (float)dfd);
+      fcd[e] = fc2;
+      bflg[e] = (unsigned char)bf;
+      if (rv != kNoRev) {
+        fcd[rv] = fc2;
+        bflg[rv] = (unsigned char)bf;
+      }
+    };
+    for (int k = t; k < nown; k += 256) {
+      const int e = ownl[k];
+      owner_terms(e, rev_of(e));
+    }
+    //    entries -- reverse entry, bond index, bond order
     for (int e = t; e < total; e += 256) {
-      const int xl = eat[e];
-      rebo_pass1<float>(xl, e - pre[xl], vec + pre[xl], nlc, nll, slot, B);
+      const int xl = eat[e], yl = nloc[e];
+      const int rv = rev_of(e);
+      rev[e] = (unsigned short)rv;
+      const double rd = rdv[e];
+      int bi = kNoBond;
+      if (eflg[e] & kEntryBond) {
+        bi = 0;
+        for (int f = pre[xl]; f < e; f++) bi += eflg[f] & kEntryBond;
+      }
+      // an entry above its neighbour whose reverse is missing (an
+      // asymmetric, overflowed list) owns its bond too
+      if (xl > yl && rv == kNoRev) owner_terms(e, rv);
+      double b = 0.0;
+      bool gs_pos = false;
+      if (rd <= rebo::kDmax) {
+        const float4 u = bd[e];
+        const float Gs = bond_order_sum<float>(e - pre[xl], pre[xl + 1] - pre[xl], u.x, u.y, u.z,
+                                               u.w, [&](int kn) { return bd[pre[xl] + kn]; });
+        b = rebo::rsq(1.0 + (double)Gs);
+        gs_pos = Gs > 1e-20f;
+      }
+      bv[e] = b;
+      bidx[e] = (unsigned char)(bi | (gs_pos ? 0x80 : 0));
     }
     __syncthreads();
-    double *Fe = (double *)vec;  // pass 1 is done with vec
-    for (int e = t; e < total; e += 256) {
-      const int xl = eat[e];
-      double F[3] = {0, 0, 0};
-      rebo_pass2_entry<float, MODE, TRACE>(xl, e - pre[xl], nlc, nll, slot, B, F, E, W);
-      for (int k = 0; k < 3; k++) Fe[3 * e + k] = F[k];
+    const int tri_total = ((wtri[0] + wtri[1]) + wtri[2]) + wtri[3];
+    // triple offset of atom x: its wavefront's scan plus the earlier wavefronts'
+    auto tbase = [&](int x) {
+      int o = tpre[x];
+      for (int w = 0; w < (x >> 6); w++) o += wtri[w];
+      return o;
+    };
+    auto own = [&](int e) {
+      const int r = rev[e];
+      return (eat[e] < nloc[e] || r == kNoRev) ? e : r;
+    };
+    if (tri_total > kReboTriSlots * n) {
+      fast = false;  // more angular triples than the LDS holds: the global path
+    } else {
+      // D: angular triples (c, p, t), one thread per bond entry c -> p
+      for (int e = t; e < total; e += 256) {
+        const int bi = bidx[e] & kNoBond;
+        const int xl = eat[e], nbx = nb[xl];
+        if (bi == kNoBond || nbx < 2) continue;
+        const int bf = bflg[e];
+        const bool ang = (bf & kBondPair) && (bidx[e] & 0x80) && (bf & kBondVaPos);
+        const float4 u = bd[e];
+        const float d3[3] = {u.x, u.y, u.z};
+        const float coeff = ang ? angular_coeff(P[own(e)].y, bv[e]) : 0.f;
+        float *out = tri + 6 * (tbase(xl) + bi * (nbx - 1));
+        for (int f = pre[xl]; f < pre[xl + 1]; f++) {
+          if (f == e || (bidx[f] & kNoBond) == kNoBond) continue;
+          float ft[3] = {0.f, 0.f, 0.f}, fp[3] = {0.f, 0.f, 0.f};
+          if (ang && (bflg[f] & kBondThird)) {
+            const float4 w4 = bd[f];
+            const float dt[3] = {w4.x, w4.y, w4.z};
+            const float2 fc2 = fcd[f];
+            rebo::angular(d3, u.w, dt, w4.w, fc2.x, fc2.y, coeff, ft, fp);
+            if (MODE == kReboPhysical) triple_virial<TRACE>(dt, ft, d3, fp, W);
+          }
+          for (int k = 0; k < 3; k++) {
+            out[k] = ft[k];
+            out[3 + k] = fp[k];
+          }
+          out += 6;
+        }
+      }
+      __syncthreads();
+      // E: every force component landing on the entry's atom, one thread per
+      // entry; the forces stay in registers until every thread is done with P
+      double Fq[kReboPer][3];
+#pragma unroll
+      for (int q = 0; q < kReboPer; q++) {
+        Fq[q][0] = Fq[q][1] = Fq[q][2] = 0.0;
+        const int e = t + 256 * q;
+        if (e >= total) continue;
+        double *F = Fq[q];
+        const int xl = eat[e], yl = nloc[e], rv = rev[e], bf = bflg[e];
+        const int bi = bidx[e] & kNoBond;
+        if (bf & kBondPair) {
+          const float4 u = bd[e];
+          const double bbar = 0.5 * (bv[e] + bv[rv != kNoRev ? rv : e]);
+          const double4 P4 = P[own(e)];
+          // pair force (AB:326-330): F_x += fpair d_xy
+          const double fpair = P4.z - bbar * P4.w;
+          F[0] += fpair * (double)u.x;
+          F[1] += fpair * (double)u.y;
+          F[2] += fpair * (double)u.z;
+          if (xl < yl) {
+            E += P4.x - bbar * P4.y;
+            const double dd[3] = {u.x, u.y, u.z};
+            if (TRACE) {
+              W[0] -= fpair * (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+            } else {
+              for (int a = 0; a < 3; a++)
+                for (int cc = 0; cc < 3; cc++) W[3 * a + cc] -= dd[a] * fpair * dd[cc];
+            }
+          }
+          // triples centred on x with partner y
+          const int nbx = nb[xl];
+          if (bi != kNoBond && nbx >= 2) {
+            const float *tp = tri + 6 * (tbase(xl) + bi * (nbx - 1));
+            for (int q2 = 0; q2 < nbx - 1; q2++, tp += 6) {
+              if (MODE == kReboCompat) {
+                for (int k = 0; k < 3; k++) F[k] += (double)tp[k];
+              } else {
+                for (int k = 0; k < 3; k++) F[k] -= (double)(tp[k] + tp[3 + k]);
+              }
+            }
+          }
+        }
+        // x as the third atom (and, physical, as the partner) of triples
+        // centred on its neighbour y
+        if (rv == kNoRev || bi == kNoBond) continue;
+        const int bix = bidx[rv] & kNoBond;  // x among y's bonds
+        const int nby = nb[yl];
+        if (nby < 2) continue;
+        const int yb = tbase(yl);
+        if (bf & kBondThird) {
+          for (int pb = 0; pb < nby; pb++) {
+            if (pb == bix) continue;
+            const float *tp = tri + 6 * (yb + pb * (nby - 1) + (bix < pb ? bix : bix - 1));
+            if (MODE == kReboCompat)
+              for (int k = 0; k < 3; k++) F[k] -= (double)tp[k];
+            else
+              for (int k = 0; k < 3; k++) F[k] += (double)tp[k];
+          }
+        }
+        if (MODE == kReboPhysical) {
+          const float *tp = tri + 6 * (yb + bix * (nby - 1));
+          for (int q2 = 0; q2 < nby - 1; q2++, tp += 6)
+            for (int k = 0; k < 3; k++) F[k] += (double)tp[3 + k];
+        }
+      }
+      __syncthreads();
+      double *Fe = reinterpret_cast<double *>(P);  // E is done with P
+#pragma unroll
+      for (int q = 0; q < kReboPer; q++) {
+        const int e = t + 256 * q;
+        if (e < total)
+          for (int k = 0; k < 3; k++) Fe[3 * e + k] = Fq[q][k];
+      }
+      __syncthreads();
+      if (t < n) {  // atom t, its c entries
+        double F[3] = {0, 0, 0};
+        for (int e = off + v - c; e < off + v; e++)
+          for (int k = 0; k < 3; k++) F[k] += Fe[3 * e + k];
+        for (int k = 0; k < 3; k++) d.Frebo[3 * ((size_t)s0 + t) + k] = F[k];
+      }
     }
-    __syncthreads();
-    if (t < n) {  // atom t, its c entries
-      double F[3] = {0, 0, 0};
-      for (int e = off + v - c; e < off + v; e++)
-        for (int k = 0; k < 3; k++) F[k] += Fe[3 * e + k];
-      for (int k = 0; k < 3; k++) d.Frebo[3 * ((size_t)s0 + t) + k] = F[k];
-    }
-  } else {
+  }
+  if (!fast) {
     double4 *vg = lds;  // [64 atoms][maxr]
     int *pre = (int *)(vg + 64 * p.maxr), *eat = pre + p.apad + 1;
     if (t < n) {
@@ -825,15 +1056,14 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
       for (int k = 0; k < 3; k++) d.Frebo[3 * (size_t)x + k] = F[k];
     }
   }
-  double part[kAtPart];
   if (TRACE) {
-    for (int k = 0; k < 9; k++) part[k] = 0;
-    part[0] = part[4] = part[8] = W[0] / 3.0;
+    block_sum_trace(W[0], E, d.rebo_mpart + m, p.Nm);
   } else {
+    double part[kAtPart];
     for (int k = 0; k < 9; k++) part[k] = W[k];
+    part[9] = E;
+    block_sum<kAtPart>(part, d.rebo_mpart + m, p.Nm);
   }
-  part[9] = E;
-  block_sum<kAtPart>(part, d.rebo_mpart + m, p.Nm);
 }
 
 // -------------------------------------------------------------------- LJ
@@ -1419,25 +1649,25 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
     flush();
   }
   __shared__ double fsum[4][2 * 64][3];
-  double acc[10];
-  for (int k = 0; k < 10; k++) acc[k] = 0;
+  // the virial's 9 components and the energy (time steps: the trace, in
+  // acc[0], and the energy)
+  constexpr int NR = TRACE ? 2 : 10;
+  double acc[NR];
+  for (int k = 0; k < NR; k++) acc[k] = 0;
   const double c48 = 48.0 * p.eps;
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     for (int k = 0; k < 3; k++) fsum[wv][lane + 64 * s][k] = -c48 * GA[s][k];
     if (TRACE) {
-      const double tr = 0.5 * c48 * WA[s][0];
-      acc[0] += tr / 3.0;
-      acc[4] += tr / 3.0;
-      acc[8] += tr / 3.0;
+      acc[0] += 0.5 * c48 * WA[s][0];
     } else {
       const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
       for (int q = 0; q < 9; q++) acc[q] += 0.5 * c48 * WA[s][map[q]];
     }
-    acc[9] += EA[s];
+    acc[NR - 1] += EA[s];
   }
   __shared__ double red[4][10];
-  for (int k = 0; k < 10; k++) {
+  for (int k = 0; k < NR; k++) {
     const double v = wave_sum(acc[k]);
     if (lane == 0) red[wv][k] = v;
   }
@@ -1451,8 +1681,10 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
   }
   if (threadIdx.x < 10) {
     const int k = threadIdx.x;
-    double v = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    const int kr = TRACE ? (k == 9 ? 1 : 0) : k;
+    double v = ((red[0][kr] + red[1][kr]) + red[2][kr]) + red[3][kr];
     if (k == 9) v *= 2.0 * p.eps;  // 1/2 * 4 eps * sum of reduced energies
+    else if (TRACE) v = (k == 0 || k == 4 || k == 8) ? v / 3.0 : 0.0;
     d.lj_part[(size_t)k * p.Nm + mi] = v;
   }
 }

@@ -477,9 +477,13 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
       HIPCHK(hipEventCreate(&ev));
       e->ev.push_back(ev);
     }
+    // slot k is timed on every t_stride-th slot and on list rebuilds (a
+    // graph replay rebuilds at its slot 0 only); events order the stream and
+    // write back caches, so sparse sampling keeps the timed run unperturbed
+    auto timed = [&](int k, int rb) { return e->timing && (k % e->t_stride == 0 || rb); };
     // events of step slot k (graph replays reuse slots 0 .. K-1)
-    auto step_events = [&](int k, unsigned flags, StepEvents *out) -> const StepEvents * {
-      if (!e->timing) return nullptr;
+    auto step_events = [&](int k, int rb, unsigned flags, StepEvents *out) -> const StepEvents * {
+      if (!timed(k, rb)) return nullptr;
       const hipEvent_t *v = &e->ev[(size_t)kEvPerStep * k];
       *out = StepEvents{v[0], v[1], v[2], v[3], v[4], flags};
       return out;
@@ -495,6 +499,7 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
     // of this chunk
     auto take_times = [&](int k0, int m) -> int {
       for (int k = 0; k < m; k++) {
+        if (!timed(k, rebuild[k0 + k])) continue;
         const hipEvent_t *v = &e->ev[(size_t)kEvPerStep * k];
         if (elapsed(v[0], v[2], kPhaseForce) || elapsed(v[0], v[1], kPhaseIntra) ||
             elapsed(v[1], v[2], kPhaseLJ))
@@ -507,7 +512,8 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
       HIPCHK(hipMemcpyAsync(d_ctrl, e->h_ctrl, n * cb, hipMemcpyHostToDevice, s));
       for (int k = 0; k < n; k++) {
         StepEvents sev;
-        HIPCHK(launch(k, rebuild[k], rescale[k], step_events(k, hipEventRecordDefault, &sev)));
+        HIPCHK(launch(k, rebuild[k], rescale[k],
+                      step_events(k, rebuild[k], hipEventRecordDefault, &sev)));
       }
       if (thermo)
         HIPCHK(hipMemcpyAsync(e->h_thermo, d_thermo, n * tb, hipMemcpyDeviceToHost, s));
@@ -522,14 +528,15 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
       for (int k = 0; k < n;) {
         int K = 1;
         while (k + K < n && !rebuild[k + K] && K < kMaxGraphSteps) K++;
-        std::string key = std::to_string(K) + (rebuild[k] ? "R" : "-") + (e->timing ? "T" : "-");
+        std::string key = std::to_string(K) + (rebuild[k] ? "R" : "-") +
+                          (e->timing ? "T" + std::to_string(e->t_stride) : "-");
         for (int j = 0; j < K; j++) key += rescale[k + j] ? '1' : '0';
         hipGraphExec_t exec;
         int rc = engine_graph(e, key, [&]() -> int {
           for (int j = 0; j < K; j++) {
             StepEvents sev;
-            HIPCHK(launch(j, j == 0 ? rebuild[k] : 0, rescale[k + j],
-                          step_events(j, hipEventRecordExternal, &sev)));
+            const int rb = j == 0 ? rebuild[k] : 0;
+            HIPCHK(launch(j, rb, rescale[k + j], step_events(j, rb, hipEventRecordExternal, &sev)));
           }
           return FMD_OK;
         }, &exec);
@@ -677,6 +684,7 @@ int fmd_engine_counters(fmd_engine *e, long long *out3) {
 int fmd_engine_set_timing(fmd_engine *e, int on) {
   if (!e) return set_error(FMD_ERR_ARG, "null engine");
   e->timing = on != 0;
+  e->t_stride = on > 1 ? on : 1;
   for (int k = 0; k < 4; k++) {
     e->t_sum[k] = 0;
     e->t_n[k] = 0;

@@ -44,6 +44,7 @@ struct fmd_engine {
   // timing of the step phases (HIP events on `stream`): kPhase* sums of the
   // durations (ms) and their counts since the last read
   bool timing = false;
+  int t_stride = 1;            // events on every t_stride-th step slot (and on rebuilds)
   std::vector<hipEvent_t> ev;  // kEvPerStep per step of a run chunk
   double t_sum[4] = {0, 0, 0, 0};
   int t_n[4] = {0, 0, 0, 0};

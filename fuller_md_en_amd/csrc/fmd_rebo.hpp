@@ -28,15 +28,21 @@ FMD_HD double si(double x) { return sin(x); }
 FMD_HD float rsq(float x) { return 1.0f / sqrtf(x); }
 FMD_HD double rsq(double x) { return 1.0 / sqrt(x); }
 
+// The FP32 terms below are evaluated without FMA contraction, so every
+// place they are inlined into rounds them the same way and the REBO kernel's
+// two paths agree bit for bit.
+//
 // Cosine switch fc and its derivative (AB:165-171).
 template <class R>
 FMD_HD R fc(R r) {
+#pragma clang fp contract(off)
   if (r <= R(kDmin)) return R(1);
   if (r >= R(kDmax)) return R(0);
   return R(0.5) * (R(1) + co(R(kPi) * (r - R(kDmin)) / R(kDmax - kDmin)));
 }
 template <class R>
 FMD_HD R dfc(R r) {
+#pragma clang fp contract(off)
   if (r <= R(kDmin) || r >= R(kDmax)) return R(0);
   return R(-0.5 * kPi / (kDmax - kDmin)) * si(R(kPi) * (r - R(kDmin)) / R(kDmax - kDmin));
 }
@@ -61,12 +67,14 @@ FMD_HD void pair_terms(R r, R &vr, R &dvr, R &va, R &dva) {
 // cancellation, which keeps FP32 at ~1e-7 relative.
 template <class R>
 FMD_HD R G(R x) {
+#pragma clang fp contract(off)
   const R c2 = R(kGc0 * kGc0), d2 = R(kGd0 * kGd0);
   R hh = R(1) + x, h2 = hh * hh;
   return R(kGa0) * (R(1) + c2 * h2 / (d2 * (d2 + h2)));
 }
 template <class R>
 FMD_HD R dG(R x) {
+#pragma clang fp contract(off)
   const R c2 = R(kGc0 * kGc0), d2 = R(kGd0 * kGd0);
   R hh = R(1) + x, dn = d2 + hh * hh;
   return R(kGa0 * 2.0) * c2 * hh / (dn * dn);
@@ -84,6 +92,7 @@ FMD_HD R clamp1(R x) {
 template <class R>
 FMD_HD void angular(const R *rcp, R r_cp, const R *rct, R r_ct, R fc_ct, R dfc_ct, R coeff,
                     R *ft, R *fp) {
+#pragma clang fp contract(off)
   R inv_ct = R(1) / r_ct, inv_cp = R(1) / r_cp;
   R uct[3] = {rct[0] * inv_ct, rct[1] * inv_ct, rct[2] * inv_ct};
   R ucp[3] = {rcp[0] * inv_cp, rcp[1] * inv_cp, rcp[2] * inv_cp};

@@ -28,14 +28,14 @@ enum : int { kBondPair = 1, kBondThird = 2, kBondAngular = 4 };
 // SlotPrefix):
 //   bd[4*slot]  dx dy dz r      FP32 (r_j - r_i, minimum image): angular terms
 //   ba[4*slot]  fc dfc coeff -  FP32 (coeff = 1/2 fc VA db/dGs)
-//   bp[4*slot]  vr dvr va dva   FP64 pair terms
+//   bp[4*slot]  P = bond_terms  FP64 pair quantities (f VR, f VA, C, D)
 //   bq[4*slot]  r fc dfc b      FP64 (b = (1 + Gs)^-1/2)
 //   bflag[slot] kBond* bits,  brev[slot] slot of i in j's list
 // The pair force is the difference of the repulsive and attractive
 // derivatives, each ~77 eV/A at the C60 bond length against a ~1 eV/A
 // result; FP32 rounding of r alone moves exp(-alpha r) by ~4e-7 relative, so
-// the pair terms and their combination are FP64 (4 exponentials per directed
-// bond).  Bond-order sums and angular terms are FP32.
+// the pair terms and their combination are FP64 (4 exponentials per bond).
+// Bond-order sums and angular terms are FP32.
 struct ReboBonds {
   float *bd, *ba;
   double *bp, *bq;
@@ -52,6 +52,73 @@ struct SlotPrefix {
   const int *pre;
   FMD_HD int operator()(int i, int jn) const { return pre[i] + jn; }
 };
+
+// FP64 terms of a bond of length rd (the same for both directions): switch
+// fd = fc(rd) and dfd, and the pair quantities P = (f VR, f VA, C, D) with
+// C = (dfc VR + f VR') / r and D = (dfc VA + f VA') / r, so that the pair
+// energy is P.x - bbar P.y and the pair force on i along r_j - r_i is
+// (P.z - bbar P.w) per unit of that vector (AB:315-330).  Returns the
+// kBondPair / kBondThird bits; 0 (and zeros) beyond the switch.
+FMD_HD int bond_terms(double rd, double &fd, double &dfd, double4 &P) {
+  using namespace rebo;
+  if (rd > kDmax) {
+    fd = dfd = 0;
+    P = make_double4(0, 0, 0, 0);
+    return 0;
+  }
+  fd = fc(rd);
+  dfd = dfc(rd);
+  int flag = 0;
+  if (!(fd < 1e-15 && dfd == 0.0)) flag |= kBondPair;
+  if (!(fd < 1e-15)) flag |= kBondThird;
+  double vr, dvr, va, dva;
+  pair_terms(rd, vr, dvr, va, dva);
+  P = make_double4(fd * vr, fd * va, (dfd * vr + fd * dvr) / rd, (dfd * va + fd * dva) / rd);
+  return flag;
+}
+
+// Bond-order sum Gs_ij = sum over k in N(i)\{j} of fc(r_ik) G(cos theta_ijk)
+// (AB:292-305) from FP32 bond vectors: bv(kn) gives (dx, dy, dz, r) of i's
+// entry kn.
+template <class R, class BV>
+FMD_HD R bond_order_sum(int jn, int nni, R ux, R uy, R uz, R r, BV bv) {
+#pragma clang fp contract(off)  // the same rounding wherever it is inlined
+  using namespace rebo;
+  R Gs = 0;
+  for (int kn = 0; kn < nni; kn++) {
+    if (kn == jn) continue;
+    const float4 e = bv(kn);
+    if (e.w > R(kDmax)) continue;
+    const R rk = e.w;
+    const R fk = fc(rk);
+    if (fk < R(1e-15)) continue;
+    R cs = clamp1((ux * (R)e.x + uy * (R)e.y + uz * (R)e.z) / (r * rk));
+    Gs += fk * G(cs);
+  }
+  return Gs;
+}
+
+// d b / d Gs = -1/2 (1 + Gs)^-3/2 and the angular coefficient
+// 1/2 fc VA db/dGs of a direction with bond order b (P.y = fc VA).
+FMD_HD float angular_coeff(double Py, double b) {
+  const double dbp = -rebo::kDelta * b * b * b;
+  return (float)(0.5 * Py * dbp);
+}
+
+// Virial share of an angular triple, d_ct (x) f_t + d_cp (x) f_p (TRACE: its
+// trace into W[0]).  Products of FP32 numbers are exact in FP64, so the sums
+// round the same way wherever this is inlined.
+template <bool TRACE>
+FMD_HD void triple_virial(const float *dt, const float *ft, const float *dp, const float *fp,
+                          double *W) {
+  if (TRACE) {
+    W[0] += (((double)dt[0] * ft[0] + (double)dt[1] * ft[1]) + (double)dt[2] * ft[2]) +
+            (((double)dp[0] * fp[0] + (double)dp[1] * fp[1]) + (double)dp[2] * fp[2]);
+  } else {
+    for (int a = 0; a < 3; a++)
+      for (int c = 0; c < 3; c++) W[3 * a + c] += (double)dt[a] * ft[c] + (double)dp[a] * fp[c];
+  }
+}
 
 // Entry (i -> j = nll[slot(i, jn)]); vec[kn] = (dx, dy, dz, r) of i's entry
 // kn (minimum image r_k - r_i, FP64), so every bond vector of i is taken once.
@@ -72,41 +139,28 @@ FMD_HD void rebo_pass1(int i, int jn, const double4 *vec, const int *nlc, const 
   B.bd[4 * s + 1] = (float)dy;
   B.bd[4 * s + 2] = (float)dz;
   B.bd[4 * s + 3] = (float)rd;
+  double fd, dfd;
+  double4 P;
+  int flag = bond_terms(rd, fd, dfd, P);
   if (rd > kDmax) {
     B.bflag[s] = 0;
     return;
   }
-  const double fd = fc(rd), dfd = dfc(rd);
-  int flag = 0;
-  if (!(fd < 1e-15 && dfd == 0.0)) flag |= kBondPair;
-  if (!(fd < 1e-15)) flag |= kBondThird;
-  double vr, dvr, va, dva;
-  pair_terms(rd, vr, dvr, va, dva);
-  const R ux = (R)dx, uy = (R)dy, uz = (R)dz;
-  R Gs = 0;
-  for (int kn = 0; kn < nni; kn++) {
-    if (kn == jn) continue;
-    const double ex = vec[kn].x, ey = vec[kn].y, ez = vec[kn].z, rkd = vec[kn].w;
-    if (rkd > kDmax) continue;
-    const R rk = (R)rkd;
-    const R fk = fc(rk);
-    if (fk < R(1e-15)) continue;
-    R cs = clamp1((ux * (R)ex + uy * (R)ey + uz * (R)ez) / (r * rk));
-    Gs += fk * G(cs);
-  }
+  const R Gs = bond_order_sum<R>(jn, nni, (R)dx, (R)dy, (R)dz, r, [&](int kn) {
+    return make_float4((float)vec[kn].x, (float)vec[kn].y, (float)vec[kn].z,
+                       (float)vec[kn].w);
+  });
   const double b = rsq(1.0 + (double)Gs);
-  const double dbp = -kDelta * b * b * b;  // d b / d Gs = -1/2 (1 + Gs)^-3/2
-  const double coeff = 0.5 * fd * va * dbp;
-  if ((flag & kBondPair) && Gs > R(1e-20) && va > 1e-20) flag |= kBondAngular;
+  if ((flag & kBondPair) && Gs > R(1e-20) && P.y > 0.0) flag |= kBondAngular;
   B.bflag[s] = flag;
   B.ba[4 * s] = (float)fd;
   B.ba[4 * s + 1] = (float)dfd;
-  B.ba[4 * s + 2] = (float)coeff;
+  B.ba[4 * s + 2] = angular_coeff(P.y, b);
   B.ba[4 * s + 3] = 0.f;
-  B.bp[4 * s] = vr;
-  B.bp[4 * s + 1] = dvr;
-  B.bp[4 * s + 2] = va;
-  B.bp[4 * s + 3] = dva;
+  B.bp[4 * s] = P.x;
+  B.bp[4 * s + 1] = P.y;
+  B.bp[4 * s + 2] = P.z;
+  B.bp[4 * s + 3] = P.w;
   B.bq[4 * s] = rd;
   B.bq[4 * s + 1] = fd;
   B.bq[4 * s + 2] = dfd;
@@ -131,16 +185,13 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, Slot
     const int y = nll[s];
     const R d[3] = {B.bd[4 * s], B.bd[4 * s + 1], B.bd[4 * s + 2]};
     const R r = B.bd[4 * s + 3];
-    const double rd = B.bq[4 * s], f = B.bq[4 * s + 1], df = B.bq[4 * s + 2];
     const int rv = B.brev[s];  // -1 only for an asymmetric (overflowed) list
     const double bbar = 0.5 * (B.bq[4 * s + 3] + B.bq[4 * (rv >= 0 ? rv : s) + 3]);
-    const double vr = B.bp[4 * s], dvr = B.bp[4 * s + 1], va = B.bp[4 * s + 2],
-                 dva = B.bp[4 * s + 3];
     // pair force (AB:326-330): F_x += fpair d_xy
-    const double fpair = (df * (vr - bbar * va) + f * (dvr - bbar * dva)) / rd;
+    const double fpair = B.bp[4 * s + 2] - bbar * B.bp[4 * s + 3];
     for (int k = 0; k < 3; k++) F[k] += fpair * (double)d[k];
     if (x < y) {
-      E += f * (vr - bbar * va);
+      E += B.bp[4 * s] - bbar * B.bp[4 * s + 1];
       if (TRACE) {
         W[0] -= fpair * ((double)d[0] * d[0] + (double)d[1] * d[1] + (double)d[2] * d[2]);
       } else {
@@ -163,14 +214,7 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, Slot
           for (int k = 0; k < 3; k++) F[k] += (double)ft[k];
         } else {
           for (int k = 0; k < 3; k++) F[k] -= (double)(ft[k] + fp[k]);
-          if (TRACE) {
-            W[0] += (double)(dt[0] * ft[0] + dt[1] * ft[1] + dt[2] * ft[2] + d[0] * fp[0] +
-                             d[1] * fp[1] + d[2] * fp[2]);
-          } else {
-            for (int a = 0; a < 3; a++)
-              for (int c = 0; c < 3; c++)
-                W[3 * a + c] += (double)(dt[a] * ft[c] + d[a] * fp[c]);
-          }
+          triple_virial<TRACE>(dt, ft, d, fp, W);
         }
       }
     }

@@ -164,8 +164,10 @@ class RigidEngine:
         N.check(N.lib().fmd_engine_counters(self._h, out))
         return list(out)
 
-    def set_timing(self, on=True):
-        N.check(N.lib().fmd_engine_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on=True, stride=1):
+        """Event timing of the step phases on every `stride`-th step (and on
+        list rebuilds)."""
+        N.check(N.lib().fmd_engine_set_timing(self._h, stride if on else 0))
 
     def set_graph(self, on=True):
         N.check(N.lib().fmd_engine_set_graph(self._h, 1 if on else 0))

@@ -447,7 +447,9 @@ int fmd_engine_counters(fmd_engine *e, long long *out3);
 
 /* Mean duration (ms) of the force kernel over the launches timed since the
  * last call (HIP events on the engine's stream; enabled by
- * fmd_engine_set_timing(e, 1)).  *n receives the number of launches. */
+ * fmd_engine_set_timing(e, on): on = 1 times every step, on = k > 1 every
+ * k-th step of a run and every list rebuild, 0 none).  *n receives the
+ * number of launches timed. */
 int fmd_engine_set_timing(fmd_engine *e, int on);
 int fmd_engine_kernel_time(fmd_engine *e, double *mean_ms, int *n);
 /* Mean durations (ms) and counts of the timed step phases since the last
