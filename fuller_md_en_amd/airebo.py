@@ -83,7 +83,7 @@ class AtomicEngine:
     """HBM-resident AIREBO NPT engine (fmd_atomic_create / fmd_engine_*)."""
 
     def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
-                 nlup=20, lj=None, rebo_rlist=None):
+                 nlup=20, lj=None, rebo_rlist=None, clamp_vg=None):
         self._mol = _c(mol_id, np.int32)
         self._mass = _c(mass)
         cfg = N.AtomicConfig()
@@ -96,6 +96,8 @@ class AtomicEngine:
             cfg.lj = lj
         if rebo_rlist is not None:
             cfg.rebo_rlist = rebo_rlist
+        if clamp_vg is not None:
+            cfg.clamp_vg = clamp_vg
         self.cfg = cfg
         self.n = len(self._mol)
         h = C.c_void_p()

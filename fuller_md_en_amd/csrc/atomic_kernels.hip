@@ -199,11 +199,65 @@ __global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
       d.qcen[(size_t)m * (p.apad >> 2) + (a >> 2)] =
           c4 > 0.f ? make_float4(qx, qy, qz, rq + 1e-4f) : make_float4(1e6f, 1e6f, 1e6f, 0.f);
   }
+  // drift since the per-atom LJ lists were built: the largest change of a
+  // tile offset plus the centre's displacement (minimum image)
+  float dof = 0.f;
+  bool built = false;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int a = lane + 64 * s;
+    if (a < n) {
+      const float4 q = d.al_pos[s0 + a];
+      built |= q.w != 0.f;
+      const float ex = (float)(da[s][0] - mx) - q.x, ey = (float)(da[s][1] - my) - q.y,
+                  ez = (float)(da[s][2] - mz) - q.z;
+      dof = fmaxf(dof, sqrtf(ex * ex + ey * ey + ez * ez));
+    }
+  }
+  dof = wave_max(dof);
+  built = __any(built);
   if (lane == 0) {
     d.center[3 * m] = rx + mx;
     d.center[3 * m + 1] = ry + my;
     d.center[3 * m + 2] = rz + mz;
     d.radius[m] = rmax;
+    if (built) {
+      double cx = rx + mx - d.al_cen[3 * m], cy = ry + my - d.al_cen[3 * m + 1],
+             cz = rz + mz - d.al_cen[3 * m + 2];
+      mimg(cx, cy, cz, hi, h);
+      const float drift = (float)sqrt(cx * cx + cy * cy + cz * cz) + dof;
+      atomicMax(&d.al_dmax[d.mol_rep[m]], __float_as_int(drift));
+    }
+  }
+}
+
+// ------------------------------------------------------------ image modes
+// At a list rebuild, for replica r (one workgroup each), from its cell and
+// the molecule radii of this rebuild: kImgMol when rc + 2 R_max + 1 A is
+// below half the narrowest cell width (every atom pair within rc + 1 A is
+// then the centres' minimum image plus the two offsets, which the
+// per-molecule-image kernels assume), kImgScan when the same holds for the
+// half-list radius (the FP32 list walk through the tiles is then exact;
+// otherwise the walk takes every pair's own FP64 image).  An engine stepping
+// with per-molecule images counts the loss of kImgMol as an image fault.
+__global__ __launch_bounds__(256) void k_at_image_check(AtomicDev d, AtomicParams p, HHRef hh) {
+  __shared__ float wm[4];
+  const int r = blockIdx.x;
+  float rm = 0.f;
+  for (int m = d.rep_mol[r] + (int)threadIdx.x; m < d.rep_mol[r + 1]; m += 256)
+    rm = fmaxf(rm, d.radius[m]);
+  rm = wave_max(rm);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = rm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double rmax = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+    const double half = 0.5 * min_width(hh_of(hh, r));
+    int bits = 0;
+    if (sqrt(p.lj.rc2d) + 2.0 * rmax + 1.0 < half) bits |= kImgMol;
+    if (sqrt(p.lj_rlist2) + 2.0 * rmax + 1.0 < half) bits |= kImgScan;
+    d.img[r] = bits;
+    d.al_dmax[r] = 0;  // largest drift since this rebuild (per-atom lists)
+    if (!p.small_box && !(bits & kImgMol)) atomicAdd(&d.flags[3], 1);
   }
 }
 
@@ -323,9 +377,10 @@ __global__ __launch_bounds__(256) void k_at_scan_half(AtomicDev d, AtomicParams 
     done[s] = lane + 64 * s >= n;
   }
   // FP32 prefilter through the molecule tiles (when the cell is wide enough
-  // that the centres' minimum image is every listed atom pair's): only pairs
-  // within 1e-5 relative of the list radius take the reference's FP64 test.
-  const bool fast = p.scan_fast;
+  // that the centres' minimum image is every listed atom pair's, as decided
+  // at this rebuild): only pairs within 1e-5 relative of the list radius
+  // take the reference's FP64 test.
+  const bool fast = p.scan_fast && (d.img[d.mol_rep[m]] & kImgScan);
   const float rl2 = (float)p.lj_rlist2, lo = rl2 * (1.0f - 1e-5f), hi2 = rl2 * (1.0f + 1e-5f);
   const float rl = sqrtf(rl2);
   const double cix = d.center[3 * m], ciy = d.center[3 * m + 1], ciz = d.center[3 * m + 2];
@@ -419,7 +474,7 @@ __global__ __launch_bounds__(256) void k_at_scan_jmax(AtomicDev d, AtomicParams 
   const int lane = threadIdx.x & 63, wv = uniform(threadIdx.x >> 6);
   const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
-  const bool fast = p.scan_fast;
+  const bool fast = p.scan_fast && (d.img[d.mol_rep[m]] & kImgScan);
   const float rl2 = (float)p.lj_rlist2, lo = rl2 * (1.0f - 1e-5f), hi2 = rl2 * (1.0f + 1e-5f);
   const float rl = sqrtf(rl2);
   const float reach_m = sqrtf(hi2) + d.radius[m] + 0.05f;
@@ -1132,12 +1187,11 @@ __device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, i
 // TRACE (time steps on the molecule-image path): only the virial trace, the
 // part the barostat uses (AB:527, 583), one FP64 FMA per (lane, neighbour)
 // instead of nine; the diagonal then carries trace / 3.
+// The body for molecule mi (a whole workgroup); k_at_lj is one workgroup per
+// molecule, and k_at_lj_al takes this path when its per-atom lists are stale.
 template <int NS, bool PERPAIR, bool TRACE>
-__global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRef hh) {
-  const int per = (p.Nm + kXcd - 1) / kXcd;
-  const int bid = blockIdx.x;
-  const int mi = (bid % kXcd) * per + bid / kXcd;
-  if (mi >= p.Nm) return;
+__device__ __forceinline__ void lj_tiles(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                                         int mi) {
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
   const double *h = hh_of(hh, d.mol_rep[mi]), *hi = h + 9;
@@ -1317,6 +1371,15 @@ __global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRe
   }
 }
 
+template <int NS, bool PERPAIR, bool TRACE>
+__global__ __launch_bounds__(256) void k_at_lj(AtomicDev d, AtomicParams p, HHRef hh) {
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int bid = blockIdx.x;
+  const int mi = (bid % kXcd) * per + bid / kXcd;
+  if (mi >= p.Nm) return;
+  lj_tiles<NS, PERPAIR, TRACE>(d, p, hh, mi);
+}
+
 // ------------------------------------------------- LJ per-atom lists
 // Engines on cells with one image per molecule pair keep, per tile slot a of
 // molecule i, a list of the atoms within rc + 1 A at the list rebuild, so the
@@ -1370,25 +1433,17 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
     ia[s] = s0 + a;
   }
   const float rl2 = p.al_r2, rl = sqrtf(rl2);
-  // the lists just used were valid if no atom has moved half the skin since
-  // they were built (a pair closes by at most twice that)
+  // the state these lists are built from: tile offsets, centre, cell; the
+  // drift since then is tracked by k_at_centers
   if (wv == 0) {
-    const double half_skin = 0.5 * (sqrt((double)rl2) - (double)sqrtf(p.lj.rc2));
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const int a = lane + 64 * s;
-      if (a < n) {
-        const int x = s0 + a;
-        const double px = d.pos[3 * x], py = d.pos[3 * x + 1], pz = d.pos[3 * x + 2];
-        const float4 q = d.al_pos[x];
-        if (q.w != 0.f) {
-          double dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-          mimg(dx, dy, dz, hi, h);
-          if (dx * dx + dy * dy + dz * dz > half_skin * half_skin) atomicAdd(&d.flags[2], 1);
-        }
-        d.al_pos[x] = make_float4((float)px, (float)py, (float)pz, 1.f);
-      }
+      if (a < n) d.al_pos[s0 + a] = make_float4(rax[s], ray[s], raz[s], 1.f);
     }
+    if (lane < 3) d.al_cen[3 * mi + lane] = d.center[3 * mi + lane];
+    const int r = d.mol_rep[mi];
+    if (mi == d.rep_mol[r] && lane < 9) d.al_h[9 * r + lane] = h[lane];
   }
   const float reach_i = rl + d.radius[mi] + 0.05f;
   const int i_last = s0 + n - 1;
@@ -1516,7 +1571,12 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
     while (fill[s] != 0) push(s, kAlPad);
     d.al_n[((size_t)mi * kAlWaves + wv) * NA + lane + 64 * s] =
         (unsigned short)(cnt[s] < nbk ? cnt[s] : nbk);
-    if (cnt[s] > nbk) atomicAdd(&d.flags[2], 1);
+    if (cnt[s] > nbk) {
+      // a segment that overflowed lost entries: the replica's lists count as
+      // stale until the next rebuild, so that the forces take every tile pair
+      atomicAdd(&d.flags[2], 1);
+      atomicMax(&d.al_dmax[d.mol_rep[mi]], __float_as_int(1.0e30f));
+    }
   }
 }
 
@@ -1535,6 +1595,27 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
   if (mi >= p.Nm) return;
+  {
+    // The lists hold every pair within rc + skin at their build.  A pair
+    // vector (centre image + offsets) has since changed by at most the two
+    // molecules' drifts (centre displacement plus largest offset change) and
+    // the cell change along the image, |dh n| <= sum of the column changes;
+    // while that stays below the skin no pair within rc is missing.
+    // Otherwise every tile pair is taken until the next rebuild.
+    const int r = d.mol_rep[mi];
+    const double *hn = hh_of(hh, r), *h0 = d.al_h + 9 * r;
+    double dh = 0;
+    for (int c = 0; c < 3; c++) {
+      const double x = hn[c] - h0[c], y = hn[3 + c] - h0[3 + c], z = hn[6 + c] - h0[6 + c];
+      dh += sqrt(x * x + y * y + z * z);
+    }
+    const double skin = sqrt((double)p.al_r2) - sqrt(p.lj.rc2d) - 1e-3;
+    if (2.0 * (double)__int_as_float(d.al_dmax[r]) + dh > skin) {
+      if (threadIdx.x == 0 && mi == d.rep_mol[r]) atomicAdd(&d.flags[2], 1);
+      lj_tiles<NS, false, TRACE>(d, p, hh, mi);
+      return;
+    }
+  }
   constexpr int NA = NS * 64;
   constexpr int NW = TRACE ? 1 : 6;
   const int lane = threadIdx.x & 63;
@@ -1896,6 +1977,7 @@ hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, HHRef hh
 
 hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                            int rebo_list, hipStream_t s) {
+  k_at_image_check<<<p.R, 256, 0, s>>>(d, p, hh);
   k_at_mol_nlist<<<p.Nm, 64, 0, s>>>(d, p, hh);
   dim3 g((p.Nm + 3) / 4);
   const bool one = ns_of(p) == 1;

@@ -137,7 +137,7 @@ static void free_rigid(RigidDev &d) {
                   d.sites,    d.sites_d,  d.body,     d.qbody,    d.qcen,     d.hh,
                   d.nl_count, d.nl_list,  d.hl_count, d.hl_list,  d.in_count, d.in_list,
                   d.pairout,  d.flags,    d.molpart,  d.blockpart, d.state,   d.mid,
-                  d.ctrl,     d.thermo,   d.hl_item,  d.ord_ctl};
+                  d.ctrl,     d.thermo,   d.hl_item,  d.ord_ctl,  d.nl_pos,   d.nl_h};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   memset(&d, 0, sizeof d);
@@ -167,6 +167,8 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   AL(d.hh, 18);
   AL(d.nl_count, N);
   AL(d.nl_list, N * p.max_neigh);
+  AL(d.nl_pos, 3 * N);
+  AL(d.nl_h, 10);
   if (p.use_half) {
     AL(d.qbody, 4 * (size_t)p.nq);
     AL(d.qcen, N * p.nq);
@@ -194,6 +196,7 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   }
   (void)hipMemset(d.flags, 0, 4 * sizeof(int));
   (void)hipMemset(d.nl_count, 0, N * sizeof(int));
+  (void)hipMemset(d.nl_h, 0, 10 * sizeof(double));
   if (d.ord_ctl) (void)hipMemset(d.ord_ctl, 0, sizeof(int));
   (void)hipMemset(d.Fv, 0, 3 * N * sizeof(double));
   (void)hipMemset(d.Tv, 0, 3 * N * sizeof(double));
@@ -372,6 +375,8 @@ int fmd_rigid_set_state(fmd_engine *e, const double *pos, const double *vel, con
   }
   S.scale = 1.0;
   HIPCHK(hipMemcpyAsync(d.state, &S, sizeof S, hipMemcpyHostToDevice, s));
+  // a new state: the next list build has no previous one to check
+  if (pos || h) HIPCHK(hipMemsetAsync(d.nl_h + 9, 0, sizeof(double), s));
   HIPCHK(hipStreamSynchronize(s));
   e->prepared = false;
   return FMD_OK;
@@ -570,7 +575,7 @@ static int rigid_prepare(fmd_engine *e) {
   hipStream_t s = e->stream;
   const double *hh = state_hh(d);
   HIPCHK(rigid_launch_apply_pbc(d, p, hh, s));
-  HIPCHK(rigid_launch_nlist(d, p, hh, s));
+  HIPCHK(rigid_launch_rebuild(d, p, hh, s));
   if (p.use_half) HIPCHK(rigid_launch_halflist(d, p, hh, s));
   HIPCHK(rigid_launch_sites(d, p, s));
   HIPCHK(rigid_launch_forces(d, p, hh, s));
@@ -580,13 +585,15 @@ static int rigid_prepare(fmd_engine *e) {
   return FMD_OK;
 }
 
+// List overflows and list faults since the last read, into the fault counter.
 static int read_overflow(fmd_engine *e, int *flags_dev) {
-  int f = 0;
-  HIPCHK(hipMemcpyAsync(&f, flags_dev, sizeof f, hipMemcpyDeviceToHost, e->stream));
+  int f[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(f, flags_dev, sizeof f, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
-  if (f) {
-    e->overflow += f;
-    HIPCHK(hipMemsetAsync(flags_dev, 0, sizeof(int), e->stream));
+  if (f[0] || f[1]) {
+    e->overflow += f[0] + f[1];
+    HIPCHK(hipMemsetAsync(flags_dev, 0, sizeof f, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
   }
   return FMD_OK;
 }

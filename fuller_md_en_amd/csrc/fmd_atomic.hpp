@@ -60,6 +60,12 @@
 namespace fmd {
 
 constexpr int kAtPart = 10;  // W9 + E
+constexpr int kAtFlags = 4;  // AtomicDev::flags
+// image modes of a replica at a list rebuild (AtomicDev::img)
+enum : int {
+  kImgMol = 1,   // atom pairs within rc + 1 A are the centres' image + offsets
+  kImgScan = 2,  // the same within the half-list radius: FP32 prefiltered walk
+};
 
 struct AtomicState {
   double h[9], hi[9];  // adjacent: kernels take them as one 18-double block
@@ -122,15 +128,26 @@ struct AtomicDev {
   double *rebo_fe;  // 3 x Na*maxr: per-entry REBO forces (global-memory REBO path)
   unsigned short *al_e;  // per-atom LJ lists: [Nm][4][al_cap / 8][NA] blocks of 8 entries
   unsigned short *al_n;  // segment lengths in blocks: [Nm][4][NA]  (NA = 64 * NS)
-  float4 *al_pos;        // per atom: position at the last list build (w = 1 once set)
+  // per-atom list staleness: at the last list build, each atom's tile offset
+  // (al_pos, w = 1 once set), each molecule's centre (al_cen) and each
+  // replica's cell (al_h); al_dmax[r], the largest molecule drift since
+  // then (centre displacement plus largest offset change, float bits; reset
+  // by the image check that opens every rebuild, 1e30 once a list segment
+  // overflowed)
+  float4 *al_pos;
+  double *al_cen, *al_h;
+  int *al_dmax;
+  int *img;  // per replica, decided at every list rebuild: kImgMol | kImgScan
   double *hh;  // 18 doubles for the stateless entry points
   AtomicState *state;
   AtomicMid *mid;
   StepCtrl *ctrl;
   Thermo *thermo;
   int *flags;  // [0] REBO list overflow, [1] molecule list overflow,
-               // [2] per-atom LJ list overflow or an atom moved half the
-               // list skin between rebuilds
+               // [2] per-atom LJ list overflow, or a step whose per-atom
+               // lists had gone stale (the LJ kernel then takes every tile
+               // pair), [3] image fault: the cell became too narrow for
+               // per-molecule images while the engine stepped with them
   int *mol_rep, *atom_rep, *rep_mol, *rep_atom;
   int *blk_a0, *blk_a1, *blk_rep, *rep_blk;
   int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)

@@ -39,18 +39,6 @@ int mol_ranges(const int *mol_id, int na, std::vector<int> &start, int &apad) {
   return FMD_OK;
 }
 
-// Smallest perpendicular width of the cell h (columns are the cell vectors).
-double min_width(const double *h) {
-  const double a[3] = {h[0], h[3], h[6]}, b[3] = {h[1], h[4], h[7]}, c[3] = {h[2], h[5], h[8]};
-  auto cross_norm = [](const double *u, const double *v) {
-    const double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2],
-                 z = u[0] * v[1] - u[1] * v[0];
-    return std::sqrt(x * x + y * y + z * z);
-  };
-  const double V = std::fabs(det9(h));
-  return std::min(V / cross_norm(b, c), std::min(V / cross_norm(c, a), V / cross_norm(a, b)));
-}
-
 // Largest atom distance from its molecule's centre (host, FP64).
 double max_radius(const double *pos, const std::vector<int> &start, const double *h) {
   double hi[9];
@@ -87,7 +75,7 @@ void free_atomic_dev(AtomicDev &d) {
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.rebo_mpart,
                   d.lj_part,   d.ke_part,   d.rebo_fe,   d.al_e,       d.al_n,
-                  d.al_pos,
+                  d.al_pos,    d.al_cen,    d.al_h,     d.al_dmax,   d.img,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
                   d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
                   d.blk_rep,   d.rep_blk};
@@ -192,7 +180,11 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
     AL(d.al_e, Nm * 4 * (size_t)p.al_cap * na);
     AL(d.al_n, Nm * 4 * na);
     AL(d.al_pos, Na);
+    AL(d.al_cen, 3 * Nm);
+    AL(d.al_h, 9 * R);
+    AL(d.al_dmax, R);
   }
+  AL(d.img, R);
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
   AL(d.rebo_mpart, Nm * kAtPart);
   AL(d.lj_part, Nm * kAtPart);
@@ -202,7 +194,7 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   AL(d.mid, R);
   AL(d.ctrl, ring * R);
   AL(d.thermo, ring * R);
-  AL(d.flags, 4);
+  AL(d.flags, kAtFlags);
   AL(d.mol_rep, Nm);
   AL(d.atom_rep, Na);
   AL(d.rep_mol, R + 1);
@@ -216,8 +208,9 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
     free_atomic_dev(d);
     return hip_fail(e, "hipMalloc (atomic engine)");
   }
-  (void)hipMemset(d.flags, 0, 4 * sizeof(int));
+  (void)hipMemset(d.flags, 0, kAtFlags * sizeof(int));
   (void)hipMemset(d.al_pos, 0, Na * sizeof(float4));
+  (void)hipMemset(d.al_dmax, 0, R * sizeof(int));
   (void)hipMemset(d.F, 0, 3 * Na * sizeof(double));
   (void)hipMemset(d.state, 0, R * sizeof(AtomicState));
   (void)hipMemset(d.nlc_r, 0, Na * sizeof(int));
@@ -268,6 +261,8 @@ int decide_scan_fast(const AtomicParams &p, const double *h, double rmax) {
 
 namespace fmd {
 
+int read_flags(fmd_engine *e);
+
 int engine_prepare_atomic(fmd_engine *e) {
   AtomicEngine &A = *e->at;
   const AtomicDev &d = A.d;
@@ -281,11 +276,47 @@ int engine_prepare_atomic(fmd_engine *e) {
   HIPCHK(at_launch_observe(d, p, kAtObsKE | kAtObsForce, s));
   e->rebuilds++;
   e->prepared = true;
-  int f[3] = {0, 0, 0};
-  HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  e->overflow += f[0] + f[1] + f[2];
-  if (f[0] || f[1] || f[2]) HIPCHK(hipMemset(d.flags, 0, 3 * sizeof(int)));
+  return read_flags(e);
+}
+
+// Device flags since the last read into the list-fault counter.  After an
+// image fault (the cell became too narrow for per-molecule images while the
+// engine stepped with them), the image mode is decided again from the
+// current state, so that the following steps take every pair's own image.
+int read_flags(fmd_engine *e) {
+  AtomicEngine &A = *e->at;
+  int f[kAtFlags] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(f, A.d.flags, sizeof f, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  int any = 0;
+  for (int k = 0; k < kAtFlags; k++) {
+    e->overflow += f[k];
+    any |= f[k];
+  }
+  if (any) HIPCHK(hipMemsetAsync(A.d.flags, 0, sizeof f, e->stream));
+  if (f[3]) {
+    for (int r = 0; r < A.p.R; r++) {
+      const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
+      std::vector<double> pos(3 * na);
+      AtomicState S;
+      HIPCHK(hipMemcpyAsync(pos.data(), A.d.pos + 3 * a0, 3 * na * 8, hipMemcpyDeviceToHost,
+                            e->stream));
+      HIPCHK(hipMemcpyAsync(&S, A.d.state + r, sizeof S, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      std::vector<int> start;
+      for (int m = A.rep_mol[r]; m <= A.rep_mol[r + 1]; m++)
+        start.push_back(A.mol_start[m] - (int)a0);
+      const double rmax = max_radius(pos.data(), start, S.h);
+      A.small[r] = decide_small_box(A.p, S.h, rmax);
+      A.fast[r] = decide_scan_fast(A.p, S.h, rmax);
+    }
+    A.p.small_box = 0;
+    A.p.scan_fast = 1;
+    for (int x : A.small) A.p.small_box |= x;
+    for (int x : A.fast) A.p.scan_fast &= x;
+    engine_drop_graphs(e);  // captured launches hold the old parameters
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
   return FMD_OK;
 }
 
@@ -316,14 +347,7 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
     if (judge && ((long long)first_step + done) % every == 0 && (rc = surrogate_judge(e)))
       return rc;
   }
-  int f[3] = {0, 0, 0};
-  HIPCHK(hipMemcpyAsync(f, d.flags, sizeof f, hipMemcpyDeviceToHost, e->stream));
-  HIPCHK(hipStreamSynchronize(e->stream));
-  if (f[0] || f[1] || f[2]) {
-    e->overflow += f[0] + f[1] + f[2];
-    HIPCHK(hipMemset(d.flags, 0, 3 * sizeof(int)));
-  }
-  return FMD_OK;
+  return read_flags(e);
 }
 
 int engine_forces_atomic(fmd_engine *e, double *F, double *T, double *W9, double *energies) {

@@ -38,6 +38,19 @@ FMD_HD double det9(const double *h) {  // C1:137-141
          h[2] * (h[3] * h[7] - h[4] * h[6]);
 }
 
+// Smallest perpendicular width of the cell h (columns are the cell vectors).
+FMD_HD double min_width(const double *h) {
+  const double a[3] = {h[0], h[3], h[6]}, b[3] = {h[1], h[4], h[7]}, c[3] = {h[2], h[5], h[8]};
+  auto cross_norm = [](const double *u, const double *v) {
+    const double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2],
+                 z = u[0] * v[1] - u[1] * v[0];
+    return sqrt(x * x + y * y + z * z);
+  };
+  const double V = fabs(det9(h));
+  const double wa = V / cross_norm(b, c), wb = V / cross_norm(c, a), wc = V / cross_norm(a, b);
+  return wa < wb ? (wa < wc ? wa : wc) : (wb < wc ? wb : wc);
+}
+
 FMD_HD void inv9(const double *h, double *hi) {  // C1:147-158
 #if defined(__clang__)
 #pragma clang fp contract(off)

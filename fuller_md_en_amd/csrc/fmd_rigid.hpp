@@ -99,7 +99,10 @@ struct RigidDev {
   double *qbody; // nq x 4 doubles: body-frame quad centre and radius
   float4 *qcen;  // N*nq: lab-frame quad centre offsets (w = quad radius)
   double *hh;    // 18 doubles: h, h^-1 for the stateless entry points
-  int *nl_count, *nl_list, *flags;  // flags[0]: list overflow count
+  int *nl_count, *nl_list, *flags;  // flags[0]: list overflow count, [1]: list
+                                    // faults (a pair could have closed the skin)
+  // molecule centres and cell (h, then 1 once set) at the last list build
+  double *nl_pos, *nl_h;
   // pairs owned by each molecule (half list) and, per molecule, the slots of
   // the pairs owned by its neighbours that carry its force / torque
   int *hl_count, *hl_list, *in_count, *in_list;
@@ -125,6 +128,10 @@ hipError_t rigid_launch_apply_pbc(const RigidDev &d, const RigidParams &p, const
                                   hipStream_t s);
 hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const double *hh,
                               hipStream_t s);
+// The engine's rebuild: checks the list being replaced (flags[1]), builds the
+// new one and records the state it was built from.
+hipError_t rigid_launch_rebuild(const RigidDev &d, const RigidParams &p, const double *hh,
+                                hipStream_t s);
 // Half list and incoming slots from the symmetric list; with the pair kernel
 // also its work order (centre distances from the cell `hh`).
 hipError_t rigid_launch_halflist(const RigidDev &d, const RigidParams &p, const double *hh,

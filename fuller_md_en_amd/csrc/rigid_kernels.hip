@@ -169,6 +169,43 @@ __global__ __launch_bounds__(64) void k_rigid_nlist(const double *__restrict__ p
   }
 }
 
+// Before the engine replaces its list: it held every pair within the list
+// radius at its build, and a pair has closed by at most the two centres'
+// displacements plus the cell change along the image (at most the sum of
+// the column changes) since then.  A molecule that may have closed the skin
+// beyond the force loop's cutoff with any neighbour counts a list fault.
+__global__ __launch_bounds__(256) void k_rigid_list_check(const double *__restrict__ pos,
+                                                          const double *__restrict__ nl_pos,
+                                                          const double *__restrict__ nl_h,
+                                                          const double *__restrict__ hh,
+                                                          int *__restrict__ flags, RigidParams p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.N || nl_h[9] == 0.0) return;
+  const double *h = hh, *hi = hh + 9;
+  double dh = 0;
+  for (int c = 0; c < 3; c++) {
+    const double x = h[c] - nl_h[c], y = h[3 + c] - nl_h[3 + c], z = h[6 + c] - nl_h[6 + c];
+    dh += sqrt(x * x + y * y + z * z);
+  }
+  double dx = pos[3 * i] - nl_pos[3 * i], dy = pos[3 * i + 1] - nl_pos[3 * i + 1],
+         dz = pos[3 * i + 2] - nl_pos[3 * i + 2];
+  mimg(dx, dy, dz, hi, h);
+  const double skin = sqrt(p.rlist2) - sqrt(p.rmcut2);
+  if (2.0 * sqrt(dx * dx + dy * dy + dz * dz) + dh > skin) atomicAdd(&flags[1], 1);
+}
+
+// The state a new list is built from.
+__global__ __launch_bounds__(256) void k_rigid_list_mark(const double *__restrict__ pos,
+                                                         double *__restrict__ nl_pos,
+                                                         double *__restrict__ nl_h,
+                                                         const double *__restrict__ hh, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    for (int k = 0; k < 3; k++) nl_pos[3 * i + k] = pos[3 * i + k];
+  if (i < 9) nl_h[i] = hh[i];
+  if (i == 9) nl_h[9] = 1.0;
+}
+
 // Ownership of a molecule pair: i owns (i, j) iff (j - i) mod N < N / 2 (ties
 // to the lower index), so every molecule owns about half of its neighbours
 // whatever its index.
@@ -1215,6 +1252,16 @@ hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const dou
   return hipGetLastError();
 }
 
+hipError_t rigid_launch_rebuild(const RigidDev &d, const RigidParams &p, const double *hh,
+                                hipStream_t s) {
+  const int g = (p.N + 255) / 256;
+  k_rigid_list_check<<<g, 256, 0, s>>>(d.pos, d.nl_pos, d.nl_h, hh, d.flags, p);
+  hipError_t e = rigid_launch_nlist(d, p, hh, s);
+  if (e != hipSuccess) return e;
+  k_rigid_list_mark<<<g, 256, 0, s>>>(d.pos, d.nl_pos, d.nl_h, hh, p.N);
+  return hipGetLastError();
+}
+
 hipError_t rigid_launch_forces(const RigidDev &d, const RigidParams &p, const double *hh,
                                hipStream_t s) {
   const int per = (p.N + kXcd - 1) / kXcd;
@@ -1278,7 +1325,7 @@ hipError_t rigid_launch_step(const RigidDev &d, const RigidParams &p, int ctrl_s
       reinterpret_cast<const char *>(d.mid) + offsetof(RigidMid, h));
   if (rebuild) {
     if (ev && (e = hipEventRecordWithFlags(ev->l0, s, evf)) != hipSuccess) return e;
-    if ((e = rigid_launch_nlist(d, p, state_hh, s)) != hipSuccess) return e;
+    if ((e = rigid_launch_rebuild(d, p, state_hh, s)) != hipSuccess) return e;
     if (p.use_half && (e = rigid_launch_halflist(d, p, state_hh, s)) != hipSuccess) return e;
     if (ev && (e = hipEventRecordWithFlags(ev->l1, s, evf)) != hipSuccess) return e;
   }

@@ -40,6 +40,10 @@ Fixtures
                           +-0.01 A jitter): list counts and last LJ entries,
                           compute_forces, and 5 NPT steps (float32 inputs
                           and forces, stored exactly).
+  airebo_rebuild_nc3.npz  AB on a jittered 3x3x3 C60 crystal (6480 atoms): 25
+                          NPT steps with both lists rebuilt before step 20 as
+                          AB's main loop does (AB:881-887); per-step energies
+                          and the end state.
   core_ensemble_nc3.npz   the core program's main loop (C1:683-733) at nc=3
                           (config 1: 108 molecules, 1000 steps) for seeds
                           42..49: per-step T, P, a, Ecoh of seed 42 (the
@@ -302,6 +306,44 @@ def gen_airebo_cfg3():
          dpos_end=dpos.astype(np.float32), h_end=hh, npt_end=state)
 
 
+def gen_airebo_rebuild():
+    """C60 FCC 3x3x3 (6480 atoms, +-0.01 A jitter, seed 4242), 25 NPT steps of
+    0.5 fs at 300 K with both lists rebuilt before step 20 (AB:881-887): the
+    per-step energies and the end state.  Inputs and end state rounded to
+    float32 for storage; the reference ran on the rounded inputs."""
+    L = ref.airebo()
+    pos, h, mol, a0 = airebo_system(3, seed=4242)
+    pos = pos.astype(np.float32).astype(np.float64)
+    na = len(pos)
+    rng = np.random.default_rng(7)
+    vel = np.sqrt(oracle.KB * 300.0 * oracle.CONV / 12.011) * rng.normal(size=(na, 3))
+    vel = (vel - vel.mean(axis=0)).astype(np.float32).astype(np.float64)
+    mass = np.full(na, 12.011)
+    lists = ref.ab_nlists(pos, h, mol)
+    E, F, W = ref.ab_compute_forces(pos, h, lists)
+    state = np.zeros(15)
+    L.refab_make_npt(300.0, 0.0, na, state)
+    npt0 = state.copy()
+    P, V, FF, WW, hh, hi = (pos.reshape(-1).copy(), vel.reshape(-1).copy(),
+                            F.reshape(-1).copy(), W.copy(), h.copy(), oracle.inv9(h))
+    out, traj = np.zeros(4), []
+    r_cnt, r_lst, l_cnt, l_lst = (x.reshape(-1).copy() for x in lists)
+    for g in range(1, 26):
+        if g % 20 == 0:
+            L.refab_mat_inv9(hh, hi)
+            nl = ref.ab_nlists(P.reshape(na, 3), hh, mol)
+            r_cnt, r_lst, l_cnt, l_lst = (x.reshape(-1).copy() for x in nl)
+        L.refab_step_npt(P, V, FF, WW, hh, hi, mass, na, 0.5, state, r_cnt, r_lst, l_cnt,
+                         l_lst, out)
+        traj.append(out.copy())
+    L0 = np.diag(h.reshape(3, 3))
+    dpos = P.reshape(na, 3) - pos
+    dpos -= np.round(dpos / L0) * L0
+    save("airebo_rebuild_nc3.npz", pos=pos.astype(np.float32), vel=vel.astype(np.float32),
+         h=h, mol_id=mol, npt0=npt0, traj=np.array(traj), dpos_end=dpos,
+         vel_end=V.reshape(na, 3), h_end=hh, npt_end=state)
+
+
 MM_K4 = np.array([469.0, 63.0, 14.5, 15.0]) / 23.06054783  # MM:1039-1042
 
 
@@ -416,6 +458,9 @@ if __name__ == "__main__":
         sys.exit(0)
     if sys.argv[1:] == ["airebo_cfg3"]:
         gen_airebo_cfg3()
+        sys.exit(0)
+    if sys.argv[1:] == ["airebo_rebuild"]:
+        gen_airebo_rebuild()
         sys.exit(0)
     if sys.argv[1:] == ["core_ensemble"]:
         gen_core_ensemble()

@@ -194,21 +194,141 @@ def test_rebo_lds_and_global_paths_agree(mode):
     assert rel_max(Fa, F0) < TOL and rel_max(Wa, W0) < TOL
 
 
-def test_engine_counts_a_stale_lj_list():
-    """The engine's per-atom LJ lists carry a 1 A skin; an atom moving more
-    than half of it between two list rebuilds is counted as a list fault.
-    A uniform drift of 0.15 A/fs (twenty times the thermal speed at 300 K,
-    every bond unchanged) covers about 1 A in the 20 steps of 0.5 fs between
-    rebuilds; the same crystal at rest counts none."""
-    pos, h, mol = crystal(3, 7, 0.01)
-    for drift, faults in ((0.0, False), (0.15, True)):
-        print(f"case drift={drift}", flush=True)
-        vel = np.zeros_like(pos)
-        vel[:, 0] = drift
-        eng = _engine(pos, h, mol, vel=vel, mode=airebo.REBO_PHYSICAL)
-        eng.run(0, 21)
-        assert (eng.counters()[0] > 0) == faults
+def _frozen_npt(n):
+    """NPT state whose thermostat and barostat masses (1e30) keep xi and the
+    cell velocity where they are set."""
+    npt = airebo.make_npt(300.0, 0.0, n)
+    npt.Q = npt.W = 1e30
+    return npt
+
+
+def test_stale_lj_lists_fall_back_to_every_pair():
+    """The per-atom LJ lists hold the pairs within rc + 1 A at their build.
+    Molecules of a 3x3x3 crystal (lattice constant 1.1 times the default)
+    given random rigid velocities of up to 0.05 A/fs per component (cell and
+    thermostat frozen, REBO bonds unchanged) close in on each other by more
+    than that skin before the rebuild at step 20: the engine counts the stale
+    lists and takes every tile pair, so its forces at step 19 equal the
+    oracle's on complete lists of that state, while the pairs within rc + 1 A
+    of the initial state miss some (6e-4 of the largest force).  At 1/20 of
+    the speed nothing is counted.  LJ cut at 5 A, where the pairs entering
+    the cutoff carry forces well above the tolerance; untruncated lists
+    (lj_max_neigh = 0), so that both sides mean every pair within rc."""
+    from fuller_md_en_amd import rigid, system
+    m0 = system.molecule("C60")
+    cen, h = system.make_crystal("FCC", 1.1 * system.default_a0(m0.dmol, "FCC", 1.0), 3)
+    pos = (cen[:, None, :] + m0.coords[None]).reshape(-1, 3)
+    pos = pos + np.random.default_rng(7).uniform(-0.01, 0.01, pos.shape)
+    p = pos.reshape(-1).copy()
+    oracle.lib().orc_apply_pbc(p, h, oracle.inv9(h), len(pos))
+    pos = p.reshape(-1, 3)
+    mol = np.repeat(np.arange(1, len(cen) + 1), m0.natom).astype(np.int32)
+    n = len(pos)
+    lj, lj_o = rigid.lj_params(3.4, 0.00284, 5.0), oracle.lj_params_custom(3.4, 0.00284, 5.0)
+    vm = np.random.default_rng(3).uniform(-0.05, 0.05, (mol.max(), 3))
+    old = (*oracle.nlist_rebo(pos, h, mol),
+           *oracle.nlist_lj(pos, h, mol, rlist=6.0, max_neigh=2000))
+    for scale, stale in ((0.05, False), (1.0, True)):
+        print(f"case speed scale={scale}", flush=True)
+        eng = airebo.AtomicEngine(mol, np.full(n, 12.011), mode=airebo.REBO_PHYSICAL,
+                                  lj_max_neigh=0, lj=lj)
+        eng.set_state(pos=pos, vel=vm[mol - 1] * scale, h=h, npt=_frozen_npt(n))
+        eng.run(0, 19)  # lists of step 0 throughout
+        assert (eng.counters()[0] > 0) == stale
+        st = eng.get_state()
+        E, F, W = eng.forces()
+        fresh = (*oracle.nlist_rebo(st["pos"], st["h"], mol),
+                 *oracle.nlist_lj(st["pos"], st["h"], mol, rlist=7.0, max_neigh=2000))
+        assert (fresh[2] < 2000).all()
+        E0, F0, W0 = oracle.airebo_forces(st["pos"], st["h"], fresh, oracle.REBO_PHYSICAL,
+                                          params=lj_o)
+        assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
+        for a, b in zip(E, E0):
+            assert rel(a, b) < TOL
+        if stale:
+            E1, F1, W1 = oracle.airebo_forces(st["pos"], st["h"], old, oracle.REBO_PHYSICAL,
+                                              params=lj_o)
+            assert rel_max(F1, F0) > 20 * TOL
         eng.close()
+
+
+def _radius(pos, h, mol):
+    """Largest atom distance from its molecule's centre (minimum image)."""
+    hi = oracle.inv9(h).reshape(3, 3)
+    hm = h.reshape(3, 3)
+    out = 0.0
+    for m in range(1, mol.max() + 1):
+        x = pos[mol == m]
+        d = x - x[0]
+        d -= (np.round(d @ hi.T)) @ hm.T
+        c = x[0] + d.mean(axis=0)
+        out = max(out, np.linalg.norm(x[0] + d - c, axis=1).max())
+    return out
+
+
+def test_image_modes_follow_a_compressed_cell():
+    """The list walk takes every molecule pair's image from the molecule
+    centres in FP32 only while the cell's half width exceeds the list radius
+    (rc + 2 A) + 2 R_max + 1 A; the engine decides this on the device at every
+    rebuild.  A 3x3x3 crystal starts 0.1 A above that bound and is compressed
+    at a constant cell velocity of -0.08 A/fs (barostat frozen) to below it
+    by the rebuild before step 20.  The step-20 forces equal the oracle's on
+    the reference's truncated lists of the rebuild state.  On the way the
+    cell columns shrink by 0.04 A a step, which makes the per-atom lists of
+    step 0 stale (counted) after about eight steps."""
+    from fuller_md_en_amd import system
+    m0 = system.molecule("C60")
+    rm = np.linalg.norm(m0.coords - m0.coords.mean(axis=0), axis=1).max()
+    a0 = (12.2 + 2 * rm + 1.0 + 0.1) / 1.5
+    cen, h = system.make_crystal("FCC", a0, 3)
+    pos = (cen[:, None, :] + m0.coords[None]).reshape(-1, 3)
+    pos = pos + np.random.default_rng(31).uniform(-0.01, 0.01, pos.shape)
+    p = pos.reshape(-1).copy()
+    oracle.lib().orc_apply_pbc(p, h, oracle.inv9(h), len(pos))
+    pos = p.reshape(-1, 3)
+    mol = np.repeat(np.arange(1, len(cen) + 1), m0.natom).astype(np.int32)
+    n = len(pos)
+    eng = airebo.AtomicEngine(mol, np.full(n, 12.011), mode=airebo.REBO_COMPAT, clamp_vg=0.1)
+    npt = _frozen_npt(n)
+    npt.Vg[0] = npt.Vg[4] = npt.Vg[8] = -0.08
+    eng.set_state(pos=pos, vel=np.zeros_like(pos), h=h, npt=npt)
+    eng.run(0, 19)
+    S = eng.get_state()
+    bound = lambda x, hh: 12.2 + 2 * _radius(x, hh, mol) + 1.0  # noqa: E731
+    assert 0.5 * h[0] > bound(pos, h) and 0.5 * S["h"][0] < bound(S["pos"], S["h"])
+    lists = (*oracle.nlist_rebo(S["pos"], S["h"], mol), *oracle.nlist_lj(S["pos"], S["h"], mol))
+    assert (lists[2] == 400).any()  # truncation active
+    eng.run(19, 1)  # step 20: both lists rebuilt from S
+    S2 = eng.get_state()
+    E, F, W = eng.forces()
+    E0, F0, W0 = oracle.airebo_forces(S2["pos"], S2["h"], lists, oracle.REBO_COMPAT)
+    assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
+    for a, b in zip(E, E0):
+        assert rel(a, b) < TOL
+    assert eng.counters()[0] > 0
+    eng.close()
+
+
+def test_engine_trajectory_across_rebuild_matches_reference():
+    """25 NPT steps of a 3x3x3 crystal (per-atom LJ lists, molecule images)
+    with both lists rebuilt before step 20, against the reference build."""
+    g = golden("airebo_rebuild_nc3.npz")
+    pos, vel = g["pos"].astype(np.float64), g["vel"].astype(np.float64)
+    eng = _engine(pos, g["h"], g["mol_id"], vel=vel)
+    eng.set_state(npt=airebo.NPTState.from_array(g["npt0"]))
+    th = eng.run(0, 25)
+    ep = np.array([[t.Ep, t.Ep_rebo, t.Ep_lj, t.KE] for t in th])
+    for k in range(4):
+        assert rel_max(ep[:, k], g["traj"][:, k]) < TOL
+    st = eng.get_state()
+    L0 = np.diag(g["h"].reshape(3, 3))
+    d = st["pos"] - pos
+    d -= np.round(d / L0) * L0
+    assert np.abs(d - g["dpos_end"]).max() < 1e-6
+    assert rel_max(st["vel"], g["vel_end"]) < 1e-4
+    assert np.abs(st["h"] - g["h_end"]).max() < 1e-9
+    assert eng.counters()[0] == 0 and eng.counters()[1] >= 2
+    eng.close()
 
 
 def test_engine_trajectory_matches_reference():

@@ -202,3 +202,25 @@ def test_engine_forces_newton3():
     assert np.abs(F.sum(axis=0)).max() < 1e-5 * np.abs(F).max() * np.sqrt(len(F))
     assert rel_max(W, W.reshape(3, 3).T.ravel()) < 1e-6  # LJ virial is symmetric
     eng.close()
+
+
+def test_engine_counts_a_stale_list():
+    """The molecule list holds the pairs within rmcut + skin at its build.
+    When the next rebuild comes (step 25), a molecule whose pairs may have
+    closed in by more than the skin since (twice its centre displacement plus
+    the cell change) is counted as a list fault.  The nc=3 state at 300 K
+    with the 3 A skin counts none; at ten times the speeds with a 0.1 A skin
+    it counts.  Thermostat and barostat frozen (masses 1e30)."""
+    g = golden("rigid_core_nc3.npz")
+    n = len(g["pos"])
+    for skin, speed, faults in ((3.0, 1.0, False), (0.1, 10.0, True)):
+        print(f"case skin={skin} speed x{speed}", flush=True)
+        eng = rigid.RigidEngine(g["body"], n, float(g["I0"]), float(g["Mmol"]), rmol=3.55,
+                                rmcut=float(g["rmcut"]), skin=skin)
+        npt = rigid.make_npt(300.0, 0.0, n)
+        npt.Q = npt.W = 1e30
+        eng.set_state(pos=g["pos"], vel=g["vel"] * speed, qv=g["qv"], omg=g["omg"], h=g["h"],
+                      npt=npt)
+        eng.run(0, 25)
+        assert (eng.counters()[0] > 0) == faults
+        eng.close()
