@@ -127,13 +127,46 @@ __global__ __launch_bounds__(256) void k_at_apply_pbc(double *__restrict__ pos,
   pos[3 * i + 2] = h[6] * s0 + h[7] * s1 + h[8] * s2;
 }
 
+// The per-atom lists hold every pair within rc + skin at their build.  A
+// pair vector (centre image + offsets) has since changed by at most the two
+// molecules' drifts (centre displacement plus largest offset change, the
+// replica's largest in al_dmax) and the cell change along the image,
+// |dh n| <= sum of the column changes; while that stays below the skin no
+// pair within rc is missing from replica r's lists.
+// By one whole wavefront: lanes 0-8 carry component k = lane of the cell
+// now (hn) and at the build (h0), lanes 0-7 one drift maximum each (dm,
+// float bits; the others 0).  Every lane returns the verdict.
+__device__ __forceinline__ bool al_stale_wave(const AtomicParams &p, double hn, double h0,
+                                              int dm) {
+  const double x = hn - h0, y = x * x;
+  double dh = 0;
+  for (int c = 0; c < 3; c++)  // column c: components c, c + 3, c + 6
+    dh += sqrt(__shfl(y, c) + __shfl(y, c + 3) + __shfl(y, c + 6));
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) dm = max(dm, __shfl_xor(dm, m, 64));
+  dm = max(dm, __float_as_int(kAlDriftFloor));
+  const double skin = sqrt((double)p.al_r2) - sqrt(p.lj.rc2d) - 1e-3;
+  return 2.0 * (double)__int_as_float(dm) + dh > skin;
+}
+
 // ------------------------------------------------------------- centres
 template <int NS>
 __global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p, HHRef hh) {
   const int lane = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (m >= p.Nm) return;
-  const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
+  const int rep = d.mol_rep[m];
+  const double *h = hh_of(hh, rep), *hi = h + 9;
+  // the wavefront of the replica's first molecule also judges its per-atom
+  // lists (inputs loaded here, ahead of their use at the end)
+  const bool judge = p.lj_alist && !p.small_box && m == d.rep_mol[rep];
+  double jn = 0, j0 = 0;
+  int jd = 0;
+  if (judge && lane < 9) {
+    jn = h[lane];
+    j0 = d.al_h[9 * rep + lane];
+  }
+  if (judge && lane < kAlDriftSlots) jd = d.al_dmax[kAlDriftSlots * rep + lane];
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
   const double rx = d.pos[3 * s0], ry = d.pos[3 * s0 + 1], rz = d.pos[3 * s0 + 2];
   double da[NS][3];
@@ -199,35 +232,15 @@ __global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
       d.qcen[(size_t)m * (p.apad >> 2) + (a >> 2)] =
           c4 > 0.f ? make_float4(qx, qy, qz, rq + 1e-4f) : make_float4(1e6f, 1e6f, 1e6f, 0.f);
   }
-  // drift since the per-atom LJ lists were built: the largest change of a
-  // tile offset plus the centre's displacement (minimum image)
-  float dof = 0.f;
-  bool built = false;
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int a = lane + 64 * s;
-    if (a < n) {
-      const float4 q = d.al_pos[s0 + a];
-      built |= q.w != 0.f;
-      const float ex = (float)(da[s][0] - mx) - q.x, ey = (float)(da[s][1] - my) - q.y,
-                  ez = (float)(da[s][2] - mz) - q.z;
-      dof = fmaxf(dof, sqrtf(ex * ex + ey * ey + ez * ez));
-    }
-  }
-  dof = wave_max(dof);
-  built = __any(built);
+  // whether the replica's per-atom lists must be rebuilt before this step's
+  // LJ walks them (k_at_lj_al; the drifts are final after k_at_kick_drift)
+  const bool redo = judge && al_stale_wave(p, jn, j0, jd);
   if (lane == 0) {
     d.center[3 * m] = rx + mx;
     d.center[3 * m + 1] = ry + my;
     d.center[3 * m + 2] = rz + mz;
     d.radius[m] = rmax;
-    if (built) {
-      double cx = rx + mx - d.al_cen[3 * m], cy = ry + my - d.al_cen[3 * m + 1],
-             cz = rz + mz - d.al_cen[3 * m + 2];
-      mimg(cx, cy, cz, hi, h);
-      const float drift = (float)sqrt(cx * cx + cy * cy + cz * cz) + dof;
-      atomicMax(&d.al_dmax[d.mol_rep[m]], __float_as_int(drift));
-    }
+    if (judge) d.al_redo[rep] = redo;
   }
 }
 
@@ -256,14 +269,17 @@ __global__ __launch_bounds__(256) void k_at_image_check(AtomicDev d, AtomicParam
     if (sqrt(p.lj.rc2d) + 2.0 * rmax + 1.0 < half) bits |= kImgMol;
     if (sqrt(p.lj_rlist2) + 2.0 * rmax + 1.0 < half) bits |= kImgScan;
     d.img[r] = bits;
-    d.al_dmax[r] = 0;  // largest drift since this rebuild (per-atom lists)
+    for (int k = 0; k < kAlDriftSlots; k++)  // largest drift since this rebuild
+      d.al_dmax[kAlDriftSlots * r + k] = 0;
+    d.al_redo[r] = 0;
     if (!p.small_box && !(bits & kImgMol)) atomicAdd(&d.flags[3], 1);
   }
 }
 
 // --------------------------------------------------------- molecule list
-__global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p, HHRef hh) {
-  const int m = blockIdx.x, lane = threadIdx.x;
+// Row m of the molecule list, by one wavefront (lane = 0..63).
+__device__ __forceinline__ void mol_row(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                                        int m, int lane) {
   const int r = d.mol_rep[m];
   const double *h = hh_of(hh, r), *hi = h + 9;
   const double cx = d.center[3 * m], cy = d.center[3 * m + 1], cz = d.center[3 * m + 2];
@@ -293,6 +309,10 @@ __global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p
     d.mol_nl_count[m] = count < p.mol_cap ? count : p.mol_cap;
     if (count > p.mol_cap) atomicAdd(&d.flags[1], 1);
   }
+}
+
+__global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p, HHRef hh) {
+  mol_row(d, p, hh, blockIdx.x, threadIdx.x);
 }
 
 // -------------------------------------------------------------- REBO list
@@ -749,6 +769,27 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
   d.pos[3 * i + 1] = h[3] * sx + h[4] * sy + h[5] * sz;
   d.pos[3 * i + 2] = h[6] * sx + h[7] * sy + h[8] * sz;
   for (int a = 0; a < 3; a++) d.vel[3 * i + a] = v[a];
+  if (p.lj_alist && !p.small_box) {
+    // the atom's drift since the per-atom LJ lists were built, |h (s - s0)|
+    // with s0 its fractional position then (minimum image); the pair
+    // vectors have changed by at most two drifts plus the cell change
+    const float4 q = d.al_pos[i];
+    if (q.w != 0.f) {
+      float ex = (float)sx - q.x, ey = (float)sy - q.y, ez = (float)sz - q.z;
+      ex -= rintf(ex);
+      ey -= rintf(ey);
+      ez -= rintf(ez);
+      const float dx = (float)h[0] * ex + (float)h[1] * ey + (float)h[2] * ez;
+      const float dy = (float)h[3] * ex + (float)h[4] * ey + (float)h[5] * ez;
+      const float dz = (float)h[6] * ex + (float)h[7] * ey + (float)h[8] * ez;
+      const float dr = sqrtf(dx * dx + dy * dy + dz * dz);
+      // drifts up to kAlDriftFloor count as that floor and need no atomic;
+      // larger ones go to one of the replica's kAlDriftSlots maxima
+      if (dr > kAlDriftFloor)
+        atomicMax(d.al_dmax + kAlDriftSlots * rep + (b & (kAlDriftSlots - 1)),
+                  __float_as_int(dr));
+    }
+  }
 }
 
 // ------------------------------------------------------------------ REBO
@@ -1188,7 +1229,7 @@ __device__ __forceinline__ void lj_group(const float4 *rb, const int *jb_jmax, i
 // part the barostat uses (AB:527, 583), one FP64 FMA per (lane, neighbour)
 // instead of nine; the diagonal then carries trace / 3.
 // The body for molecule mi (a whole workgroup); k_at_lj is one workgroup per
-// molecule, and k_at_lj_al takes this path when its per-atom lists are stale.
+// molecule.
 template <int NS, bool PERPAIR, bool TRACE>
 __device__ __forceinline__ void lj_tiles(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                                          int mi) {
@@ -1398,16 +1439,18 @@ constexpr int kAlMolCap = 160;  // the molecule list stride these kernels stage 
 // table, tile slot 0 of molecule 0), never within the cutoff
 constexpr unsigned kAlPad = 255u << 8;
 
-template <int NS>
-__global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p, HHRef hh) {
+// The lists of molecule mi's atoms, by the whole workgroup.  FULL (the
+// scheduled rebuild) also records the replica's cell; a rebuild within a
+// step (stale lists) leaves that to k_at_kick2, since the other workgroups
+// of the step's LJ kernel still read it.
+template <int NS, bool FULL>
+__device__ __forceinline__ void al_build(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                                         int mi) {
   // per neighbour k: centre offset (FP64 minimum image), live four-atom
   // groups (0: none within reach), molecule, first atom, atom count, PRED
   __shared__ double dmk[kAlMolCap][3];
   __shared__ unsigned long long qmk[kAlMolCap];
   __shared__ int mjk[kAlMolCap], t0k[kAlMolCap], tnk[kAlMolCap], prk[kAlMolCap];
-  const int per = (p.Nm + kXcd - 1) / kXcd;
-  const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
-  if (mi >= p.Nm) return;
   constexpr int NA = NS * 64;
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
@@ -1439,11 +1482,17 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const int a = lane + 64 * s;
-      if (a < n) d.al_pos[s0 + a] = make_float4(rax[s], ray[s], raz[s], 1.f);
+      if (a < n) {
+        const double *x = d.pos + 3 * (s0 + a);
+        double fx = hi[0] * x[0] + hi[1] * x[1] + hi[2] * x[2];
+        double fy = hi[3] * x[0] + hi[4] * x[1] + hi[5] * x[2];
+        double fz = hi[6] * x[0] + hi[7] * x[1] + hi[8] * x[2];
+        d.al_pos[s0 + a] = make_float4((float)(fx - floor(fx)), (float)(fy - floor(fy)),
+                                       (float)(fz - floor(fz)), 1.f);
+      }
     }
-    if (lane < 3) d.al_cen[3 * mi + lane] = d.center[3 * mi + lane];
     const int r = d.mol_rep[mi];
-    if (mi == d.rep_mol[r] && lane < 9) d.al_h[9 * r + lane] = h[lane];
+    if (FULL && mi == d.rep_mol[r] && lane < 9) d.al_h[9 * r + lane] = h[lane];
   }
   const float reach_i = rl + d.radius[mi] + 0.05f;
   const int i_last = s0 + n - 1;
@@ -1571,14 +1620,18 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
     while (fill[s] != 0) push(s, kAlPad);
     d.al_n[((size_t)mi * kAlWaves + wv) * NA + lane + 64 * s] =
         (unsigned short)(cnt[s] < nbk ? cnt[s] : nbk);
-    if (cnt[s] > nbk) {
-      // a segment that overflowed lost entries: the replica's lists count as
-      // stale until the next rebuild, so that the forces take every tile pair
-      atomicAdd(&d.flags[2], 1);
-      atomicMax(&d.al_dmax[d.mol_rep[mi]], __float_as_int(1.0e30f));
-    }
+    if (cnt[s] > nbk) atomicAdd(&d.flags[4], 1);  // entries lost: the host turns to k_at_lj
   }
 }
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p, HHRef hh) {
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+  if (mi >= p.Nm) return;
+  al_build<NS, true>(d, p, hh, mi);
+}
+
 
 // LJ from the per-atom lists: lane = atom slot of molecule i, wavefront w
 // walks segment w of the atom's list; each block is one 16-byte load of eight
@@ -1595,32 +1648,26 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
   if (mi >= p.Nm) return;
-  {
-    // The lists hold every pair within rc + skin at their build.  A pair
-    // vector (centre image + offsets) has since changed by at most the two
-    // molecules' drifts (centre displacement plus largest offset change) and
-    // the cell change along the image, |dh n| <= sum of the column changes;
-    // while that stays below the skin no pair within rc is missing.
-    // Otherwise every tile pair is taken until the next rebuild.
-    const int r = d.mol_rep[mi];
-    const double *hn = hh_of(hh, r), *h0 = d.al_h + 9 * r;
-    double dh = 0;
-    for (int c = 0; c < 3; c++) {
-      const double x = hn[c] - h0[c], y = hn[3 + c] - h0[3 + c], z = hn[6 + c] - h0[6 + c];
-      dh += sqrt(x * x + y * y + z * z);
-    }
-    const double skin = sqrt((double)p.al_r2) - sqrt(p.lj.rc2d) - 1e-3;
-    if (2.0 * (double)__int_as_float(d.al_dmax[r]) + dh > skin) {
-      if (threadIdx.x == 0 && mi == d.rep_mol[r]) atomicAdd(&d.flags[2], 1);
-      lj_tiles<NS, false, TRACE>(d, p, hh, mi);
-      return;
-    }
-  }
   constexpr int NA = NS * 64;
   constexpr int NW = TRACE ? 1 : 6;
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
-  const double *h = hh_of(hh, d.mol_rep[mi]), *hi = h + 9;
+  const int rep = d.mol_rep[mi];
+  if (__builtin_expect(d.al_redo[rep] != 0, 0)) {
+    // The replica's lists went stale (k_at_centers decided, alike for every
+    // workgroup): rebuilt here from this step's positions, the molecule's
+    // row of the molecule list, then its atoms' lists, keeping the
+    // truncation points of the scheduled rebuild; k_at_kick2 records the
+    // rebuilt state.
+    if (threadIdx.x < 64) mol_row(d, p, hh, mi, threadIdx.x);
+    __threadfence();
+    __syncthreads();
+    al_build<NS, false>(d, p, hh, mi);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0 && mi == d.rep_mol[rep]) atomicAdd(&d.flags[2], 1);
+  }
+  const double *h = hh_of(hh, rep), *hi = h + 9;
   const int apad = p.apad;
   const int s0 = d.mol_start[mi], n = d.mol_start[mi + 1] - s0;
   const int nq = d.mol_nl_count[mi];
@@ -1772,6 +1819,16 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
 
 // ------------------------------------------------------- kick 2 + partials
 __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, int kick) {
+  if (kick && p.lj_alist && !p.small_box && threadIdx.x == 0) {
+    // lists rebuilt within this step's LJ kernel: their cell, drift zero
+    const int r = d.blk_rep[blockIdx.x];
+    if (blockIdx.x == d.rep_blk[r] && d.al_redo[r]) {
+      const double *hm = d.mid[r].h;
+      for (int k = 0; k < 9; k++) d.al_h[9 * r + k] = hm[k];
+      for (int k = 0; k < kAlDriftSlots; k++) d.al_dmax[kAlDriftSlots * r + k] = 0;
+      d.al_redo[r] = 0;
+    }
+  }
   const int i = d.blk_a0[blockIdx.x] + threadIdx.x;
   double v2[1] = {0};
   if (i < d.blk_a1[blockIdx.x]) {

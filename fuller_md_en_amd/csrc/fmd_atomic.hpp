@@ -60,7 +60,9 @@
 namespace fmd {
 
 constexpr int kAtPart = 10;  // W9 + E
-constexpr int kAtFlags = 4;  // AtomicDev::flags
+constexpr int kAtFlags = 5;  // AtomicDev::flags
+constexpr int kAlDriftSlots = 8;  // AtomicDev::al_dmax entries per replica
+constexpr float kAlDriftFloor = 0.125f;  // drifts below it count as it (A)
 // image modes of a replica at a list rebuild (AtomicDev::img)
 enum : int {
   kImgMol = 1,   // atom pairs within rc + 1 A are the centres' image + offsets
@@ -128,15 +130,17 @@ struct AtomicDev {
   double *rebo_fe;  // 3 x Na*maxr: per-entry REBO forces (global-memory REBO path)
   unsigned short *al_e;  // per-atom LJ lists: [Nm][4][al_cap / 8][NA] blocks of 8 entries
   unsigned short *al_n;  // segment lengths in blocks: [Nm][4][NA]  (NA = 64 * NS)
-  // per-atom list staleness: at the last list build, each atom's tile offset
-  // (al_pos, w = 1 once set), each molecule's centre (al_cen) and each
-  // replica's cell (al_h); al_dmax[r], the largest molecule drift since
-  // then (centre displacement plus largest offset change, float bits; reset
-  // by the image check that opens every rebuild, 1e30 once a list segment
-  // overflowed)
+  // per-atom list staleness: at the last list build, each atom's fractional
+  // position (al_pos, w = 1 once set) and each replica's cell (al_h);
+  // al_dmax[kAlDriftSlots r + k], the largest atom drift |h (s - s0)| since
+  // then over replica r's integrator blocks b = k mod kAlDriftSlots (float
+  // bits, drifts below kAlDriftFloor not recorded; reset by the image check
+  // that opens every scheduled rebuild, and by k_at_kick2 after a rebuild
+  // within a step)
   float4 *al_pos;
-  double *al_cen, *al_h;
+  double *al_h;
   int *al_dmax;
+  int *al_redo;  // per replica: lists to rebuild within this step (k_at_centers)
   int *img;  // per replica, decided at every list rebuild: kImgMol | kImgScan
   double *hh;  // 18 doubles for the stateless entry points
   AtomicState *state;
@@ -144,10 +148,11 @@ struct AtomicDev {
   StepCtrl *ctrl;
   Thermo *thermo;
   int *flags;  // [0] REBO list overflow, [1] molecule list overflow,
-               // [2] per-atom LJ list overflow, or a step whose per-atom
-               // lists had gone stale (the LJ kernel then takes every tile
-               // pair), [3] image fault: the cell became too narrow for
-               // per-molecule images while the engine stepped with them
+               // [2] a step whose per-atom lists had gone stale (the LJ
+               // kernel rebuilt them from that step's positions), [3] image
+               // fault: the cell became too narrow for per-molecule images
+               // while the engine stepped with them, [4] per-atom list
+               // overflow (the engine then turns to the tile kernel)
   int *mol_rep, *atom_rep, *rep_mol, *rep_atom;
   int *blk_a0, *blk_a1, *blk_rep, *rep_blk;
   int nblk;    // per-atom integrator blocks (<= 256 atoms of one replica each)

@@ -75,7 +75,7 @@ void free_atomic_dev(AtomicDev &d) {
                   d.mol_nl,    d.nlc_r,     d.nll_r,    d.bonds.bd,  d.bonds.ba,   d.bonds.bp,
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.rebo_mpart,
                   d.lj_part,   d.ke_part,   d.rebo_fe,   d.al_e,       d.al_n,
-                  d.al_pos,    d.al_cen,    d.al_h,     d.al_dmax,   d.img,
+                  d.al_pos,    d.al_h,      d.al_dmax,  d.al_redo,   d.img,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
                   d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
                   d.blk_rep,   d.rep_blk};
@@ -180,9 +180,9 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
     AL(d.al_e, Nm * 4 * (size_t)p.al_cap * na);
     AL(d.al_n, Nm * 4 * na);
     AL(d.al_pos, Na);
-    AL(d.al_cen, 3 * Nm);
     AL(d.al_h, 9 * R);
-    AL(d.al_dmax, R);
+    AL(d.al_dmax, kAlDriftSlots * R);
+    AL(d.al_redo, R);
   }
   AL(d.img, R);
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
@@ -210,7 +210,8 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   }
   (void)hipMemset(d.flags, 0, kAtFlags * sizeof(int));
   (void)hipMemset(d.al_pos, 0, Na * sizeof(float4));
-  (void)hipMemset(d.al_dmax, 0, R * sizeof(int));
+  (void)hipMemset(d.al_dmax, 0, kAlDriftSlots * R * sizeof(int));
+  (void)hipMemset(d.al_redo, 0, R * sizeof(int));
   (void)hipMemset(d.F, 0, 3 * Na * sizeof(double));
   (void)hipMemset(d.state, 0, R * sizeof(AtomicState));
   (void)hipMemset(d.nlc_r, 0, Na * sizeof(int));
@@ -283,9 +284,10 @@ int engine_prepare_atomic(fmd_engine *e) {
 // image fault (the cell became too narrow for per-molecule images while the
 // engine stepped with them), the image mode is decided again from the
 // current state, so that the following steps take every pair's own image.
+// After a per-atom list overflow the engine evaluates LJ by molecule tiles.
 int read_flags(fmd_engine *e) {
   AtomicEngine &A = *e->at;
-  int f[kAtFlags] = {0, 0, 0, 0};
+  int f[kAtFlags] = {};
   HIPCHK(hipMemcpyAsync(f, A.d.flags, sizeof f, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   int any = 0;
@@ -294,6 +296,10 @@ int read_flags(fmd_engine *e) {
     any |= f[k];
   }
   if (any) HIPCHK(hipMemsetAsync(A.d.flags, 0, sizeof f, e->stream));
+  if (f[4] && A.p.lj_alist) {
+    A.p.lj_alist = 0;
+    engine_drop_graphs(e);
+  }
   if (f[3]) {
     for (int r = 0; r < A.p.R; r++) {
       const size_t a0 = A.rep_atom[r], na = A.rep_atom[r + 1] - a0;
