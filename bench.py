@@ -102,6 +102,8 @@ def parse():
     ap.add_argument("--mm", choices=["compat", "physical"], default="physical",
                     help="mm workload: the reference's forces (which heat the crystal without "
                          "bound) or the exact gradient")
+    ap.add_argument("--al-skin", type=float, default=0.0,
+                    help="airebo workload: per-atom LJ list skin (default: the autotuner's)")
     ap.add_argument("--rebo", choices=["compat", "physical"], default="physical",
                     help="REBO force mode of the timed run")
     ap.add_argument("--sg-iters", type=int, default=30000,
@@ -340,6 +342,34 @@ def airebo_cpu_leg(s, nc):
     return cpu, out
 
 
+def airebo_engine(s, mode, params):
+    from fuller_md_en_amd import airebo
+    eng = airebo.AtomicEngine(s["mol_id"], s["mass"], dt=0.5, mode=mode,
+                              al_skin=params["al_skin"])
+    eng.set_graph(bool(params["graph"]))
+    eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"],
+                  npt=airebo.make_npt(T_K, 0.0, len(s["pos"])))
+    return eng
+
+
+def airebo_params(args, s, nc, mode):
+    """The per-atom list skin and step replay: --al-skin, the defaults
+    (--no-tune), or the autotuner's choice for this system (cached per device,
+    system, REBO mode, temperature and pressure)."""
+    from fuller_md_en_amd import autotune
+    default = {k: v[0] for k, v in autotune.ATOMIC_SPACE.items()}
+    if args.al_skin:
+        return {**default, "al_skin": args.al_skin}, None
+    if args.no_tune:
+        return default, None
+    key = (f"{autotune.device_name()}|airebo|C60-FCC{nc}|{args.rebo}|T={T_K:g}|P=0|"
+           f"seed={SEED}")
+    entry = autotune.tune(key, autotune.engine_runner(lambda p: airebo_engine(s, mode, p),
+                                                      steps=200, warmup=20),
+                          autotune.ATOMIC_SPACE)
+    return entry["params"], entry
+
+
 def main_airebo(args):
     import numpy as np
     from fuller_md_en_amd import airebo, system
@@ -347,10 +377,10 @@ def main_airebo(args):
     s = system.airebo_system(nc, "C60", T=T_K, seed=SEED)
     na, nm = len(s["pos"]), s["n_mol"]
     mode = airebo.REBO_PHYSICAL if args.rebo == "physical" else airebo.REBO_COMPAT
-    eng = airebo.AtomicEngine(s["mol_id"], s["mass"], dt=0.5, mode=mode)
-    eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"], npt=airebo.make_npt(T_K, 0.0, na))
+    params, tuned = airebo_params(args, s, nc, mode)
+    eng = airebo_engine(s, mode, params)
     dt, th, ph, rebuild = timed_run(eng, args.warmup, args.steps, 20)
-    faults = eng.counters()[0]
+    faults, rebuilds = eng.counters()[:2]
     eng.close()
     value = na * args.steps / dt
     lj_ms, rebo_ms = ph["lj"][0], ph["intra"][0]
@@ -368,7 +398,8 @@ def main_airebo(args):
                value, args.steps, args.warmup, dt, "fp32 pair/bond-order, fp64 integrator",
                lj, kernels=[rebo, force], rebuild=rebuild,
                value_amortized=na / (rebuild["ms_per_step_amortized"] * 1e-3),
-               list_faults=faults,
+               list_faults=faults, list_rebuilds=rebuilds, params=params,
+               tuned=tuned and {k: tuned[k] for k in ("seconds", "default_seconds", "tried")},
                final={"T": last.T, "P": last.P, "E_rebo_per_mol": last.Ep_rebo / nm,
                       "E_lj_per_mol": last.Ep_lj / nm})
     if not args.no_cpu:
@@ -400,7 +431,7 @@ def tuned_params(args, nc):
     if args.no_tune:
         return {k: v[0] for k, v in autotune.RIGID_SPACE.items()}, None
     key = f"{autotune.device_name()}|rigid|C60-FCC{nc}|a=14.17|T={T_K:g}|P=0"
-    entry = autotune.tune(key, autotune.rigid_runner(lambda p: rigid_engine(p, nc)[0],
+    entry = autotune.tune(key, autotune.engine_runner(lambda p: rigid_engine(p, nc)[0],
                                                      steps=200, warmup=20),
                           autotune.RIGID_SPACE)
     return entry["params"], entry

@@ -57,7 +57,8 @@ class AtomicConfig(C.Structure):
                 ("mass", C.POINTER(C.c_double)), ("lj", LJParams), ("lj_skin", C.c_double),
                 ("lj_max_neigh", C.c_int), ("rebo_rlist", C.c_double),
                 ("rebo_max_neigh", C.c_int), ("rebo_mode", C.c_int), ("nlup", C.c_int),
-                ("dt", C.c_double), ("clamp_xi", C.c_double), ("clamp_vg", C.c_double)]
+                ("dt", C.c_double), ("clamp_xi", C.c_double), ("clamp_vg", C.c_double),
+                ("al_skin", C.c_double)]
 
 
 class StepCtrl(C.Structure):

@@ -83,7 +83,7 @@ class AtomicEngine:
     """HBM-resident AIREBO NPT engine (fmd_atomic_create / fmd_engine_*)."""
 
     def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
-                 nlup=20, lj=None, rebo_rlist=None, clamp_vg=None):
+                 nlup=20, lj=None, rebo_rlist=None, clamp_vg=None, al_skin=None):
         self._mol = _c(mol_id, np.int32)
         self._mass = _c(mass)
         cfg = N.AtomicConfig()
@@ -98,6 +98,8 @@ class AtomicEngine:
             cfg.rebo_rlist = rebo_rlist
         if clamp_vg is not None:
             cfg.clamp_vg = clamp_vg
+        if al_skin is not None:
+            cfg.al_skin = al_skin
         self.cfg = cfg
         self.n = len(self._mol)
         h = C.c_void_p()
@@ -179,7 +181,7 @@ class ReplicaBatch:
     HIP graphs.  replicas: list of (mol_id, mass) per replica."""
 
     def __init__(self, replicas, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH, nlup=20,
-                 lj=None, graph=True):
+                 lj=None, graph=True, al_skin=None):
         self._keep = []
         cfgs = (N.AtomicConfig * len(replicas))()
         for r, (mol_id, mass) in enumerate(replicas):
@@ -193,6 +195,8 @@ class ReplicaBatch:
             cfgs[r].lj_max_neigh, cfgs[r].nlup = lj_max_neigh, nlup
             if lj is not None:
                 cfgs[r].lj = lj
+            if al_skin is not None:
+                cfgs[r].al_skin = al_skin
         self.sizes = [len(mol_id) for mol_id, _ in replicas]
         self.R = len(replicas)
         h = C.c_void_p()

@@ -11,8 +11,13 @@ Tunables:
   rigid    skin (list radius beyond the force cutoff), nlup (rebuild
            interval), half_pairs (1 pair-once per owned pair, 2 pair-once
            per molecule, 0 both sides), graph (step replay)
-  atomic   graph (step replay); the list radius and rebuild interval are
-           part of the reference's truncated-half-list semantics and stay
+  atomic   al_skin (the per-atom LJ lists' radius beyond the cutoff: a
+           thinner skin walks fewer candidates per step but rebuilds the
+           lists within steps more often), graph (step replay); the half-list
+           radius and rebuild interval are part of the reference's
+           truncated-half-list semantics and stay
+
+A candidate that counts a list fault is rejected.
 """
 from __future__ import annotations
 
@@ -53,7 +58,7 @@ def store(key: str, entry: dict, path: str | None = None) -> None:
 
 RIGID_SPACE = {"skin": [3.0, 2.0, 1.5], "nlup": [25, 10, 50], "half_pairs": [1, 2, 0],
                "graph": [0, 1]}
-ATOMIC_SPACE = {"graph": [0, 1]}
+ATOMIC_SPACE = {"al_skin": [1.0, 0.75, 0.6, 0.5], "graph": [0, 1]}
 
 
 def candidates(space: dict):
@@ -115,9 +120,9 @@ def device_name() -> str:
         return "unknown"
 
 
-def rigid_runner(build, steps: int = 200, warmup: int = 20):
-    """run(params) for tune(): build(params) -> a prepared-state RigidEngine
-    (fresh, same initial state every call)."""
+def engine_runner(build, steps: int = 200, warmup: int = 20):
+    """run(params) for tune(): build(params) -> an engine (RigidEngine or
+    AtomicEngine) in the same initial state on every call."""
     def run(params):
         eng = build(params)
         eng.set_graph(bool(params.get("graph", 0)))
@@ -133,3 +138,4 @@ def rigid_runner(build, steps: int = 200, warmup: int = 20):
         eng.close()
         return out
     return run
+

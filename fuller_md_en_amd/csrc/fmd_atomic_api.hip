@@ -235,7 +235,7 @@ void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Na, int Nm, in
   const double rl = c.lj.rcut + c.lj_skin;
   p.lj_rlist2 = rl * rl;  // (LJ_RCUT + 2.0)^2, AB:265
   p.mol_rlist = rl + 1.0;  // + 1 A for centre drift between rebuilds
-  p.al_r2 = (float)((c.lj.rcut + 1.0) * (c.lj.rcut + 1.0));  // per-atom lists: 1 A skin
+  p.al_r2 = (float)((c.lj.rcut + c.al_skin) * (c.lj.rcut + c.al_skin));  // per-atom lists
   p.al_cap = 384;  // per wavefront segment: about 3.4 times a C60 crystal's share
   p.dt = c.dt;
   p.clamp_xi = c.clamp_xi;
@@ -292,7 +292,10 @@ int read_flags(fmd_engine *e) {
   HIPCHK(hipStreamSynchronize(e->stream));
   int any = 0;
   for (int k = 0; k < kAtFlags; k++) {
-    e->overflow += f[k];
+    if (k == 2)
+      e->rebuilds += f[k];  // per-atom lists rebuilt within a step
+    else
+      e->overflow += f[k];
     any |= f[k];
   }
   if (any) HIPCHK(hipMemsetAsync(A.d.flags, 0, sizeof f, e->stream));
@@ -502,6 +505,7 @@ void fmd_atomic_config_default(fmd_atomic_config *c) {
   c->dt = 0.5;
   c->clamp_xi = 0.05;
   c->clamp_vg = 0.005;
+  c->al_skin = 1.0;
 }
 
 int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
@@ -517,6 +521,8 @@ int fmd_batch_create(const fmd_atomic_config *c, int R, fmd_engine **out) {
 // R replicas with the intramolecular force field ff (kFfAirebo / kFfMM).
 int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine **out) {
   if (!c || !out || R < 1) return set_error(FMD_ERR_ARG, "fmd_batch_create: bad argument");
+  if (!(c[0].al_skin > 0.0 && c[0].al_skin <= c[0].lj_skin + 1.0))
+    return set_error(FMD_ERR_ARG, "fmd_batch_create: al_skin must lie in (0, lj_skin + 1]");
   for (int r = 0; r < R; r++) {
     if (!c[r].mol_id || !c[r].mass || c[r].n_atoms < 1)
       return set_error(FMD_ERR_ARG, "fmd_batch_create: replica %d: bad configuration", r);
@@ -525,7 +531,7 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
         a.lj_skin != b.lj_skin || a.lj_max_neigh != b.lj_max_neigh ||
         a.rebo_rlist != b.rebo_rlist || a.rebo_max_neigh != b.rebo_max_neigh ||
         a.rebo_mode != b.rebo_mode || a.nlup != b.nlup || a.dt != b.dt ||
-        a.clamp_xi != b.clamp_xi || a.clamp_vg != b.clamp_vg)
+        a.clamp_xi != b.clamp_xi || a.clamp_vg != b.clamp_vg || a.al_skin != b.al_skin)
       return set_error(FMD_ERR_ARG,
                        "fmd_batch_create: replica %d: potential, lists, dt, nlup and clamps "
                        "must equal replica 0's",

@@ -314,6 +314,11 @@ typedef struct {
   int nlup;              /* list rebuild interval (20, AB:860) */
   double dt;             /* fs (0.5) */
   double clamp_xi, clamp_vg; /* 0.05, 0.005 (AB:524, 530) */
+  double al_skin;        /* engines: the per-atom LJ lists hold the pairs within
+                            rcut + al_skin (1.0; no counterpart in the reference,
+                            whose LJ list is the half list above).  Results do
+                            not depend on it beyond summation order: stale lists
+                            are rebuilt within the step. */
 } fmd_atomic_config;
 
 /* Defaults of the reference AIREBO program. */
@@ -440,9 +445,11 @@ int fmd_engine_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_ctr
  * F (Na x 3), T unused.  W9, energies {Ep, Ep_rebo, Ep_lj}.  Any may be NULL. */
 int fmd_engine_forces(fmd_engine *e, double *F, double *T, double *W9, double *energies);
 
-/* Counters: [0] list faults since creation (a list overflow, or an atom of an
- * atomic engine moving half its per-atom LJ list skin, 0.5 A, between
- * rebuilds), [1] list rebuilds, [2] steps. */
+/* Counters: [0] list faults since creation (a list overflow; a rigid
+ * molecule list found stale at its rebuild; an atomic engine's cell grown too
+ * narrow for per-molecule images), [1] list rebuilds: the scheduled ones, and
+ * for atomic engines also each replica's per-atom LJ lists rebuilt within a
+ * step because an atom pair may have closed their skin, [2] steps. */
 int fmd_engine_counters(fmd_engine *e, long long *out3);
 
 /* Mean duration (ms) of the force kernel over the launches timed since the

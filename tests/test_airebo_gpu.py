@@ -202,18 +202,19 @@ def _frozen_npt(n):
     return npt
 
 
-def test_stale_lj_lists_fall_back_to_every_pair():
+def test_stale_lj_lists_are_rebuilt_within_the_step():
     """The per-atom LJ lists hold the pairs within rc + 1 A at their build.
     Molecules of a 3x3x3 crystal (lattice constant 1.1 times the default)
     given random rigid velocities of up to 0.05 A/fs per component (cell and
     thermostat frozen, REBO bonds unchanged) close in on each other by more
-    than that skin before the rebuild at step 20: the engine counts the stale
-    lists and rebuilds them within the step, so its forces at step 19 equal
-    the oracle's on complete lists of that state, while the pairs within
+    than that skin before the scheduled rebuild at step 20: the engine
+    rebuilds the lists within the steps where a pair may have closed the
+    skin (counted with the rebuilds), so its forces at step 19 equal the
+    oracle's on complete lists of that state, while the pairs within
     rc + 1 A of the initial state miss some (6e-4 of the largest force).  At
-    1/20 of the speed nothing is counted.  LJ cut at 5 A, where the pairs entering
-    the cutoff carry forces well above the tolerance; untruncated lists
-    (lj_max_neigh = 0), so that both sides mean every pair within rc."""
+    1/20 of the speed nothing is rebuilt.  LJ cut at 5 A, where the pairs
+    entering the cutoff carry forces well above the tolerance; untruncated
+    lists (lj_max_neigh = 0), so that both sides mean every pair within rc."""
     from fuller_md_en_amd import rigid, system
     m0 = system.molecule("C60")
     cen, h = system.make_crystal("FCC", 1.1 * system.default_a0(m0.dmol, "FCC", 1.0), 3)
@@ -233,8 +234,9 @@ def test_stale_lj_lists_fall_back_to_every_pair():
         eng = airebo.AtomicEngine(mol, np.full(n, 12.011), mode=airebo.REBO_PHYSICAL,
                                   lj_max_neigh=0, lj=lj)
         eng.set_state(pos=pos, vel=vm[mol - 1] * scale, h=h, npt=_frozen_npt(n))
-        eng.run(0, 19)  # lists of step 0 throughout
-        assert (eng.counters()[0] > 0) == stale
+        eng.run(0, 19)  # no scheduled rebuild after the first
+        faults, rebuilds = eng.counters()[:2]
+        assert faults == 0 and (rebuilds > 1) == stale
         st = eng.get_state()
         E, F, W = eng.forces()
         fresh = (*oracle.nlist_rebo(st["pos"], st["h"], mol),
@@ -275,7 +277,7 @@ def test_image_modes_follow_a_compressed_cell():
     by the rebuild before step 20.  The step-20 forces equal the oracle's on
     the reference's truncated lists of the rebuild state.  On the way the
     cell columns shrink by 0.04 A a step, which makes the per-atom lists
-    stale (counted, rebuilt within the step) every few steps."""
+    stale every few steps (rebuilt within the step)."""
     from fuller_md_en_amd import system
     m0 = system.molecule("C60")
     rm = np.linalg.norm(m0.coords - m0.coords.mean(axis=0), axis=1).max()
@@ -305,7 +307,8 @@ def test_image_modes_follow_a_compressed_cell():
     assert rel_max(F, F0) < TOL and rel_max(W, W0) < TOL
     for a, b in zip(E, E0):
         assert rel(a, b) < TOL
-    assert eng.counters()[0] > 0
+    faults, rebuilds = eng.counters()[:2]
+    assert faults == 0 and rebuilds > 2  # the scheduled ones: the first and step 20
     eng.close()
 
 

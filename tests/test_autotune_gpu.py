@@ -24,7 +24,7 @@ def _build(g):
 
 def test_tuned_rigid_parameters(tmp_path):
     g = golden("rigid_core_nc3.npz")
-    run = autotune.rigid_runner(_build(g), steps=120, warmup=10)
+    run = autotune.engine_runner(_build(g), steps=120, warmup=10)
     entry = autotune.tune("test|rigid|nc3", run, autotune.RIGID_SPACE,
                           path=str(tmp_path / "tune.json"))
     print(entry["params"], entry["seconds"], entry["default_seconds"], flush=True)
