@@ -80,7 +80,8 @@ FLOP_PER_ATOM_MM_LJ = 42.0 * 169.6
 # WRITE x 1.0 (profiles/pmc_traffic_*.md).
 RIGID_TRAFFIC = {1: (9710.9 * 2.0 + 23524.8 * 1.0) * 1024.0,
                  2: (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0}
-AIREBO_LJ_TRAFFIC = (66058.9 * 2.0 + 3040.2 * 1.0) * 1024.0
+AIREBO_LJ_TRAFFIC = (57568.3 * 2.0 + 3040.1 * 1.0) * 1024.0  # tuned 0.5 A list skin
+AIREBO_REBO_TRAFFIC = (4730.8 * 2.0 + 3040.0 * 1.0) * 1024.0
 RIGID_KERNEL = {1: "k_rigid_pairs", 2: "k_rigid_forces_half", 0: "k_rigid_forces"}
 TIMING_STRIDE = 10
 
@@ -387,7 +388,8 @@ def main_airebo(args):
     lj = kernel_rate("k_at_lj_al", FLOP_PER_ATOM_LJ * na, lj_ms, PEAK_FP32_TFLOPS,
                      traffic=AIREBO_LJ_TRAFFIC)
     lj["launches"] = ph["lj"][1]
-    rebo = kernel_rate("k_rebo_mol", FLOP_PER_ATOM_REBO * na, rebo_ms, PEAK_FP32_TFLOPS)
+    rebo = kernel_rate("k_rebo_mol", FLOP_PER_ATOM_REBO * na, rebo_ms, PEAK_FP32_TFLOPS,
+                       traffic=AIREBO_REBO_TRAFFIC)
     force = kernel_rate("force phase (k_rebo_mol + k_at_lj_al)",
                         (FLOP_PER_ATOM_LJ + FLOP_PER_ATOM_REBO) * na, ph["force"][0],
                         PEAK_FP32_TFLOPS)

@@ -1633,6 +1633,30 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
 }
 
 
+// The lists of molecule mi rebuilt from this step's positions: its row of
+// the molecule list, then its atoms' lists, keeping the truncation points of
+// the scheduled rebuild (every workgroup of the replica decides alike;
+// k_at_kick2 records the rebuilt state).
+template <int NS>
+__device__ __forceinline__ void al_relist(const AtomicDev &d, const AtomicParams &p, HHRef hh,
+                                          int mi) {
+  if (threadIdx.x < 64) mol_row(d, p, hh, mi, threadIdx.x);
+  __threadfence();
+  __syncthreads();
+  al_build<NS, false>(d, p, hh, mi);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0 && mi == d.rep_mol[d.mol_rep[mi]]) atomicAdd(&d.flags[2], 1);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_at_lj_relist(AtomicDev d, AtomicParams p, HHRef hh) {
+  const int per = (p.Nm + kXcd - 1) / kXcd;
+  const int mi = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+  if (mi >= p.Nm || !d.al_redo[d.mol_rep[mi]]) return;
+  al_relist<NS>(d, p, hh, mi);
+}
+
 // LJ from the per-atom lists: lane = atom slot of molecule i, wavefront w
 // walks segment w of the atom's list; each block is one 16-byte load of eight
 // entries, taken four at a time (the neighbours' centre offsets from LDS,
@@ -1653,20 +1677,11 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
   const int lane = threadIdx.x & 63;
   const int wv = uniform(threadIdx.x >> 6);
   const int rep = d.mol_rep[mi];
-  if (__builtin_expect(d.al_redo[rep] != 0, 0)) {
-    // The replica's lists went stale (k_at_centers decided, alike for every
-    // workgroup): rebuilt here from this step's positions, the molecule's
-    // row of the molecule list, then its atoms' lists, keeping the
-    // truncation points of the scheduled rebuild; k_at_kick2 records the
-    // rebuilt state.
-    if (threadIdx.x < 64) mol_row(d, p, hh, mi, threadIdx.x);
-    __threadfence();
-    __syncthreads();
-    al_build<NS, false>(d, p, hh, mi);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0 && mi == d.rep_mol[rep]) atomicAdd(&d.flags[2], 1);
-  }
+  // Stale lists (k_at_centers decided for the replica) are rebuilt from this
+  // step's positions before the walk: here for one atom per lane; for two
+  // the rebuild's registers would cost the walk a wavefront per SIMD, and
+  // k_at_lj_relist runs before this kernel instead.
+  if (NS == 1 && __builtin_expect(d.al_redo[rep] != 0, 0)) al_relist<NS>(d, p, hh, mi);
   const double *h = hh_of(hh, rep), *hi = h + 9;
   const int apad = p.apad;
   const int s0 = d.mol_start[mi], n = d.mol_start[mi + 1] - s0;
@@ -2130,6 +2145,7 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
     else
       k_at_lj<2, true, false><<<g, 256, 0, s>>>(d, p, hh);
   } else if (p.lj_alist) {
+    if (!one) k_at_lj_relist<2><<<g, 256, 0, s>>>(d, p, hh);
     if (p.lj_trace) {
       if (one)
         k_at_lj_al<1, true><<<g, 256, 0, s>>>(d, p, hh);

@@ -202,9 +202,11 @@ def _frozen_npt(n):
     return npt
 
 
-def test_stale_lj_lists_are_rebuilt_within_the_step():
+@pytest.mark.parametrize("spec,expand", [("C60", 1.1), ("C70", 1.0)])
+def test_stale_lj_lists_are_rebuilt_within_the_step(spec, expand):
     """The per-atom LJ lists hold the pairs within rc + 1 A at their build.
-    Molecules of a 3x3x3 crystal (lattice constant 1.1 times the default)
+    Molecules of a 3x3x3 crystal (C60: lattice constant 1.1 times the
+    default; C70, whose default is wider: 1.0)
     given random rigid velocities of up to 0.05 A/fs per component (cell and
     thermostat frozen, REBO bonds unchanged) close in on each other by more
     than that skin before the scheduled rebuild at step 20: the engine
@@ -214,10 +216,12 @@ def test_stale_lj_lists_are_rebuilt_within_the_step():
     rc + 1 A of the initial state miss some (6e-4 of the largest force).  At
     1/20 of the speed nothing is rebuilt.  LJ cut at 5 A, where the pairs
     entering the cutoff carry forces well above the tolerance; untruncated
-    lists (lj_max_neigh = 0), so that both sides mean every pair within rc."""
+    lists (lj_max_neigh = 0), so that both sides mean every pair within rc.
+    C70 takes two atoms per lane, whose lists are rebuilt by a kernel of
+    their own before the walk."""
     from fuller_md_en_amd import rigid, system
-    m0 = system.molecule("C60")
-    cen, h = system.make_crystal("FCC", 1.1 * system.default_a0(m0.dmol, "FCC", 1.0), 3)
+    m0 = system.molecule(spec)
+    cen, h = system.make_crystal("FCC", expand * system.default_a0(m0.dmol, "FCC", 1.0), 3)
     pos = (cen[:, None, :] + m0.coords[None]).reshape(-1, 3)
     pos = pos + np.random.default_rng(7).uniform(-0.01, 0.01, pos.shape)
     p = pos.reshape(-1).copy()
@@ -230,7 +234,7 @@ def test_stale_lj_lists_are_rebuilt_within_the_step():
     old = (*oracle.nlist_rebo(pos, h, mol),
            *oracle.nlist_lj(pos, h, mol, rlist=6.0, max_neigh=2000))
     for scale, stale in ((0.05, False), (1.0, True)):
-        print(f"case speed scale={scale}", flush=True)
+        print(f"case {spec} speed scale={scale}", flush=True)
         eng = airebo.AtomicEngine(mol, np.full(n, 12.011), mode=airebo.REBO_PHYSICAL,
                                   lj_max_neigh=0, lj=lj)
         eng.set_state(pos=pos, vel=vm[mol - 1] * scale, h=h, npt=_frozen_npt(n))
