@@ -61,9 +61,15 @@ EV2GPA = 160.21766208
 # half list keeps 19.08 M in-cutoff pairs of the C60 FCC 8^3 crystal, 155.3
 # per atom (6.5% of the true pairs are truncated).
 FLOP_PER_ATOM_LJ = 42.0 * 155.3
-# REBO: about 880 FLOP and 32 transcendentals per bond (two ends, two side
-# neighbours each), 1.5 bonds per C60 atom.
-FLOP_PER_ATOM_REBO = 880.0 * 1.5
+# REBO, exact-gradient mode, per bond of a C60 crystal (three bonds per atom,
+# all inside the switch's inner radius, so two side bonds at each end): bond
+# vector, minimum image, length and reciprocal 42; pair terms 25 (+4 exp);
+# unit vector 3; four angles: cosine 5 + G, dG and the G sum 12 each (68);
+# bond orders at both ends 8 (+2 rsqrt); energy and pair-force scalar 10;
+# pair force on both atoms and the virial tensor 27; four angular force
+# triples 28 each (112); the exact gradient's partner term and triple virial
+# 18 each (72): 370 FLOP (and 7 transcendentals) per bond, 1.5 bonds per atom.
+FLOP_PER_ATOM_REBO = 370.0 * 1.5
 # Rigid LJ: 69 FLOP per unique in-cutoff site pair (11 displacement + r^2, 13
 # LJ scalar incl. energy, 3 force, 6 force accumulation, 24 torque, 12 virial)
 # x 163.2 unique pairs per site (C60 FCC, a = 14.17 A); 162.6 for the core
