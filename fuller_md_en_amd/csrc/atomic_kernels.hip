@@ -1892,6 +1892,36 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
     }
   }
   const int m0 = d.rep_mol[r], m1 = d.rep_mol[r + 1];
+  if (p.lj_trace && !p.use_sg && !p.small_box) {
+    // time steps: the LJ and REBO partials hold the virial trace / 3 on the
+    // diagonal and zeros off it, so two of their ten components carry all
+    // (the same sums as below, in the same order; the per-pair-image LJ
+    // kernel of narrow cells keeps the full tensor)
+    for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
+      const double lt = d.lj_part[m], le = d.lj_part[9 * nm + m];
+      double x[13];
+      if (MM) {
+#pragma unroll
+        for (int k = 0; k < 13; k++) x[k] = d.mm.part[k * nm + m];
+      } else {
+        x[0] = d.rebo_mpart[m];
+        x[9] = d.rebo_mpart[9 * nm + m];
+      }
+      v[1] += lt;
+      v[5] += lt;
+      v[9] += lt;
+      v[11] += le;
+      if (MM) {
+        for (int k = 0; k < 9; k++) v[1 + k] += x[k];
+        for (int k = 0; k < 4; k++) v[12 + k] += x[9 + k];
+      } else {
+        v[1] += x[0];
+        v[5] += x[0];
+        v[9] += x[0];
+        v[10] += x[9];
+      }
+    }
+  } else {
   for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
     double x[MM ? kAtPart + 13 : 2 * kAtPart];
 #pragma unroll
@@ -1912,6 +1942,7 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
       for (int k = 0; k < 9; k++) v[1 + k] += x[kAtPart + k];
       v[10] += x[kAtPart + 9];
     }
+  }
   }
   __shared__ double tot[kAtSums];
   block_sum<kAtSums, kFinThreads / 64>(v, tot);
@@ -2206,9 +2237,9 @@ hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_sl
   if (ev && (e = hipEventRecordWithFlags(ev->f1, s, evf)) != hipSuccess) return e;
   k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
   if (p.ff == kFfMM)
-    k_at_finish<true><<<p.R, kFinThreads, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+    k_at_finish<true><<<p.R, kFinThreads, 0, s>>>(d, ps, ctrl_slot, thermo_slot);
   else
-    k_at_finish<false><<<p.R, kFinThreads, 0, s>>>(d, p, ctrl_slot, thermo_slot);
+    k_at_finish<false><<<p.R, kFinThreads, 0, s>>>(d, ps, ctrl_slot, thermo_slot);
   if (rescale) k_at_rescale<<<d.nblk, 256, 0, s>>>(d, p);
   return hipGetLastError();
 }
