@@ -5,8 +5,11 @@ reference's 400-entry half-list semantics), C60 FCC 8x8x8 = 2048 molecules /
 122,880 atoms at the reference lattice constant, seed-42 Maxwell velocities at
 300 K, NPT at 0 GPa, dt = 0.5 fs, lists rebuilt every 20 steps, exact-gradient
 REBO (the reference's own REBO forces are not a gradient and heat the crystal
-without bound).  One "step" = one full NPT step; value = atoms x steps / s over
-the timed steps, inputs resident in HBM.
+without bound), the autotuner's parameters for this system (per-atom list skin,
+step replay; cached per device and system, --no-tune for the defaults).  One
+"step" = one full NPT step, list rebuilds included; value = atoms x steps / s
+over the timed steps, inputs resident in HBM.  vs_baseline stays null: no
+published number exists for this metric on this configuration.
 
     python bench.py [--workload airebo|rigid|core|batch|surrogate|mm|copy]
                     [--steps K] [--warmup W] [--gpus 1] [--no-cpu]

@@ -16,8 +16,10 @@
 // algebraically instead of through atan2, cos and sin; each contribution
 // equals the reference's to a few ulps (FP64 throughout).
 //
-// Intermolecular LJ (MM:711-765) is the atomic engine's k_at_lj with the MM
-// parameters.
+// Intermolecular LJ (MM:711-765) is the atomic engine's LJ with the MM
+// parameters: the per-atom lists (k_at_lj_alist, k_at_lj_al) in the engines,
+// k_at_lj (every molecule-tile pair) in the drop-ins and in cells narrower
+// than twice the list reach.
 #include <climits>
 #include <cstddef>
 
