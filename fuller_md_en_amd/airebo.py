@@ -83,7 +83,8 @@ class AtomicEngine:
     """HBM-resident AIREBO NPT engine (fmd_atomic_create / fmd_engine_*)."""
 
     def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
-                 nlup=20, lj=None, rebo_rlist=None, clamp_vg=None, al_skin=None):
+                 nlup=20, lj=None, rebo_rlist=None, clamp_vg=None, al_skin=None,
+                 cell_min_molecules=None):
         self._mol = _c(mol_id, np.int32)
         self._mass = _c(mass)
         cfg = N.AtomicConfig()
@@ -100,6 +101,8 @@ class AtomicEngine:
             cfg.clamp_vg = clamp_vg
         if al_skin is not None:
             cfg.al_skin = al_skin
+        if cell_min_molecules is not None:
+            cfg.cell_min_molecules = cell_min_molecules
         self.cfg = cfg
         self.n = len(self._mol)
         h = C.c_void_p()

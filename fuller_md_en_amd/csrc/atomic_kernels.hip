@@ -36,6 +36,7 @@ namespace fmd {
 namespace {
 
 constexpr int kXcd = 8;
+constexpr int kMolHitCap = 256;  // molecule-list hits gathered from the cells
 
 __device__ __forceinline__ double clampd(double x, double lo, double hi) {
   return x < lo ? lo : (x > hi ? hi : x);
@@ -255,6 +256,7 @@ __global__ __launch_bounds__(256) void k_at_centers(AtomicDev d, AtomicParams p,
 // with per-molecule images counts the loss of kImgMol as an image fault.
 __global__ __launch_bounds__(256) void k_at_image_check(AtomicDev d, AtomicParams p, HHRef hh) {
   __shared__ float wm[4];
+  __shared__ int cdim[1];
   const int r = blockIdx.x;
   float rm = 0.f;
   for (int m = d.rep_mol[r] + (int)threadIdx.x; m < d.rep_mol[r + 1]; m += 256)
@@ -273,33 +275,124 @@ __global__ __launch_bounds__(256) void k_at_image_check(AtomicDev d, AtomicParam
       d.al_dmax[kAlDriftSlots * r + k] = 0;
     d.al_redo[r] = 0;
     if (!p.small_box && !(bits & kImgMol)) atomicAdd(&d.flags[3], 1);
+    // the molecule-list grid: cells at least one pair reach wide along each
+    // cell vector (perpendicular width / cells), three or more per axis so
+    // that the 27 neighbouring cells are distinct, at most the replica's
+    // allotment; otherwise every molecule is scanned
+    const double *h = hh_of(hh, r);
+    const double reach = p.mol_rlist + 2.0 * rmax + 1e-3;
+    int n[3];
+    for (int a = 0; a < 3; a++) {
+      const int b = (a + 1) % 3, c = (a + 2) % 3;
+      const double x = h[3 * 1 + b] * h[3 * 2 + c] - h[3 * 2 + b] * h[3 * 1 + c],
+                   y = h[3 * 2 + b] * h[c] - h[b] * h[3 * 2 + c],
+                   z = h[b] * h[3 * 1 + c] - h[3 * 1 + b] * h[c];
+      const double w = fabs(det9(h)) / sqrt(x * x + y * y + z * z);
+      n[a] = (int)(w / reach);
+    }
+    const int cap = d.cell_base[r + 1] - d.cell_base[r];
+    while ((long long)n[0] * n[1] * n[2] > cap) {
+      const int a = n[0] >= n[1] ? (n[0] >= n[2] ? 0 : 2) : (n[1] >= n[2] ? 1 : 2);
+      n[a]--;
+    }
+    const int ok = p.mol_cells && !p.small_box && n[0] >= 3 && n[1] >= 3 && n[2] >= 3 &&
+                   d.rep_mol[r + 1] - d.rep_mol[r] >= p.cell_min;
+    d.cell_dim[r] = make_int4(n[0], n[1], n[2], ok);
+    cdim[0] = ok ? n[0] * n[1] * n[2] : 0;
   }
+  __syncthreads();
+  for (int c = threadIdx.x; c < cdim[0]; c += 256) d.cell_n[d.cell_base[r] + c] = 0;
+}
+
+// The cell of a molecule centre: fractional coordinates in [0, 1).
+__device__ __forceinline__ int3 mol_cell(const double *c, const double *hi, int4 cd) {
+  double s[3];
+  for (int a = 0; a < 3; a++) {
+    s[a] = hi[3 * a] * c[0] + hi[3 * a + 1] * c[1] + hi[3 * a + 2] * c[2];
+    s[a] -= floor(s[a]);
+  }
+  return make_int3(min((int)(s[0] * cd.x), cd.x - 1), min((int)(s[1] * cd.y), cd.y - 1),
+                   min((int)(s[2] * cd.z), cd.z - 1));
+}
+
+// Molecules into their replica's cells (grids decided by k_at_image_check);
+// a full cell sends the replica back to the scan of every molecule.
+__global__ __launch_bounds__(256) void k_at_cell_bin(AtomicDev d, AtomicParams p, HHRef hh) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= p.Nm) return;
+  const int r = d.mol_rep[m];
+  const int4 cd = d.cell_dim[r];
+  if (!cd.w) return;
+  const double *hi = hh_of(hh, r) + 9;
+  const int3 k = mol_cell(d.center + 3 * m, hi, cd);
+  const int c = d.cell_base[r] + (k.x * cd.y + k.y) * cd.z + k.z;
+  const int slot = atomicAdd(&d.cell_n[c], 1);
+  if (slot < kCellCap)
+    d.cell_m[(size_t)c * kCellCap + slot] = m;
+  else
+    atomicExch(&d.cell_dim[r].w, 0);
 }
 
 // --------------------------------------------------------- molecule list
-// Row m of the molecule list, by one wavefront (lane = 0..63).
+// Row m of the molecule list, by one wavefront (lane = 0..63): the molecules
+// q != m of its replica whose centre lies within mol_rlist + R_m + R_q, in
+// ascending order.  From the 27 cells around m's when its replica was binned
+// (CELLS; the hits then take their places by rank), else by a scan of every
+// molecule of the replica.
+template <bool CELLS>
 __device__ __forceinline__ void mol_row(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                                         int m, int lane) {
   const int r = d.mol_rep[m];
   const double *h = hh_of(hh, r), *hi = h + 9;
   const double cx = d.center[3 * m], cy = d.center[3 * m + 1], cz = d.center[3 * m + 2];
   const double rm = d.radius[m];
+  auto within = [&](int q) {
+    if (p.small_box) return true;
+    double dx = d.center[3 * q] - cx, dy = d.center[3 * q + 1] - cy,
+           dz = d.center[3 * q + 2] - cz;
+    mimg(dx, dy, dz, hi, h);
+    const double rl = p.mol_rlist + rm + d.radius[q];
+    return dx * dx + dy * dy + dz * dz < rl * rl;
+  };
+  if (CELLS) {
+    const int4 cd = d.cell_dim[r];
+    if (cd.w) {
+      __shared__ int hit[kMolHitCap];
+      const int3 k = mol_cell(d.center + 3 * m, hi, cd);
+      const int cb = d.cell_base[r];
+      int nh = 0;
+      for (int o = 0; o < 27; o++) {
+        const int ix = (k.x + o / 9 - 1 + cd.x) % cd.x, iy = (k.y + (o / 3) % 3 - 1 + cd.y) % cd.y,
+                  iz = (k.z + o % 3 - 1 + cd.z) % cd.z;
+        const int c = cb + (ix * cd.y + iy) * cd.z + iz;
+        const int nc = min(d.cell_n[c], kCellCap);
+        const int q = lane < nc ? d.cell_m[(size_t)c * kCellCap + lane] : -1;
+        const bool in = q >= 0 && q != m && within(q);
+        const unsigned long long bits = __ballot(in);
+        const int rank = __popcll(bits & ((1ull << lane) - 1ull));
+        if (in && nh + rank < kMolHitCap) hit[nh + rank] = q;
+        nh += __popcll(bits);
+      }
+      __syncthreads();
+      const int nk = min(nh, kMolHitCap);
+      for (int t = lane; t < nk; t += 64) {
+        const int q = hit[t];
+        int rank = 0;
+        for (int u = 0; u < nk; u++) rank += hit[u] < q;
+        if (rank < p.mol_cap) d.mol_nl[(size_t)m * p.mol_cap + rank] = q;
+      }
+      if (lane == 0) {
+        d.mol_nl_count[m] = nh < p.mol_cap ? nh : p.mol_cap;
+        if (nh > p.mol_cap) atomicAdd(&d.flags[1], 1);
+      }
+      return;
+    }
+  }
   const int q0 = d.rep_mol[r], q1 = d.rep_mol[r + 1];  // molecules of this replica
   int count = 0;
   for (int n0 = q0; n0 < q1; n0 += 64) {
     const int q = n0 + lane;
-    bool in = false;
-    if (q < q1 && q != m) {
-      if (p.small_box) {
-        in = true;
-      } else {
-        double dx = d.center[3 * q] - cx, dy = d.center[3 * q + 1] - cy,
-               dz = d.center[3 * q + 2] - cz;
-        mimg(dx, dy, dz, hi, h);
-        const double rl = p.mol_rlist + rm + d.radius[q];
-        in = dx * dx + dy * dy + dz * dz < rl * rl;
-      }
-    }
+    const bool in = q < q1 && q != m && within(q);
     unsigned long long bits = __ballot(in);
     int rank = __popcll(bits & ((1ull << lane) - 1ull));
     if (in && count + rank < p.mol_cap) d.mol_nl[(size_t)m * p.mol_cap + count + rank] = q;
@@ -312,7 +405,7 @@ __device__ __forceinline__ void mol_row(const AtomicDev &d, const AtomicParams &
 }
 
 __global__ __launch_bounds__(64) void k_at_mol_nlist(AtomicDev d, AtomicParams p, HHRef hh) {
-  mol_row(d, p, hh, blockIdx.x, threadIdx.x);
+  mol_row<true>(d, p, hh, blockIdx.x, threadIdx.x);
 }
 
 // -------------------------------------------------------------- REBO list
@@ -1640,7 +1733,7 @@ __global__ __launch_bounds__(256) void k_at_lj_alist(AtomicDev d, AtomicParams p
 template <int NS>
 __device__ __forceinline__ void al_relist(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                                           int mi) {
-  if (threadIdx.x < 64) mol_row(d, p, hh, mi, threadIdx.x);
+  if (threadIdx.x < 64) mol_row<false>(d, p, hh, mi, threadIdx.x);
   __threadfence();
   __syncthreads();
   al_build<NS, false>(d, p, hh, mi);
@@ -2081,6 +2174,7 @@ hipError_t at_launch_centers(const AtomicDev &d, const AtomicParams &p, HHRef hh
 hipError_t at_launch_lists(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                            int rebo_list, hipStream_t s) {
   k_at_image_check<<<p.R, 256, 0, s>>>(d, p, hh);
+  if (p.mol_cells) k_at_cell_bin<<<(p.Nm + 255) / 256, 256, 0, s>>>(d, p, hh);
   k_at_mol_nlist<<<p.Nm, 64, 0, s>>>(d, p, hh);
   dim3 g((p.Nm + 3) / 4);
   const bool one = ns_of(p) == 1;

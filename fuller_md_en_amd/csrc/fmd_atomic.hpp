@@ -63,6 +63,7 @@ constexpr int kAtPart = 10;  // W9 + E
 constexpr int kAtFlags = 5;  // AtomicDev::flags
 constexpr int kAlDriftSlots = 8;  // AtomicDev::al_dmax entries per replica
 constexpr float kAlDriftFloor = 0.125f;  // drifts below it count as it (A)
+constexpr int kCellCap = 64;  // molecules per cell of the molecule-list grid
 // image modes of a replica at a list rebuild (AtomicDev::img)
 enum : int {
   kImgMol = 1,   // atom pairs within rc + 1 A are the centres' image + offsets
@@ -103,6 +104,9 @@ struct AtomicParams {
   int lj_alist;    // 1: LJ from per-atom lists (engines; cells with one image
                    // per molecule pair); 0: every tile pair each step (drop-ins)
   int al_cap;      // per-atom list capacity per wavefront segment (entries)
+  int cell_total;  // molecule-list cells over all replicas
+  int mol_cells;   // 1: some replica builds its molecule list from cells
+  int cell_min;    // the molecules a replica needs for that
   float al_r2;     // per-atom list radius^2: (rc + 1 A)^2
   int scan_fast;   // 1: list walk prefiltered in FP32 through the molecule
                    // tiles (cell wide enough for the list radius + 2 R_max)
@@ -142,6 +146,12 @@ struct AtomicDev {
   int *al_dmax;
   int *al_redo;  // per replica: lists to rebuild within this step (k_at_centers)
   int *img;  // per replica, decided at every list rebuild: kImgMol | kImgScan
+  // molecule-list cells, per replica at every list rebuild: the grid (cells
+  // per cell vector, w = 1 when the molecules were binned, 0 for a scan of
+  // every molecule), the first of the replica's cells (cell_base[R] = all),
+  // each cell's molecule count and ids (kCellCap per cell)
+  int4 *cell_dim;
+  int *cell_base, *cell_n, *cell_m;
   double *hh;  // 18 doubles for the stateless entry points
   AtomicState *state;
   AtomicMid *mid;

@@ -76,6 +76,7 @@ void free_atomic_dev(AtomicDev &d) {
                   d.bonds.bq,  d.bonds.bflag, d.bonds.brev, d.rebo_part, d.rebo_mpart,
                   d.lj_part,   d.ke_part,   d.rebo_fe,   d.al_e,       d.al_n,
                   d.al_pos,    d.al_h,      d.al_dmax,  d.al_redo,   d.img,
+                  d.cell_dim,  d.cell_base, d.cell_n,   d.cell_m,
                   d.hh,        d.state,     d.mid,      d.ctrl,      d.thermo,     d.flags,
                   d.mol_rep,   d.atom_rep,  d.rep_mol,  d.rep_atom,  d.blk_a0,     d.blk_a1,
                   d.blk_rep,   d.rep_blk};
@@ -93,6 +94,7 @@ hipError_t dalloc(T **p, size_t n) {
 struct Layout {
   std::vector<int> mol_start, mol_rep, atom_rep, rep_mol, rep_atom;
   std::vector<int> blk_a0, blk_a1, blk_rep, rep_blk;
+  std::vector<int> cell_base;  // per replica: max(27, its molecules) cells
 };
 
 // Concatenate the replicas' molecule layouts.
@@ -102,6 +104,7 @@ int build_layout(const fmd_atomic_config *c, int R, Layout &L, int &apad) {
   L.rep_mol.push_back(0);
   L.rep_atom.push_back(0);
   L.rep_blk.push_back(0);
+  L.cell_base.push_back(0);
   apad = 4;
   int a0 = 0;
   for (int r = 0; r < R; r++) {
@@ -124,6 +127,8 @@ int build_layout(const fmd_atomic_config *c, int R, Layout &L, int &apad) {
     L.rep_mol.push_back((int)L.mol_rep.size());
     L.rep_atom.push_back(a0);
     L.rep_blk.push_back((int)L.blk_rep.size());
+    const int nm = L.rep_mol[r + 1] - L.rep_mol[r];
+    L.cell_base.push_back(L.cell_base.back() + std::max(27, nm));
   }
   return FMD_OK;
 }
@@ -141,6 +146,7 @@ int upload_layout(AtomicDev &d, const Layout &L) {
   if (e == hipSuccess) e = up(d.blk_a1, L.blk_a1);
   if (e == hipSuccess) e = up(d.blk_rep, L.blk_rep);
   if (e == hipSuccess) e = up(d.rep_blk, L.rep_blk);
+  if (e == hipSuccess) e = up(d.cell_base, L.cell_base);
   if (e != hipSuccess) return hip_fail(e, "upload replica layout");
   return FMD_OK;
 }
@@ -185,6 +191,10 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
     AL(d.al_redo, R);
   }
   AL(d.img, R);
+  AL(d.cell_dim, R);
+  AL(d.cell_base, R + 1);
+  AL(d.cell_n, (size_t)p.cell_total);
+  AL(d.cell_m, (size_t)p.cell_total * kCellCap);
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
   AL(d.rebo_mpart, Nm * kAtPart);
   AL(d.lj_part, Nm * kAtPart);
@@ -506,6 +516,7 @@ void fmd_atomic_config_default(fmd_atomic_config *c) {
   c->clamp_xi = 0.05;
   c->clamp_vg = 0.005;
   c->al_skin = 1.0;
+  c->cell_min_molecules = 4096;
 }
 
 int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
@@ -564,6 +575,9 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
   A.fast.assign(R, 0);
   const int ring = std::max(64, 2048 / R);
   rc = engine_init_common(e, ring, R);
+  A.p.cell_total = L.cell_base.back();
+  A.p.cell_min = std::max(1, c[0].cell_min_molecules);
+  for (int r = 0; r < R; r++) A.p.mol_cells |= L.rep_mol[r + 1] - L.rep_mol[r] >= A.p.cell_min;
   if (rc == FMD_OK) rc = atomic_alloc(A.d, A.p, e->ring, (int)L.blk_rep.size());
   if (rc == FMD_OK) rc = upload_layout(A.d, L);
   if (rc == FMD_OK) {

@@ -319,6 +319,9 @@ typedef struct {
                             whose LJ list is the half list above).  Results do
                             not depend on it beyond summation order: stale lists
                             are rebuilt within the step. */
+  int cell_min_molecules; /* engines: molecule lists from cells of a grid for
+                            replicas of at least this many molecules (4096),
+                            below it a scan of every molecule; same lists */
 } fmd_atomic_config;
 
 /* Defaults of the reference AIREBO program. */

@@ -364,3 +364,30 @@ def test_engine_deterministic_and_list_rebuild():
         assert eng.counters()[1] >= 2
         eng.close()
     assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+def test_molecule_lists_from_cells_equal_the_scan():
+    """Replicas of at least cell_min_molecules molecules bin the molecule
+    centres into cells (at least one pair reach wide along each cell vector)
+    and take each molecule's list from its 27 cells, sorted; below that every
+    molecule is scanned.  Both give the same lists, so on a 5x5x5 crystal
+    (three cells per axis) the forces after the first build and 25 steps
+    across the step-20 rebuild are bitwise equal."""
+    from fuller_md_en_amd import system
+    s = system.airebo_system(5, "C60", T=300.0, seed=42)
+    out = []
+    for cmin in (1, 1 << 30):
+        print(f"case cell_min_molecules={cmin}", flush=True)
+        eng = airebo.AtomicEngine(s["mol_id"], s["mass"], mode=airebo.REBO_PHYSICAL,
+                                  cell_min_molecules=cmin)
+        eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"],
+                      npt=airebo.make_npt(300.0, 0.0, len(s["pos"])))
+        eng.prepare()
+        E, F, W = eng.forces()
+        th = eng.run(0, 25)
+        out.append((F, np.array([t.Ep for t in th]), eng.get_state()["pos"].copy()))
+        assert eng.counters()[0] == 0
+        eng.close()
+    (Fa, ea, pa), (Fb, eb, pb) = out
+    assert np.array_equal(Fa, Fb) and np.array_equal(ea, eb) and np.array_equal(pa, pb)
+
