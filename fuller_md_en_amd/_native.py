@@ -49,7 +49,8 @@ class RigidConfig(C.Structure):
                 ("I0", C.c_double), ("Mmol", C.c_double), ("lj", LJParams),
                 ("rmcut", C.c_double), ("skin", C.c_double), ("max_neigh", C.c_int),
                 ("nlup", C.c_int), ("dt", C.c_double), ("clamp_xi", C.c_double),
-                ("clamp_vg", C.c_double), ("half_pairs", C.c_int)]
+                ("clamp_vg", C.c_double), ("half_pairs", C.c_int),
+                ("cell_min_molecules", C.c_int)]
 
 
 class AtomicConfig(C.Structure):

@@ -36,7 +36,6 @@ namespace fmd {
 namespace {
 
 constexpr int kXcd = 8;
-constexpr int kMolHitCap = 256;  // molecule-list hits gathered from the cells
 
 __device__ __forceinline__ double clampd(double x, double lo, double hi) {
   return x < lo ? lo : (x > hi ? hi : x);
@@ -279,40 +278,15 @@ __global__ __launch_bounds__(256) void k_at_image_check(AtomicDev d, AtomicParam
     // cell vector (perpendicular width / cells), three or more per axis so
     // that the 27 neighbouring cells are distinct, at most the replica's
     // allotment; otherwise every molecule is scanned
-    const double *h = hh_of(hh, r);
-    const double reach = p.mol_rlist + 2.0 * rmax + 1e-3;
-    int n[3];
-    for (int a = 0; a < 3; a++) {
-      const int b = (a + 1) % 3, c = (a + 2) % 3;
-      const double x = h[3 * 1 + b] * h[3 * 2 + c] - h[3 * 2 + b] * h[3 * 1 + c],
-                   y = h[3 * 2 + b] * h[c] - h[b] * h[3 * 2 + c],
-                   z = h[b] * h[3 * 1 + c] - h[3 * 1 + b] * h[c];
-      const double w = fabs(det9(h)) / sqrt(x * x + y * y + z * z);
-      n[a] = (int)(w / reach);
-    }
-    const int cap = d.cell_base[r + 1] - d.cell_base[r];
-    while ((long long)n[0] * n[1] * n[2] > cap) {
-      const int a = n[0] >= n[1] ? (n[0] >= n[2] ? 0 : 2) : (n[1] >= n[2] ? 1 : 2);
-      n[a]--;
-    }
-    const int ok = p.mol_cells && !p.small_box && n[0] >= 3 && n[1] >= 3 && n[2] >= 3 &&
+    const int4 g = cell_grid(hh_of(hh, r), p.mol_rlist + 2.0 * rmax + 1e-3,
+                             d.cell_base[r + 1] - d.cell_base[r]);
+    const int ok = p.mol_cells && !p.small_box && g.w &&
                    d.rep_mol[r + 1] - d.rep_mol[r] >= p.cell_min;
-    d.cell_dim[r] = make_int4(n[0], n[1], n[2], ok);
-    cdim[0] = ok ? n[0] * n[1] * n[2] : 0;
+    d.cell_dim[r] = make_int4(g.x, g.y, g.z, ok);
+    cdim[0] = ok ? g.x * g.y * g.z : 0;
   }
   __syncthreads();
   for (int c = threadIdx.x; c < cdim[0]; c += 256) d.cell_n[d.cell_base[r] + c] = 0;
-}
-
-// The cell of a molecule centre: fractional coordinates in [0, 1).
-__device__ __forceinline__ int3 mol_cell(const double *c, const double *hi, int4 cd) {
-  double s[3];
-  for (int a = 0; a < 3; a++) {
-    s[a] = hi[3 * a] * c[0] + hi[3 * a + 1] * c[1] + hi[3 * a + 2] * c[2];
-    s[a] -= floor(s[a]);
-  }
-  return make_int3(min((int)(s[0] * cd.x), cd.x - 1), min((int)(s[1] * cd.y), cd.y - 1),
-                   min((int)(s[2] * cd.z), cd.z - 1));
 }
 
 // Molecules into their replica's cells (grids decided by k_at_image_check);
@@ -323,22 +297,17 @@ __global__ __launch_bounds__(256) void k_at_cell_bin(AtomicDev d, AtomicParams p
   const int r = d.mol_rep[m];
   const int4 cd = d.cell_dim[r];
   if (!cd.w) return;
-  const double *hi = hh_of(hh, r) + 9;
-  const int3 k = mol_cell(d.center + 3 * m, hi, cd);
-  const int c = d.cell_base[r] + (k.x * cd.y + k.y) * cd.z + k.z;
-  const int slot = atomicAdd(&d.cell_n[c], 1);
-  if (slot < kCellCap)
-    d.cell_m[(size_t)c * kCellCap + slot] = m;
-  else
-    atomicExch(&d.cell_dim[r].w, 0);
+  cell_put(m, d.center + 3 * m, hh_of(hh, r) + 9, cd, d.cell_base[r], d.cell_n, d.cell_m,
+           &d.cell_dim[r].w);
 }
 
 // --------------------------------------------------------- molecule list
 // Row m of the molecule list, by one wavefront (lane = 0..63): the molecules
 // q != m of its replica whose centre lies within mol_rlist + R_m + R_q, in
 // ascending order.  From the 27 cells around m's when its replica was binned
-// (CELLS; the hits then take their places by rank), else by a scan of every
-// molecule of the replica.
+// (CELLS; the hits then take their places by rank, and a row of more than
+// kCellHitCap hits is scanned), else by a scan of every molecule of the
+// replica.
 template <bool CELLS>
 __device__ __forceinline__ void mol_row(const AtomicDev &d, const AtomicParams &p, HHRef hh,
                                         int m, int lane) {
@@ -357,35 +326,17 @@ __device__ __forceinline__ void mol_row(const AtomicDev &d, const AtomicParams &
   if (CELLS) {
     const int4 cd = d.cell_dim[r];
     if (cd.w) {
-      __shared__ int hit[kMolHitCap];
-      const int3 k = mol_cell(d.center + 3 * m, hi, cd);
-      const int cb = d.cell_base[r];
-      int nh = 0;
-      for (int o = 0; o < 27; o++) {
-        const int ix = (k.x + o / 9 - 1 + cd.x) % cd.x, iy = (k.y + (o / 3) % 3 - 1 + cd.y) % cd.y,
-                  iz = (k.z + o % 3 - 1 + cd.z) % cd.z;
-        const int c = cb + (ix * cd.y + iy) * cd.z + iz;
-        const int nc = min(d.cell_n[c], kCellCap);
-        const int q = lane < nc ? d.cell_m[(size_t)c * kCellCap + lane] : -1;
-        const bool in = q >= 0 && q != m && within(q);
-        const unsigned long long bits = __ballot(in);
-        const int rank = __popcll(bits & ((1ull << lane) - 1ull));
-        if (in && nh + rank < kMolHitCap) hit[nh + rank] = q;
-        nh += __popcll(bits);
+      __shared__ int hit[kCellHitCap], seg[64];
+      const int nh = cell_hits(cd, cell_of(d.center + 3 * m, hi, cd), d.cell_base[r], d.cell_n,
+                               d.cell_m, m, lane, within, hit, seg);
+      if (nh <= kCellHitCap) {  // else the scan below
+        cell_place(hit, nh, lane, d.mol_nl + (size_t)m * p.mol_cap, p.mol_cap);
+        if (lane == 0) {
+          d.mol_nl_count[m] = nh < p.mol_cap ? nh : p.mol_cap;
+          if (nh > p.mol_cap) atomicAdd(&d.flags[1], 1);
+        }
+        return;
       }
-      __syncthreads();
-      const int nk = min(nh, kMolHitCap);
-      for (int t = lane; t < nk; t += 64) {
-        const int q = hit[t];
-        int rank = 0;
-        for (int u = 0; u < nk; u++) rank += hit[u] < q;
-        if (rank < p.mol_cap) d.mol_nl[(size_t)m * p.mol_cap + rank] = q;
-      }
-      if (lane == 0) {
-        d.mol_nl_count[m] = nh < p.mol_cap ? nh : p.mol_cap;
-        if (nh > p.mol_cap) atomicAdd(&d.flags[1], 1);
-      }
-      return;
     }
   }
   const int q0 = d.rep_mol[r], q1 = d.rep_mol[r + 1];  // molecules of this replica

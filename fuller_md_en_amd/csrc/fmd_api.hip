@@ -125,6 +125,7 @@ void fmd_rigid_config_default(fmd_rigid_config *c, double rmol) {
   c->clamp_xi = 0.1;
   c->clamp_vg = 0.01;
   c->half_pairs = 1;
+  c->cell_min_molecules = 3000;  // the scan is faster below about 2,500
 }
 
 }  // extern "C"
@@ -137,7 +138,8 @@ static void free_rigid(RigidDev &d) {
                   d.sites,    d.sites_d,  d.body,     d.qbody,    d.qcen,     d.hh,
                   d.nl_count, d.nl_list,  d.hl_count, d.hl_list,  d.in_count, d.in_list,
                   d.pairout,  d.flags,    d.molpart,  d.blockpart, d.state,   d.mid,
-                  d.ctrl,     d.thermo,   d.hl_item,  d.ord_ctl,  d.nl_pos,   d.nl_h};
+                  d.ctrl,     d.thermo,   d.hl_item,  d.ord_ctl,  d.nl_pos,   d.nl_h,
+                  d.cell_dim, d.cell_n,   d.cell_m};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   memset(&d, 0, sizeof d);
@@ -181,6 +183,12 @@ int rigid_alloc(RigidDev &d, const RigidParams &p, int ring) {
   if (p.pair_kernel) {
     AL(d.hl_item, N * p.max_neigh);
     AL(d.ord_ctl, 1);
+  }
+  if (p.cell_min > 0 && p.N >= p.cell_min) {
+    const size_t nc = std::max<size_t>(27, N);
+    AL(d.cell_dim, 1);
+    AL(d.cell_n, nc);
+    AL(d.cell_m, nc * kCellCap);
   }
   AL(d.flags, 4);
   AL(d.molpart, N * kMolPart);
@@ -326,6 +334,7 @@ int fmd_rigid_create(const fmd_rigid_config *c, fmd_engine **out) {
                         c->clamp_xi, c->clamp_vg, body4.data(),
                         c->half_pairs ? &qbody : nullptr);
   if (c->half_pairs == 2) e->rp.pair_kernel = 0;  // the per-molecule pair-once kernel
+  e->rp.cell_min = std::max(0, c->cell_min_molecules);
   if (rc == FMD_OK) rc = engine_init_common(e, 4096, 1);
   if (rc == FMD_OK) rc = rigid_alloc(e->rd, e->rp, e->ring);
   if (rc != FMD_OK) {
@@ -786,9 +795,11 @@ int rigid_workspace(int N, int natom, int max_neigh, const double *body, double 
   }
   fmd_engine *e = g_rws.e;
   std::vector<double> body4(4 * 128);
+  const int cell_min = e->rp.cell_min;  // as allocated
   int rc = rigid_params(e->rp, N, natom, body, I0, Mmol, *lj, rmcut2, 0.0, max_neigh, dt, 0.1,
                         0.01, body4.data(), nullptr);
   if (rc) return rc;
+  e->rp.cell_min = cell_min;
   HIPCHK(hipMemcpyAsync(e->rd.body, body4.data(), 4 * e->rp.natom_pad * sizeof(double),
                         hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));

@@ -51,6 +51,7 @@
 #include <cstddef>
 #include <vector>
 
+#include "fmd_cells.hpp"
 #include "fmd_common.hpp"
 #include "fmd_mm.hpp"
 #include "fmd_rebo_passes.hpp"
@@ -63,7 +64,6 @@ constexpr int kAtPart = 10;  // W9 + E
 constexpr int kAtFlags = 5;  // AtomicDev::flags
 constexpr int kAlDriftSlots = 8;  // AtomicDev::al_dmax entries per replica
 constexpr float kAlDriftFloor = 0.125f;  // drifts below it count as it (A)
-constexpr int kCellCap = 64;  // molecules per cell of the molecule-list grid
 // image modes of a replica at a list rebuild (AtomicDev::img)
 enum : int {
   kImgMol = 1,   // atom pairs within rc + 1 A are the centres' image + offsets

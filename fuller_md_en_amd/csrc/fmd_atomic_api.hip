@@ -516,7 +516,7 @@ void fmd_atomic_config_default(fmd_atomic_config *c) {
   c->clamp_xi = 0.05;
   c->clamp_vg = 0.005;
   c->al_skin = 1.0;
-  c->cell_min_molecules = 4096;
+  c->cell_min_molecules = 2048;
 }
 
 int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {

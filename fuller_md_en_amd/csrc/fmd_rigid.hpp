@@ -10,6 +10,8 @@
 //                                                 R(q_i) body_a, FP32; padding
 //                                                 sites sit at +1e6 A
 //   nl_count / nl_list      int[N] / int[N*max_neigh]  symmetric full list
+//   cell_dim / cell_n / cell_m  int4 / int[C] / int[C*kCellCap]  the list's
+//                           cell grid (C = max(27, N) cells) when N >= cell_min
 //   hl_* / in_*             int, same shapes          owned pairs / incoming slots
 //   pairout                 double[N*max_neigh*16]    per owned pair, both sides
 //   hl_item                 int2[max_neigh*N]         pair kernel work items, rank-major
@@ -22,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fmd_cells.hpp"
 #include "fmd_common.hpp"
 
 namespace fmd {
@@ -87,6 +90,7 @@ struct RigidParams {
   // the r^2 clamp over sigma^2, and 48 eps / sigma
   float isig, rc2_lo_s, rc2_hi_s, rmin2_s;
   double c48s;
+  int cell_min;       // lists from a cell grid for N >= cell_min (0: never)
 };
 
 // Device buffers of one rigid system.
@@ -101,6 +105,10 @@ struct RigidDev {
   double *hh;    // 18 doubles: h, h^-1 for the stateless entry points
   int *nl_count, *nl_list, *flags;  // flags[0]: list overflow count, [1]: list
                                     // faults (a pair could have closed the skin)
+  // the list's cell grid (cells per axis, w = 1 while usable), each cell's
+  // molecule count and ids; allocated when N >= cell_min
+  int4 *cell_dim;
+  int *cell_n, *cell_m;
   // molecule centres and cell (h, then 1 once set) at the last list build
   double *nl_pos, *nl_h;
   // pairs owned by each molecule (half list) and, per molecule, the slots of

@@ -2,7 +2,8 @@
 //
 //   k_rigid_nlist       symmetric molecule list, one wavefront per molecule,
 //                       ballot/popcount compaction keeps ascending order
-//                       (nlist_build_sym C1:280-301).
+//                       (nlist_build_sym C1:280-301); from a cell grid
+//                       (k_rigid_cell_grid, k_rigid_cell_bin) for large N.
 //   k_rigid_sites       lab-frame FP32 site offsets R(q) body (C1:354-364).
 //   k_rigid_kick_drift  NPT first half + half-kick + fractional drift + cell
 //                       update + quaternion update + sites, wavefront per
@@ -140,24 +141,75 @@ __device__ __forceinline__ void write_quads(int i, const double *q, const double
 }
 
 // ------------------------------------------------------------------ list
+// The list's cell grid for the cell hh: cells at least the list radius wide
+// along each cell vector, at most max(27, N); the counts cleared.
+__global__ __launch_bounds__(256) void k_rigid_cell_grid(const double *__restrict__ hh,
+                                                         int4 *__restrict__ cell_dim,
+                                                         int *__restrict__ cell_n, RigidParams p) {
+  __shared__ int nc;
+  if (threadIdx.x == 0) {
+    const int4 g = cell_grid(hh, sqrt(p.rlist2) + 1e-3, max(27, p.N));
+    *cell_dim = g;
+    nc = g.w ? g.x * g.y * g.z : 0;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nc; c += 256) cell_n[c] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_rigid_cell_bin(const double *__restrict__ pos,
+                                                        const double *__restrict__ hh,
+                                                        int4 *cell_dim, int *__restrict__ cell_n,
+                                                        int *__restrict__ cell_m, int N) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int4 cd = *cell_dim;
+  if (i >= N || !cd.w) return;
+  cell_put(i, pos + 3 * i, hh + 9, cd, 0, cell_n, cell_m, &cell_dim->w);
+}
+
+// Row i of the symmetric list by one wavefront: the molecules j != i whose
+// centre lies within the list radius (the reference's FP64 minimum image and
+// r^2), ascending, the first max_neigh kept and all counted.  CELLS: from the
+// 27 cells around i's while the grid is usable (the hits take their places by
+// rank; a row of more than kCellHitCap hits is scanned), else by a scan of
+// every molecule, 64 per pass with ballot compaction.
+template <bool CELLS>
 __global__ __launch_bounds__(64) void k_rigid_nlist(const double *__restrict__ pos,
                                                     const double *__restrict__ hh,
                                                     int *__restrict__ nl_count,
                                                     int *__restrict__ nl_list,
-                                                    int *__restrict__ flags, RigidParams p) {
+                                                    int *__restrict__ flags,
+                                                    const int4 *__restrict__ cell_dim,
+                                                    const int *__restrict__ cell_n,
+                                                    const int *__restrict__ cell_m, RigidParams p) {
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   const double *h = hh, *hi = hh + 9;
   const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+  auto within = [&](int j) {
+    double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
+    mimg_exact(dx, dy, dz, hi, h);
+    return norm2_exact(dx, dy, dz) < p.rlist2;
+  };
+  if (CELLS) {
+    const int4 cd = *cell_dim;
+    if (cd.w) {
+      __shared__ int hit[kCellHitCap], seg[64];
+      const int nh = cell_hits(cd, cell_of(pos + 3 * i, hi, cd), 0, cell_n, cell_m, i, lane,
+                               within, hit, seg);
+      if (nh <= kCellHitCap) {
+        cell_place(hit, nh, lane, nl_list + (size_t)i * p.max_neigh, p.max_neigh);
+        if (lane == 0) {
+          nl_count[i] = nh < p.max_neigh ? nh : p.max_neigh;
+          if (nh > p.max_neigh) atomicAdd(&flags[0], 1);
+        }
+        return;
+      }
+    }
+  }
   int count = 0;
   for (int j0 = 0; j0 < p.N; j0 += 64) {
-    int j = j0 + lane;
-    bool in = false;
-    if (j < p.N && j != i) {
-      double dx = pos[3 * j] - pix, dy = pos[3 * j + 1] - piy, dz = pos[3 * j + 2] - piz;
-      mimg_exact(dx, dy, dz, hi, h);
-      in = norm2_exact(dx, dy, dz) < p.rlist2;
-    }
+    const int j = j0 + lane;
+    const bool in = j < p.N && j != i && within(j);
     unsigned long long m = __ballot(in);
     int rank = __popcll(m & ((1ull << lane) - 1ull));
     if (in && count + rank < p.max_neigh) nl_list[(size_t)i * p.max_neigh + count + rank] = j;
@@ -1248,7 +1300,16 @@ hipError_t rigid_launch_apply_pbc(const RigidDev &d, const RigidParams &p, const
 
 hipError_t rigid_launch_nlist(const RigidDev &d, const RigidParams &p, const double *hh,
                               hipStream_t s) {
-  k_rigid_nlist<<<p.N, 64, 0, s>>>(d.pos, hh, d.nl_count, d.nl_list, d.flags, p);
+  if (d.cell_dim) {  // allocated for N >= cell_min
+    k_rigid_cell_grid<<<1, 256, 0, s>>>(hh, d.cell_dim, d.cell_n, p);
+    k_rigid_cell_bin<<<(p.N + 255) / 256, 256, 0, s>>>(d.pos, hh, d.cell_dim, d.cell_n,
+                                                       d.cell_m, p.N);
+    k_rigid_nlist<true><<<p.N, 64, 0, s>>>(d.pos, hh, d.nl_count, d.nl_list, d.flags,
+                                           d.cell_dim, d.cell_n, d.cell_m, p);
+  } else {
+    k_rigid_nlist<false><<<p.N, 64, 0, s>>>(d.pos, hh, d.nl_count, d.nl_list, d.flags,
+                                            nullptr, nullptr, nullptr, p);
+  }
   return hipGetLastError();
 }
 

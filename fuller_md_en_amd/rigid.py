@@ -93,7 +93,8 @@ class RigidEngine:
     """HBM-resident rigid-body NPT engine (fmd_rigid_create / fmd_engine_*)."""
 
     def __init__(self, body, n_mol, I0, Mmol, rmol, dt=1.0, lj=None, nlup=25,
-                 max_neigh=MAX_NEIGH, rmcut=None, skin=3.0, half_pairs=1):
+                 max_neigh=MAX_NEIGH, rmcut=None, skin=3.0, half_pairs=1,
+                 cell_min_molecules=None):
         self.body = _c(body).reshape(-1, 3)
         cfg = N.RigidConfig()
         N.lib().fmd_rigid_config_default(C.byref(cfg), rmol)
@@ -104,6 +105,8 @@ class RigidEngine:
         cfg.I0, cfg.Mmol, cfg.dt, cfg.nlup, cfg.max_neigh = I0, Mmol, dt, nlup, max_neigh
         cfg.skin = skin
         cfg.half_pairs = int(half_pairs)
+        if cell_min_molecules is not None:
+            cfg.cell_min_molecules = int(cell_min_molecules)
         if lj is not None:
             cfg.lj = lj
             cfg.rmcut = lj.rcut + 2 * rmol + 1.0

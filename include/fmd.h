@@ -287,6 +287,9 @@ typedef struct {
                            wavefront per pair, 2: the same per molecule (its owned
                            pairs in one workgroup); 0: every pair from both sides
                            and the full tensor, as the reference computes */
+  int cell_min_molecules; /* molecule lists from cells of a grid for systems of at
+                           least this many molecules (3000; 0 = never), below it a
+                           scan of every molecule; same lists */
 } fmd_rigid_config;
 
 /* Defaults of the reference rigid programs for the given molecule radius. */
@@ -320,7 +323,7 @@ typedef struct {
                             not depend on it beyond summation order: stale lists
                             are rebuilt within the step. */
   int cell_min_molecules; /* engines: molecule lists from cells of a grid for
-                            replicas of at least this many molecules (4096),
+                            replicas of at least this many molecules (2048),
                             below it a scan of every molecule; same lists */
 } fmd_atomic_config;
 

@@ -375,6 +375,10 @@ def test_molecule_lists_from_cells_equal_the_scan():
     across the step-20 rebuild are bitwise equal."""
     from fuller_md_en_amd import system
     s = system.airebo_system(5, "C60", T=300.0, seed=42)
+    # three cells per axis: a third of the box holds the pair reach of the
+    # molecule lists (rc + list skin + 1 A drift + two molecule radii)
+    reach = 10.2 + 2.0 + 1.0 + 2.0 * _radius(s["pos"], s["h"], s["mol_id"])
+    assert s["h"][0] / 3.0 >= reach and s["h"][0] / 4.0 < reach
     out = []
     for cmin in (1, 1 << 30):
         print(f"case cell_min_molecules={cmin}", flush=True)

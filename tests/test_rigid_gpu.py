@@ -224,3 +224,38 @@ def test_engine_counts_a_stale_list():
         eng.run(0, 25)
         assert (eng.counters()[0] > 0) == faults
         eng.close()
+
+
+def test_lists_from_cells_equal_the_scan():
+    """Systems of at least cell_min_molecules molecules bin the centres into
+    cells at least the list radius wide and take each list row from the 27
+    cells around its own, sorted; below that every molecule is scanned.  On a
+    5x5x5 C60 crystal (500 molecules, three cells per axis) both give the same
+    lists, so 60 steps across the rebuilds at steps 25 and 50 are bitwise
+    equal.  The stateless list of a 10x10x10 crystal (4000 molecules: from
+    the cells by default) equals the reference build's."""
+    from fuller_md_en_amd import system
+    mol = system.molecule("C60")
+    s = system.rigid_system(nc=5, spec="C60", T=300.0, seed=42)
+    n = len(s["pos"])
+    rmcut = 3.0 * 3.431 + 2.0 * mol.rmol + 1.0
+    assert s["h"][0] / 3.0 >= rmcut + 3.0 > s["h"][0] / 4.0  # three cells per axis
+    out = []
+    for cmin in (1, 1 << 30):
+        print(f"case nc=5 cell_min_molecules={cmin}", flush=True)
+        eng = rigid.RigidEngine(mol.coords, n, mol.I0, mol.mass, rmol=mol.rmol,
+                                cell_min_molecules=cmin)
+        eng.set_state(pos=s["pos"], vel=s["vel"], qv=s["qv"], omg=s["omg"], h=s["h"],
+                      npt=rigid.make_npt(300.0, 0.0, n))
+        th = eng.run(0, 60)
+        out.append((np.array([[t.Ep, t.P] for t in th]), eng.get_state()["pos"].copy()))
+        assert eng.counters()[0] == 0
+        eng.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    print("case stateless nc=10", flush=True)
+    s = system.rigid_system(nc=10, spec="C60", T=300.0, seed=42)
+    cnt, lst = rigid.nlist_build_sym(s["pos"], s["h"], oracle.inv9(s["h"]), rmcut)
+    c0, l0 = oracle.rigid_nlist(s["pos"], s["h"], rmcut)
+    assert (cnt == c0).all() and cnt.min() > 0
+    for i in range(len(cnt)):
+        assert (lst[i, :cnt[i]] == l0[i, :cnt[i]]).all()
