@@ -114,6 +114,10 @@ def parse():
                          "bound) or the exact gradient")
     ap.add_argument("--al-skin", type=float, default=0.0,
                     help="airebo workload: per-atom LJ list skin (default: the autotuner's)")
+    ap.add_argument("--cell-min", type=int, default=None,
+                    help="airebo / rigid workloads: molecule lists from a cell grid for systems "
+                         "of at least this many molecules (default: the engine's, 2048 AIREBO, "
+                         "3000 rigid; a larger value keeps the scan of every molecule)")
     ap.add_argument("--rebo", choices=["compat", "physical"], default="physical",
                     help="REBO force mode of the timed run")
     ap.add_argument("--sg-iters", type=int, default=30000,
@@ -352,10 +356,10 @@ def airebo_cpu_leg(s, nc):
     return cpu, out
 
 
-def airebo_engine(s, mode, params):
+def airebo_engine(s, mode, params, cell_min=None):
     from fuller_md_en_amd import airebo
     eng = airebo.AtomicEngine(s["mol_id"], s["mass"], dt=0.5, mode=mode,
-                              al_skin=params["al_skin"])
+                              al_skin=params["al_skin"], cell_min_molecules=cell_min)
     eng.set_graph(bool(params["graph"]))
     eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"],
                   npt=airebo.make_npt(T_K, 0.0, len(s["pos"])))
@@ -388,7 +392,7 @@ def main_airebo(args):
     na, nm = len(s["pos"]), s["n_mol"]
     mode = airebo.REBO_PHYSICAL if args.rebo == "physical" else airebo.REBO_COMPAT
     params, tuned = airebo_params(args, s, nc, mode)
-    eng = airebo_engine(s, mode, params)
+    eng = airebo_engine(s, mode, params, args.cell_min)
     dt, th, ph, rebuild = timed_run(eng, args.warmup, args.steps, 20)
     faults, rebuilds = eng.counters()[:2]
     eng.close()
@@ -423,14 +427,14 @@ def main_airebo(args):
 
 
 # ------------------------------------------------------------------ rigid
-def rigid_engine(params, nc=10, a=14.17):
+def rigid_engine(params, nc=10, a=14.17, cell_min=None):
     from fuller_md_en_amd import rigid, system
     mol = system.molecule("C60")
     scale = a / system.default_a0(mol.dmol, "FCC", 1.0)
     s = system.rigid_system(nc=nc, spec="C60", T=T_K, seed=SEED, init_scale=scale)
     eng = rigid.RigidEngine(mol.coords, len(s["pos"]), mol.I0, mol.mass, rmol=mol.rmol,
                             dt=1.0, skin=params["skin"], nlup=params["nlup"],
-                            half_pairs=int(params["half_pairs"]))
+                            half_pairs=int(params["half_pairs"]), cell_min_molecules=cell_min)
     eng.set_graph(bool(params["graph"]))
     eng.set_state(pos=s["pos"], vel=s["vel"], qv=s["qv"], omg=s["omg"], h=s["h"],
                   npt=rigid.make_npt(T_K, 0.0, len(s["pos"])))
@@ -496,7 +500,7 @@ def rigid_cpu_baseline(nc, scale):
 def main_rigid(args):
     nc = args.nc or 10
     params, entry = tuned_params(args, nc)
-    eng, mol, s, scale = rigid_engine(params, nc)
+    eng, mol, s, scale = rigid_engine(params, nc, cell_min=args.cell_min)
     n_sites = len(s["pos"]) * mol.natom
     dt, th, ph, rebuild = timed_run(eng, args.warmup, args.steps, int(params["nlup"]))
     faults = eng.counters()[0]
