@@ -74,6 +74,14 @@ __device__ __noinline__ double r2_ref(const double *__restrict__ pos, int i, int
 
 __device__ __forceinline__ const double *hh_of(HHRef hh, int r) { return hh.p + r * hh.stride; }
 
+// Sum of NW per-wavefront integer totals in a fixed order.
+template <int NW>
+__device__ __forceinline__ int wave_tot(const int *w) {
+  int s = w[0];
+  for (int k = 1; k < NW; k++) s += w[k];
+  return s;
+}
+
 // Block sum of K doubles in a fixed order (NW wavefronts of 64 threads);
 // sum k goes to out[k * stride].
 template <int K, int NW = 4>
@@ -97,10 +105,11 @@ __device__ __forceinline__ void block_sum(const double *v, double *out, int stri
 // Per-record partials of a trace-only evaluation: the block sums of the
 // virial trace w and the energy e give W = (w/3) 1 and E, written as the
 // kAtPart components (W9 row-major, E) at out[k * stride].
+template <int NW = 4>
 __device__ __forceinline__ void block_sum_trace(double w, double e, double *out, int stride) {
   const double v[2] = {w, e};
   __shared__ double tot[2];
-  block_sum<2>(v, tot);
+  block_sum<2, NW>(v, tot);
   if (threadIdx.x < kAtPart) {
     const int k = threadIdx.x;
     out[(size_t)k * stride] = k == 9 ? tot[1] : (k == 0 || k == 4 || k == 8) ? tot[0] / 3.0 : 0.0;
@@ -837,67 +846,70 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 }
 
 // ------------------------------------------------------------------ REBO
-// One 256-thread workgroup per molecule (the REBO list never leaves a
-// molecule).  Energy and virial: every thread's share, one fixed-order
-// workgroup sum.
+// One workgroup per molecule (the REBO list never leaves a molecule): three
+// wavefronts when the tile holds at most 192 atoms, else four.  Energy and
+// virial: every thread's share, one fixed-order workgroup sum.
 //
-// Fast path (the molecule has at most kReboSlots list entries and
-// kReboTriSlots angular triples per atom on average, and the caller sized the
-// LDS for them): the molecule's entries are packed atom after atom in LDS by
-// prefix sums of the list counts, with local atom indices, and
-//   B  one thread per entry: the FP64 minimum-image bond vector, kept as FP32
-//      (x, y, z, r) and the FP64 length;
-//   C  one thread per entry: the reverse entry, the entry's place among its
-//      atom's bonds (entries shorter than the 2.0 A switch, the only ones that
-//      take part in angular terms), the FP32 bond-order sum Gs and
-//      b = (1 + Gs)^-1/2; the bond's owner (lower atom index) computes the
-//      FP64 switch and pair quantities once per bond (bond_terms);
+// Fast path (the molecule has at most kReboSlots bond entries -- list
+// entries shorter than the 2.0 A switch -- and kReboTriSlots angular triples
+// per atom on average, and the caller sized the LDS for them):
+//   B  one thread per list entry: the FP64 minimum-image bond vector and
+//      length; the bond entries are then packed atom after atom in LDS, in
+//      list order, with local atom indices (entries beyond the switch add
+//      nothing to any force, energy or bond order and are dropped);
+//   C  one thread per bond entry: the reverse entry, the FP32 bond-order sum
+//      Gs and b = (1 + Gs)^-1/2; the bond's owner (lower atom index) computes
+//      the FP64 switch and pair quantities once per bond (bond_terms);
 //   D  one thread per bond entry c -> p: every angular triple (c, p, t) over
 //      c's other bonds t, stored once (force ft on t, fp on p);
-//   E  one thread per entry x -> y: every force component landing on x
+//   E  one thread per bond entry x -> y: every force component landing on x
 //      through it, in a fixed order -- the pair term, the triples centred on
 //      x with partner y, those centred on y with x as the third atom and (in
 //      the physical mode) with x as the partner;
-// then each atom's force is the sum of its entries' in list order.  Each
-// angular triple is evaluated once (the entry-based form evaluates it from
-// all three of its atoms).
+// then each atom's force is the sum of its bond entries' in list order.
+// Each angular triple is evaluated once (the entry-based form evaluates it
+// from all three of its atoms).
 //
 // Otherwise the entries go through the global ReboBonds arrays in the
 // reference list layout and the entry forces through rebo_fe, evaluating
 // the triples where they are used (rebo_pass2_entry); both paths compute the
 // same FP32 triple terms and FP64 pair terms and add them in the same order,
 // and the path depends on the molecule alone.
-constexpr int kReboSlots = 6;
-constexpr int kReboTriSlots = 7;
-// fast-path LDS bytes per entry: P (double4), FP64 length and b (double
+constexpr int kReboSlots = 4;     // bond entries per atom (C60: 3)
+constexpr int kReboTriSlots = 6;  // angular triples per atom (C60: 6)
+constexpr int kReboPer = 2;       // list entries per thread in the gather
+// fast-path LDS bytes per bond entry: P (double4), FP64 length and b (double
 // each), bond vector (float4), fc / dfc (float2), reverse entry and owner
-// list (u16 each), local neighbour, owning atom, bond index, bond flags,
-// entry flags (u8 each)
-constexpr int kReboSlotBytes = 32 + 16 + 16 + 8 + 4 + 5;
+// list (u16 each), local neighbour, owning atom, bond index, bond flags (u8
+// each)
+constexpr int kReboSlotBytes = 32 + 16 + 16 + 8 + 4 + 4;
 constexpr int kReboTriBytes = 24;  // ft, fp (float3 each)
 constexpr size_t kReboFastLdsMax = 64 * 1024;
 constexpr int kNoRev = 0xffff;  // no reverse entry (an asymmetric, overflowed list)
 constexpr int kNoBond = 0x7f;   // entry beyond the switch: not one of the atom's bonds
 enum : int { kEntryBond = 1, kBondVaPos = 8 };
 
+// (plus the prefix sums of bonds, triples and list entries per atom, the
+// atoms' bond counts, and each list entry's atom and bond flag)
 __host__ __device__ inline size_t rebo_fast_lds(int apad) {
   return (size_t)kReboSlots * apad * kReboSlotBytes + (size_t)kReboTriSlots * apad * kReboTriBytes +
-         4 * (2 * (size_t)apad + 1) + (size_t)apad + 16;
+         4 * (3 * (size_t)apad + 2) + (size_t)apad + 2 * 256 * kReboPer + 16;
 }
 // global path: bond vectors of 64 atoms, prefix sums and owning atoms
 __host__ __device__ inline size_t rebo_slow_lds(int apad, int maxr) {
   return (size_t)64 * maxr * sizeof(double4) + 4 * ((size_t)apad + 1 + (size_t)apad * maxr);
 }
 
-template <int MODE, bool TRACE>
-__global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, HHRef hh,
+template <int MODE, bool TRACE, int NT>
+__global__ __launch_bounds__(NT) void k_rebo_mol(AtomicDev d, AtomicParams p, HHRef hh,
                                                    int fast_ok) {
   extern __shared__ double4 lds[];
-  __shared__ int wtot[4], wtri[4], nown;
+  constexpr int NW = NT / 64;
+  __shared__ int wtot[NW], wtri[NW], nown;
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int m = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
   if (m >= p.Nm) return;
-  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;  // n <= 256 (host check)
+  const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;  // n <= NT (host check)
   const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   // exclusive prefix sums of the list counts (one atom per thread)
@@ -911,12 +923,15 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
   __syncthreads();
   int off = 0;
   for (int w = 0; w < wv; w++) off += wtot[w];
-  const int total = ((wtot[0] + wtot[1]) + wtot[2]) + wtot[3];
-  // the choice depends on the molecule alone (not on the tile width apad);
-  // the gather keeps kReboPer entries per thread in registers
-  constexpr int kReboPer = 2;
-  bool fast = fast_ok && total <= kReboSlots * n && total <= 256 * kReboPer;
+  const int total = wave_tot<NW>(wtot);
+  // the choice depends on the molecule alone (not on the tile width apad)
+  bool fast = fast_ok && total <= NT * kReboPer;
   double E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int nbond = 0;  // the molecule's bond entries (list entries shorter than kDmax)
+  int yq[kReboPer];         // this thread's list entries: local neighbour
+  double dq[kReboPer][4];   // (-1: none), FP64 bond vector and length
+#pragma unroll
+  for (int q = 0; q < kReboPer; q++) yq[q] = -1;
   if (fast) {
     const int S = kReboSlots * p.apad, TS = kReboTriSlots * p.apad;
     char *base = reinterpret_cast<char *>(lds);
@@ -926,58 +941,116 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
     float4 *bd = reinterpret_cast<float4 *>(bv + S);
     float2 *fcd = reinterpret_cast<float2 *>(bd + S);
     float *tri = reinterpret_cast<float *>(fcd + S);  // 6 floats per triple
-    int *pre = reinterpret_cast<int *>(tri + 6 * TS);
+    int *pre = reinterpret_cast<int *>(tri + 6 * TS);  // each atom's first bond entry
     int *tpre = pre + p.apad + 1;  // triple offsets, exclusive within each wavefront
-    unsigned short *rev = reinterpret_cast<unsigned short *>(tpre + p.apad);
+    int *rpre = tpre + p.apad;     // each atom's first list entry
+    unsigned short *rev = reinterpret_cast<unsigned short *>(rpre + p.apad + 1);
     unsigned short *ownl = rev + S;  // the entries that own their bond, packed
     unsigned char *nloc = reinterpret_cast<unsigned char *>(ownl + S);
-    unsigned char *eat = nloc + S, *bidx = eat + S, *bflg = bidx + S, *eflg = bflg + S;
-    unsigned char *nb = eflg + S;
+    unsigned char *eat = nloc + S, *bidx = eat + S, *bflg = bidx + S;
+    unsigned char *nb = bflg + S;
+    unsigned char *rat = nb + p.apad, *rbd = rat + NT * kReboPer;  // per list entry: atom, bond
     if (t < n) {
-      pre[t] = off + v - c;
-      for (int jn = 0; jn < c; jn++) eat[off + v - c + jn] = (unsigned char)t;
+      rpre[t] = off + v - c;
+      for (int jn = 0; jn < c; jn++) rat[off + v - c + jn] = (unsigned char)t;
     }
     if (t == 0) {
-      pre[n] = total;
+      rpre[n] = total;
       nown = 0;
     }
     __syncthreads();
-    // B: local neighbours and bond vectors (FP64 minimum image); the entries
-    // that own their bond (lower atom index first) are listed in any order --
-    // which thread takes a bond does not change its result -- so that the
-    // FP64 bond terms run on as few wavefronts as possible
-    for (int e = t; e < total; e += 256) {
-      const int xl = eat[e], i = s0 + xl;
-      const int j = d.nll_r[(size_t)i * p.maxr + (e - pre[xl])];
-      nloc[e] = (unsigned char)(j - s0);
-      double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
-             dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
-      mimg(dx, dy, dz, hi, h);
-      const double rd = sqrt(dx * dx + dy * dy + dz * dz);
-      bd[e] = make_float4((float)dx, (float)dy, (float)dz, (float)rd);
-      rdv[e] = rd;
-      eflg[e] = rd < rebo::kDmax ? kEntryBond : 0;
-      if (xl < j - s0) ownl[atomicAdd(&nown, 1)] = (unsigned short)e;
+    // B: list entries, at most kReboPer per thread: local neighbour, FP64
+    // minimum-image bond vector and length (kept in registers)
+#pragma unroll
+    for (int q = 0; q < kReboPer; q++) {
+      const int e = t + NT * q;
+      if (e < total) {
+        const int xl = rat[e], i = s0 + xl;
+        const int j = d.nll_r[(size_t)i * p.maxr + (e - rpre[xl])];
+        double dx = d.pos[3 * j] - d.pos[3 * i], dy = d.pos[3 * j + 1] - d.pos[3 * i + 1],
+               dz = d.pos[3 * j + 2] - d.pos[3 * i + 2];
+        mimg(dx, dy, dz, hi, h);
+        const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+        yq[q] = j - s0;
+        dq[q][0] = dx;
+        dq[q][1] = dy;
+        dq[q][2] = dz;
+        dq[q][3] = rd;
+        rbd[e] = rd < rebo::kDmax;
+      }
     }
     __syncthreads();
-    // C: atoms -- bond counts and the triple offsets (within each wavefront)
-    int ntri = 0;
-    if (t < n) {
-      int k = 0;
-      for (int e = pre[t]; e < pre[t + 1]; e++) k += eflg[e] & kEntryBond;
-      nb[t] = (unsigned char)k;
-      ntri = k * (k - 1);
-    }
-    {
-      int w = ntri;
-      for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(w, o, 64);
-        if (lane >= o) w += u;
+    // atoms: bond counts, the bond entries' prefix sums and the triple
+    // offsets (within each wavefront); only bonds take part from here on
+    // (entries beyond kDmax add nothing to any force, energy or bond order)
+    int k = 0;
+    if (t < n)
+      for (int e = rpre[t]; e < rpre[t + 1]; e++) k += rbd[e];
+    int wk = k, w3 = t < n ? k * (k - 1) : 0;
+    const int ntri = w3;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(wk, o, 64), u3 = __shfl_up(w3, o, 64);
+      if (lane >= o) {
+        wk += u;
+        w3 += u3;
       }
-      if (t < n) tpre[t] = w - ntri;
-      if (lane == 63) wtri[wv] = w;
     }
-    //    owners: the bond's FP64 terms, into its own slot and its reverse's
+    if (lane == 63) {
+      wtot[wv] = wk;  // the list totals are no longer needed
+      wtri[wv] = w3;
+    }
+    if (t < n) {
+      nb[t] = (unsigned char)k;
+      tpre[t] = w3 - ntri;
+    }
+    __syncthreads();
+    int koff = 0;
+    for (int w = 0; w < wv; w++) koff += wtot[w];
+    nbond = wave_tot<NW>(wtot);
+    if (t < n) pre[t] = koff + wk - k;
+    if (t == 0) pre[n] = nbond;
+    if (nbond > kReboSlots * n) fast = false;  // more bonds than the LDS holds
+  }
+  if (fast) {
+    const int S = kReboSlots * p.apad, TS = kReboTriSlots * p.apad;
+    char *base = reinterpret_cast<char *>(lds);
+    double4 *P = reinterpret_cast<double4 *>(base);
+    double *rdv = reinterpret_cast<double *>(P + S);
+    double *bv = rdv + S;
+    float4 *bd = reinterpret_cast<float4 *>(bv + S);
+    float2 *fcd = reinterpret_cast<float2 *>(bd + S);
+    float *tri = reinterpret_cast<float *>(fcd + S);
+    int *pre = reinterpret_cast<int *>(tri + 6 * TS);
+    int *tpre = pre + p.apad + 1;
+    int *rpre = tpre + p.apad;
+    unsigned short *rev = reinterpret_cast<unsigned short *>(rpre + p.apad + 1);
+    unsigned short *ownl = rev + S;
+    unsigned char *nloc = reinterpret_cast<unsigned char *>(ownl + S);
+    unsigned char *eat = nloc + S, *bidx = eat + S, *bflg = bidx + S;
+    unsigned char *nb = bflg + S;
+    unsigned char *rat = nb + p.apad, *rbd = rat + NT * kReboPer;
+    const int total = nbond;
+    __syncthreads();
+    // the bonds into their slots, atom after atom in list order; those that
+    // own their bond (lower atom index first) are listed in any order --
+    // which thread takes a bond does not change its result -- so that the
+    // FP64 bond terms run on as few wavefronts as possible
+#pragma unroll
+    for (int q = 0; q < kReboPer; q++) {
+      const int e = t + NT * q;
+      if (yq[q] < 0 || !rbd[e]) continue;
+      const int xl = rat[e];
+      int bi = 0;
+      for (int f = rpre[xl]; f < e; f++) bi += rbd[f];
+      const int ce = pre[xl] + bi;
+      nloc[ce] = (unsigned char)yq[q];
+      eat[ce] = (unsigned char)xl;
+      bd[ce] = make_float4((float)dq[q][0], (float)dq[q][1], (float)dq[q][2], (float)dq[q][3]);
+      rdv[ce] = dq[q][3];
+      if (xl < yq[q]) ownl[atomicAdd(&nown, 1)] = (unsigned short)ce;
+    }
+    __syncthreads();
+    // C: owners -- the bond's FP64 terms, into its own slot and its reverse's
     auto rev_of = [&](int e) {
       const int xl = eat[e], yl = nloc[e];
       int rv = kNoRev;
@@ -999,38 +1072,27 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
         bflg[rv] = (unsigned char)bf;
       }
     };
-    for (int k = t; k < nown; k += 256) {
+    for (int k = t; k < nown; k += NT) {
       const int e = ownl[k];
       owner_terms(e, rev_of(e));
     }
-    //    entries -- reverse entry, bond index, bond order
-    for (int e = t; e < total; e += 256) {
+    //    bonds -- reverse entry, bond index, bond order
+    for (int e = t; e < total; e += NT) {
       const int xl = eat[e], yl = nloc[e];
       const int rv = rev_of(e);
       rev[e] = (unsigned short)rv;
-      const double rd = rdv[e];
-      int bi = kNoBond;
-      if (eflg[e] & kEntryBond) {
-        bi = 0;
-        for (int f = pre[xl]; f < e; f++) bi += eflg[f] & kEntryBond;
-      }
+      const int bi = e - pre[xl];
       // an entry above its neighbour whose reverse is missing (an
       // asymmetric, overflowed list) owns its bond too
       if (xl > yl && rv == kNoRev) owner_terms(e, rv);
-      double b = 0.0;
-      bool gs_pos = false;
-      if (rd <= rebo::kDmax) {
-        const float4 u = bd[e];
-        const float Gs = bond_order_sum<float>(e - pre[xl], pre[xl + 1] - pre[xl], u.x, u.y, u.z,
-                                               u.w, [&](int kn) { return bd[pre[xl] + kn]; });
-        b = rebo::rsq(1.0 + (double)Gs);
-        gs_pos = Gs > 1e-20f;
-      }
-      bv[e] = b;
-      bidx[e] = (unsigned char)(bi | (gs_pos ? 0x80 : 0));
+      const float4 u = bd[e];
+      const float Gs = bond_order_sum<float>(bi, pre[xl + 1] - pre[xl], u.x, u.y, u.z, u.w,
+                                             [&](int kn) { return bd[pre[xl] + kn]; });
+      bv[e] = rebo::rsq(1.0 + (double)Gs);
+      bidx[e] = (unsigned char)(bi | (Gs > 1e-20f ? 0x80 : 0));
     }
     __syncthreads();
-    const int tri_total = ((wtri[0] + wtri[1]) + wtri[2]) + wtri[3];
+    const int tri_total = wave_tot<NW>(wtri);
     // triple offset of atom x: its wavefront's scan plus the earlier wavefronts'
     auto tbase = [&](int x) {
       int o = tpre[x];
@@ -1045,7 +1107,7 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
       fast = false;  // more angular triples than the LDS holds: the global path
     } else {
       // D: angular triples (c, p, t), one thread per bond entry c -> p
-      for (int e = t; e < total; e += 256) {
+      for (int e = t; e < total; e += NT) {
         const int bi = bidx[e] & kNoBond;
         const int xl = eat[e], nbx = nb[xl];
         if (bi == kNoBond || nbx < 2) continue;
@@ -1079,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
 #pragma unroll
       for (int q = 0; q < kReboPer; q++) {
         Fq[q][0] = Fq[q][1] = Fq[q][2] = 0.0;
-        const int e = t + 256 * q;
+        const int e = t + NT * q;
         if (e >= total) continue;
         double *F = Fq[q];
         const int xl = eat[e], yl = nloc[e], rv = rev[e], bf = bflg[e];
@@ -1143,20 +1205,21 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
       double *Fe = reinterpret_cast<double *>(P);  // E is done with P
 #pragma unroll
       for (int q = 0; q < kReboPer; q++) {
-        const int e = t + 256 * q;
+        const int e = t + NT * q;
         if (e < total)
           for (int k = 0; k < 3; k++) Fe[3 * e + k] = Fq[q][k];
       }
       __syncthreads();
       if (t < n) {  // atom t, its c entries
         double F[3] = {0, 0, 0};
-        for (int e = off + v - c; e < off + v; e++)
+        for (int e = pre[t]; e < pre[t + 1]; e++)
           for (int k = 0; k < 3; k++) F[k] += Fe[3 * e + k];
         for (int k = 0; k < 3; k++) d.Frebo[3 * ((size_t)s0 + t) + k] = F[k];
       }
     }
   }
   if (!fast) {
+    __syncthreads();  // the fast path may have used the LDS (fast is uniform)
     double4 *vg = lds;  // [64 atoms][maxr]
     int *pre = (int *)(vg + 64 * p.maxr), *eat = pre + p.apad + 1;
     if (t < n) {
@@ -1165,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
     }
     const SlotStride slot{p.maxr};
     const int q = t & 3;
-    for (int base = 0; base < n; base += 64) {
+    for (int base = 0; base < n; base += NT / 4) {
       const int xl = base + (t >> 2), i = s0 + xl;
       const int cx = xl < n ? d.nlc_r[i] : 0;
       double4 *vi = vg + (size_t)(t >> 2) * p.maxr;
@@ -1181,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
         rebo_pass1<float>(i, jn, vi, d.nlc_r, d.nll_r, slot, d.bonds);
       __syncthreads();
     }
-    for (int e = t; e < total; e += 256) {
+    for (int e = t; e < total; e += NT) {
       const int x = s0 + eat[e], yn = e - pre[eat[e]];
       double F[3] = {0, 0, 0};
       rebo_pass2_entry<float, MODE, TRACE>(x, yn, d.nlc_r, d.nll_r, slot, d.bonds, F, E, W);
@@ -1197,12 +1260,12 @@ __global__ __launch_bounds__(256) void k_rebo_mol(AtomicDev d, AtomicParams p, H
     }
   }
   if (TRACE) {
-    block_sum_trace(W[0], E, d.rebo_mpart + m, p.Nm);
+    block_sum_trace<NW>(W[0], E, d.rebo_mpart + m, p.Nm);
   } else {
     double part[kAtPart];
     for (int k = 0; k < 9; k++) part[k] = W[k];
     part[9] = E;
-    block_sum<kAtPart>(part, d.rebo_mpart + m, p.Nm);
+    block_sum<kAtPart, NW>(part, d.rebo_mpart + m, p.Nm);
   }
 }
 
@@ -2185,14 +2248,27 @@ hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, i
   const size_t shm = fast_ok && fast > slow ? fast : slow;
   const int per = (p.Nm + kXcd - 1) / kXcd;
   dim3 g(per * kXcd);
-  if (mode == kReboPhysical && p.lj_trace)
-    k_rebo_mol<kReboPhysical, true><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
-  else if (mode == kReboPhysical)
-    k_rebo_mol<kReboPhysical, false><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
-  else if (p.lj_trace)
-    k_rebo_mol<kReboCompat, true><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
-  else
-    k_rebo_mol<kReboCompat, false><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
+  // three wavefronts when the molecule's atoms fit (five workgroups per CU
+  // within the VGPR budget), else four
+  if (p.apad <= 192) {
+    if (mode == kReboPhysical && p.lj_trace)
+      k_rebo_mol<kReboPhysical, true, 192><<<g, 192, shm, s>>>(d, p, hh, fast_ok);
+    else if (mode == kReboPhysical)
+      k_rebo_mol<kReboPhysical, false, 192><<<g, 192, shm, s>>>(d, p, hh, fast_ok);
+    else if (p.lj_trace)
+      k_rebo_mol<kReboCompat, true, 192><<<g, 192, shm, s>>>(d, p, hh, fast_ok);
+    else
+      k_rebo_mol<kReboCompat, false, 192><<<g, 192, shm, s>>>(d, p, hh, fast_ok);
+  } else {
+    if (mode == kReboPhysical && p.lj_trace)
+      k_rebo_mol<kReboPhysical, true, 256><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
+    else if (mode == kReboPhysical)
+      k_rebo_mol<kReboPhysical, false, 256><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
+    else if (p.lj_trace)
+      k_rebo_mol<kReboCompat, true, 256><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
+    else
+      k_rebo_mol<kReboCompat, false, 256><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
+  }
   return hipGetLastError();
 }
 
