@@ -359,7 +359,9 @@ def airebo_cpu_leg(s, nc):
 def airebo_engine(s, mode, params, cell_min=None):
     from fuller_md_en_amd import airebo
     eng = airebo.AtomicEngine(s["mol_id"], s["mass"], dt=0.5, mode=mode,
-                              al_skin=params["al_skin"], cell_min_molecules=cell_min)
+                              al_skin=params["al_skin"], cell_min_molecules=cell_min,
+                              rebo_slots=params.get("rebo_slots"),
+                              rebo_threads=params.get("rebo_threads"))
     eng.set_graph(bool(params["graph"]))
     eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"],
                   npt=airebo.make_npt(T_K, 0.0, len(s["pos"])))
@@ -371,7 +373,7 @@ def airebo_params(args, s, nc, mode):
     (--no-tune), or the autotuner's choice for this system (cached per device,
     system, REBO mode, temperature and pressure)."""
     from fuller_md_en_amd import autotune
-    default = {k: v[0] for k, v in autotune.ATOMIC_SPACE.items()}
+    default = autotune.defaults(autotune.ATOMIC_SPACE)
     if args.al_skin:
         return {**default, "al_skin": args.al_skin}, None
     if args.no_tune:

@@ -59,7 +59,8 @@ class AtomicConfig(C.Structure):
                 ("lj_max_neigh", C.c_int), ("rebo_rlist", C.c_double),
                 ("rebo_max_neigh", C.c_int), ("rebo_mode", C.c_int), ("nlup", C.c_int),
                 ("dt", C.c_double), ("clamp_xi", C.c_double), ("clamp_vg", C.c_double),
-                ("al_skin", C.c_double), ("cell_min_molecules", C.c_int)]
+                ("al_skin", C.c_double), ("cell_min_molecules", C.c_int),
+                ("rebo_lds_slots", C.c_int), ("rebo_threads", C.c_int)]
 
 
 class StepCtrl(C.Structure):

@@ -84,7 +84,7 @@ class AtomicEngine:
 
     def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
                  nlup=20, lj=None, rebo_rlist=None, clamp_vg=None, al_skin=None,
-                 cell_min_molecules=None):
+                 cell_min_molecules=None, rebo_slots=None, rebo_threads=None):
         self._mol = _c(mol_id, np.int32)
         self._mass = _c(mass)
         cfg = N.AtomicConfig()
@@ -103,6 +103,10 @@ class AtomicEngine:
             cfg.al_skin = al_skin
         if cell_min_molecules is not None:
             cfg.cell_min_molecules = cell_min_molecules
+        if rebo_slots is not None:
+            cfg.rebo_lds_slots = int(rebo_slots)
+        if rebo_threads is not None:
+            cfg.rebo_threads = int(rebo_threads)
         self.cfg = cfg
         self.n = len(self._mol)
         h = C.c_void_p()
@@ -184,7 +188,7 @@ class ReplicaBatch:
     HIP graphs.  replicas: list of (mol_id, mass) per replica."""
 
     def __init__(self, replicas, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH, nlup=20,
-                 lj=None, graph=True, al_skin=None):
+                 lj=None, graph=True, al_skin=None, rebo_slots=None, rebo_threads=None):
         self._keep = []
         cfgs = (N.AtomicConfig * len(replicas))()
         for r, (mol_id, mass) in enumerate(replicas):
@@ -200,6 +204,10 @@ class ReplicaBatch:
                 cfgs[r].lj = lj
             if al_skin is not None:
                 cfgs[r].al_skin = al_skin
+            if rebo_slots is not None:
+                cfgs[r].rebo_lds_slots = int(rebo_slots)
+            if rebo_threads is not None:
+                cfgs[r].rebo_threads = int(rebo_threads)
         self.sizes = [len(mol_id) for mol_id, _ in replicas]
         self.R = len(replicas)
         h = C.c_void_p()

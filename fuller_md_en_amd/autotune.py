@@ -13,11 +13,16 @@ Tunables:
            per molecule, 0 both sides), graph (step replay)
   atomic   al_skin (the per-atom LJ lists' radius beyond the cutoff: a
            thinner skin walks fewer candidates per step but rebuilds the
-           lists within steps more often), graph (step replay); the half-list
-           radius and rebuild interval are part of the reference's
-           truncated-half-list semantics and stay
+           lists within steps more often), graph (step replay); then, with
+           those fixed, the REBO kernel's LDS budget (bond entries per atom:
+           fewer fit more workgroups per CU, more keep hot molecules off the
+           global-memory path) and workgroup size (three or four
+           wavefronts); the half-list radius and rebuild interval are part
+           of the reference's truncated-half-list semantics and stay
 
-A candidate that counts a list fault is rejected.
+A space is one dict of axes (every combination is tried) or a list of such
+stages: each stage tries its axes' combinations with the earlier stages'
+choices fixed.  A candidate that counts a list fault is rejected.
 """
 from __future__ import annotations
 
@@ -58,7 +63,20 @@ def store(key: str, entry: dict, path: str | None = None) -> None:
 
 RIGID_SPACE = {"skin": [3.0, 2.0, 1.5], "nlup": [25, 10, 50], "half_pairs": [1, 2, 0],
                "graph": [0, 1]}
-ATOMIC_SPACE = {"al_skin": [1.0, 0.75, 0.6, 0.5], "graph": [0, 1]}
+ATOMIC_SPACE = [{"al_skin": [1.0, 0.75, 0.6, 0.5], "graph": [0, 1]},
+                {"rebo_slots": [4, 6], "rebo_threads": [192, 256]}]
+
+
+def stages(space) -> list:
+    return list(space) if isinstance(space, (list, tuple)) else [space]
+
+
+def defaults(space) -> dict:
+    """The first value of every axis of every stage."""
+    out = {}
+    for st in stages(space):
+        out.update({k: v[0] for k, v in st.items()})
+    return out
 
 
 def candidates(space: dict):
@@ -80,7 +98,7 @@ def same_results(ref: dict, res: dict, rtol: float) -> bool:
     return True
 
 
-def tune(key: str, run, space: dict, rtol: float = 1e-7, path: str | None = None,
+def tune(key: str, run, space, rtol: float = 1e-7, path: str | None = None,
          use_cache: bool = True) -> dict:
     """run(params) -> {"seconds": s, "pos": final positions, "ep": energies,
     "overflow": list overflows}.  Returns the cache entry
@@ -92,16 +110,20 @@ def tune(key: str, run, space: dict, rtol: float = 1e-7, path: str | None = None
     tried = []
     ref = None
     best = None
-    for params in candidates(space):
-        res = run(params)
-        ok = res.get("overflow", 0) == 0
-        if ref is None:
-            ref = res
-        else:
-            ok = ok and same_results(ref, res, rtol)
-        tried.append({"params": params, "seconds": res["seconds"], "valid": bool(ok)})
-        if ok and (best is None or res["seconds"] < best[1]):
-            best = (params, res["seconds"])
+    fixed = defaults(space)
+    for st in stages(space):
+        for sub in candidates(st):
+            params = {**fixed, **sub}
+            res = run(params)
+            ok = res.get("overflow", 0) == 0
+            if ref is None:
+                ref = res
+            else:
+                ok = ok and same_results(ref, res, rtol)
+            tried.append({"params": params, "seconds": res["seconds"], "valid": bool(ok)})
+            if ok and (best is None or res["seconds"] < best[1]):
+                best = (params, res["seconds"])
+        fixed = dict(best[0])
     entry = {"params": best[0], "seconds": best[1], "default_seconds": tried[0]["seconds"],
              "tried": tried}
     store(key, entry, path)

@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 // wavefronts when the tile holds at most 192 atoms, else four.  Energy and
 // virial: every thread's share, one fixed-order workgroup sum.
 //
-// Fast path (the molecule has at most kReboSlots bond entries -- list
+// Fast path (the molecule has at most rebo_slots bond entries -- list
 // entries shorter than the 2.0 A switch -- and kReboTriSlots angular triples
 // per atom on average, and the caller sized the LDS for them):
 //   B  one thread per list entry: the FP64 minimum-image bond vector and
@@ -875,8 +875,8 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
 // the triples where they are used (rebo_pass2_entry); both paths compute the
 // same FP32 triple terms and FP64 pair terms and add them in the same order,
 // and the path depends on the molecule alone.
-constexpr int kReboSlots = 4;     // bond entries per atom (C60: 3)
-constexpr int kReboTriSlots = 6;  // angular triples per atom (C60: 6)
+constexpr int kReboTriSlots = 6;  // angular triples per atom (C60: 6); bond
+                                  // entries per atom: AtomicParams::rebo_slots
 constexpr int kReboPer = 2;       // list entries per thread in the gather
 // fast-path LDS bytes per bond entry: P (double4), FP64 length and b (double
 // each), bond vector (float4), fc / dfc (float2), reverse entry and owner
@@ -891,8 +891,8 @@ enum : int { kEntryBond = 1, kBondVaPos = 8 };
 
 // (plus the prefix sums of bonds, triples and list entries per atom, the
 // atoms' bond counts, and each list entry's atom and bond flag)
-__host__ __device__ inline size_t rebo_fast_lds(int apad) {
-  return (size_t)kReboSlots * apad * kReboSlotBytes + (size_t)kReboTriSlots * apad * kReboTriBytes +
+__host__ __device__ inline size_t rebo_fast_lds(int apad, int slots) {
+  return (size_t)slots * apad * kReboSlotBytes + (size_t)kReboTriSlots * apad * kReboTriBytes +
          4 * (3 * (size_t)apad + 2) + (size_t)apad + 2 * 256 * kReboPer + 16;
 }
 // global path: bond vectors of 64 atoms, prefix sums and owning atoms
@@ -933,7 +933,7 @@ __global__ __launch_bounds__(NT) void k_rebo_mol(AtomicDev d, AtomicParams p, HH
 #pragma unroll
   for (int q = 0; q < kReboPer; q++) yq[q] = -1;
   if (fast) {
-    const int S = kReboSlots * p.apad, TS = kReboTriSlots * p.apad;
+    const int S = p.rebo_slots * p.apad, TS = kReboTriSlots * p.apad;
     char *base = reinterpret_cast<char *>(lds);
     double4 *P = reinterpret_cast<double4 *>(base);
     double *rdv = reinterpret_cast<double *>(P + S);  // FP64 length
@@ -1009,10 +1009,10 @@ __global__ __launch_bounds__(NT) void k_rebo_mol(AtomicDev d, AtomicParams p, HH
     nbond = wave_tot<NW>(wtot);
     if (t < n) pre[t] = koff + wk - k;
     if (t == 0) pre[n] = nbond;
-    if (nbond > kReboSlots * n) fast = false;  // more bonds than the LDS holds
+    if (nbond > p.rebo_slots * n) fast = false;  // more bonds than the LDS holds
   }
   if (fast) {
-    const int S = kReboSlots * p.apad, TS = kReboTriSlots * p.apad;
+    const int S = p.rebo_slots * p.apad, TS = kReboTriSlots * p.apad;
     char *base = reinterpret_cast<char *>(lds);
     double4 *P = reinterpret_cast<double4 *>(base);
     double *rdv = reinterpret_cast<double *>(P + S);
@@ -2243,14 +2243,15 @@ hipError_t at_launch_halflist(const AtomicDev &d, const AtomicParams &p, HHRef h
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,
                           hipStream_t s) {
   if (p.apad > 256) return hipErrorInvalidValue;  // one thread per atom in the prefix sums
-  const size_t fast = rebo_fast_lds(p.apad), slow = rebo_slow_lds(p.apad, p.maxr);
+  const size_t fast = rebo_fast_lds(p.apad, p.rebo_slots), slow = rebo_slow_lds(p.apad, p.maxr);
   const int fast_ok = fast <= kReboFastLdsMax;
   const size_t shm = fast_ok && fast > slow ? fast : slow;
   const int per = (p.Nm + kXcd - 1) / kXcd;
   dim3 g(per * kXcd);
-  // three wavefronts when the molecule's atoms fit (five workgroups per CU
-  // within the VGPR budget), else four
-  if (p.apad <= 192) {
+  // three wavefronts when asked for and the molecule's atoms fit (with four
+  // bond slots per atom, five workgroups per CU within the VGPR budget), else
+  // four
+  if (p.rebo_nt == 192 && p.apad <= 192) {
     if (mode == kReboPhysical && p.lj_trace)
       k_rebo_mol<kReboPhysical, true, 192><<<g, 192, shm, s>>>(d, p, hh, fast_ok);
     else if (mode == kReboPhysical)

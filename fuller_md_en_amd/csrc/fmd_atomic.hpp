@@ -104,6 +104,8 @@ struct AtomicParams {
   int lj_alist;    // 1: LJ from per-atom lists (engines; cells with one image
                    // per molecule pair); 0: every tile pair each step (drop-ins)
   int al_cap;      // per-atom list capacity per wavefront segment (entries)
+  int rebo_slots;  // REBO kernel: bond entries per atom in LDS
+  int rebo_nt;     // REBO kernel: workgroup size (192 used for apad <= 192)
   int cell_total;  // molecule-list cells over all replicas
   int mol_cells;   // 1: some replica builds its molecule list from cells
   int cell_min;    // the molecules a replica needs for that

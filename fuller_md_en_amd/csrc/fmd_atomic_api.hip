@@ -247,6 +247,8 @@ void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Na, int Nm, in
   p.mol_rlist = rl + 1.0;  // + 1 A for centre drift between rebuilds
   p.al_r2 = (float)((c.lj.rcut + c.al_skin) * (c.lj.rcut + c.al_skin));  // per-atom lists
   p.al_cap = 384;  // per wavefront segment: about 3.4 times a C60 crystal's share
+  p.rebo_slots = c.rebo_lds_slots > 0 ? c.rebo_lds_slots : 4;
+  p.rebo_nt = c.rebo_threads == 256 ? 256 : 192;
   p.dt = c.dt;
   p.clamp_xi = c.clamp_xi;
   p.clamp_vg = c.clamp_vg;
@@ -517,6 +519,8 @@ void fmd_atomic_config_default(fmd_atomic_config *c) {
   c->clamp_vg = 0.005;
   c->al_skin = 1.0;
   c->cell_min_molecules = 2048;
+  c->rebo_lds_slots = 4;
+  c->rebo_threads = 192;
 }
 
 int fmd_atomic_create(const fmd_atomic_config *c, fmd_engine **out) {
@@ -534,6 +538,10 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
   if (!c || !out || R < 1) return set_error(FMD_ERR_ARG, "fmd_batch_create: bad argument");
   if (!(c[0].al_skin > 0.0 && c[0].al_skin <= c[0].lj_skin + 1.0))
     return set_error(FMD_ERR_ARG, "fmd_batch_create: al_skin must lie in (0, lj_skin + 1]");
+  if (c[0].rebo_lds_slots != 0 && (c[0].rebo_lds_slots < 3 || c[0].rebo_lds_slots > 8))
+    return set_error(FMD_ERR_ARG, "fmd_batch_create: rebo_lds_slots must lie in [3, 8]");
+  if (c[0].rebo_threads != 0 && c[0].rebo_threads != 192 && c[0].rebo_threads != 256)
+    return set_error(FMD_ERR_ARG, "fmd_batch_create: rebo_threads must be 192 or 256");
   for (int r = 0; r < R; r++) {
     if (!c[r].mol_id || !c[r].mass || c[r].n_atoms < 1)
       return set_error(FMD_ERR_ARG, "fmd_batch_create: replica %d: bad configuration", r);
@@ -542,7 +550,8 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
         a.lj_skin != b.lj_skin || a.lj_max_neigh != b.lj_max_neigh ||
         a.rebo_rlist != b.rebo_rlist || a.rebo_max_neigh != b.rebo_max_neigh ||
         a.rebo_mode != b.rebo_mode || a.nlup != b.nlup || a.dt != b.dt ||
-        a.clamp_xi != b.clamp_xi || a.clamp_vg != b.clamp_vg || a.al_skin != b.al_skin)
+        a.clamp_xi != b.clamp_xi || a.clamp_vg != b.clamp_vg || a.al_skin != b.al_skin ||
+        a.rebo_lds_slots != b.rebo_lds_slots || a.rebo_threads != b.rebo_threads)
       return set_error(FMD_ERR_ARG,
                        "fmd_batch_create: replica %d: potential, lists, dt, nlup and clamps "
                        "must equal replica 0's",

@@ -325,6 +325,11 @@ typedef struct {
   int cell_min_molecules; /* engines: molecule lists from cells of a grid for
                             replicas of at least this many molecules (2048),
                             below it a scan of every molecule; same lists */
+  int rebo_lds_slots;    /* REBO kernel: bond entries per atom its LDS holds (4,
+                            3..8); a molecule with more takes the global-memory
+                            path, with the same forces */
+  int rebo_threads;      /* REBO kernel: workgroup size, 192 (molecules of up to
+                            192 atoms; else 256) or 256 */
 } fmd_atomic_config;
 
 /* Defaults of the reference AIREBO program. */

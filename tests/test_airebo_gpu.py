@@ -168,12 +168,14 @@ def test_engine_forces_other_lattices_against_oracle(spec, lattice, nc):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_rebo_lds_and_global_paths_agree(mode):
-    """The REBO kernel stages a molecule's list entries in LDS when they fit
-    six per atom on average, and works from the global entry arrays
-    otherwise.  A 2.6 A REBO list (nine entries per C60 atom: the extra ones
-    beyond the 2.0 A switch carry no force) sends every molecule down the
-    global path; forces, virial and energies must equal those of the 2.3 A
-    list (LDS path) up to summation order, and both the oracle's."""
+    """The REBO kernel stages a molecule's bond entries in LDS when its list
+    has at most two entries per thread and its bonds fit the LDS budget, and
+    works from the global entry arrays otherwise.  A 2.6 A REBO list (nine
+    entries per C60 atom, 540 per molecule against 384 gathered by 192
+    threads; the extra ones beyond the 2.0 A switch carry no force) sends
+    every molecule down the global path; forces, virial and energies must
+    equal those of the 2.3 A list (LDS path) up to summation order, and both
+    the oracle's."""
     pos, h, mol = crystal(2, 11, 0.02)
     lists = (*oracle.nlist_rebo(pos, h, mol), *oracle.nlist_lj(pos, h, mol))
     E0, F0, W0 = oracle.airebo_forces(pos, h, lists, mode)
@@ -192,6 +194,30 @@ def test_rebo_lds_and_global_paths_agree(mode):
     for a, b in zip(Ea, Eb):
         assert rel(a, b) < 1e-12
     assert rel_max(Fa, F0) < TOL and rel_max(Wa, W0) < TOL
+
+
+@pytest.mark.parametrize("slots,threads", [(3, 192), (6, 256), (8, 192)])
+def test_rebo_kernel_parameters_keep_forces(slots, threads):
+    """The REBO kernel's tunables -- bond entries per atom held in LDS and the
+    workgroup size -- change which threads sum which energy and virial shares,
+    never an atom's force: forces equal the default's bit for bit, energies and
+    virial to summation order.  Three slots hold a C60's three bonds per atom
+    exactly."""
+    pos, h, mol = crystal(2, 11, 0.02)
+    out = []
+    for sl, th in ((None, None), (slots, threads)):
+        print(f"case rebo_slots={sl} rebo_threads={th}", flush=True)
+        eng = airebo.AtomicEngine(mol, np.full(len(pos), 12.011), mode=airebo.REBO_PHYSICAL,
+                                  rebo_slots=sl, rebo_threads=th)
+        eng.set_state(pos=pos, vel=np.zeros_like(pos), h=h,
+                      npt=airebo.make_npt(300.0, 0.0, len(pos)))
+        eng.prepare()
+        out.append(eng.forces())
+        eng.close()
+    (Ea, Fa, Wa), (Eb, Fb, Wb) = out
+    assert np.array_equal(Fa, Fb) and rel_max(Wa, Wb) < 1e-12
+    for a, b in zip(Ea, Eb):
+        assert rel(a, b) < 1e-12
 
 
 def _frozen_npt(n):

@@ -47,3 +47,22 @@ def test_same_results_tolerance():
     assert autotune.same_results(a, b, 1e-7)
     b["ep"] = np.array([-3.1])
     assert not autotune.same_results(a, b, 1e-7)
+
+
+def test_stages_fix_earlier_choices(tmp_path):
+    """A list of spaces is tuned stage by stage: the second stage's axes are
+    swept with the first stage's choice fixed, and every candidate is judged
+    against the all-default run."""
+    space = [{"a": [1, 2]}, {"b": [0, 1]}]
+    assert autotune.defaults(space) == {"a": 1, "b": 0}
+    costs = {(("a", 1), ("b", 0)): 1.0, (("a", 2), ("b", 0)): 0.7,
+             (("a", 2), ("b", 1)): 0.6, (("a", 1), ("b", 1)): 0.1}
+    calls = []
+
+    def run(p):
+        calls.append(dict(p))
+        return _fake_run(costs)(p)
+    e = autotune.tune("k3", run, space, path=str(tmp_path / "t.json"))
+    # (a=1, b=1) is never tried: stage 2 runs with a = 2 fixed
+    assert calls == [{"a": 1, "b": 0}, {"a": 2, "b": 0}, {"a": 2, "b": 0}, {"a": 2, "b": 1}]
+    assert e["params"] == {"a": 2, "b": 1} and e["default_seconds"] == 1.0
