@@ -1765,7 +1765,8 @@ __global__ __launch_bounds__(256) void k_at_lj_relist(AtomicDev d, AtomicParams 
 }
 
 // LJ from the per-atom lists: lane = atom slot of molecule i, wavefront w
-// walks segment w of the atom's list; each block is one 16-byte load of eight
+// walks quarter w of the atom's blocks (its four build segments one after the
+// other); each block is one 16-byte load of eight
 // entries, taken four at a time (the neighbours' centre offsets from LDS,
 // then their tile slots, then the pairs).  Pair vector
 // r_b - r_a = (off_b - off_a) + (centre_j - centre_i) in FP32 with the
@@ -1820,10 +1821,28 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
     for (int k = 0; k < 3; k++) GA[s][k] = 0;
     for (int k = 0; k < NW; k++) WA[s][k] = 0;
     EA[s] = 0;
-    // the segment of the atom's list this wavefront's build counterpart wrote
-    const int cnt = a < n ? d.al_n[((size_t)mi * kAlWaves + wv) * NA + a] : 0;
+    // The atom's list is its four build segments one after the other; this
+    // wavefront walks the quarter [lo, lo + cnt) of their blocks.  (Segment
+    // w holds the groups the build's wavefront w took, and their lengths
+    // differ from atom to atom.)
+    const int nbk = p.al_cap >> 3;
+    int p1 = 0, p2 = 0, p3 = 0, tot = 0;
+    if (a < n) {
+      const unsigned short *cn = d.al_n + (size_t)mi * kAlWaves * NA + a;
+      p1 = cn[0];
+      p2 = p1 + cn[NA];
+      p3 = p2 + cn[2 * NA];
+      tot = p3 + cn[3 * NA];
+    }
+    const int lo = (tot * wv) >> 2, cnt = ((tot * (wv + 1)) >> 2) - lo;
     const uint4 *lst = reinterpret_cast<const uint4 *>(d.al_e) +
-                       ((size_t)mi * kAlWaves + wv) * (p.al_cap >> 3) * NA + a;
+                       (size_t)mi * kAlWaves * nbk * NA + a;
+    // block t of the concatenation: segment (t >= p1) + (t >= p2) + (t >= p3)
+    auto blk_at = [&](int t) {
+      const int pre = t >= p3 ? p3 : t >= p2 ? p2 : t >= p1 ? p1 : 0;
+      const int seg = (t >= p1) + (t >= p2) + (t >= p3);
+      return lst[(size_t)(seg * nbk + (t - pre)) * NA];
+    };
     float rx = 0.f, ry = 0.f, rz = 0.f;
     if (a < n) {
       const float4 r = off[(size_t)mi * apad + a];
@@ -1847,11 +1866,11 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
       e = 0.f;
     };
     const char *offb = reinterpret_cast<const char *>(off);
-    uint4 nxt = cnt > 0 ? lst[0] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 nxt = cnt > 0 ? blk_at(lo) : make_uint4(0u, 0u, 0u, 0u);
     for (int ib = 0; ib < cnt; ib++) {
       // the next block's load is in flight while this one is walked
       const uint4 blk = nxt;
-      if (ib + 1 < cnt) nxt = lst[(size_t)(ib + 1) * NA];
+      if (ib + 1 < cnt) nxt = blk_at(lo + ib + 1);
       const unsigned wd[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
       for (int h4 = 0; h4 < 2; h4++) {

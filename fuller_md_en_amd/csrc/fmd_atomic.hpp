@@ -25,11 +25,13 @@
 //                          molecule i, 16-bit entries k << 8 | b (atom slot b
 //                          of the k-th neighbour in mol_nl) of the pairs
 //                          within rc + 1 A that the reference's truncated half
-//                          list keeps, in four segments (one per wavefront) of
-//                          blocks of eight (16 bytes, padded with a never-near
-//                          entry): block c of segment w of slot a at
-//                          ((i * 4 + w) * al_cap / 8 + c) * NA + a (NA = 64 NS);
-//                          al_n counts blocks per segment
+//                          list keeps, in four segments (one per build
+//                          wavefront) of blocks of eight (16 bytes, padded
+//                          with a never-near entry): block c of segment w of
+//                          slot a at ((i * 4 + w) * al_cap / 8 + c) * NA + a
+//                          (NA = 64 NS); al_n counts blocks per segment; the
+//                          walk splits each atom's blocks evenly over its
+//                          four wavefronts
 //   bonds, rebo_fe         per directed REBO entry (fmd_rebo_passes.hpp), used
 //                          for molecules whose entries do not fit the LDS
 //   rebo_mpart, lj_part    fixed-order partial sums per molecule, and
