@@ -111,6 +111,7 @@ def _load():
         "orc_build_nlist_rebo": (None, [_dp, _dp, _dp, _i, _ip, _d, _i, _ip, _ip]),
         "orc_build_nlist_lj": (None, [_dp, _dp, _dp, _i, _ip, _d, _i, _ip, _ip]),
         "orc_compute_rebo": (_d, [_dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i, _i]),
+        "orc_rebo_ext": (_d, [_dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i, _dp, _d, _dp]),
         "orc_compute_lj": (_d, [_dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _i,
                                 C.POINTER(LJParams)]),
         "orc_airebo_forces": (None, [_dp, _dp, _dp, _dp, _dp, _ip, _ip, _i, _ip, _ip, _i,
@@ -234,6 +235,26 @@ def compute_rebo(pos, h, nlc, nll, mode=REBO_COMPAT):
     e = lib().orc_compute_rebo(F, W, pos, h, inv9(h), _c(nlc, np.int32), nll.reshape(-1),
                                nll.size // na, na, mode)
     return e, F.reshape(na, 3), W
+
+
+# AIREBO carbon parameters of the two extension terms: P_CC(N_C, N_H = 0)
+# at N_C = 0..4 and the CCCC torsion well depth (eV)
+AIREBO_PCC = (0.0, 0.0, -0.0276030, 0.0, 0.0)
+AIREBO_TORSION_EPS = 0.3079
+
+
+def rebo_ext(pos, h, nlc, nll, pcc=AIREBO_PCC, eps_t=AIREBO_TORSION_EPS):
+    """orc_rebo_ext: exact-gradient REBO with the P_ij spline and the torsion
+    term -> (E total, F[na, 3], W9, E torsion)."""
+    pos, h = _c(pos).reshape(-1), _c(h)
+    na = pos.size // 3
+    nll = _c(nll, np.int32)
+    F = np.zeros(na * 3)
+    W = np.zeros(9)
+    et = np.zeros(1)
+    e = lib().orc_rebo_ext(F, W, pos, h, inv9(h), _c(nlc, np.int32), nll.reshape(-1),
+                           nll.size // na, na, _c(pcc), float(eps_t), et)
+    return e, F.reshape(na, 3), W, float(et[0])
 
 
 def compute_lj(pos, h, nlc, nll, params=None):

@@ -30,8 +30,16 @@
  *                        by FD gradient and strain-derivative virial; compat
  *                        bond/angle forces equal +gradient term by term.
  *   orc_mm_step_npt      golden multi-step trajectories (MM).
+ *   orc_rebo_ext         no reference counterpart (the REBO-II P_ij correction
+ *                        and the AIREBO torsion term): equal to
+ *                        orc_compute_rebo (physical) with both terms off; FD
+ *                        gradient, strain-derivative virial, closed forms of
+ *                        planar and twisted chains and of a three-coordinated
+ *                        centre.  Parity with the reference is not defined
+ *                        for these terms (the reference has neither).
  */
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "fmd_oracle.h"
@@ -688,6 +696,279 @@ double orc_compute_rebo(double *F, double *vir9, const double *pos,
     }
   }
   return Ep;
+}
+
+/* ------------------------------------------ REBO-II / AIREBO extension
+ * The reference's REBO (above) has neither the REBO-II P_ij correction nor
+ * the AIREBO torsion term.  This is an independent FP64 formulation of the
+ * exact-gradient energy with both (Brenner et al., J. Phys.: Condens. Matter
+ * 14, 783 (2002); Stuart, Tutein & Harrison, J. Chem. Phys. 112, 6472
+ * (2000)), written per bond rather than in the reference's loop order:
+ *   E = sum_{bonds i<j} fc_ij (VR_ij - bbar_ij VA_ij)
+ *       + sum_{chains i-j-k-l} w_ij w_jk w_kl s(cos th_ijk) s(cos th_jkl) V(cos om)
+ *   b_ij = (1 + S_ij)^-1/2,  S_ij = sum_{k != j} fc_ik G(cos th_jik) + P(N_ij),
+ *   N_ij = sum_{k != j} fc_ik (carbon neighbours; no hydrogen, N^H = 0),
+ *   P: cubic Hermite through pcc[n] at N = n (n = 0..4), zero slope at the
+ *      knots, pcc[4] beyond;
+ *   V(c) = eps_t (256/405 ((1 + c)/2)^5 - 1/10) = eps_t (256/405 cos^10(om/2)
+ *      - 1/10), om the dihedral of the chain (0 = cis), w = fc;
+ *   s(c) = 1 for c >= -0.995, t^2 (3 - 2t) with t = (c + 1) / 0.005 below
+ *      (the torsion fades out as a bond angle straightens, where the
+ *      dihedral is undefined).
+ * Each chain (bonds i-j, j-k, k-l, i != l, j-k its central bond) counts
+ * once.  Forces are the exact negative gradient; the virial is
+ * sum_a r_a (x) F_a with each term's atoms taken through its bond vectors. */
+static double pcc_spline(const double *pc, double N, double *dP) {
+  if (N <= 0.0) {
+    *dP = 0.0;
+    return pc[0];
+  }
+  if (N >= 4.0) {
+    *dP = 0.0;
+    return pc[4];
+  }
+  const int n = (int)floor(N);
+  const double t = N - n, dp = pc[n + 1] - pc[n];
+  *dP = dp * 6.0 * t * (1.0 - t);
+  return pc[n] + dp * t * t * (3.0 - 2.0 * t);
+}
+
+static double tors_switch(double c, double *ds) {
+  const double c0 = -1.0, c1 = -0.995;
+  if (c >= c1) {
+    *ds = 0.0;
+    return 1.0;
+  }
+  if (c <= c0) {
+    *ds = 0.0;
+    return 0.0;
+  }
+  const double t = (c - c0) / (c1 - c0);
+  *ds = 6.0 * t * (1.0 - t) / (c1 - c0);
+  return t * t * (3.0 - 2.0 * t);
+}
+
+static void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* One torsion term of the chain with bond vectors b1 = r_j - r_i,
+ * b2 = r_k - r_j, b3 = r_l - r_k, switches (w, dw/dr) of the three bonds.
+ * Returns the energy; g1, g2, g3 = dE/db1, dE/db2, dE/db3. */
+static double tors_term(const double *b1, const double *b2, const double *b3,
+                        const double *w, const double *dw, double eps_t,
+                        double *g1, double *g2, double *g3) {
+  const double r1 = sqrt(dot3(b1, b1)), r2 = sqrt(dot3(b2, b2)), r3 = sqrt(dot3(b3, b3));
+  double u1[3], u2[3], u3[3];
+  for (int a = 0; a < 3; a++) {
+    u1[a] = b1[a] / r1;
+    u2[a] = b2[a] / r2;
+    u3[a] = b3[a] / r3;
+  }
+  for (int a = 0; a < 3; a++) g1[a] = g2[a] = g3[a] = 0.0;
+  /* bond angles at j (between i and k) and at k (between j and l) */
+  const double cj = -dot3(u1, u2), ck = -dot3(u2, u3);
+  double dsj, dsk;
+  const double sj = tors_switch(cj, &dsj), sk = tors_switch(ck, &dsk);
+  if (sj == 0.0 || sk == 0.0) return 0.0;
+  double n1[3], n2[3];
+  cross3(b1, b2, n1);
+  cross3(b2, b3, n2);
+  const double m1 = sqrt(dot3(n1, n1)), m2 = sqrt(dot3(n2, n2));
+  if (m1 == 0.0 || m2 == 0.0) return 0.0;
+  double c = dot3(n1, n2) / (m1 * m2);
+  if (c > 1.0) c = 1.0;
+  if (c < -1.0) c = -1.0;
+  const double hc = 0.5 * (1.0 + c), hc4 = hc * hc * hc * hc;
+  const double V = eps_t * (256.0 / 405.0 * hc4 * hc - 0.1);
+  const double dV = eps_t * 256.0 / 405.0 * 2.5 * hc4;
+  const double W = w[0] * w[1] * w[2], S = sj * sk;
+  /* dc/dn1, dc/dn2 */
+  double A[3], B[3];
+  for (int a = 0; a < 3; a++) {
+    A[a] = (n2[a] / m2 - c * n1[a] / m1) / m1;
+    B[a] = (n1[a] / m1 - c * n2[a] / m2) / m2;
+  }
+  /* dc/db1 = b2 x A, dc/db2 = A x b1 + b3 x B, dc/db3 = B x b2 */
+  double c1[3], c2a[3], c2b[3], c3[3];
+  cross3(b2, A, c1);
+  cross3(A, b1, c2a);
+  cross3(b3, B, c2b);
+  cross3(B, b2, c3);
+  const double kc = W * S * dV;
+  /* d cj / db1 = -(u2 + cj u1) / r1, d cj / db2 = -(u1 + cj u2) / r2;
+     d ck / db2 = -(u3 + ck u2) / r2, d ck / db3 = -(u2 + ck u3) / r3 */
+  const double kj = W * V * dsj * sk, kk = W * V * sj * dsk;
+  const double kw1 = S * V * dw[0] * w[1] * w[2], kw2 = S * V * w[0] * dw[1] * w[2],
+               kw3 = S * V * w[0] * w[1] * dw[2];
+  for (int a = 0; a < 3; a++) {
+    g1[a] = kc * c1[a] - kj * (u2[a] + cj * u1[a]) / r1 + kw1 * u1[a];
+    g2[a] = kc * (c2a[a] + c2b[a]) - kj * (u1[a] + cj * u2[a]) / r2 -
+            kk * (u3[a] + ck * u2[a]) / r2 + kw2 * u2[a];
+    g3[a] = kc * c3[a] - kk * (u2[a] + ck * u3[a]) / r3 + kw3 * u3[a];
+  }
+  return W * S * V;
+}
+
+double orc_rebo_ext(double *F, double *vir9, const double *pos, const double *h,
+                    const double *hi, const int *nlc, const int *nll, int max_neigh, int na,
+                    const double *pcc, double eps_t, double *e_tors) {
+  /* the bonds of every atom (list entries shorter than DMAX_CC) */
+  int *nb = (int *)calloc((size_t)na, sizeof(int));
+  int *bj = (int *)malloc((size_t)na * max_neigh * sizeof(int));
+  double *bv = (double *)malloc((size_t)na * max_neigh * 6 * sizeof(double));
+  for (int i = 0; i < na; i++) {
+    for (int jn = 0; jn < nlc[i]; jn++) {
+      const int j = nll[i * max_neigh + jn];
+      double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1],
+                     pos[3 * j + 2] - pos[3 * i + 2]};
+      orc_mimg(d, hi, h);
+      const double r = sqrt(dot3(d, d));
+      if (r >= DMAX_CC) continue;
+      const int q = i * max_neigh + nb[i]++;
+      bj[q] = j;
+      double *e = bv + 6 * (size_t)q;
+      e[0] = d[0];
+      e[1] = d[1];
+      e[2] = d[2];
+      e[3] = r;
+      e[4] = orc_rebo_fc(r);
+      e[5] = orc_rebo_dfc(r);
+    }
+  }
+  double Ep = 0.0;
+  /* bond energies and forces: each bond i<j, both directions' bond orders */
+  for (int i = 0; i < na; i++) {
+    for (int a = 0; a < nb[i]; a++) {
+      const int j = bj[i * max_neigh + a];
+      if (j <= i) continue;
+      const double *e = bv + 6 * (size_t)(i * max_neigh + a);
+      const double r = e[3], fc = e[4], dfc = e[5];
+      const double vr = orc_rebo_vr(r), dvr = orc_rebo_dvr(r);
+      const double va = orc_rebo_va(r), dva = orc_rebo_dva(r);
+      /* bond orders of the two directions: centre c, partner p */
+      double bdir[2], Sdir[2], dPdir[2];
+      int cq[2] = {i, j}, pq[2] = {j, i};
+      for (int side = 0; side < 2; side++) {
+        const int c = cq[side], pp = pq[side];
+        const double sg = side == 0 ? 1.0 : -1.0;
+        const double ucp[3] = {sg * e[0] / r, sg * e[1] / r, sg * e[2] / r};
+        double Gs = 0.0, N = 0.0;
+        for (int t = 0; t < nb[c]; t++) {
+          if (bj[c * max_neigh + t] == pp) continue;
+          const double *et = bv + 6 * (size_t)(c * max_neigh + t);
+          double cs = (ucp[0] * et[0] + ucp[1] * et[1] + ucp[2] * et[2]) / et[3];
+          if (cs > 1.0) cs = 1.0;
+          if (cs < -1.0) cs = -1.0;
+          Gs += et[4] * orc_rebo_G(cs);
+          N += et[4];
+        }
+        double dP;
+        const double P = pcc_spline(pcc, N, &dP);
+        Sdir[side] = Gs + P;
+        dPdir[side] = dP;
+        bdir[side] = 1.0 / sqrt(1.0 + Sdir[side]);
+      }
+      const double bbar = 0.5 * (bdir[0] + bdir[1]);
+      Ep += fc * (vr - bbar * va);
+      /* pair force at fixed bond orders, on i along d = r_j - r_i */
+      const double fpair = (dfc * (vr - bbar * va) + fc * (dvr - bbar * dva)) / r;
+      for (int x = 0; x < 3; x++) {
+        F[3 * i + x] += fpair * e[x];
+        F[3 * j + x] -= fpair * e[x];
+      }
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) vir9[3 * x + y] -= e[x] * fpair * e[y];
+      /* through each direction's S: dE/dS = 1/4 fc VA b^3 */
+      for (int side = 0; side < 2; side++) {
+        const int c = cq[side], pp = pq[side];
+        const double sg = side == 0 ? 1.0 : -1.0;
+        const double dcp[3] = {sg * e[0], sg * e[1], sg * e[2]};
+        const double ucp[3] = {dcp[0] / r, dcp[1] / r, dcp[2] / r};
+        const double b = bdir[side];
+        const double dEdS = 0.25 * fc * va * b * b * b;
+        for (int t = 0; t < nb[c]; t++) {
+          if (bj[c * max_neigh + t] == pp) continue;
+          const int tt = bj[c * max_neigh + t];
+          const double *et = bv + 6 * (size_t)(c * max_neigh + t);
+          const double rt = et[3];
+          const double uct[3] = {et[0] / rt, et[1] / rt, et[2] / rt};
+          double cs = dot3(ucp, uct);
+          if (cs > 1.0) cs = 1.0;
+          if (cs < -1.0) cs = -1.0;
+          const double g = orc_rebo_G(cs), dg = orc_rebo_dG(cs);
+          double ft[3], fp[3];
+          for (int x = 0; x < 3; x++) {
+            /* dS/dd_ct and dS/dd_cp of this third atom */
+            const double st = et[5] * (g + dPdir[side]) * uct[x] +
+                              et[4] * dg * (ucp[x] - cs * uct[x]) / rt;
+            const double sp = et[4] * dg * (uct[x] - cs * ucp[x]) / r;
+            ft[x] = -dEdS * st;
+            fp[x] = -dEdS * sp;
+          }
+          for (int x = 0; x < 3; x++) {
+            F[3 * tt + x] += ft[x];
+            F[3 * pp + x] += fp[x];
+            F[3 * c + x] -= ft[x] + fp[x];
+          }
+          for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++) vir9[3 * x + y] += et[x] * ft[y] + dcp[x] * fp[y];
+        }
+      }
+    }
+  }
+  /* torsion: chains i-j-k-l around each bond j<k */
+  double Et = 0.0;
+  if (eps_t != 0.0) {
+    for (int j = 0; j < na; j++) {
+      for (int a = 0; a < nb[j]; a++) {
+        const int k = bj[j * max_neigh + a];
+        if (k <= j) continue;
+        const double *ejk = bv + 6 * (size_t)(j * max_neigh + a);
+        for (int bi = 0; bi < nb[j]; bi++) {
+          const int i = bj[j * max_neigh + bi];
+          if (i == k) continue;
+          const double *eji = bv + 6 * (size_t)(j * max_neigh + bi);
+          for (int cl = 0; cl < nb[k]; cl++) {
+            const int l = bj[k * max_neigh + cl];
+            if (l == j || l == i) continue;
+            const double *ekl = bv + 6 * (size_t)(k * max_neigh + cl);
+            const double b1[3] = {-eji[0], -eji[1], -eji[2]};
+            const double b2[3] = {ejk[0], ejk[1], ejk[2]};
+            const double b3[3] = {ekl[0], ekl[1], ekl[2]};
+            const double w[3] = {eji[4], ejk[4], ekl[4]}, dw[3] = {eji[5], ejk[5], ekl[5]};
+            double g1[3], g2[3], g3[3];
+            Et += tors_term(b1, b2, b3, w, dw, eps_t, g1, g2, g3);
+            /* r_i - r_j = -b1, r_k - r_j = b2, r_l - r_j = b2 + b3 */
+            double Fi[3], Fj[3], Fk[3], Fl[3];
+            for (int x = 0; x < 3; x++) {
+              Fi[x] = g1[x];
+              Fj[x] = g2[x] - g1[x];
+              Fk[x] = g3[x] - g2[x];
+              Fl[x] = -g3[x];
+              F[3 * i + x] += Fi[x];
+              F[3 * j + x] += Fj[x];
+              F[3 * k + x] += Fk[x];
+              F[3 * l + x] += Fl[x];
+            }
+            for (int x = 0; x < 3; x++)
+              for (int y = 0; y < 3; y++)
+                vir9[3 * x + y] += -b1[x] * Fi[y] + b2[x] * Fk[y] + (b2[x] + b3[x]) * Fl[y];
+          }
+        }
+      }
+    }
+  }
+  free(nb);
+  free(bj);
+  free(bv);
+  if (e_tors) *e_tors = Et;
+  return Ep + Et;
 }
 
 /* compute_lj AB:424-473: shifted 12-6 over the half list with r^2 <= rc^2

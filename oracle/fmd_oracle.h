@@ -71,6 +71,13 @@ double orc_compute_rebo(double *F, double *vir9, const double *pos, const double
 double orc_compute_lj(double *F, double *vir9, const double *pos, const double *h,
                       const double *hi, const int *nlc, const int *nll, int max_neigh,
                       int na, const orc_lj_params *p);
+/* Exact-gradient REBO with the REBO-II P_ij correction (pcc[5]: P_CC at
+ * N = 0..4) and the AIREBO torsion term (well depth eps_t); no reference
+ * counterpart.  Adds into F / vir9; returns the total energy, the torsion
+ * part in *e_tors. */
+double orc_rebo_ext(double *F, double *vir9, const double *pos, const double *h,
+                    const double *hi, const int *nlc, const int *nll, int max_neigh, int na,
+                    const double *pcc, double eps_t, double *e_tors);
 void orc_airebo_forces(double *F, double *vir9, const double *pos, const double *h,
                        const double *hi, const int *nlc_r, const int *nll_r, int max_r,
                        const int *nlc_l, const int *nll_l, int max_l, int na,
