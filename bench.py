@@ -120,6 +120,9 @@ def parse():
                          "3000 rigid; a larger value keeps the scan of every molecule)")
     ap.add_argument("--rebo", choices=["compat", "physical"], default="physical",
                     help="REBO force mode of the timed run")
+    ap.add_argument("--rebo-ext", action="store_true",
+                    help="airebo workload: add the REBO-II P_ij spline and the AIREBO torsion "
+                         "term (AIREBO carbon values; no reference counterpart, no CPU leg)")
     ap.add_argument("--sg-iters", type=int, default=30000,
                     help="surrogate workload: training iterations")
     return ap.parse_args()
@@ -356,12 +359,14 @@ def airebo_cpu_leg(s, nc):
     return cpu, out
 
 
-def airebo_engine(s, mode, params, cell_min=None):
+def airebo_engine(s, mode, params, cell_min=None, ext=False):
     from fuller_md_en_amd import airebo
     eng = airebo.AtomicEngine(s["mol_id"], s["mass"], dt=0.5, mode=mode,
                               al_skin=params["al_skin"], cell_min_molecules=cell_min,
                               rebo_slots=params.get("rebo_slots"),
-                              rebo_threads=params.get("rebo_threads"))
+                              rebo_threads=params.get("rebo_threads"),
+                              pcc=airebo.AIREBO_PCC if ext else None,
+                              torsion_eps=airebo.AIREBO_TORSION_EPS if ext else 0.0)
     eng.set_graph(bool(params["graph"]))
     eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"],
                   npt=airebo.make_npt(T_K, 0.0, len(s["pos"])))
@@ -393,8 +398,10 @@ def main_airebo(args):
     s = system.airebo_system(nc, "C60", T=T_K, seed=SEED)
     na, nm = len(s["pos"]), s["n_mol"]
     mode = airebo.REBO_PHYSICAL if args.rebo == "physical" else airebo.REBO_COMPAT
+    if args.rebo_ext and mode != airebo.REBO_PHYSICAL:
+        raise SystemExit("--rebo-ext needs --rebo physical")
     params, tuned = airebo_params(args, s, nc, mode)
-    eng = airebo_engine(s, mode, params, args.cell_min)
+    eng = airebo_engine(s, mode, params, args.cell_min, ext=args.rebo_ext)
     dt, th, ph, rebuild = timed_run(eng, args.warmup, args.steps, 20)
     faults, rebuilds = eng.counters()[:2]
     eng.close()
@@ -403,15 +410,18 @@ def main_airebo(args):
     lj = kernel_rate("k_at_lj_al", FLOP_PER_ATOM_LJ * na, lj_ms, PEAK_FP32_TFLOPS,
                      traffic=AIREBO_LJ_TRAFFIC)
     lj["launches"] = ph["lj"][1]
-    rebo = kernel_rate("k_rebo_mol", FLOP_PER_ATOM_REBO * na, rebo_ms, PEAK_FP32_TFLOPS,
-                       traffic=AIREBO_REBO_TRAFFIC)
+    rebo = kernel_rate("k_rebo_mol + k_rebo_tors" if args.rebo_ext else "k_rebo_mol",
+                       FLOP_PER_ATOM_REBO * na, rebo_ms, PEAK_FP32_TFLOPS,
+                       traffic=None if args.rebo_ext else AIREBO_REBO_TRAFFIC)
     force = kernel_rate("force phase (k_rebo_mol + k_at_lj_al)",
                         (FLOP_PER_ATOM_LJ + FLOP_PER_ATOM_REBO) * na, ph["force"][0],
                         PEAK_FP32_TFLOPS)
     last = th[args.steps - 1]
+    ext = (" with the REBO-II P_ij spline and the AIREBO torsion term (no reference "
+           "counterpart)") if args.rebo_ext else ""
     res = line({"workload": f"AIREBO C60 FCC {nc}x{nc}x{nc} ({nm} mol, {na} atoms), REBO "
-                            f"{args.rebo} + LJ (400-entry half-list semantics), 300 K, 0 GPa, "
-                            "dt=0.5 fs, NPT, lists every 20 steps", "atoms": na},
+                            f"{args.rebo}{ext} + LJ (400-entry half-list semantics), 300 K, "
+                            "0 GPa, dt=0.5 fs, NPT, lists every 20 steps", "atoms": na},
                value, args.steps, args.warmup, dt, "fp32 pair/bond-order, fp64 integrator",
                lj, kernels=[rebo, force], rebuild=rebuild,
                value_amortized=na / (rebuild["ms_per_step_amortized"] * 1e-3),
@@ -419,7 +429,7 @@ def main_airebo(args):
                tuned=tuned and {k: tuned[k] for k in ("seconds", "default_seconds", "tried")},
                final={"T": last.T, "P": last.P, "E_rebo_per_mol": last.Ep_rebo / nm,
                       "E_lj_per_mol": last.Ep_lj / nm})
-    if not args.no_cpu:
+    if not args.no_cpu and not args.rebo_ext:  # the reference has neither term
         cpu, err = airebo_cpu_leg(s, nc)
         res["cpu_baseline"] = cpu
         res["fp64_error"] = err

@@ -60,7 +60,8 @@ class AtomicConfig(C.Structure):
                 ("rebo_max_neigh", C.c_int), ("rebo_mode", C.c_int), ("nlup", C.c_int),
                 ("dt", C.c_double), ("clamp_xi", C.c_double), ("clamp_vg", C.c_double),
                 ("al_skin", C.c_double), ("cell_min_molecules", C.c_int),
-                ("rebo_lds_slots", C.c_int), ("rebo_threads", C.c_int)]
+                ("rebo_lds_slots", C.c_int), ("rebo_threads", C.c_int),
+                ("rebo_pcc", C.c_double * 5), ("torsion_eps", C.c_double)]
 
 
 class StepCtrl(C.Structure):

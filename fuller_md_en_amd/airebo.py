@@ -19,6 +19,11 @@ REBO_COMPAT = 0
 REBO_PHYSICAL = 1
 MAX_REBO_NEIGH = 12  # AB:126
 MAX_LJ_NEIGH = 400  # AB:127
+# AIREBO carbon values of the two REBO-II / AIREBO terms the reference does
+# not have (engines, REBO_PHYSICAL only): P_CC(N, N_H = 0) at N = 0..4 and the
+# torsion well depth (eV)
+AIREBO_PCC = (0.0, 0.0, -0.0276030, 0.0, 0.0)
+AIREBO_TORSION_EPS = 0.3079
 
 
 def _c(a, dt=np.float64):
@@ -79,12 +84,24 @@ def step_npt(pos, vel, F, vir9, h, hi, mass, dt, npt, lists, mol_id, mode=REBO_C
     return tuple(out)
 
 
+def _extension_terms(cfg, pcc, torsion_eps):
+    """The REBO-II P_ij knots (5 values, P_CC at N = 0..4) and the AIREBO
+    torsion well depth into an engine configuration (None / 0: off)."""
+    if pcc is not None:
+        if len(pcc) != 5:
+            raise ValueError("pcc: P_CC at N = 0..4 (5 values)")
+        for k in range(5):
+            cfg.rebo_pcc[k] = float(pcc[k])
+    cfg.torsion_eps = float(torsion_eps)
+
+
 class AtomicEngine:
     """HBM-resident AIREBO NPT engine (fmd_atomic_create / fmd_engine_*)."""
 
     def __init__(self, mol_id, mass, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH,
                  nlup=20, lj=None, rebo_rlist=None, clamp_vg=None, al_skin=None,
-                 cell_min_molecules=None, rebo_slots=None, rebo_threads=None):
+                 cell_min_molecules=None, rebo_slots=None, rebo_threads=None, pcc=None,
+                 torsion_eps=0.0):
         self._mol = _c(mol_id, np.int32)
         self._mass = _c(mass)
         cfg = N.AtomicConfig()
@@ -107,6 +124,7 @@ class AtomicEngine:
             cfg.rebo_lds_slots = int(rebo_slots)
         if rebo_threads is not None:
             cfg.rebo_threads = int(rebo_threads)
+        _extension_terms(cfg, pcc, torsion_eps)
         self.cfg = cfg
         self.n = len(self._mol)
         h = C.c_void_p()
@@ -188,7 +206,8 @@ class ReplicaBatch:
     HIP graphs.  replicas: list of (mol_id, mass) per replica."""
 
     def __init__(self, replicas, dt=0.5, mode=REBO_COMPAT, lj_max_neigh=MAX_LJ_NEIGH, nlup=20,
-                 lj=None, graph=True, al_skin=None, rebo_slots=None, rebo_threads=None):
+                 lj=None, graph=True, al_skin=None, rebo_slots=None, rebo_threads=None,
+                 pcc=None, torsion_eps=0.0):
         self._keep = []
         cfgs = (N.AtomicConfig * len(replicas))()
         for r, (mol_id, mass) in enumerate(replicas):
@@ -208,6 +227,7 @@ class ReplicaBatch:
                 cfgs[r].rebo_lds_slots = int(rebo_slots)
             if rebo_threads is not None:
                 cfgs[r].rebo_threads = int(rebo_threads)
+            _extension_terms(cfgs[r], pcc, torsion_eps)
         self.sizes = [len(mol_id) for mol_id, _ in replicas]
         self.R = len(replicas)
         h = C.c_void_p()

@@ -108,6 +108,11 @@ struct AtomicParams {
   int al_cap;      // per-atom list capacity per wavefront segment (entries)
   int rebo_slots;  // REBO kernel: bond entries per atom in LDS
   int rebo_nt;     // REBO kernel: workgroup size (192 used for apad <= 192)
+  // REBO-II / AIREBO terms the reference does not have (exact-gradient REBO
+  // only; zero: off): the P_ij spline of the bond order, and the torsion
+  // term's well depth (eV, k_rebo_tors)
+  rebo::ReboPcc pcc;
+  double tors_eps;
   int cell_total;  // molecule-list cells over all replicas
   int mol_cells;   // 1: some replica builds its molecule list from cells
   int cell_min;    // the molecules a replica needs for that

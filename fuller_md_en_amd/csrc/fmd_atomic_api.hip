@@ -249,6 +249,12 @@ void fill_params(AtomicParams &p, const fmd_atomic_config &c, int Na, int Nm, in
   p.al_cap = 384;  // per wavefront segment: about 3.4 times a C60 crystal's share
   p.rebo_slots = c.rebo_lds_slots > 0 ? c.rebo_lds_slots : 4;
   p.rebo_nt = c.rebo_threads == 256 ? 256 : 192;
+  p.pcc.on = 0;
+  for (int k = 0; k < 5; k++) {
+    p.pcc.v[k] = (float)c.rebo_pcc[k];
+    p.pcc.on |= c.rebo_pcc[k] != 0.0;
+  }
+  p.tors_eps = c.torsion_eps;
   p.dt = c.dt;
   p.clamp_xi = c.clamp_xi;
   p.clamp_vg = c.clamp_vg;
@@ -542,6 +548,12 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
     return set_error(FMD_ERR_ARG, "fmd_batch_create: rebo_lds_slots must lie in [3, 8]");
   if (c[0].rebo_threads != 0 && c[0].rebo_threads != 192 && c[0].rebo_threads != 256)
     return set_error(FMD_ERR_ARG, "fmd_batch_create: rebo_threads must be 192 or 256");
+  bool ext = c[0].torsion_eps != 0.0;
+  for (int k = 0; k < 5; k++) ext |= c[0].rebo_pcc[k] != 0.0;
+  if (ext && (c[0].rebo_mode != FMD_REBO_PHYSICAL || ff != kFfAirebo))
+    return set_error(FMD_ERR_ARG,
+                     "fmd_batch_create: rebo_pcc and torsion_eps need FMD_REBO_PHYSICAL "
+                     "(the reference's REBO has neither term)");
   for (int r = 0; r < R; r++) {
     if (!c[r].mol_id || !c[r].mass || c[r].n_atoms < 1)
       return set_error(FMD_ERR_ARG, "fmd_batch_create: replica %d: bad configuration", r);
@@ -551,7 +563,8 @@ int fmd::atomic_create_ff(const fmd_atomic_config *c, int R, int ff, fmd_engine 
         a.rebo_rlist != b.rebo_rlist || a.rebo_max_neigh != b.rebo_max_neigh ||
         a.rebo_mode != b.rebo_mode || a.nlup != b.nlup || a.dt != b.dt ||
         a.clamp_xi != b.clamp_xi || a.clamp_vg != b.clamp_vg || a.al_skin != b.al_skin ||
-        a.rebo_lds_slots != b.rebo_lds_slots || a.rebo_threads != b.rebo_threads)
+        a.rebo_lds_slots != b.rebo_lds_slots || a.rebo_threads != b.rebo_threads ||
+        a.torsion_eps != b.torsion_eps || memcmp(a.rebo_pcc, b.rebo_pcc, sizeof a.rebo_pcc))
       return set_error(FMD_ERR_ARG,
                        "fmd_batch_create: replica %d: potential, lists, dt, nlup and clamps "
                        "must equal replica 0's",

@@ -80,6 +80,45 @@ FMD_HD R dG(R x) {
   return R(kGa0 * 2.0) * c2 * hh / (dn * dn);
 }
 
+// The REBO-II P_ij correction of a carbon bond order, P_CC(N, N_H = 0): a
+// cubic Hermite through v[n] at N = n (n = 0..4) with zero slope at every
+// knot, v[4] beyond (Brenner et al. 2002; the AIREBO carbon values have only
+// v[2] nonzero).  No counterpart in the reference, whose REBO has no P term;
+// all zero (the default) leaves its bond order unchanged.
+struct ReboPcc {
+  float v[5];
+  int on;  // any v nonzero (0: no P term, the reference's bond order)
+};
+template <class R>
+FMD_HD R pcc(const ReboPcc &pc, R N, R &dP) {
+#pragma clang fp contract(off)
+  if (!(N > R(0))) {
+    dP = R(0);
+    return R(pc.v[0]);
+  }
+  if (N >= R(4)) {
+    dP = R(0);
+    return R(pc.v[4]);
+  }
+  R lo = R(pc.v[3]), hi = R(pc.v[4]), n0 = R(3);
+  if (N < R(1)) {
+    lo = R(pc.v[0]);
+    hi = R(pc.v[1]);
+    n0 = R(0);
+  } else if (N < R(2)) {
+    lo = R(pc.v[1]);
+    hi = R(pc.v[2]);
+    n0 = R(1);
+  } else if (N < R(3)) {
+    lo = R(pc.v[2]);
+    hi = R(pc.v[3]);
+    n0 = R(2);
+  }
+  const R t = N - n0, dv = hi - lo;
+  dP = dv * R(6) * t * (R(1) - t);
+  return lo + dv * t * t * (R(3) - R(2) * t);
+}
+
 template <class R>
 FMD_HD R clamp1(R x) {
   return x < R(-1) ? R(-1) : (x > R(1) ? R(1) : x);
@@ -88,17 +127,19 @@ FMD_HD R clamp1(R x) {
 // Angular term of the bond order of bond c->p at centre c with third atom t
 // (vectors from c).  coeff = 1/2 fc(r_cp) VA(r_cp) db/dGs.  Physical forces:
 // ft on t, fp on p, the centre gets -(ft + fp).  The reference's force on t
-// is -ft, with no fp term (AB:360-419).
+// is -ft, with no fp term (AB:360-419).  dP: dP/dN of the bond's P_ij term
+// (N sums fc over c's other bonds, so t's switch carries it too; 0 without
+// the term).
 template <class R>
 FMD_HD void angular(const R *rcp, R r_cp, const R *rct, R r_ct, R fc_ct, R dfc_ct, R coeff,
-                    R *ft, R *fp) {
+                    R dP, R *ft, R *fp) {
 #pragma clang fp contract(off)
   R inv_ct = R(1) / r_ct, inv_cp = R(1) / r_cp;
   R uct[3] = {rct[0] * inv_ct, rct[1] * inv_ct, rct[2] * inv_ct};
   R ucp[3] = {rcp[0] * inv_cp, rcp[1] * inv_cp, rcp[2] * inv_cp};
   R cs = clamp1((rcp[0] * rct[0] + rcp[1] * rct[1] + rcp[2] * rct[2]) * (inv_cp * inv_ct));
   R g = G(cs), dg = dG(cs);
-  R a = coeff * dfc_ct * g, b = coeff * fc_ct * dg;
+  R a = coeff * dfc_ct * (g + dP), b = coeff * fc_ct * dg;
   for (int k = 0; k < 3; k++) {
     ft[k] = a * uct[k] + b * (ucp[k] - cs * uct[k]) * inv_ct;
     fp[k] = b * (uct[k] - cs * ucp[k]) * inv_cp;

@@ -27,9 +27,9 @@ enum : int { kBondPair = 1, kBondThird = 2, kBondAngular = 4 };
 // Per directed-entry storage (slot of entry jn of atom i, see SlotStride /
 // SlotPrefix):
 //   bd[4*slot]  dx dy dz r      FP32 (r_j - r_i, minimum image): angular terms
-//   ba[4*slot]  fc dfc coeff -  FP32 (coeff = 1/2 fc VA db/dGs)
+//   ba[4*slot]  fc dfc coeff dP FP32 (coeff = 1/2 fc VA db/dGs, dP = dP_ij/dN)
 //   bp[4*slot]  P = bond_terms  FP64 pair quantities (f VR, f VA, C, D)
-//   bq[4*slot]  r fc dfc b      FP64 (b = (1 + Gs)^-1/2)
+//   bq[4*slot]  r fc dfc b      FP64 (b = (1 + Gs + P)^-1/2)
 //   bflag[slot] kBond* bits,  brev[slot] slot of i in j's list
 // The pair force is the difference of the repulsive and attractive
 // derivatives, each ~77 eV/A at the C60 bond length against a ~1 eV/A
@@ -79,12 +79,14 @@ FMD_HD int bond_terms(double rd, double &fd, double &dfd, double4 &P) {
 
 // Bond-order sum Gs_ij = sum over k in N(i)\{j} of fc(r_ik) G(cos theta_ijk)
 // (AB:292-305) from FP32 bond vectors: bv(kn) gives (dx, dy, dz, r) of i's
-// entry kn.
+// entry kn.  Also the P_ij term P(N_ij), N_ij = sum of the same fc(r_ik), and
+// its slope dP (both 0 when pc is all zero, the reference's bond order).
 template <class R, class BV>
-FMD_HD R bond_order_sum(int jn, int nni, R ux, R uy, R uz, R r, BV bv) {
+FMD_HD R bond_order_sum(int jn, int nni, R ux, R uy, R uz, R r, BV bv, const rebo::ReboPcc &pc,
+                        R &P, R &dP) {
 #pragma clang fp contract(off)  // the same rounding wherever it is inlined
   using namespace rebo;
-  R Gs = 0;
+  R Gs = 0, N = 0;
   for (int kn = 0; kn < nni; kn++) {
     if (kn == jn) continue;
     const float4 e = bv(kn);
@@ -94,6 +96,12 @@ FMD_HD R bond_order_sum(int jn, int nni, R ux, R uy, R uz, R r, BV bv) {
     if (fk < R(1e-15)) continue;
     R cs = clamp1((ux * (R)e.x + uy * (R)e.y + uz * (R)e.z) / (r * rk));
     Gs += fk * G(cs);
+    N += fk;
+  }
+  if (pc.on) {
+    P = rebo::pcc(pc, N, dP);
+  } else {
+    P = dP = R(0);
   }
   return Gs;
 }
@@ -124,7 +132,7 @@ FMD_HD void triple_virial(const float *dt, const float *ft, const float *dp, con
 // kn (minimum image r_k - r_i, FP64), so every bond vector of i is taken once.
 template <class R, class Slot>
 FMD_HD void rebo_pass1(int i, int jn, const double4 *vec, const int *nlc, const int *nll,
-                       Slot slot, ReboBonds B) {
+                       Slot slot, ReboBonds B, const rebo::ReboPcc &pc) {
   using namespace rebo;
   const int s = slot(i, jn);
   const int j = nll[s];
@@ -146,17 +154,21 @@ FMD_HD void rebo_pass1(int i, int jn, const double4 *vec, const int *nlc, const 
     B.bflag[s] = 0;
     return;
   }
-  const R Gs = bond_order_sum<R>(jn, nni, (R)dx, (R)dy, (R)dz, r, [&](int kn) {
-    return make_float4((float)vec[kn].x, (float)vec[kn].y, (float)vec[kn].z,
-                       (float)vec[kn].w);
-  });
-  const double b = rsq(1.0 + (double)Gs);
+  R Pn, dPn;
+  const R Gs = bond_order_sum<R>(
+      jn, nni, (R)dx, (R)dy, (R)dz, r,
+      [&](int kn) {
+        return make_float4((float)vec[kn].x, (float)vec[kn].y, (float)vec[kn].z,
+                           (float)vec[kn].w);
+      },
+      pc, Pn, dPn);
+  const double b = rsq((1.0 + (double)Gs) + (double)Pn);
   if ((flag & kBondPair) && Gs > R(1e-20) && P.y > 0.0) flag |= kBondAngular;
   B.bflag[s] = flag;
   B.ba[4 * s] = (float)fd;
   B.ba[4 * s + 1] = (float)dfd;
   B.ba[4 * s + 2] = angular_coeff(P.y, b);
-  B.ba[4 * s + 3] = 0.f;
+  B.ba[4 * s + 3] = (float)dPn;
   B.bp[4 * s] = P.x;
   B.bp[4 * s + 1] = P.y;
   B.bp[4 * s + 2] = P.z;
@@ -208,8 +220,8 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, Slot
         if (!(B.bflag[t] & kBondThird)) continue;
         const R dt[3] = {B.bd[4 * t], B.bd[4 * t + 1], B.bd[4 * t + 2]};
         R ft[3], fp[3];
-        angular(d, r, dt, (R)B.bd[4 * t + 3], (R)B.ba[4 * t], (R)B.ba[4 * t + 1], coeff, ft,
-                fp);
+        angular(d, r, dt, (R)B.bd[4 * t + 3], (R)B.ba[4 * t], (R)B.ba[4 * t + 1], coeff,
+                (R)B.ba[4 * s + 3], ft, fp);
         if (MODE == kReboCompat) {
           for (int k = 0; k < 3; k++) F[k] += (double)ft[k];
         } else {
@@ -235,7 +247,8 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, Slot
       if (sp == sx || !(B.bflag[sp] & kBondAngular)) continue;
       const R dcp[3] = {B.bd[4 * sp], B.bd[4 * sp + 1], B.bd[4 * sp + 2]};
       R ft[3], fp[3];
-      angular(dcp, (R)B.bd[4 * sp + 3], dcx, rcx, fcx, dfcx, (R)B.ba[4 * sp + 2], ft, fp);
+      angular(dcp, (R)B.bd[4 * sp + 3], dcx, rcx, fcx, dfcx, (R)B.ba[4 * sp + 2],
+              (R)B.ba[4 * sp + 3], ft, fp);
       if (MODE == kReboCompat)
         for (int k = 0; k < 3; k++) F[k] -= (double)ft[k];
       else
@@ -250,7 +263,7 @@ FMD_HD void rebo_pass2_entry(int x, int yn, const int *nlc, const int *nll, Slot
       const R dct[3] = {B.bd[4 * st], B.bd[4 * st + 1], B.bd[4 * st + 2]};
       R ft[3], fp[3];
       angular(dcx, rcx, dct, (R)B.bd[4 * st + 3], (R)B.ba[4 * st], (R)B.ba[4 * st + 1], coeff,
-              ft, fp);
+              (R)B.ba[4 * sx + 3], ft, fp);
       for (int k = 0; k < 3; k++) F[k] += (double)fp[k];
     }
   }

@@ -330,7 +330,24 @@ typedef struct {
                             path, with the same forces */
   int rebo_threads;      /* REBO kernel: workgroup size, 192 (molecules of up to
                             192 atoms; else 256) or 256 */
+  /* REBO-II / AIREBO terms the reference's REBO does not have (no reference
+     counterpart; FMD_REBO_PHYSICAL only; all zero, the default, is the
+     reference's potential):
+       rebo_pcc[n]  P_CC(N, N_H = 0) at N = n = 0..4, the bond-order correction
+                    b_ij = (1 + Gs_ij + P(N_ij))^-1/2, N_ij = sum over i's
+                    other bonds of fc; a cubic Hermite with zero slope at the
+                    knots (FMD_AIREBO_PCC2 at N = 2 for AIREBO carbon);
+       torsion_eps  well depth of the AIREBO torsion term
+                    w w w eps (256/405 cos^10(omega/2) - 1/10) over bond chains
+                    (FMD_AIREBO_TORSION_EPS); its energy is part of Ep_rebo. */
+  double rebo_pcc[5];
+  double torsion_eps;
 } fmd_atomic_config;
+
+/* AIREBO carbon values of the two terms above (Stuart, Tutein & Harrison,
+ * J. Chem. Phys. 112, 6472 (2000)). */
+#define FMD_AIREBO_PCC2 (-0.0276030)
+#define FMD_AIREBO_TORSION_EPS 0.3079
 
 /* Defaults of the reference AIREBO program. */
 void fmd_atomic_config_default(fmd_atomic_config *c);
