@@ -1289,85 +1289,95 @@ __global__ __launch_bounds__(NT) void k_rebo_mol(AtomicDev d, AtomicParams p, HH
 //   V(c) = eps (256/405 ((1 + c)/2)^5 - 1/10) = eps (256/405 cos^10(om/2) - 1/10),
 // om the chain's dihedral (0 = cis), w the REBO switch fc, s a cubic fade to
 // 0 as a bond angle straightens (cos th from -0.995 to -1), where the
-// dihedral is undefined.  FP64.
+// dihedral is undefined.  Chains in FP32, their sums in FP64.
 namespace tors {
-constexpr double kC0 = -1.0, kC1 = -0.995;
+constexpr float kC0 = -1.0f, kC1 = -0.995f;
 
-__device__ __forceinline__ double angle_switch(double c, double &ds) {
+__device__ __forceinline__ float angle_switch(float c, float &ds) {
   if (c >= kC1) {
-    ds = 0.0;
-    return 1.0;
+    ds = 0.f;
+    return 1.f;
   }
   if (c <= kC0) {
-    ds = 0.0;
-    return 0.0;
+    ds = 0.f;
+    return 0.f;
   }
-  const double t = (c - kC0) / (kC1 - kC0);
-  ds = 6.0 * t * (1.0 - t) / (kC1 - kC0);
-  return t * t * (3.0 - 2.0 * t);
+  constexpr float kInv = 1.0f / (kC1 - kC0);
+  const float t = (c - kC0) * kInv;
+  ds = 6.f * t * (1.f - t) * kInv;
+  return t * t * (3.f - 2.f * t);
 }
 
-__device__ __forceinline__ void cross(const double *a, const double *b, double *c) {
+__device__ __forceinline__ void cross(const float *a, const float *b, float *c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
-__device__ __forceinline__ double dot(const double *a, const double *b) {
+__device__ __forceinline__ float dot(const float *a, const float *b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// The chain with bond vectors b1 = r_j - r_i, b2 = r_k - r_j, b3 = r_l - r_k,
-// lengths r, switches w and their slopes dw: its energy, and into f the force
-// on i (END) or on j (!END), -dE/dr_i = dE/db1 or -dE/dr_j = dE/db2 - dE/db1.
+// Bond x -> y of the table: vector r_y - r_x (minimum image), 1 / length,
+// switch and its slope.
+struct Bond {
+  float d[3], ir, w, dw;
+};
+
+// The chain with bonds b1 = r_j - r_i, b2 = r_k - r_j, b3 = r_l - r_k: its
+// energy, and into f the force on i (END) or on j (!END),
+// -dE/dr_i = dE/db1 or -dE/dr_j = dE/db2 - dE/db1.  FP32 (the energy and
+// forces are summed in FP64).
 template <bool END>
-__device__ __forceinline__ double term(const double *b1, const double *b2, const double *b3,
-                                       const double *r, const double *w, const double *dw,
-                                       double eps, double *f) {
-  double u1[3], u2[3], u3[3];
+__device__ __forceinline__ float term(const Bond &e1, const Bond &e2, const Bond &e3, float eps,
+                                      float *f) {
+  const float *b1 = e1.d, *b2 = e2.d, *b3 = e3.d;
+  float u1[3], u2[3], u3[3];
   for (int a = 0; a < 3; a++) {
-    u1[a] = b1[a] / r[0];
-    u2[a] = b2[a] / r[1];
-    u3[a] = b3[a] / r[2];
-    f[a] = 0.0;
+    u1[a] = b1[a] * e1.ir;
+    u2[a] = b2[a] * e2.ir;
+    u3[a] = b3[a] * e3.ir;
+    f[a] = 0.f;
   }
-  const double cj = -dot(u1, u2), ck = -dot(u2, u3);
-  double dsj, dsk;
-  const double sj = angle_switch(cj, dsj), sk = angle_switch(ck, dsk);
-  if (sj == 0.0 || sk == 0.0) return 0.0;
-  double n1[3], n2[3];
+  const float cj = -dot(u1, u2), ck = -dot(u2, u3);
+  float dsj, dsk;
+  const float sj = angle_switch(cj, dsj), sk = angle_switch(ck, dsk);
+  if (sj == 0.f || sk == 0.f) return 0.f;
+  float n1[3], n2[3];
   cross(b1, b2, n1);
   cross(b2, b3, n2);
-  const double m1 = sqrt(dot(n1, n1)), m2 = sqrt(dot(n2, n2));
-  if (m1 == 0.0 || m2 == 0.0) return 0.0;
-  const double c = fmin(1.0, fmax(-1.0, dot(n1, n2) / (m1 * m2)));
-  const double hc = 0.5 * (1.0 + c), hc4 = (hc * hc) * (hc * hc);
-  const double V = eps * (256.0 / 405.0 * hc4 * hc - 0.1);
-  const double dV = eps * (256.0 / 405.0 * 2.5) * hc4;
-  const double Wt = w[0] * w[1] * w[2], S = sj * sk;
-  double A[3], B[3];
+  const float q1 = dot(n1, n1), q2 = dot(n2, n2);
+  if (q1 == 0.f || q2 == 0.f) return 0.f;
+  // 1/|n| by the hardware reciprocal square root (1 ulp)
+  const float im1 = __builtin_amdgcn_rsqf(q1), im2 = __builtin_amdgcn_rsqf(q2);
+  const float c = fminf(1.f, fmaxf(-1.f, dot(n1, n2) * (im1 * im2)));
+  const float hc = 0.5f * (1.f + c), hc4 = (hc * hc) * (hc * hc);
+  const float V = eps * (256.f / 405.f * hc4 * hc - 0.1f);
+  const float dV = eps * (256.f / 405.f * 2.5f) * hc4;
+  const float Wt = e1.w * e2.w * e3.w, S = sj * sk;
+  float A[3], B[3];
   for (int a = 0; a < 3; a++) {
-    A[a] = (n2[a] / m2 - c * n1[a] / m1) / m1;  // dc/dn1
-    B[a] = (n1[a] / m1 - c * n2[a] / m2) / m2;  // dc/dn2
+    A[a] = (n2[a] * im2 - c * n1[a] * im1) * im1;  // dc/dn1
+    B[a] = (n1[a] * im1 - c * n2[a] * im2) * im2;  // dc/dn2
   }
-  const double kc = Wt * S * dV, kj = Wt * V * dsj * sk, kk = Wt * V * sj * dsk;
+  const float kc = Wt * S * dV, kj = Wt * V * dsj * sk, kk = Wt * V * sj * dsk;
   // dE/db1 = kc (b2 x A) - kj (u2 + cj u1) / r1 + S V dw1 w2 w3 u1
-  double g1[3], x1[3];
+  float g1[3], x1[3];
   cross(b2, A, x1);
-  const double kw1 = S * V * dw[0] * w[1] * w[2];
+  const float kw1 = S * V * e1.dw * e2.w * e3.w;
   for (int a = 0; a < 3; a++)
-    g1[a] = kc * x1[a] - kj * (u2[a] + cj * u1[a]) / r[0] + kw1 * u1[a];
+    g1[a] = kc * x1[a] - kj * (u2[a] + cj * u1[a]) * e1.ir + kw1 * u1[a];
   if (END) {
     for (int a = 0; a < 3; a++) f[a] = g1[a];
   } else {
     // dE/db2 = kc (A x b1 + b3 x B) - kj (u1 + cj u2) / r2 - kk (u3 + ck u2) / r2
     //          + S V w1 dw2 w3 u2
-    double xa[3], xb[3];
+    float xa[3], xb[3];
     cross(A, b1, xa);
     cross(b3, B, xb);
-    const double kw2 = S * V * w[0] * dw[1] * w[2];
+    const float kw2 = S * V * e1.w * e2.dw * e3.w;
     for (int a = 0; a < 3; a++) {
-      const double g2 = kc * (xa[a] + xb[a]) - kj * (u1[a] + cj * u2[a]) / r[1] -
-                        kk * (u3[a] + ck * u2[a]) / r[1] + kw2 * u2[a];
+      const float g2 = kc * (xa[a] + xb[a]) - kj * (u1[a] + cj * u2[a]) * e2.ir -
+                       kk * (u3[a] + ck * u2[a]) * e2.ir + kw2 * u2[a];
       f[a] = g2 - g1[a];
     }
   }
@@ -1377,26 +1387,30 @@ __device__ __forceinline__ double term(const double *b1, const double *b2, const
 
 constexpr int kTorB = 4;  // bonds per atom the torsion kernel holds (carbon cages: 3)
 
-// One workgroup per molecule, one thread per atom x: every chain holding x,
-// with x at an end (x-j-k-l) or in the middle (i-x-k-l), evaluated by the
-// thread itself (so each chain four times, once per atom: no atomics, a fixed
-// order); its force on x is added to Frebo, a quarter of its energy and
-// (r_x - r_ref) (x) f_x (r_ref: the central bond's lower-index atom) to the
-// molecule's REBO partials.
-template <bool TRACE, int NT>
-__global__ __launch_bounds__(NT) void k_rebo_tors(AtomicDev d, AtomicParams p, HHRef hh) {
-  __shared__ int nbk[NT], bjk[NT][kTorB];
-  __shared__ double4 bvk[NT][kTorB];  // r_j - r_x (minimum image) and its length
-  __shared__ double2 bwk[NT][kTorB];  // fc and its slope
-  constexpr int NW = NT / 64;
+// One workgroup of 256 threads per molecule, TPA = 256 / NA threads per atom
+// (NA = the tile width rounded up to 64): the chains that hold atom x, with x
+// at an end (x-j-k-l) or in the middle (i-x-k-l), in a fixed order, dealt
+// round robin to x's TPA threads, each evaluated by its thread (so a chain
+// is evaluated once for each of its four atoms and nothing is accumulated
+// across atoms); x's force is the sum of its threads' in thread order, added
+// to the REBO forces.  A quarter of each chain's energy and
+// (r_x - r_ref) (x) f_x (r_ref: the central bond's lower-index atom) go to
+// the molecule's REBO partials.
+template <bool TRACE, int NA>
+__global__ __launch_bounds__(256) void k_rebo_tors(AtomicDev d, AtomicParams p, HHRef hh) {
+  constexpr int TPA = 256 / NA;
+  __shared__ int nbk[NA], bjk[NA][kTorB];
+  __shared__ tors::Bond bk[NA][kTorB];
+  __shared__ double fq[TPA][NA][3];
   const int per = (p.Nm + kXcd - 1) / kXcd;
   const int m = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
   if (m >= p.Nm) return;
   const int t = threadIdx.x;
+  const int x = t % NA, q = t / NA;  // atom, part
   const double *h = hh_of(hh, d.mol_rep[m]), *hi = h + 9;
   const int s0 = d.mol_start[m], n = d.mol_start[m + 1] - s0;
-  if (t < n) {
-    const int i = s0 + t;
+  if (q == 0 && x < n) {
+    const int i = s0 + x;
     int c = 0;
     for (int jn = 0; jn < d.nlc_r[i]; jn++) {
       const int j = d.nll_r[(size_t)i * p.maxr + jn];
@@ -1409,54 +1423,50 @@ __global__ __launch_bounds__(NT) void k_rebo_tors(AtomicDev d, AtomicParams p, H
         atomicAdd(&d.flags[0], 1);  // more bonds than held: counted as an overflow
         break;
       }
-      bjk[t][c] = j - s0;  // the REBO list never leaves the molecule
-      bvk[t][c] = make_double4(dx, dy, dz, r);
-      bwk[t][c] = make_double2(rebo::fc(r), rebo::dfc(r));
+      bjk[x][c] = j - s0;  // the REBO list never leaves the molecule
+      tors::Bond &b = bk[x][c];
+      b.d[0] = (float)dx;
+      b.d[1] = (float)dy;
+      b.d[2] = (float)dz;
+      b.ir = (float)(1.0 / r);
+      b.w = (float)rebo::fc(r);
+      b.dw = (float)rebo::dfc(r);
       c++;
     }
-    nbk[t] = c;
+    nbk[x] = c;
   }
   __syncthreads();
   double F[3] = {0.0, 0.0, 0.0}, E = 0.0, W[TRACE ? 1 : 9];
   for (int k = 0; k < (TRACE ? 1 : 9); k++) W[k] = 0.0;
-  const double eps = p.tors_eps;
-  auto add = [&](double e4, const double *f, const double *rel) {
-    E += 0.25 * e4;
-    for (int a = 0; a < 3; a++) F[a] += f[a];
+  const float eps = (float)p.tors_eps;
+  auto add = [&](float e4, const float *f, const float *rel) {
+    E += 0.25 * (double)e4;
+    for (int a = 0; a < 3; a++) F[a] += (double)f[a];
     if (TRACE) {
-      W[0] += rel[0] * f[0] + rel[1] * f[1] + rel[2] * f[2];
+      W[0] += (double)rel[0] * f[0] + (double)rel[1] * f[1] + (double)rel[2] * f[2];
     } else {
       for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) W[3 * a + b] += rel[a] * f[b];
+        for (int b = 0; b < 3; b++) W[3 * a + b] += (double)rel[a] * f[b];
     }
   };
-  if (t < n) {
-    const int x = t;
+  if (x < n) {
+    int ord = 0;  // the chain's place in x's sequence; this thread takes ord = q mod TPA
     // x at an end: chains x - j - k - l
     for (int a = 0; a < nbk[x]; a++) {
       const int j = bjk[x][a];
-      const double4 e1 = bvk[x][a];
-      const double2 w1 = bwk[x][a];
       for (int b = 0; b < nbk[j]; b++) {
         const int k = bjk[j][b];
         if (k == x) continue;
-        const double4 e2 = bvk[j][b];
-        const double2 w2 = bwk[j][b];
         for (int c = 0; c < nbk[k]; c++) {
           const int l = bjk[k][c];
           if (l == j || l == x) continue;
-          const double4 e3 = bvk[k][c];
-          const double2 w3 = bwk[k][c];
-          const double b1[3] = {e1.x, e1.y, e1.z}, b2[3] = {e2.x, e2.y, e2.z},
-                       b3[3] = {e3.x, e3.y, e3.z};
-          const double r[3] = {e1.w, e2.w, e3.w}, w[3] = {w1.x, w2.x, w3.x},
-                       dw[3] = {w1.y, w2.y, w3.y};
-          double f[3];
-          const double e = tors::term<true>(b1, b2, b3, r, w, dw, eps, f);
+          if (ord++ % TPA != q) continue;
+          const tors::Bond e1 = bk[x][a], e2 = bk[j][b], e3 = bk[k][c];
+          float f[3];
+          const float e = tors::term<true>(e1, e2, e3, eps, f);
           // r_x - r_ref: -b1 (ref j) or -(b1 + b2) (ref k)
-          const double rel[3] = {j < k ? -b1[0] : -(b1[0] + b2[0]),
-                                 j < k ? -b1[1] : -(b1[1] + b2[1]),
-                                 j < k ? -b1[2] : -(b1[2] + b2[2])};
+          float rel[3];
+          for (int u = 0; u < 3; u++) rel[u] = j < k ? -e1.d[u] : -(e1.d[u] + e2.d[u]);
           add(e, f, rel);
         }
       }
@@ -1464,44 +1474,45 @@ __global__ __launch_bounds__(NT) void k_rebo_tors(AtomicDev d, AtomicParams p, H
     // x in the middle: chains i - x - k - l
     for (int b = 0; b < nbk[x]; b++) {
       const int k = bjk[x][b];
-      const double4 e2 = bvk[x][b];
-      const double2 w2 = bwk[x][b];
       for (int a = 0; a < nbk[x]; a++) {
         if (a == b) continue;
         const int i = bjk[x][a];
-        const double4 ei = bvk[x][a];
-        const double2 w1 = bwk[x][a];
         for (int c = 0; c < nbk[k]; c++) {
           const int l = bjk[k][c];
           if (l == x || l == i) continue;
-          const double4 e3 = bvk[k][c];
-          const double2 w3 = bwk[k][c];
-          const double b1[3] = {-ei.x, -ei.y, -ei.z}, b2[3] = {e2.x, e2.y, e2.z},
-                       b3[3] = {e3.x, e3.y, e3.z};
-          const double r[3] = {ei.w, e2.w, e3.w}, w[3] = {w1.x, w2.x, w3.x},
-                       dw[3] = {w1.y, w2.y, w3.y};
-          double f[3];
-          const double e = tors::term<false>(b1, b2, b3, r, w, dw, eps, f);
+          if (ord++ % TPA != q) continue;
+          tors::Bond e1 = bk[x][a];
+          for (int u = 0; u < 3; u++) e1.d[u] = -e1.d[u];  // r_x - r_i
+          const tors::Bond e2 = bk[x][b], e3 = bk[k][c];
+          float f[3];
+          const float e = tors::term<false>(e1, e2, e3, eps, f);
           // r_x - r_ref: 0 (ref x) or -b2 (ref k)
-          const double rel[3] = {x < k ? 0.0 : -b2[0], x < k ? 0.0 : -b2[1],
-                                 x < k ? 0.0 : -b2[2]};
+          float rel[3];
+          for (int u = 0; u < 3; u++) rel[u] = x < k ? 0.f : -e2.d[u];
           add(e, f, rel);
         }
       }
     }
-    for (int a = 0; a < 3; a++) d.Frebo[3 * (size_t)(s0 + x) + a] += F[a];
   }
+  for (int a = 0; a < 3; a++) fq[q][x][a] = F[a];
   // the molecule's energy and virial into its REBO partials (written by
   // k_rebo_mol earlier on the stream)
   __shared__ double tot[TRACE ? 2 : kAtPart];
   if (TRACE) {
     const double v[2] = {W[0], E};
-    block_sum<2, NW>(v, tot);
+    block_sum<2, 4>(v, tot);
   } else {
     double v[kAtPart];
     for (int k = 0; k < 9; k++) v[k] = W[k];
     v[9] = E;
-    block_sum<kAtPart, NW>(v, tot);
+    block_sum<kAtPart, 4>(v, tot);
+  }
+  if (q == 0 && x < n) {  // block_sum ended with a barrier: fq is complete
+    for (int a = 0; a < 3; a++) {
+      double s = fq[0][x][a];
+      for (int u = 1; u < TPA; u++) s += fq[u][x][a];
+      d.Frebo[3 * (size_t)(s0 + x) + a] += s;
+    }
   }
   if (t < kAtPart) {
     double add_k;
@@ -2534,17 +2545,17 @@ hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, i
       k_rebo_mol<kReboCompat, false, 256><<<g, 256, shm, s>>>(d, p, hh, fast_ok);
   }
   if (p.tors_eps != 0.0) {
-    // the torsion term over the same molecules (one thread per atom)
+    // the torsion term over the same molecules (256 threads, 1 to 4 per atom)
     if (p.apad <= 64) {
       if (p.lj_trace)
-        k_rebo_tors<true, 64><<<g, 64, 0, s>>>(d, p, hh);
+        k_rebo_tors<true, 64><<<g, 256, 0, s>>>(d, p, hh);
       else
-        k_rebo_tors<false, 64><<<g, 64, 0, s>>>(d, p, hh);
+        k_rebo_tors<false, 64><<<g, 256, 0, s>>>(d, p, hh);
     } else if (p.apad <= 128) {
       if (p.lj_trace)
-        k_rebo_tors<true, 128><<<g, 128, 0, s>>>(d, p, hh);
+        k_rebo_tors<true, 128><<<g, 256, 0, s>>>(d, p, hh);
       else
-        k_rebo_tors<false, 128><<<g, 128, 0, s>>>(d, p, hh);
+        k_rebo_tors<false, 128><<<g, 256, 0, s>>>(d, p, hh);
     } else {
       if (p.lj_trace)
         k_rebo_tors<true, 256><<<g, 256, 0, s>>>(d, p, hh);
