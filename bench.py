@@ -89,8 +89,8 @@ FLOP_PER_ATOM_MM_LJ = 42.0 * 169.6
 # WRITE x 1.0 (profiles/pmc_traffic_*.md).
 RIGID_TRAFFIC = {1: (9710.9 * 2.0 + 23524.8 * 1.0) * 1024.0,
                  2: (11032.8 * 2.0 + 5618.9 * 1.0) * 1024.0}
-AIREBO_LJ_TRAFFIC = (57568.3 * 2.0 + 3040.1 * 1.0) * 1024.0  # tuned 0.5 A list skin
-AIREBO_REBO_TRAFFIC = (4730.8 * 2.0 + 3040.0 * 1.0) * 1024.0
+AIREBO_LJ_TRAFFIC = (62639.7 * 2.0 + 3040.3 * 1.0) * 1024.0  # tuned 0.5 A list skin
+AIREBO_REBO_TRAFFIC = (4766.0 * 2.0 + 3040.0 * 1.0) * 1024.0
 RIGID_KERNEL = {1: "k_rigid_pairs", 2: "k_rigid_forces_half", 0: "k_rigid_forces"}
 TIMING_STRIDE = 10
 
