@@ -7,9 +7,12 @@
 //                (v_mfma_f32_16x16x16_bf16, FP32 accumulation): one
 //                wavefront per 16 atoms, activations kept transposed
 //                (feature x atom) so every layer's accumulator tile is the
-//                next layer's B operand without a shuffle
-//   k_sg_force   forces and per-atom virial shares: lane = centre atom i,
-//                dE_i/dd_ij per neighbour; the neighbour side is a
+//                next layer's B operand without a shuffle; a persistent
+//                grid with the weights staged in LDS once per workgroup
+//   k_sg_force   forces and per-atom virial shares: one workgroup per
+//                molecule, lane = centre atom i, dE_i/dd_ij per neighbour,
+//                the neighbours split over four wavefronts (their shares
+//                summed in wavefront order); the neighbour side is a
 //                fixed-order wavefront sum per neighbour (radial terms) or an
 //                LDS exchange over the bonded lists (angular terms)
 //   k_sg_parts   per-integrator-block energy / virial partials (REBO layout,
@@ -36,6 +39,15 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 __device__ __forceinline__ float bfround(float x) { return bf2f(f2bf(x)); }
+// The network's activation: tanh x = 1 - 2 / (e^(2x) + 1) with the hardware
+// base-2 exponential and reciprocal (absolute error ~1e-7, against the 4e-3
+// relative rounding of the BF16 activations it feeds); |x| capped at 9, where
+// tanh is 1 to FP32 precision.
+__device__ __forceinline__ float sg_tanh(float x) {
+  const float ax = fminf(fabsf(x), 9.f);
+  const float e = __builtin_amdgcn_exp2f(ax * 2.88539008177792681f);  // 2 / ln 2
+  return copysignf(1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f), x);
+}
 __device__ __forceinline__ bf4 pack4(float a, float b, float c, float d) {
   bf4 r;
   r[0] = (short)f2bf(a);
@@ -231,17 +243,49 @@ __global__ __launch_bounds__(256) void k_sg_descr(AtomicDev d, AtomicParams p, S
 }
 
 // ------------------------------------------------------------------- MLP
+// The network's BF16 weight matrices in LDS, rows padded (+4 BF16) so that
+// the 16 rows an MFMA operand reads fall in distinct bank pairs.
+constexpr int kSgLdK = kSgKF + 4;  // row stride of w1t [64][48]
+constexpr int kSgLdH = kSgH + 4;   // row stride of the [.][64] matrices
+struct SgLdsW {
+  unsigned short w1t[kSgH * kSgLdK], w2t[kSgH * kSgLdH], w3t[kSgH * kSgLdH];
+  unsigned short w3[kSgH * kSgLdH], w2[kSgH * kSgLdH], w1[kSgKF * kSgLdH];
+};
+constexpr int kSgMlpGroupsPerCU = 3;  // workgroups of the persistent grid per CU (48 KB LDS each)
+
+__device__ __forceinline__ void stage_rows(unsigned short *dst, const unsigned short *src, int rows,
+                                           int cols, int ld) {
+  // four BF16 (8 bytes) per thread and step
+  const int q = cols / 4;
+  for (int e = threadIdx.x; e < rows * q; e += blockDim.x) {
+    const int r = e / q, c = 4 * (e % q);
+    *reinterpret_cast<bf4 *>(dst + r * ld + c) = ld_bf4(src + r * cols + c);
+  }
+}
+
 // One wavefront = 16 atoms.  Transposed activations: tile m of a layer
 // holds rows (features) 16m .. 16m+15 x the 16 atoms; lane l keeps rows
 // 16m + 4 (l / 16) + v (v < 4) of atom l % 16 -- the MFMA accumulator
-// layout, which is also the B-operand layout of the next product.
+// layout, which is also the B-operand layout of the next product.  A
+// persistent grid: each workgroup stages the weights in LDS once, then its
+// four wavefronts take 16-atom tiles, 64 atoms per workgroup and pass.
 __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
+  __shared__ SgLdsW L;
+  stage_rows(L.w1t, w.w1t, kSgH, kSgKF, kSgLdK);
+  stage_rows(L.w2t, w.w2t, kSgH, kSgH, kSgLdH);
+  stage_rows(L.w3t, w.w3t, kSgH, kSgH, kSgLdH);
+  stage_rows(L.w3, w.w3, kSgH, kSgH, kSgLdH);
+  stage_rows(L.w2, w.w2, kSgH, kSgH, kSgLdH);
+  stage_rows(L.w1, w.w1, kSgKF, kSgH, kSgLdH);
+  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-  if (base >= na) return;
-  const int atom = base + (lane & 15), grp = lane >> 4;
-  const bool ok = atom < na;
+  const int grp = lane >> 4;
   const float *th = w.theta;
+  for (int b0 = blockIdx.x * 64; b0 < na; b0 += gridDim.x * 64) {
+  const int base = b0 + (threadIdx.x >> 6) * 16;
+  if (base >= na) break;
+  const int atom = base + (lane & 15);
+  const bool ok = atom < na;
   // input tiles (features 16t + 4 grp + v of this lane's atom)
   bf4 x[3];
 #pragma unroll
@@ -254,9 +298,9 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 3; t++)
-      acc = mfma(ld_bf4(w.w1t + (16 * mt + (lane & 15)) * kSgKF + 16 * t + 4 * grp), x[t], acc);
+      acc = mfma(ld_bf4(L.w1t + (16 * mt + (lane & 15)) * kSgLdK + 16 * t + 4 * grp), x[t], acc);
 #pragma unroll
-    for (int v = 0; v < 4; v++) h1[mt][v] = tanhf(acc[v] + th[kSgOffB1 + 16 * mt + 4 * grp + v]);
+    for (int v = 0; v < 4; v++) h1[mt][v] = sg_tanh(acc[v] + th[kSgOffB1 + 16 * mt + 4 * grp + v]);
   }
   bf4 hb[4];
 #pragma unroll
@@ -266,9 +310,9 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      acc = mfma(ld_bf4(w.w2t + (16 * mt + (lane & 15)) * kSgH + 16 * t + 4 * grp), hb[t], acc);
+      acc = mfma(ld_bf4(L.w2t + (16 * mt + (lane & 15)) * kSgLdH + 16 * t + 4 * grp), hb[t], acc);
 #pragma unroll
-    for (int v = 0; v < 4; v++) h2[mt][v] = tanhf(acc[v] + th[kSgOffB2 + 16 * mt + 4 * grp + v]);
+    for (int v = 0; v < 4; v++) h2[mt][v] = sg_tanh(acc[v] + th[kSgOffB2 + 16 * mt + 4 * grp + v]);
   }
 #pragma unroll
   for (int t = 0; t < 4; t++) hb[t] = pack4(h2[t][0], h2[t][1], h2[t][2], h2[t][3]);
@@ -277,9 +321,9 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      acc = mfma(ld_bf4(w.w3t + (16 * mt + (lane & 15)) * kSgH + 16 * t + 4 * grp), hb[t], acc);
+      acc = mfma(ld_bf4(L.w3t + (16 * mt + (lane & 15)) * kSgLdH + 16 * t + 4 * grp), hb[t], acc);
 #pragma unroll
-    for (int v = 0; v < 4; v++) h3[mt][v] = tanhf(acc[v] + th[kSgOffB3 + 16 * mt + 4 * grp + v]);
+    for (int v = 0; v < 4; v++) h3[mt][v] = sg_tanh(acc[v] + th[kSgOffB3 + 16 * mt + 4 * grp + v]);
   }
   // energy: w4 . h3 (w4 in bf16 as the MFMA operands are)
   float e = 0.f;
@@ -304,7 +348,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      acc = mfma(ld_bf4(w.w3 + (16 * mt + (lane & 15)) * kSgH + 16 * t + 4 * grp), db[t], acc);
+      acc = mfma(ld_bf4(L.w3 + (16 * mt + (lane & 15)) * kSgLdH + 16 * t + 4 * grp), db[t], acc);
 #pragma unroll
     for (int v = 0; v < 4; v++) h2[mt][v] = acc[v] * (1.f - h2[mt][v] * h2[mt][v]);
   }
@@ -315,7 +359,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      acc = mfma(ld_bf4(w.w2 + (16 * mt + (lane & 15)) * kSgH + 16 * t + 4 * grp), db[t], acc);
+      acc = mfma(ld_bf4(L.w2 + (16 * mt + (lane & 15)) * kSgLdH + 16 * t + 4 * grp), db[t], acc);
 #pragma unroll
     for (int v = 0; v < 4; v++) h1[mt][v] = acc[v] * (1.f - h1[mt][v] * h1[mt][v]);
   }
@@ -326,7 +370,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      acc = mfma(ld_bf4(w.w1 + (16 * mt + (lane & 15)) * kSgH + 16 * t + 4 * grp), db[t], acc);
+      acc = mfma(ld_bf4(L.w1 + (16 * mt + (lane & 15)) * kSgLdH + 16 * t + 4 * grp), db[t], acc);
     if (ok) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -334,6 +378,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(int na, SgWeights w, SgDev s) {
         if (f < kSgNF) s.g[(size_t)atom * kSgNF + f] = acc[v] / w.sd[f];
       }
     }
+  }
   }
 }
 
@@ -466,19 +511,30 @@ __device__ __forceinline__ void bond_geom(const float4 *tile, const float4 &xa, 
   }
 }
 
-// One wavefront per molecule (NS atoms per lane), 4 molecules per block.
-// F_i += sum_j dE_i/dd_ij, F_j -= dE_i/dd_ij;  virial share of i:
-// -sum_j d_ij (x) dE_i/dd_ij.
-template <int NS>
+// WPM wavefronts per molecule (1 or 4; NS atoms per lane), 4 / WPM molecules
+// per workgroup: wavefront q of a molecule takes the neighbours b of share q
+// of its atoms for every atom of the molecule (four shares keep small systems,
+// with few molecules per CU, from waiting on one wavefront each).
+// F_i += sum_j dE_i/dd_ij, F_j -= dE_i/dd_ij (the latter summed over the
+// wavefront, to the lane that owns j); virial share of i:
+// -sum_j d_ij (x) dE_i/dd_ij.  A molecule's wavefronts' partial forces and
+// virials are summed in LDS in wavefront order; the bonded shell is its
+// first wavefront's.
+template <int NS, int WPM>
 __global__ __launch_bounds__(256) void k_sg_force(AtomicDev d, AtomicParams p, SgWeights w,
                                                   SgDev s, double *out) {
-  __shared__ float xf[4][128][kSgNbMax][3];
+  constexpr int NA = 64 * NS, MPB = 4 / WPM;
+  __shared__ float xf[MPB][NA][kSgNbMax][3];
+  // each wavefront's F (3) and W (9) per atom (shares of one molecule)
+  __shared__ float fw[WPM > 1 ? 4 : 1][WPM > 1 ? NA : 1][12];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int m = blockIdx.x * 4 + wv;
-  const bool valid = m < p.Nm;  // no early return: the block meets at a barrier
+  const int ml = wv / WPM, q = wv % WPM;  // molecule of the workgroup, share
+  const int m = blockIdx.x * MPB + ml;
+  const bool valid = m < p.Nm;  // no early return: the block meets at barriers
   const int a0 = valid ? d.mol_start[m] : 0, n = valid ? d.mol_start[m + 1] - a0 : 0;
   const float4 *tile = d.off + (size_t)(valid ? m : 0) * p.apad;
   const float rc2 = kSgRc * kSgRc;
+  const int b_lo = (n * q) / WPM, b_hi = (n * (q + 1)) / WPM;
   float F[NS][3], W[NS][9];
 #pragma unroll
   for (int si = 0; si < NS; si++) {
@@ -492,7 +548,7 @@ __global__ __launch_bounds__(256) void k_sg_force(AtomicDev d, AtomicParams p, S
     const float4 xa = act ? tile[a] : make_float4(0.f, 0.f, 0.f, 0.f);
     SgCoef c;
     sg_coef(s, act ? a0 + a : a0, c);
-    for (int b = 0; b < n; b++) {
+    for (int b = b_lo; b < b_hi; b++) {
       const float4 xb = tile[b];
       const float dv[3] = {xb.x - xa.x, xb.y - xa.y, xb.z - xa.z};
       const float r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
@@ -521,15 +577,17 @@ __global__ __launch_bounds__(256) void k_sg_force(AtomicDev d, AtomicParams p, S
       }
     }
   }
-  // bonded shell: own share now, the neighbour's through LDS
+  // bonded shell (the molecule's first wavefront): own share now, the
+  // neighbour's through LDS
   int nbs[NS], nbls[NS][kSgNbMax];
 #pragma unroll
   for (int si = 0; si < NS; si++) {
     const int a = lane + 64 * si;
-    const bool act = a < n;
+    const bool act = q == 0 && a < n;
     nbs[si] = act ? s.nbc[a0 + a] : 0;
 #pragma unroll
-    for (int q = 0; q < kSgNbMax; q++) nbls[si][q] = act ? s.nbl[(size_t)(a0 + a) * kSgNbMax + q] : 0;
+    for (int qb = 0; qb < kSgNbMax; qb++)
+      nbls[si][qb] = act ? s.nbl[(size_t)(a0 + a) * kSgNbMax + qb] : 0;
     if (!act) continue;
     const float4 xa = tile[a];
     float bu[kSgNbMax][3], br[kSgNbMax], bd[kSgNbMax][3], bw[kSgNbMax][2], bdw[kSgNbMax][2];
@@ -542,13 +600,13 @@ __global__ __launch_bounds__(256) void k_sg_force(AtomicDev d, AtomicParams p, S
     float fq[kSgNbMax][3];
     sg_bond_grad(gI, nbs[si], bu, br, bw, bdw, fq);
 #pragma unroll
-    for (int q = 0; q < kSgNbMax; q++) {
+    for (int qb = 0; qb < kSgNbMax; qb++) {
 #pragma unroll
       for (int x = 0; x < 3; x++) {
-        F[si][x] += fq[q][x];
-        xf[wv][a][q][x] = fq[q][x];
+        F[si][x] += fq[qb][x];
+        xf[ml][a][qb][x] = fq[qb][x];
 #pragma unroll
-        for (int y = 0; y < 3; y++) W[si][3 * x + y] -= bd[q][x] * fq[q][y];
+        for (int y = 0; y < 3; y++) W[si][3 * x + y] -= bd[qb][x] * fq[qb][y];
       }
     }
   }
@@ -556,21 +614,48 @@ __global__ __launch_bounds__(256) void k_sg_force(AtomicDev d, AtomicParams p, S
 #pragma unroll
   for (int si = 0; si < NS; si++) {
     const int a = lane + 64 * si;
-    if (a >= n) continue;
-    for (int q = 0; q < nbs[si]; q++) {
-      const int j = nbls[si][q];  // a is in j's bonded list (same distance test)
-      const int cj = s.nbc[a0 + j];
-      const int *lj = s.nbl + (size_t)(a0 + j) * kSgNbMax;
-      for (int q2 = 0; q2 < cj; q2++)
-        if (lj[q2] == a) {
-          F[si][0] -= xf[wv][j][q2][0];
-          F[si][1] -= xf[wv][j][q2][1];
-          F[si][2] -= xf[wv][j][q2][2];
-        }
+    if (q == 0 && a < n) {
+      for (int qb = 0; qb < nbs[si]; qb++) {
+        const int j = nbls[si][qb];  // a is in j's bonded list (same distance test)
+        const int cj = s.nbc[a0 + j];
+        const int *lj = s.nbl + (size_t)(a0 + j) * kSgNbMax;
+        for (int q2 = 0; q2 < cj; q2++)
+          if (lj[q2] == a) {
+            F[si][0] -= xf[ml][j][q2][0];
+            F[si][1] -= xf[ml][j][q2][1];
+            F[si][2] -= xf[ml][j][q2][2];
+          }
+      }
     }
-    const size_t i = a0 + a;
-    for (int x = 0; x < 3; x++) out[3 * i + x] = (double)F[si][x];
-    for (int k = 0; k < 9; k++) s.vir[9 * i + k] = W[si][k];
+    if constexpr (WPM == 1) {
+      if (a < n) {
+        const size_t i = a0 + a;
+        for (int x = 0; x < 3; x++) out[3 * i + x] = (double)F[si][x];
+        for (int k = 0; k < 9; k++) s.vir[9 * i + k] = W[si][k];
+      }
+    } else {
+      for (int x = 0; x < 3; x++) fw[wv][a][x] = F[si][x];
+      for (int k = 0; k < 9; k++) fw[wv][a][3 + k] = W[si][k];
+    }
+  }
+  if constexpr (WPM == 1) return;
+  __syncthreads();
+  // each molecule's shares, in wavefront order: its wavefront q = 0 .. WPM-1
+  // takes atoms lane + 64 si
+  if (valid) {
+#pragma unroll
+    for (int si = 0; si < NS; si++) {
+      const int a = lane + 64 * si;
+      if (q != 0 || a >= n) continue;
+      const size_t i = a0 + a;
+      float f[12];
+      for (int k = 0; k < 12; k++) {
+        f[k] = fw[wv][a][k];
+        for (int u = 1; u < WPM; u++) f[k] += fw[wv + u][a][k];
+      }
+      for (int x = 0; x < 3; x++) out[3 * i + x] = (double)f[x];
+      for (int k = 0; k < 9; k++) s.vir[9 * i + k] = f[3 + k];
+    }
   }
 }
 
@@ -796,7 +881,7 @@ __global__ __launch_bounds__(256) void k_sg_train(int na, SgWeights w, SgDev s, 
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const float hv = tanhf(az[v] + th[boff[ly] + 16 * mt + 4 * grp + v]);
+          const float hv = sg_tanh(az[v] + th[boff[ly] + 16 * mt + 4 * grp + v]);
           h[ly][mt][v] = hv;
           sdot[ly][mt][v] = at[v];
           hd[ly][mt][v] = (1.f - hv * hv) * at[v];
@@ -1032,17 +1117,35 @@ hipError_t sg_launch_descr(const AtomicDev &d, const AtomicParams &p, const SgWe
 
 hipError_t sg_launch_mlp(const AtomicParams &p, const SgWeights &w, const SgDev &s,
                          hipStream_t st) {
-  k_sg_mlp<<<(p.Na + 63) / 64, 256, 0, st>>>(p.Na, w, s);
+  static int ncu = 0;  // CUs of the device (one GPU per process)
+  if (!ncu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+  }
+  const int groups = std::min((p.Na + 63) / 64, kSgMlpGroupsPerCU * ncu);
+  k_sg_mlp<<<groups, 256, 0, st>>>(p.Na, w, s);
   return hipGetLastError();
 }
 
 hipError_t sg_launch_force(const AtomicDev &d, const AtomicParams &p, const SgWeights &w,
                            const SgDev &s, double *out, hipStream_t st) {
-  dim3 g((p.Nm + 3) / 4);
-  if (ns_of(p) == 1)
-    k_sg_force<1><<<g, 256, 0, st>>>(d, p, w, s, out);
-  else
-    k_sg_force<2><<<g, 256, 0, st>>>(d, p, w, s, out);
+  // four wavefronts per molecule for systems too small to give every SIMD
+  // several molecules (the training frames), else one
+  const bool split = p.Nm < 1024;
+  dim3 g(split ? p.Nm : (p.Nm + 3) / 4);
+  if (ns_of(p) == 1) {
+    if (split)
+      k_sg_force<1, 4><<<g, 256, 0, st>>>(d, p, w, s, out);
+    else
+      k_sg_force<1, 1><<<g, 256, 0, st>>>(d, p, w, s, out);
+  } else {
+    if (split)
+      k_sg_force<2, 4><<<g, 256, 0, st>>>(d, p, w, s, out);
+    else
+      k_sg_force<2, 1><<<g, 256, 0, st>>>(d, p, w, s, out);
+  }
   return hipGetLastError();
 }
 
