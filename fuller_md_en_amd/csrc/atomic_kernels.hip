@@ -749,9 +749,12 @@ struct AtHalf1 {
   double xi1, sc_nh, sc_v;
 };
 
-// step_npt AB:522-555 with the phase controls of AB:875-879.
-__device__ __forceinline__ void at_half1(const AtomicState &S, const StepCtrl &c,
-                                         const AtomicParams &p, AtHalf1 &o) {
+// step_npt AB:522-555 with the phase controls of AB:875-879.  St: the
+// finished state (AtomicState, or AtFinish when the finish of the previous
+// step is folded into the kick-drift).
+template <class St>
+__device__ __forceinline__ void at_half1(const St &S, const StepCtrl &c, const AtomicParams &p,
+                                         AtHalf1 &o) {
   const double hdt = 0.5 * p.dt;
   double xi = S.xi, Vg[9];
   for (int k = 0; k < 9; k++) Vg[k] = S.Vg[k];
@@ -779,32 +782,17 @@ __device__ __forceinline__ void at_half1(const AtomicState &S, const StepCtrl &c
   o.xi1 = xi;
 }
 
-// One block per integrator block of a single replica; every thread derives
-// its replica's first-half NPT scalars (redundantly, from the same inputs).
-__global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams p,
-                                                       int ctrl_slot) {
-  const int b = blockIdx.x;
-  const int rep = d.blk_rep[b];
-  const AtomicState &S = d.state[rep];
-  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + rep];
-  AtHalf1 H;
-  at_half1(S, c, p, H);
-  if (b == d.rep_blk[rep] && threadIdx.x == 0) {
-    AtomicMid &M = d.mid[rep];
-    for (int k = 0; k < 9; k++) {
-      M.h[k] = H.h1[k];
-      M.hi[k] = H.hi1[k];
-      M.Vg[k] = H.Vg1[k];
-    }
-    M.xi = H.xi1;
-    M.sc_nh = H.sc_nh;
-  }
-  const int i = d.blk_a0[b] + threadIdx.x;
-  if (i >= d.blk_a1[b]) return;
+// First half-kick, drift and wrap of atom i (AB:557-568) with the step's
+// first-half NPT scalars H, its velocity first multiplied by vs (the previous
+// step's rescale factor when that step's finish is folded in, else 1).
+__device__ __forceinline__ void kick_drift_atom(const AtomicDev &d, const AtomicParams &p,
+                                                const AtHalf1 &H, int i, int rep, int b,
+                                                double vs) {
   const double hdt = 0.5 * p.dt, mi = kConv / d.mass[i];
   double v[3], r[3];
   for (int a = 0; a < 3; a++) {
-    v[a] = d.vel[3 * i + a] * H.sc_v + hdt * d.F[3 * i + a] * mi;
+    v[a] = (vs == 1.0 ? d.vel[3 * i + a] : d.vel[3 * i + a] * vs) * H.sc_v +
+           hdt * d.F[3 * i + a] * mi;
     r[a] = d.pos[3 * i + a];
   }
   const double *hi = H.hi;
@@ -843,6 +831,33 @@ __global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams
                   __float_as_int(dr));
     }
   }
+}
+
+__device__ __forceinline__ void write_mid(AtomicMid &M, const AtHalf1 &H) {
+  for (int k = 0; k < 9; k++) {
+    M.h[k] = H.h1[k];
+    M.hi[k] = H.hi1[k];
+    M.Vg[k] = H.Vg1[k];
+  }
+  M.xi = H.xi1;
+  M.sc_nh = H.sc_nh;
+}
+
+// One block per integrator block of a single replica; every thread derives
+// its replica's first-half NPT scalars (redundantly, from the same inputs)
+// into midpoint buffer par.
+__global__ __launch_bounds__(256) void k_at_kick_drift(AtomicDev d, AtomicParams p,
+                                                       int ctrl_slot, int par) {
+  const int b = blockIdx.x;
+  const int rep = d.blk_rep[b];
+  const AtomicState &S = d.state[rep];
+  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + rep];
+  AtHalf1 H;
+  at_half1(S, c, p, H);
+  if (b == d.rep_blk[rep] && threadIdx.x == 0) write_mid(d.mid[(size_t)par * p.R + rep], H);
+  const int i = d.blk_a0[b] + threadIdx.x;
+  if (i >= d.blk_a1[b]) return;
+  kick_drift_atom(d, p, H, i, rep, b, 1.0);
 }
 
 // ------------------------------------------------------------------ REBO
@@ -2214,12 +2229,13 @@ __global__ __launch_bounds__(256) void k_at_lj_al(AtomicDev d, AtomicParams p, H
 }
 
 // ------------------------------------------------------- kick 2 + partials
-__global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, int kick) {
+__global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, int kick,
+                                                  int par) {
   if (kick && p.lj_alist && !p.small_box && threadIdx.x == 0) {
     // lists rebuilt within this step's LJ kernel: their cell, drift zero
     const int r = d.blk_rep[blockIdx.x];
     if (blockIdx.x == d.rep_blk[r] && d.al_redo[r]) {
-      const double *hm = d.mid[r].h;
+      const double *hm = d.mid[(size_t)par * p.R + r].h;
       for (int k = 0; k < 9; k++) d.al_h[9 * r + k] = hm[k];
       for (int k = 0; k < kAlDriftSlots; k++) d.al_dmax[kAlDriftSlots * r + k] = 0;
       d.al_redo[r] = 0;
@@ -2230,7 +2246,7 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
   if (i < d.blk_a1[blockIdx.x]) {
     double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
     if (kick) {
-      const AtomicMid &M = d.mid[d.blk_rep[blockIdx.x]];
+      const AtomicMid &M = d.mid[(size_t)par * p.R + d.blk_rep[blockIdx.x]];
       const double hdt = 0.5 * p.dt, mi = kConv / d.mass[i];
       const double eps2 = M.Vg[0] * M.hi[0] + M.Vg[4] * M.hi[4] + M.Vg[8] * M.hi[8];
       const double sc_v2 = M.sc_nh * exp(-hdt * eps2 / 3.0);
@@ -2252,7 +2268,10 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
 // Replica r's sums (one workgroup; the strided order is the same whatever
 // the replica's offset, so a replica sums bitwise as it would alone).
 constexpr int kAtSums = 17;
-constexpr int kFinThreads = 1024;  // one workgroup per replica: loads in flight
+// threads that reduce a replica's partials: the finish kernel's workgroup,
+// and every integrator block of the kick-drift that folds a finish in (both
+// must add in the same order)
+constexpr int kFinThreads = 256;
 // The partial arrays are component-major (component k of record r at
 // k * records + r), so the strided loops below read them coalesced.
 template <bool MM>
@@ -2278,28 +2297,48 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
     // diagonal and zeros off it, so two of their ten components carry all
     // (the same sums as below, in the same order; the per-pair-image LJ
     // kernel of narrow cells keeps the full tensor)
-    for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
-      const double lt = d.lj_part[m], le = d.lj_part[9 * nm + m];
-      double x[13];
-      if (MM) {
+    if (MM) {
+      for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
+        const double lt = d.lj_part[m], le = d.lj_part[9 * nm + m];
+        double x[13];
 #pragma unroll
         for (int k = 0; k < 13; k++) x[k] = d.mm.part[k * nm + m];
-      } else {
-        x[0] = d.rebo_mpart[m];
-        x[9] = d.rebo_mpart[9 * nm + m];
-      }
-      v[1] += lt;
-      v[5] += lt;
-      v[9] += lt;
-      v[11] += le;
-      if (MM) {
+        v[1] += lt;
+        v[5] += lt;
+        v[9] += lt;
+        v[11] += le;
         for (int k = 0; k < 9; k++) v[1 + k] += x[k];
         for (int k = 0; k < 4; k++) v[12 + k] += x[9 + k];
-      } else {
-        v[1] += x[0];
-        v[5] += x[0];
-        v[9] += x[0];
-        v[10] += x[9];
+      }
+    } else {
+      // eight molecules per thread loaded before any is added (their loads
+      // in flight together), added in molecule order
+      constexpr int U = 8;
+      for (int m = m0 + threadIdx.x; m < m1; m += U * kFinThreads) {
+        double lt[U], le[U], x0[U], x9[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int mu = m + u * kFinThreads;
+          lt[u] = le[u] = x0[u] = x9[u] = 0.0;
+          if (mu < m1) {
+            lt[u] = d.lj_part[mu];
+            le[u] = d.lj_part[9 * nm + mu];
+            x0[u] = d.rebo_mpart[mu];
+            x9[u] = d.rebo_mpart[9 * nm + mu];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (m + u * kFinThreads >= m1) break;
+          v[1] += lt[u];
+          v[5] += lt[u];
+          v[9] += lt[u];
+          v[11] += le[u];
+          v[1] += x0[u];
+          v[5] += x0[u];
+          v[9] += x0[u];
+          v[10] += x9[u];
+        }
       }
     }
   } else {
@@ -2363,32 +2402,28 @@ __global__ __launch_bounds__(kFinThreads) void k_at_observe(AtomicDev d, AtomicP
   }
 }
 
-// One workgroup per replica.
-template <bool MM>
-__global__ __launch_bounds__(kFinThreads) void k_at_finish(AtomicDev d, AtomicParams p, int ctrl_slot,
-                                                   int thermo_slot) {
-  const int r = blockIdx.x;
-  // the state, midpoint and control before the reduction: their latency
-  // overlaps it
-  AtomicState &S = d.state[r];
-  const AtomicMid M = d.mid[r];
-  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + r];
+// The second half of step_npt (AB:570-590) and the rescale control
+// (AB:898-908) of a replica from its sums s, midpoint M, state constants and
+// the step's control c: the finished thermostat, cell velocity, energies and
+// the velocity rescale factor.
+struct AtFinish {
+  double h[9], xi, Vg[9], KE, W9[9], Q, W, Pe, T, P, V, scale, Nf;
+};
+__device__ __forceinline__ void at_finish_math(const AtomicState &S, const AtomicMid &M,
+                                               const StepCtrl &c, const AtomicParams &p,
+                                               const double *s, AtFinish &o) {
   const double Nf = S.Nf, Q = S.Q, Pe = S.Pe, Wb = S.W;
-  double s[kAtSums];
-  at_reduce<MM>(d, p, r, s);
-  if (threadIdx.x != 0) return;
   const double hdt = 0.5 * p.dt;
   double KE = 0.5 * s[0] / kConv;
-  double W9[9];
-  for (int k = 0; k < 9; k++) W9[k] = s[1 + k];
-  double xi = M.xi, Vg[9];
-  for (int k = 0; k < 9; k++) Vg[k] = M.Vg[k];
+  for (int k = 0; k < 9; k++) o.W9[k] = s[1 + k];
+  double xi = M.xi;
+  for (int k = 0; k < 9; k++) o.Vg[k] = M.Vg[k];
   xi += hdt * (2 * KE - Nf * kB * c.Tt) / Q;
   xi = clampd(xi, -p.clamp_xi, p.clamp_xi);
   const double V2 = fabs(det9(M.h));
-  const double P = (2 * KE + W9[0] + W9[4] + W9[8]) / (3 * V2) * kEv2GPa;
-  for (int a = 0; a < 3; a++) Vg[a * 4] += hdt * V2 * (P - Pe) / (Wb * kEv2GPa);
-  for (int a = 0; a < 3; a++) Vg[a * 4] = clampd(Vg[a * 4], -p.clamp_vg, p.clamp_vg);
+  const double P = (2 * KE + o.W9[0] + o.W9[4] + o.W9[8]) / (3 * V2) * kEv2GPa;
+  for (int a = 0; a < 3; a++) o.Vg[a * 4] += hdt * V2 * (P - Pe) / (Wb * kEv2GPa);
+  for (int a = 0; a < 3; a++) o.Vg[a * 4] = clampd(o.Vg[a * 4], -p.clamp_vg, p.clamp_vg);
   double T = 2 * KE / (Nf * kB);
   double scale = 1.0;
   if ((c.flags & kCtrlRescale) && T > 0.1) {  // AB:898-908
@@ -2398,29 +2433,48 @@ __global__ __launch_bounds__(kFinThreads) void k_at_finish(AtomicDev d, AtomicPa
     T = 2 * KE / (Nf * kB);
     xi = 0;
     if (c.flags & kCtrlRescaleZeroVg)
-      for (int k = 0; k < 9; k++) Vg[k] = 0;
+      for (int k = 0; k < 9; k++) o.Vg[k] = 0;
   }
+  for (int k = 0; k < 9; k++) o.h[k] = M.h[k];
+  o.xi = xi;
+  o.KE = KE;
+  o.T = T;
+  o.P = P;
+  o.V = V2;
+  o.scale = scale;
+  o.Q = Q;
+  o.W = Wb;
+  o.Pe = Pe;
+  o.Nf = Nf;
+}
+
+// The finished state and the step's record (one thread).
+__device__ __forceinline__ void at_finish_write(const AtomicDev &d, const AtomicParams &p, int r,
+                                                const AtomicMid &M, const StepCtrl &c,
+                                                const double *s, const AtFinish &o,
+                                                int thermo_slot) {
+  AtomicState &S = d.state[r];
   for (int k = 0; k < 9; k++) {
     S.h[k] = M.h[k];
     S.hi[k] = M.hi[k];
-    S.Vg[k] = Vg[k];
-    S.W9[k] = W9[k];
+    S.Vg[k] = o.Vg[k];
+    S.W9[k] = o.W9[k];
   }
-  S.xi = xi;
+  S.xi = o.xi;
   S.Tt = c.Tt;
-  S.KE = KE;
+  S.KE = o.KE;
   at_set_energies(S, s);
-  S.scale = scale;
+  S.scale = o.scale;
   S.step += 1;
   Thermo &t = d.thermo[(size_t)thermo_slot * p.R + r];
   t.Ep = S.Ep;
-  t.KE = KE;
-  t.KEt = KE;
+  t.KE = o.KE;
+  t.KEt = o.KE;
   t.KEr = 0;
-  t.T = T;
-  t.P = P;
-  t.V = V2;
-  t.xi = xi;
+  t.T = o.T;
+  t.P = o.P;
+  t.V = o.V;
+  t.xi = o.xi;
   t.Ep_rebo = S.Ep_rebo;
   t.Ep_lj = S.Ep_lj;
   for (int k = 0; k < 9; k++) t.h[k] = M.h[k];
@@ -2429,6 +2483,57 @@ __global__ __launch_bounds__(kFinThreads) void k_at_finish(AtomicDev d, AtomicPa
   t.Ed = S.Ed;
   t.Ei = S.Ei;
   t.Ecoul = S.Ec;
+}
+
+// One workgroup per replica: the finish of the step whose midpoint is in
+// buffer par.
+template <bool MM>
+__global__ __launch_bounds__(kFinThreads) void k_at_finish(AtomicDev d, AtomicParams p,
+                                                           int ctrl_slot, int thermo_slot,
+                                                           int par) {
+  const int r = blockIdx.x;
+  // the state, midpoint and control before the reduction: their latency
+  // overlaps it
+  const AtomicState &S = d.state[r];
+  const AtomicMid M = d.mid[(size_t)par * p.R + r];
+  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + r];
+  double s[kAtSums];
+  at_reduce<MM>(d, p, r, s);
+  if (threadIdx.x != 0) return;
+  AtFinish o;
+  at_finish_math(S, M, c, p, s, o);
+  at_finish_write(d, p, r, M, c, s, o, thermo_slot);
+}
+
+// The kick-drift of a step with the previous step's finish folded in: every
+// block reduces its replica's partials itself (the finish kernel's
+// reduction, the same sums in the same order), derives the finished state,
+// rescales its atoms' velocities by the previous step's factor and takes the
+// first-half NPT scalars from that state; the replica's first block writes
+// the finished state and record and the new midpoint (buffer par; the
+// previous step's is in 1 - par).
+template <bool MM>
+__global__ __launch_bounds__(256) void k_at_kick_drift_fin(AtomicDev d, AtomicParams p,
+                                                           int ctrl_slot, int prev_slot,
+                                                           int par) {
+  const int b = blockIdx.x;
+  const int rep = d.blk_rep[b];
+  const AtomicState &S = d.state[rep];
+  const AtomicMid M = d.mid[(size_t)(1 - par) * p.R + rep];
+  const StepCtrl cp = d.ctrl[(size_t)prev_slot * p.R + rep];
+  const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + rep];
+  double s[kAtSums];
+  at_reduce<MM>(d, p, rep, s);
+  AtFinish o;
+  at_finish_math(S, M, cp, p, s, o);
+  const bool lead = b == d.rep_blk[rep] && threadIdx.x == 0;
+  if (lead) at_finish_write(d, p, rep, M, cp, s, o, prev_slot);
+  AtHalf1 H;
+  at_half1(o, c, p, H);
+  if (lead) write_mid(d.mid[(size_t)par * p.R + rep], H);
+  const int i = d.blk_a0[b] + threadIdx.x;
+  if (i >= d.blk_a1[b]) return;
+  kick_drift_atom(d, p, H, i, rep, b, o.scale);
 }
 
 // Velocity rescale by each replica's factor (1 where its control did not ask).
@@ -2621,7 +2726,7 @@ hipError_t at_launch_forces(const AtomicDev &d, const AtomicParams &p, HHRef hh,
 
 hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what,
                              hipStream_t s) {
-  k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0);
+  k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 0, 0);
   if (p.ff == kFfMM)
     k_at_observe<true><<<p.R, kFinThreads, 0, s>>>(d, p, what);
   else
@@ -2631,9 +2736,14 @@ hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what
 
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
                           int thermo_slot, int rebuild, int rescale, const StepEvents *ev,
-                          hipStream_t s) {
+                          hipStream_t s, int prev_slot, int fin) {
   hipError_t e;
   const unsigned evf = ev ? ev->flags : 0;
+  // midpoint buffers alternate with the slot, so that the one of the step
+  // before (slot - 1) is intact while this step's is written
+  const int par = ctrl_slot & 1;
+  AtomicParams ps = p;
+  ps.lj_trace = p.trace_virial;  // steps need the virial trace only
   if (rebuild) {
     // AB:881-887: both lists from the current positions and cell
     if (ev && (e = hipEventRecordWithFlags(ev->l0, s, evf)) != hipSuccess) return e;
@@ -2641,20 +2751,28 @@ hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_sl
     if ((e = at_launch_lists(d, p, hh_state(d), 1, s)) != hipSuccess) return e;
     if (ev && (e = hipEventRecordWithFlags(ev->l1, s, evf)) != hipSuccess) return e;
   }
-  k_at_kick_drift<<<d.nblk, 256, 0, s>>>(d, p, ctrl_slot);
+  if (prev_slot >= 0) {
+    if (rebuild || (prev_slot & 1) == par) return hipErrorInvalidValue;
+    if (p.ff == kFfMM)
+      k_at_kick_drift_fin<true><<<d.nblk, 256, 0, s>>>(d, ps, ctrl_slot, prev_slot, par);
+    else
+      k_at_kick_drift_fin<false><<<d.nblk, 256, 0, s>>>(d, ps, ctrl_slot, prev_slot, par);
+  } else {
+    k_at_kick_drift<<<d.nblk, 256, 0, s>>>(d, p, ctrl_slot, par);
+  }
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = at_launch_centers(d, p, hh_mid(d), s)) != hipSuccess) return e;
+  if ((e = at_launch_centers(d, p, hh_mid(d, par, p.R), s)) != hipSuccess) return e;
   if (ev && (e = hipEventRecordWithFlags(ev->f0, s, evf)) != hipSuccess) return e;
-  AtomicParams ps = p;
-  ps.lj_trace = p.trace_virial;  // steps need the virial trace only
-  if ((e = at_launch_forces(d, ps, hh_mid(d), s, ev ? ev->fm : nullptr, evf)) != hipSuccess)
+  if ((e = at_launch_forces(d, ps, hh_mid(d, par, p.R), s, ev ? ev->fm : nullptr, evf)) !=
+      hipSuccess)
     return e;
   if (ev && (e = hipEventRecordWithFlags(ev->f1, s, evf)) != hipSuccess) return e;
-  k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1);
+  k_at_kick2<<<d.nblk, 256, 0, s>>>(d, p, 1, par);
+  if (!fin) return hipGetLastError();  // finished by the next step's kick-drift
   if (p.ff == kFfMM)
-    k_at_finish<true><<<p.R, kFinThreads, 0, s>>>(d, ps, ctrl_slot, thermo_slot);
+    k_at_finish<true><<<p.R, kFinThreads, 0, s>>>(d, ps, ctrl_slot, thermo_slot, par);
   else
-    k_at_finish<false><<<p.R, kFinThreads, 0, s>>>(d, ps, ctrl_slot, thermo_slot);
+    k_at_finish<false><<<p.R, kFinThreads, 0, s>>>(d, ps, ctrl_slot, thermo_slot, par);
   if (rescale) k_at_rescale<<<d.nblk, 256, 0, s>>>(d, p);
   return hipGetLastError();
 }

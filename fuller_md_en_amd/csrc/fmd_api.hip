@@ -524,9 +524,12 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
     };
     if (!e->use_graph) {
       HIPCHK(hipMemcpyAsync(d_ctrl, e->h_ctrl, n * cb, hipMemcpyHostToDevice, s));
+      // a step's finish is folded into the next one's launch within the
+      // chunk, except before a list rebuild
       for (int k = 0; k < n; k++) {
         StepEvents sev;
-        HIPCHK(launch(k, rebuild[k], rescale[k],
+        const int fold = k > 0 && !rebuild[k], fin = k == n - 1 || rebuild[k + 1];
+        HIPCHK(launch(k, rebuild[k], rescale[k], fold, fin,
                       step_events(k, rebuild[k], hipEventRecordDefault, &sev)));
       }
       if (thermo)
@@ -550,7 +553,8 @@ int run_steps(fmd_engine *e, int R, int first_step, int nsteps, const fmd_step_c
           for (int j = 0; j < K; j++) {
             StepEvents sev;
             const int rb = j == 0 ? rebuild[k] : 0;
-            HIPCHK(launch(j, rb, rescale[k + j], step_events(j, rb, hipEventRecordExternal, &sev)));
+            HIPCHK(launch(j, rb, rescale[k + j], j > 0, j == K - 1,
+                          step_events(j, rb, hipEventRecordExternal, &sev)));
           }
           return FMD_OK;
         }, &exec);
@@ -615,7 +619,8 @@ static int rigid_run(fmd_engine *e, int first_step, int nsteps, const fmd_step_c
     int rc = rigid_prepare(e);
     if (rc) return rc;
   }
-  auto launch = [&](int slot, int rebuild, int rescale, const StepEvents *ev) -> hipError_t {
+  auto launch = [&](int slot, int rebuild, int rescale, int, int,
+                    const StepEvents *ev) -> hipError_t {
     return rigid_launch_step(d, p, slot, slot, rebuild, rescale, ev, e->stream);
   };
   int rc = run_steps(e, 1, first_step, nsteps, ctrl, thermo, &e->tt, d.ctrl, d.thermo, launch);

@@ -44,8 +44,9 @@
 // are [rep_mol[r], rep_mol[r+1]) and [rep_atom[r], rep_atom[r+1]); mol_rep /
 // atom_rep map back.  Per-atom integrator blocks never straddle replicas:
 // block b covers atoms [blk_a0[b], blk_a1[b]) of replica blk_rep[b], and
-// replica r owns blocks [rep_blk[r], rep_blk[r+1]).  state, mid: one per
-// replica; ctrl / thermo rings: slot-major, R entries per step slot.
+// replica r owns blocks [rep_blk[r], rep_blk[r+1]).  state: one per
+// replica; mid: two buffers of R (steps alternate between them); ctrl /
+// thermo rings: slot-major, R entries per step slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -192,9 +193,11 @@ inline HHRef hh_state(const AtomicDev &d) {
                                            offsetof(AtomicState, h)),
           (int)(sizeof(AtomicState) / 8)};
 }
-inline HHRef hh_mid(const AtomicDev &d) {
+// the cells of midpoint buffer par (d.mid holds two of R replicas each)
+inline HHRef hh_mid(const AtomicDev &d, int par, int R) {
   static_assert(sizeof(AtomicMid) % 8 == 0, "AtomicMid stride");
-  return {reinterpret_cast<const double *>(reinterpret_cast<const char *>(d.mid) +
+  const AtomicMid *m = d.mid + (size_t)par * R;
+  return {reinterpret_cast<const double *>(reinterpret_cast<const char *>(m) +
                                            offsetof(AtomicMid, h)),
           (int)(sizeof(AtomicMid) / 8)};
 }
@@ -222,10 +225,14 @@ hipError_t at_launch_observe(const AtomicDev &d, const AtomicParams &p, int what
 // records to thermo[thermo_slot * R + r].  rescale: launch the velocity
 // rescale (it acts on the replicas whose control asks for it).
 // ev (may be null): events around the force phase, its LJ part and the list
-// rebuild.
+// rebuild.  prev_slot >= 0: the step of slot prev_slot (= ctrl_slot - 1, the
+// step just before, no list rebuild in this one) was launched with fin = 0
+// and its finish (thermostat, barostat, record, rescale) is folded into this
+// step's kick-drift; fin = 0 leaves this step's finish to the next step.
+// The results are bitwise those of finishing every step on its own.
 hipError_t at_launch_step(const AtomicDev &d, const AtomicParams &p, int ctrl_slot,
                           int thermo_slot, int rebuild, int rescale, const StepEvents *ev,
-                          hipStream_t s);
+                          hipStream_t s, int prev_slot = -1, int fin = 1);
 // REBO forces alone into Frebo and rebo_mpart, in the given mode (needs
 // centres and the REBO list).
 hipError_t at_launch_rebo(const AtomicDev &d, const AtomicParams &p, HHRef hh, int mode,

@@ -201,7 +201,7 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   AL(d.ke_part, d.nblk);
   AL(d.hh, 18);
   AL(d.state, R);
-  AL(d.mid, R);
+  AL(d.mid, 2 * R);
   AL(d.ctrl, ring * R);
   AL(d.thermo, ring * R);
   AL(d.flags, kAtFlags);
@@ -356,8 +356,10 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
     int rc = engine_prepare_atomic(e);
     if (rc) return rc;
   }
-  auto launch = [&](int slot, int rebuild, int rescale, const StepEvents *ev) -> hipError_t {
-    return at_launch_step(d, p, slot, slot, rebuild, rescale, ev, e->stream);
+  auto launch = [&](int slot, int rebuild, int rescale, int fold, int fin,
+                    const StepEvents *ev) -> hipError_t {
+    return at_launch_step(d, p, slot, slot, rebuild, rescale, ev, e->stream,
+                          fold ? slot - 1 : -1, fin);
   };
   // with the learned surrogate on, stop at every judge step (global step a
   // multiple of judge_every) to compare it with full REBO

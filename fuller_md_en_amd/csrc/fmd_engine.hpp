@@ -62,9 +62,12 @@ enum : int { kPhaseForce = 0, kPhaseIntra = 1, kPhaseLJ = 2, kPhaseList = 3 };
 constexpr int kEvPerStep = 5;  // StepEvents f0, fm, f1, l0, l1
 int engine_init_common(fmd_engine *e, int ring, int R);
 // Launch of one step into control / record slot `slot`; ev: the step's
-// events, or null when not timed.
-using StepLaunch =
-    std::function<hipError_t(int slot, int rebuild, int rescale, const StepEvents *ev)>;
+// events, or null when not timed.  fold: the step of slot - 1 was launched
+// with fin = 0 and this launch completes it; fin = 0: the next launch (slot
+// + 1, no list rebuild) completes this step.  Engines that finish every
+// step on its own ignore both.
+using StepLaunch = std::function<hipError_t(int slot, int rebuild, int rescale, int fold,
+                                            int fin, const StepEvents *ev)>;
 // Steps first_step+1 .. first_step+nsteps of R replicas: controls
 // ctrl[k * R + r] (null: hold each replica's target tt[r]), records
 // thermo[k * R + r]; list rebuild every e->nlup global steps or on request.

@@ -73,6 +73,47 @@ def test_batch_graph_replay_equals_launches():
         assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
 
 
+def _thermo_row(t):
+    return [t.Ep, t.KE, t.T, t.P, t.V, t.xi] + list(t.h)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_folded_finish_equals_step_by_step(graph):
+    """Within a run each step's finish (thermostat, barostat, record, velocity
+    rescale) is folded into the next step's kick-drift; one call per step
+    finishes every step on its own.  Both give the same records and state bit
+    for bit, with a rescale of one replica inside the run (global steps
+    11-25, a list rebuild at step 20)."""
+    from fuller_md_en_amd import _native as N
+    systems = _systems()[:2]
+    nsteps, R = 15, 2
+    ctrl = [[N.StepCtrl(s["T"], 0.0, 0, 0) for s in systems] for _ in range(nsteps)]
+    ctrl[4][0] = N.StepCtrl(systems[0]["T"], 450.0, N.CTRL_RESCALE, 0)
+    ctrl[12][1] = N.StepCtrl(systems[1]["T"], 700.0, N.CTRL_RESCALE | N.CTRL_RESCALE_ZERO_VG, 0)
+    out = []
+    for per_step in (False, True):
+        print(f"case graph={graph} per_step={per_step}", flush=True)
+        b = airebo.ReplicaBatch([(s["mol_id"], s["mass"]) for s in systems], graph=graph)
+        for r, s in enumerate(systems):
+            b.set_state(r, pos=s["pos"], vel=s["vel"], h=s["h"], npt=_npt(s))
+        if per_step:
+            th = [b.run(10 + k, 1, ctrl=[ctrl[k]])[0] for k in range(nsteps)]
+        else:
+            th = b.run(10, nsteps, ctrl=ctrl)
+        ep = np.array([[_thermo_row(t) for t in row] for row in th])
+        out.append((ep, [b.get_state(r) for r in range(R)]))
+        assert b.counters()[0] == 0
+        b.close()
+    (ep1, st1), (ep2, st2) = out
+    assert np.array_equal(ep1, ep2)
+    for a, b in zip(st1, st2):
+        assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
+        assert np.array_equal(a["h"], b["h"])
+        assert a["npt"].xi == b["npt"].xi and list(a["npt"].Vg) == list(b["npt"].Vg)
+    # the rescaled steps' records carry the targets
+    assert abs(ep1[4, 0, 2] - 450.0) < 1e-9 and abs(ep1[12, 1, 2] - 700.0) < 1e-9
+
+
 def test_batch_replica_forces_match_oracle():
     systems = _systems()
     b = airebo.ReplicaBatch([(s["mol_id"], s["mass"]) for s in systems])
