@@ -2274,7 +2274,10 @@ constexpr int kAtSums = 17;
 constexpr int kFinThreads = 256;
 // The partial arrays are component-major (component k of record r at
 // k * records + r), so the strided loops below read them coalesced.
-template <bool MM>
+// ALL = false (kick-drift blocks that only need the finished scalars, not
+// the record): the time-step path of the AIREBO engine leaves the energy
+// sums out; every sum it does take is the full reduction's, bit for bit.
+template <bool MM, bool ALL = true>
 __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams &p, int r,
                                           double *sums) {
   double v[kAtSums];
@@ -2322,9 +2325,11 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
           lt[u] = le[u] = x0[u] = x9[u] = 0.0;
           if (mu < m1) {
             lt[u] = d.lj_part[mu];
-            le[u] = d.lj_part[9 * nm + mu];
             x0[u] = d.rebo_mpart[mu];
-            x9[u] = d.rebo_mpart[9 * nm + mu];
+            if (ALL) {
+              le[u] = d.lj_part[9 * nm + mu];
+              x9[u] = d.rebo_mpart[9 * nm + mu];
+            }
           }
         }
 #pragma unroll
@@ -2523,10 +2528,16 @@ __global__ __launch_bounds__(256) void k_at_kick_drift_fin(AtomicDev d, AtomicPa
   const StepCtrl cp = d.ctrl[(size_t)prev_slot * p.R + rep];
   const StepCtrl c = d.ctrl[(size_t)ctrl_slot * p.R + rep];
   double s[kAtSums];
-  at_reduce<MM>(d, p, rep, s);
+  // the replica's first block takes every sum for the record, the others
+  // only those the finished scalars need (uniform per block)
+  const bool first = b == d.rep_blk[rep];
+  if (first)
+    at_reduce<MM>(d, p, rep, s);
+  else
+    at_reduce<MM, false>(d, p, rep, s);
   AtFinish o;
   at_finish_math(S, M, cp, p, s, o);
-  const bool lead = b == d.rep_blk[rep] && threadIdx.x == 0;
+  const bool lead = first && threadIdx.x == 0;
   if (lead) at_finish_write(d, p, rep, M, cp, s, o, prev_slot);
   AtHalf1 H;
   at_half1(o, c, p, H);

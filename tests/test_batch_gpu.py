@@ -3,7 +3,8 @@ uneven size, species, temperature and pressure in one engine.
 
 * Each replica of a batch evolves bit for bit as the same system run alone
   in its own engine (same kernels, replica-local reduction order), across a
-  list rebuild, with graph replay on and off.
+  list rebuild, with graph replay on and off: a batch with tiles wider than
+  64 atoms (two atoms per lane) and a batch of C60 replicas (one per lane).
 * Forces of every replica against the FP64 oracle (compat REBO), 1e-5.
 """
 import numpy as np
@@ -31,6 +32,16 @@ def _systems():
     return out
 
 
+def _systems_c60():
+    """Two C60 replicas of different size, temperature and pressure."""
+    out = []
+    for nc, T, P, js in ((3, 300.0, 0.0, 1), (4, 1200.0, 3.0, 4)):
+        s = system.airebo_system(nc, "C60", T=T, seed=60 + js, jitter=0.01, jitter_seed=js)
+        s["T"], s["P"] = T, P
+        out.append(s)
+    return out
+
+
 def _npt(s):
     return airebo.make_npt(s["T"], s["P"], len(s["pos"]))
 
@@ -47,12 +58,15 @@ def _run_batch(systems, nsteps, graph):
     return st, ep
 
 
-def test_batch_replicas_equal_single_runs():
+@pytest.mark.parametrize("specs", ["mixed", "c60"])
+def test_batch_replicas_equal_single_runs(specs):
     systems = _systems()
+    if specs == "c60":
+        systems = [systems[0], _systems_c60()[1]]
     nsteps = 25  # crosses the list rebuild at step 20
     st_b, ep_b = _run_batch(systems, nsteps, graph=True)
     for r, s in enumerate(systems):
-        print(f"replica {r} {SPECS[r]}", flush=True)
+        print(f"case {specs} replica {r}", flush=True)
         eng = airebo.AtomicEngine(s["mol_id"], s["mass"])
         eng.set_state(pos=s["pos"], vel=s["vel"], h=s["h"], npt=_npt(s))
         th = eng.run(0, nsteps)
@@ -85,7 +99,7 @@ def test_folded_finish_equals_step_by_step(graph):
     for bit, with a rescale of one replica inside the run (global steps
     11-25, a list rebuild at step 20)."""
     from fuller_md_en_amd import _native as N
-    systems = _systems()[:2]
+    systems = _systems_c60()
     nsteps, R = 15, 2
     ctrl = [[N.StepCtrl(s["T"], 0.0, 0, 0) for s in systems] for _ in range(nsteps)]
     ctrl[4][0] = N.StepCtrl(systems[0]["T"], 450.0, N.CTRL_RESCALE, 0)
