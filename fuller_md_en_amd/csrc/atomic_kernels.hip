@@ -2242,7 +2242,24 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
     }
   }
   const int i = d.blk_a0[blockIdx.x] + threadIdx.x;
-  double v2[1] = {0};
+  // sums of this block: kinetic energy; for the AIREBO engines also, over
+  // this block's share of its replica's molecules (contiguous, equal counts),
+  // the LJ + REBO virial-trace partials and the two energies (the time
+  // steps' trace-only partials, which the finish then reads per block)
+  double v2[4] = {0, 0, 0, 0};
+  if (p.ff != kFfMM) {
+    const int b = blockIdx.x, r = d.blk_rep[b];
+    const int nbr = d.rep_blk[r + 1] - d.rep_blk[r];
+    const int m0 = d.rep_mol[r], nmr = d.rep_mol[r + 1] - m0;
+    const int per = (nmr + nbr - 1) / nbr;  // <= 256: a block holds 256 atoms
+    const int t = (b - d.rep_blk[r]) * per + (int)threadIdx.x;
+    if ((int)threadIdx.x < per && t < nmr) {
+      const size_t m = (size_t)m0 + t, nm = p.Nm;
+      v2[1] = d.lj_part[m] + d.rebo_mpart[m];
+      v2[2] = d.lj_part[9 * nm + m];
+      v2[3] = d.rebo_mpart[9 * nm + m];
+    }
+  }
   if (i < d.blk_a1[blockIdx.x]) {
     double vx = d.vel[3 * i], vy = d.vel[3 * i + 1], vz = d.vel[3 * i + 2];
     if (kick) {
@@ -2259,7 +2276,7 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
     }
     v2[0] = d.mass[i] * (vx * vx + vy * vy + vz * vz);
   }
-  block_sum<1>(v2, d.ke_part + blockIdx.x);
+  block_sum<4>(v2, d.ke_part + blockIdx.x, d.nblk);
 }
 
 // Fixed-order sums of the partial arrays: s[0] = sum m v^2, s[1..9] = W9,
@@ -2314,35 +2331,15 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
         for (int k = 0; k < 4; k++) v[12 + k] += x[9 + k];
       }
     } else {
-      // eight molecules per thread loaded before any is added (their loads
-      // in flight together), added in molecule order
-      constexpr int U = 8;
-      for (int m = m0 + threadIdx.x; m < m1; m += U * kFinThreads) {
-        double lt[U], le[U], x0[U], x9[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int mu = m + u * kFinThreads;
-          lt[u] = le[u] = x0[u] = x9[u] = 0.0;
-          if (mu < m1) {
-            lt[u] = d.lj_part[mu];
-            x0[u] = d.rebo_mpart[mu];
-            if (ALL) {
-              le[u] = d.lj_part[9 * nm + mu];
-              x9[u] = d.rebo_mpart[9 * nm + mu];
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          if (m + u * kFinThreads >= m1) break;
-          v[1] += lt[u];
-          v[5] += lt[u];
-          v[9] += lt[u];
-          v[11] += le[u];
-          v[1] += x0[u];
-          v[5] += x0[u];
-          v[9] += x0[u];
-          v[10] += x9[u];
+      // the integrator blocks' sums of their molecules (k_at_kick2)
+      for (int b = b0 + threadIdx.x; b < b1; b += kFinThreads) {
+        const double ct = d.ke_part[nb + b];
+        v[1] += ct;
+        v[5] += ct;
+        v[9] += ct;
+        if (ALL) {
+          v[11] += d.ke_part[2 * nb + b];
+          v[10] += d.ke_part[3 * nb + b];
         }
       }
     }

@@ -35,9 +35,12 @@
 //   bonds, rebo_fe         per directed REBO entry (fmd_rebo_passes.hpp), used
 //                          for molecules whose entries do not fit the LDS
 //   rebo_mpart, lj_part    fixed-order partial sums per molecule, and
-//   rebo_part, ke_part     per integrator block (rebo_part: the surrogate's),
-//                          component-major (component k of record r at
-//                          k * count + r)
+//   rebo_part, ke_part     per integrator block (rebo_part: the surrogate's;
+//                          ke_part: kinetic energy, then for AIREBO the sums
+//                          over the block's share of its replica's molecules
+//                          of the LJ + REBO trace partials, the LJ energy and
+//                          the REBO energy), component-major (component k of
+//                          record r at k * count + r)
 //
 // Replicas: R independent systems share every array, concatenated replica
 // after replica (atoms, molecules).  Molecule and atom ranges of replica r
