@@ -2242,12 +2242,14 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
     }
   }
   const int i = d.blk_a0[blockIdx.x] + threadIdx.x;
-  // sums of this block: kinetic energy; for the AIREBO engines also, over
-  // this block's share of its replica's molecules (contiguous, equal counts),
-  // the LJ + REBO virial-trace partials and the two energies (the time
-  // steps' trace-only partials, which the finish then reads per block)
-  double v2[4] = {0, 0, 0, 0};
-  if (p.ff != kFfMM) {
+  // sums of this block: kinetic energy, then over this block's share of its
+  // replica's molecules (contiguous, equal counts) the time steps'
+  // trace-only partials, which the finish reads per block: AIREBO the LJ +
+  // REBO virial trace and the two energies; MM the LJ trace and energy and
+  // the 13 bonded components
+  double v2[kKePart];
+  for (int k = 0; k < kKePart; k++) v2[k] = 0;
+  {
     const int b = blockIdx.x, r = d.blk_rep[b];
     const int nbr = d.rep_blk[r + 1] - d.rep_blk[r];
     const int m0 = d.rep_mol[r], nmr = d.rep_mol[r + 1] - m0;
@@ -2255,9 +2257,15 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
     const int t = (b - d.rep_blk[r]) * per + (int)threadIdx.x;
     if ((int)threadIdx.x < per && t < nmr) {
       const size_t m = (size_t)m0 + t, nm = p.Nm;
-      v2[1] = d.lj_part[m] + d.rebo_mpart[m];
-      v2[2] = d.lj_part[9 * nm + m];
-      v2[3] = d.rebo_mpart[9 * nm + m];
+      if (p.ff == kFfMM) {
+        v2[1] = d.lj_part[m];
+        v2[2] = d.lj_part[9 * nm + m];
+        for (int k = 0; k < 13; k++) v2[3 + k] = d.mm.part[k * nm + m];
+      } else {
+        v2[1] = d.lj_part[m] + d.rebo_mpart[m];
+        v2[2] = d.lj_part[9 * nm + m];
+        v2[3] = d.rebo_mpart[9 * nm + m];
+      }
     }
   }
   if (i < d.blk_a1[blockIdx.x]) {
@@ -2276,7 +2284,10 @@ __global__ __launch_bounds__(256) void k_at_kick2(AtomicDev d, AtomicParams p, i
     }
     v2[0] = d.mass[i] * (vx * vx + vy * vy + vz * vz);
   }
-  block_sum<4>(v2, d.ke_part + blockIdx.x, d.nblk);
+  if (p.ff == kFfMM)
+    block_sum<kKePart>(v2, d.ke_part + blockIdx.x, d.nblk);
+  else
+    block_sum<4>(v2, d.ke_part + blockIdx.x, d.nblk);
 }
 
 // Fixed-order sums of the partial arrays: s[0] = sum m v^2, s[1..9] = W9,
@@ -2318,17 +2329,22 @@ __device__ __forceinline__ void at_reduce(const AtomicDev &d, const AtomicParams
     // (the same sums as below, in the same order; the per-pair-image LJ
     // kernel of narrow cells keeps the full tensor)
     if (MM) {
-      for (int m = m0 + threadIdx.x; m < m1; m += kFinThreads) {
-        const double lt = d.lj_part[m], le = d.lj_part[9 * nm + m];
+      // the integrator blocks' sums of their molecules (k_at_kick2)
+      for (int b = b0 + threadIdx.x; b < b1; b += kFinThreads) {
+        const double lt = d.ke_part[nb + b];
         double x[13];
 #pragma unroll
-        for (int k = 0; k < 13; k++) x[k] = d.mm.part[k * nm + m];
+        for (int k = 0; k < 9; k++) x[k] = d.ke_part[(3 + k) * nb + b];
         v[1] += lt;
         v[5] += lt;
         v[9] += lt;
-        v[11] += le;
         for (int k = 0; k < 9; k++) v[1 + k] += x[k];
-        for (int k = 0; k < 4; k++) v[12 + k] += x[9 + k];
+        if (ALL) {
+#pragma unroll
+          for (int k = 9; k < 13; k++) x[k] = d.ke_part[(3 + k) * nb + b];
+          v[11] += d.ke_part[2 * nb + b];
+          for (int k = 0; k < 4; k++) v[12 + k] += x[9 + k];
+        }
       }
     } else {
       // the integrator blocks' sums of their molecules (k_at_kick2)

@@ -36,11 +36,13 @@
 //                          for molecules whose entries do not fit the LDS
 //   rebo_mpart, lj_part    fixed-order partial sums per molecule, and
 //   rebo_part, ke_part     per integrator block (rebo_part: the surrogate's;
-//                          ke_part: kinetic energy, then for AIREBO the sums
-//                          over the block's share of its replica's molecules
-//                          of the LJ + REBO trace partials, the LJ energy and
-//                          the REBO energy), component-major (component k of
-//                          record r at k * count + r)
+//                          ke_part, kKePart components: kinetic energy,
+//                          then sums over the block's share of its replica's
+//                          molecules of the trace-only partials: AIREBO the
+//                          LJ + REBO trace, the LJ energy and the REBO
+//                          energy; MM the LJ trace, the LJ energy and the 13
+//                          bonded components), component-major (component k
+//                          of record r at k * count + r)
 //
 // Replicas: R independent systems share every array, concatenated replica
 // after replica (atoms, molecules).  Molecule and atom ranges of replica r
@@ -67,6 +69,7 @@
 namespace fmd {
 
 constexpr int kAtPart = 10;  // W9 + E
+constexpr int kKePart = 16;  // AtomicDev::ke_part components per block
 constexpr int kAtFlags = 5;  // AtomicDev::flags
 constexpr int kAlDriftSlots = 8;  // AtomicDev::al_dmax entries per replica
 constexpr float kAlDriftFloor = 0.125f;  // drifts below it count as it (A)

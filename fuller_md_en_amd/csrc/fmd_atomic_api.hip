@@ -198,7 +198,7 @@ int atomic_alloc(AtomicDev &d, const AtomicParams &p, int ring, int nblk) {
   AL(d.rebo_part, (size_t)d.nblk * kAtPart);
   AL(d.rebo_mpart, Nm * kAtPart);
   AL(d.lj_part, Nm * kAtPart);
-  AL(d.ke_part, 4 * (size_t)d.nblk);
+  AL(d.ke_part, kKePart * (size_t)d.nblk);
   AL(d.hh, 18);
   AL(d.state, R);
   AL(d.mid, 2 * R);
