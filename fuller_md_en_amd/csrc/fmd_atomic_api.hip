@@ -356,10 +356,11 @@ int engine_run_atomic(fmd_engine *e, int first_step, int nsteps, const fmd_step_
     int rc = engine_prepare_atomic(e);
     if (rc) return rc;
   }
+  const bool fold_ok = d.nblk <= kFoldMaxBlocks;
   auto launch = [&](int slot, int rebuild, int rescale, int fold, int fin,
                     const StepEvents *ev) -> hipError_t {
     return at_launch_step(d, p, slot, slot, rebuild, rescale, ev, e->stream,
-                          fold ? slot - 1 : -1, fin);
+                          fold_ok && fold ? slot - 1 : -1, fold_ok ? fin : 1);
   };
   // with the learned surrogate on, stop at every judge step (global step a
   // multiple of judge_every) to compare it with full REBO
