@@ -72,7 +72,7 @@ constexpr int kAtPart = 10;  // W9 + E
 constexpr int kKePart = 16;  // AtomicDev::ke_part components per block
 // Steps fold the previous step's finish into their kick-drift only up to
 // this many integrator blocks: every block repeats the finish, which pays
-// while the grid runs in about one round (the 8x8x8 C60 crystal: 480
+// while the whole grid is resident at once (the 8x8x8 C60 crystal: 480
 // blocks) and costs more than the separate finish once blocks queue up
 // (64 replicas, 7,100 blocks: 2.210 -> 2.241 ms per step).  Either way the
 // results are the same bit for bit.
